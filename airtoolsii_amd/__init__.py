@@ -1,0 +1,21 @@
+"""airtoolsii_amd -- an MI355X-native engine for the SIRT methods of AIR Tools II.
+
+Call surface (AIR Tools II, MATLAB -> Python):
+    sirt, landweber, cimmino, cav, drop, sart          sirt/*.m
+    paralleltomo, fanlineartomo                        testprobs/*.m (traced on the GPU)
+    phantomgallery                                     testprobs/phantomgallery.m
+    train_relaxpar                                     train/train_relaxpar.m
+Operators follow the afun convention A(v,'notransp'|'transp'|'size').
+The compute runs in libairhip.so (HIP kernels for gfx950, C ABI in
+include/airhip.h); build it with ``python -m airtoolsii_amd.build``.
+"""
+from .operators import AirOperator, as_operator, from_matlab_csc
+from .phantoms import add_noise, phantomgallery
+from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, spectral_radius,
+                   weights)
+from .testprobs import fanlineartomo, paralleltomo
+from .train import train_relaxpar
+
+__all__ = ["AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
+           "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "spectral_radius", "weights",
+           "fanlineartomo", "paralleltomo", "train_relaxpar"]

@@ -1,0 +1,897 @@
+// air_engine.hip -- the C ABI of libairhip.so: engine handle, operators,
+// weights, relaxation parameter and the device-resident SIRT driver.
+//
+// Host logic mirrors the reference MATLAB drivers:
+//   sirt/sirt.m:132-281          driver (stop check at k=0, save list K, info)
+//   auxil/check_inputs.m:59-260  validation errors
+//   auxil/get_mfun_dfun.m:47-250 M/D diagonals (matrix branch formulas)
+//   auxil/calc_relaxpar.m:89-244 rho, default/scalar/line/psi strategies
+//   auxil/afun_matrix.m:44-53    the afun product convention
+// The heavy work (tracing, products, reductions) runs in HIP kernels on the
+// engine's stream; the host only sequences launches.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "air_internal.h"
+
+namespace air {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+struct SirtState {
+    bool prepared = false;
+    int kmax = 0, k = 0;
+    int relax_kind = AIR_RELAX_DEFAULT;
+    std::vector<double> lam;       // per-iteration relaxpar (scalar strategies)
+    double rho = NAN;
+    StopParams sp{};
+    int64_t m = 0, n = 0;
+    bool hasM = false, hasD = false;
+    DevBuf b, x, r, Mr, M, D, g, lbv, ubv, partial, hist, ctrl, Ax;
+    SirtBufs bufs{};
+    double setup_s = 0, iter_s = 0;
+};
+
+}  // namespace air
+
+struct air_engine {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    int kind = 0;   // 0 none, 1 stored matrix, 2 matrix-free geometry
+    int64_t m = 0, n = 0;
+    air::DevSparse A, AT;
+    air::Shape shA, shAT;
+    air::Geometry geo;
+    air::SirtState st;
+    // per-kernel timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    int ev_used = 0;
+    double t_rows_ms = 0, t_cols_ms = 0;
+    int64_t launches = 0;
+};
+
+using namespace air;
+
+namespace {
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return AIR_OK;
+    } catch (const Err& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return AIR_E_NOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return AIR_E_ARG;
+    }
+}
+
+void use_device(air_engine* e) { AIR_HIP(hipSetDevice(e->device)); }
+
+void need_operator(air_engine* e) {
+    AIR_CHECK(e != nullptr, AIR_E_ARG, "null engine");
+    AIR_CHECK(e->kind != 0, AIR_E_STATE, "no operator set: call air_set_matrix_* or air_build_* first");
+}
+
+template <typename T>
+void upload(DevBuf& d, const T* h, int64_t count, hipStream_t s) {
+    d.alloc(sizeof(T) * std::max<int64_t>(count, 1));
+    if (count > 0) AIR_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+// upload FP64 host values, narrow to FP32 on the device
+void upload_f32(DevBuf& d, const double* h, int64_t count, hipStream_t s) {
+    DevBuf tmp;
+    upload(tmp, h, count, s);
+    d.alloc(sizeof(float) * std::max<int64_t>(count, 1));
+    vec_f64_to_f32(tmp.as<double>(), d.as<float>(), count, s);
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
+void download_f64(const float* d, double* h, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    DevBuf tmp;
+    tmp.alloc(sizeof(double) * count);
+    vec_f32_to_f64(d, tmp.as<double>(), count, s);
+    AIR_HIP(hipMemcpyAsync(h, tmp.p, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
+template <typename V>
+__global__ void k_narrow(const double* in, V* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = from_f64<V>(in[i]);
+}
+template <typename V>
+__global__ void k_widen(const V* in, double* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = to_f64(in[i]);
+}
+
+void reset_operator(air_engine* e) {
+    e->A = DevSparse();
+    e->AT = DevSparse();
+    e->geo.rays.release();
+    e->geo.host_rays.clear();
+    e->kind = 0;
+    e->m = e->n = 0;
+    e->st.prepared = false;
+}
+
+void finish_matrix(air_engine* e) {
+    csr_to_csc(e->A, e->AT, e->stream);
+    e->shA = default_shape(e->A, e->num_cu);
+    e->shAT = default_shape(e->AT, e->num_cu);
+    e->kind = 1;
+    e->m = e->A.nrows;
+    e->n = e->A.ncols;
+}
+
+// y = A x (trans=0) or y = A' x (trans=1), device FP32 in / out.
+void apply_dev(air_engine* e, int trans, const float* in, float* out) {
+    hipStream_t s = e->stream;
+    if (e->kind == 1) {
+        if (trans) spmv(e->AT, e->shAT, in, out, s);
+        else spmv(e->A, e->shA, in, out, s);
+    } else {
+        if (trans) {
+            DevBuf acc;
+            acc.alloc(sizeof(double) * std::max<int64_t>(e->n, 1));
+            zero_f64(acc.as<double>(), e->n, s);
+            mf_adjoint(e->geo, in, acc.as<double>(), s);
+            vec_f64_to_f32(acc.as<double>(), out, e->n, s);
+            AIR_HIP(hipStreamSynchronize(s));
+        } else {
+            mf_forward(e->geo, in, out, s);
+        }
+    }
+}
+
+// FP64 row / column statistics of the operator.
+struct Stats {
+    std::vector<double> rsq, rab, rsqw, cab, cnz;
+};
+
+void operator_stats(air_engine* e, int method, Stats& S) {
+    hipStream_t s = e->stream;
+    const int64_t m = e->m, n = e->n;
+    DevBuf rsq, rab, rsqw, cab, cnz, w;
+    rsq.alloc(8 * std::max<int64_t>(m, 1));
+    rab.alloc(8 * std::max<int64_t>(m, 1));
+    rsqw.alloc(8 * std::max<int64_t>(m, 1));
+    cab.alloc(8 * std::max<int64_t>(n, 1));
+    cnz.alloc(8 * std::max<int64_t>(n, 1));
+    S.rsq.assign(m, 0); S.rab.assign(m, 0); S.rsqw.assign(m, 0); S.cab.assign(n, 0); S.cnz.assign(n, 0);
+    bool need_cols = method == AIR_CAV || method == AIR_DROP || method == AIR_SART;
+    if (e->kind == 1) {
+        if (need_cols) row_stats(e->AT, nullptr, nullptr, cab.as<double>(), nullptr, cnz.as<double>(), s);
+        if (method == AIR_CAV) {
+            row_stats(e->A, cnz.as<double>(), rsq.as<double>(), rab.as<double>(), rsqw.as<double>(), nullptr, s);
+        } else {
+            row_stats(e->A, nullptr, rsq.as<double>(), rab.as<double>(), nullptr, nullptr, s);
+        }
+    } else {
+        if (need_cols) {
+            DevBuf cnt;
+            cnt.alloc(4 * std::max<int64_t>(n, 1));
+            AIR_HIP(hipMemsetAsync(cnt.p, 0, 4 * n, s));
+            zero_f64(cab.as<double>(), n, s);
+            mf_col_stats(e->geo, cab.as<double>(), cnt.as<int>(), s);
+            std::vector<int> hc(n);
+            AIR_HIP(hipMemcpyAsync(hc.data(), cnt.p, 4 * n, hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+            for (int64_t j = 0; j < n; ++j) S.cnz[j] = hc[j];
+            upload(cnz, S.cnz.data(), n, s);
+        }
+        mf_row_stats(e->geo, method == AIR_CAV ? cnz.as<double>() : nullptr, rsq.as<double>(), rab.as<double>(),
+                     method == AIR_CAV ? rsqw.as<double>() : nullptr, s);
+    }
+    AIR_HIP(hipMemcpyAsync(S.rsq.data(), rsq.p, 8 * m, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipMemcpyAsync(S.rab.data(), rab.p, 8 * m, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipMemcpyAsync(S.rsqw.data(), rsqw.p, 8 * m, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipMemcpyAsync(S.cab.data(), cab.p, 8 * n, hipMemcpyDeviceToHost, s));
+    if (e->kind == 1) AIR_HIP(hipMemcpyAsync(S.cnz.data(), cnz.p, 8 * n, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
+inline double inf_to_zero(double v) { return std::isinf(v) ? 0.0 : v; }
+
+// get_mfun_dfun.m matrix-branch formulas; identity is returned as all ones.
+void compute_weights(air_engine* e, int method, std::vector<double>& M, std::vector<double>& D) {
+    const int64_t m = e->m, n = e->n;
+    M.assign(m, 1.0);
+    D.assign(n, 1.0);
+    if (method == AIR_LANDWEBER) return;                       // :50-54
+    AIR_CHECK(method >= AIR_CIMMINO && method <= AIR_SART, AIR_E_ARG, "SIRT method not defined.");
+    Stats S;
+    operator_stats(e, method, S);
+    switch (method) {
+    case AIR_CIMMINO:                                          // :56-88
+        for (int64_t i = 0; i < m; ++i) M[i] = inf_to_zero((1.0 / (double)m) * (1.0 / S.rsq[i]));
+        break;
+    case AIR_CAV:                                              // :90-138
+        for (int64_t i = 0; i < m; ++i) M[i] = inf_to_zero(1.0 / S.rsqw[i]);
+        break;
+    case AIR_DROP:                                             // :140-198
+        for (int64_t i = 0; i < m; ++i) M[i] = inf_to_zero(1.0 / S.rsq[i]);
+        for (int64_t j = 0; j < n; ++j) D[j] = inf_to_zero(1.0 / S.cnz[j]);
+        break;
+    case AIR_SART:                                             // :200-241
+        for (int64_t i = 0; i < m; ++i) M[i] = inf_to_zero(1.0 / S.rab[i]);
+        for (int64_t j = 0; j < n; ++j) D[j] = inf_to_zero(1.0 / S.cab[j]);
+        break;
+    }
+}
+
+// Largest eigenvalue of the symmetric T^(1/2) A' M A T^(1/2) (= rho(T A' M A)),
+// Lanczos with a Sturm-bisection solve of the tridiagonal matrix.
+double lanczos_rho(air_engine* e, const std::vector<double>& M, const std::vector<double>& D) {
+    hipStream_t s = e->stream;
+    const int64_t m = e->m, n = e->n;
+    DevBuf dM, dv, dy, dg;
+    upload_f32(dM, M.data(), m, s);
+    dv.alloc(4 * std::max<int64_t>(n, 1));
+    dy.alloc(4 * std::max<int64_t>(m, 1));
+    dg.alloc(4 * std::max<int64_t>(n, 1));
+    std::vector<double> sD(n), v(n, 1.0 / std::sqrt((double)n)), vprev(n, 0.0), w(n);
+    std::vector<float> hf(n);
+    for (int64_t j = 0; j < n; ++j) sD[j] = std::sqrt(D[j]);
+    auto op = [&](const std::vector<double>& in, std::vector<double>& out) {
+        for (int64_t j = 0; j < n; ++j) hf[j] = (float)(sD[j] * in[j]);
+        AIR_HIP(hipMemcpyAsync(dv.p, hf.data(), 4 * n, hipMemcpyHostToDevice, s));
+        apply_dev(e, 0, dv.as<float>(), dy.as<float>());
+        vec_mul_f32(dM.as<float>(), dy.as<float>(), m, s);
+        apply_dev(e, 1, dy.as<float>(), dg.as<float>());
+        AIR_HIP(hipMemcpyAsync(hf.data(), dg.p, 4 * n, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        for (int64_t j = 0; j < n; ++j) out[j] = sD[j] * (double)hf[j];
+    };
+    std::vector<double> al, be;
+    double beta = 0, theta_prev = 0, theta = 0;
+    int stable = 0;
+    auto top_eig = [&](int k) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < k; ++i) {
+            double r = (i > 0 ? std::fabs(be[i - 1]) : 0) + (i + 1 < k ? std::fabs(be[i]) : 0);
+            lo = std::min(lo, al[i] - r);
+            hi = std::max(hi, al[i] + r);
+        }
+        auto count_below = [&](double x) {
+            int c = 0;
+            double q = al[0] - x;
+            if (q < 0) c++;
+            for (int i = 1; i < k; ++i) {
+                if (q == 0) q = 1e-300;
+                q = al[i] - x - be[i - 1] * be[i - 1] / q;
+                if (q < 0) c++;
+            }
+            return c;
+        };
+        for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(hi)); ++it) {
+            double mid = 0.5 * (lo + hi);
+            if (count_below(mid) >= k) hi = mid; else lo = mid;
+        }
+        return 0.5 * (lo + hi);
+    };
+    for (int it = 0; it < 300; ++it) {
+        op(v, w);
+        double a = 0;
+        for (int64_t j = 0; j < n; ++j) { w[j] -= beta * vprev[j]; }
+        for (int64_t j = 0; j < n; ++j) a += w[j] * v[j];
+        for (int64_t j = 0; j < n; ++j) w[j] -= a * v[j];
+        double bnew = 0;
+        for (int64_t j = 0; j < n; ++j) bnew += w[j] * w[j];
+        bnew = std::sqrt(bnew);
+        al.push_back(a);
+        theta = top_eig((int)al.size());
+        if (it > 5 && std::fabs(theta - theta_prev) <= 1e-10 * std::fabs(theta)) {
+            if (++stable >= 3) break;
+        } else stable = 0;
+        theta_prev = theta;
+        if (bnew <= 1e-14 * std::fabs(theta) || bnew == 0) break;
+        be.push_back(bnew);
+        vprev = v;
+        for (int64_t j = 0; j < n; ++j) v[j] = w[j] / bnew;
+        beta = bnew;
+    }
+    return theta;
+}
+
+double calczeta_one(int k) {
+    // calc_relaxpar.m:217-244: Newton from 1, 6 steps; Horner on C = [0 2k-1 -ones(1,k-1)]
+    auto fung = [k](double x, double& f, double& df) {
+        f = 0.0;
+        df = 0.0;
+        for (int i = 1; i <= k; ++i) {
+            double c = (i == 1) ? (double)(2 * k - 1) : -1.0;
+            df = df * x + f;
+            f = f * x + c;
+        }
+    };
+    double x = 1.0, f, df;
+    fung(x, f, df);
+    double h = f / df;
+    for (int it = 0; it < 6; ++it) {
+        x = x - h;
+        fung(x, f, df);
+        h = f / df;
+    }
+    return x;
+}
+
+// calc_relaxpar.m:105-171: per-iteration relaxpar for scalar strategies.
+void relax_schedule(int kind, double relaxpar, const double* vec, double rho, int kmax, std::vector<double>& lam) {
+    lam.assign(std::max(kmax, 0), 0.0);
+    switch (kind) {
+    case AIR_RELAX_DEFAULT:
+        for (auto& v : lam) v = 1.9 / rho;
+        break;
+    case AIR_RELAX_SCALAR:
+        for (auto& v : lam) v = relaxpar;
+        break;
+    case AIR_RELAX_VECTOR:
+        AIR_CHECK(vec != nullptr || kmax == 0, AIR_E_ARG, "relax_vec must hold kmax values");
+        for (int i = 0; i < kmax; ++i) lam[i] = vec[i];
+        break;
+    case AIR_RELAX_PSI1: case AIR_RELAX_PSI1MOD: case AIR_RELAX_PSI2: case AIR_RELAX_PSI2MOD: {
+        AIR_CHECK(kmax > 2, AIR_E_ARG, "psi relaxation strategies need kmax > 2 (calczeta of 2:kmax-1)");
+        bool psi2 = kind == AIR_RELAX_PSI2 || kind == AIR_RELAX_PSI2MOD;
+        double nu = kind == AIR_RELAX_PSI1MOD ? 2.0 : (kind == AIR_RELAX_PSI2MOD ? 1.5 : 1.0);
+        lam[0] = std::sqrt(2.0) / rho;
+        lam[1] = std::sqrt(2.0) / rho;
+        for (int k = 2; k < kmax; ++k) {
+            double z = calczeta_one(k);
+            double v = psi2 ? (nu * 2) * (1 - z) / ((1 - std::pow(z, (double)k)) * (1 - std::pow(z, (double)k)))
+                            : (nu * 2) * (1 - z);
+            if (!psi2 && nu == 1.0) v = 2 * (1 - z);
+            lam[k] = v / rho;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+void timing_begin(air_engine* e, int niter) {
+    e->ev_used = 0;
+    if (!e->timing) return;
+    size_t need = 3 * (size_t)std::max(niter, 1);
+    while (e->ev.size() < need) {
+        hipEvent_t v;
+        AIR_HIP(hipEventCreate(&v));
+        e->ev.push_back(v);
+    }
+}
+
+void record(air_engine* e, int slot) {
+    if (!e->timing) return;
+    AIR_HIP(hipEventRecord(e->ev[(size_t)e->ev_used * 3 + slot], e->stream));
+}
+
+// one SIRT iteration k (sirt.m:213-243)
+void enqueue_iteration(air_engine* e, int k) {
+    SirtState& st = e->st;
+    hipStream_t s = e->stream;
+    const bool line = st.relax_kind == AIR_RELAX_LINE;
+    double lam = line ? 0.0 : st.lam[k - 1];
+    if (e->kind == 1) {
+        record(e, 0);
+        if (line) launch_line_update(e->AT, e->shAT, st.bufs, k, s);
+        else launch_update(e->AT, e->shAT, st.bufs, lam, k, s);
+        record(e, 1);
+        launch_resid(e->A, e->shA, st.bufs, st.sp, k, s);
+        record(e, 2);
+        if (e->timing) e->ev_used++;
+    } else {
+        record(e, 0);
+        launch_mf_update(e->geo, e->n, st.bufs, lam, line ? 1 : 0, k, s);
+        record(e, 1);
+        launch_mf_resid(e->geo, st.bufs, st.sp, k, st.Ax.as<float>(), s);
+        record(e, 2);
+        if (e->timing) e->ev_used++;
+    }
+}
+
+// after the stream has drained: sum the per-kernel event intervals
+void timing_end(air_engine* e) {
+    if (!e->timing) return;
+    for (int i = 0; i < e->ev_used; ++i) {
+        float a = 0, b = 0;
+        AIR_HIP(hipEventElapsedTime(&a, e->ev[3 * i], e->ev[3 * i + 1]));
+        AIR_HIP(hipEventElapsedTime(&b, e->ev[3 * i + 1], e->ev[3 * i + 2]));
+        e->t_cols_ms += a;
+        e->t_rows_ms += b;
+        e->launches += 1;
+    }
+    e->ev_used = 0;
+}
+
+void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, const double* x0, int kmax) {
+    need_operator(e);
+    AIR_CHECK(o != nullptr && b != nullptr, AIR_E_ARG, "Too few input arguments.");
+    AIR_CHECK(kmax >= 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
+    AIR_CHECK(o->stoprule != AIR_STOP_NCP, AIR_E_UNSUPPORTED, "NCP stopping rule is not available on the device");
+    AIR_CHECK(o->stoprule >= 0 && o->stoprule <= 3, AIR_E_ARG, "The chosen stopping rule is not valid.");
+    if (o->stoprule == AIR_STOP_DP || o->stoprule == AIR_STOP_ME)
+        AIR_CHECK(!std::isnan(o->taudelta), AIR_E_ARG,
+                  "For stopping rules DP and ME a scalar taudelta must be given in options.");
+    auto t0 = std::chrono::steady_clock::now();
+    use_device(e);
+    hipStream_t s = e->stream;
+    SirtState& st = e->st;
+    st.prepared = false;
+    st.m = e->m;
+    st.n = e->n;
+    const int64_t m = st.m, n = st.n;
+    // M, D (get_mfun_dfun.m)
+    std::vector<double> M, D;
+    if (o->method == AIR_CUSTOM) {
+        M.assign(m, 1.0);
+        D.assign(n, 1.0);
+        if (o->M) std::copy(o->M, o->M + m, M.begin());
+        if (o->D) std::copy(o->D, o->D + n, D.begin());
+        st.hasM = o->M != nullptr;
+        st.hasD = o->D != nullptr;
+    } else {
+        compute_weights(e, o->method, M, D);
+        st.hasM = o->method != AIR_LANDWEBER;
+        st.hasD = o->method == AIR_DROP || o->method == AIR_SART;
+    }
+    // rho and relaxpar (sirt.m:164-169, calc_relaxpar.m:89-171)
+    double rho = o->rho;
+    if (o->method == AIR_SART) rho = 1.0;
+    // calc_relaxpar.m:91-103 computes rho whenever it is not given
+    if (std::isnan(rho)) rho = lanczos_rho(e, M, D);
+    st.rho = rho;
+    st.relax_kind = o->relax_kind;
+    relax_schedule(o->relax_kind, o->relaxpar, o->relax_vec, rho, kmax, st.lam);
+    // device vectors
+    upload_f32(st.b, b, m, s);
+    if (x0) upload_f32(st.x, x0, n, s);
+    else {
+        std::vector<double> z(n, 0.0);
+        upload_f32(st.x, z.data(), n, s);
+    }
+    if (st.hasM) upload_f32(st.M, M.data(), m, s);
+    if (st.hasD) upload_f32(st.D, D.data(), n, s);
+    st.r.alloc(4 * std::max<int64_t>(m, 1));
+    st.Mr.alloc(4 * std::max<int64_t>(m, 1));
+    st.g.alloc(8 * std::max<int64_t>(n, 1));
+    st.Ax.alloc(4 * std::max<int64_t>(m, 1));
+    st.partial.alloc(8 * 16384);
+    st.hist.alloc(8 * (size_t)(kmax + 1));
+    st.ctrl.alloc(sizeof(Ctrl));
+    AIR_HIP(hipMemsetAsync(st.ctrl.p, 0, sizeof(Ctrl), s));
+    AIR_HIP(hipMemsetAsync(st.r.p, 0, 4 * std::max<int64_t>(m, 1), s));
+    SirtBufs& B = st.bufs;
+    B = SirtBufs{};
+    B.b = st.b.as<float>();
+    B.M = st.hasM ? st.M.as<float>() : nullptr;
+    B.D = st.hasD ? st.D.as<float>() : nullptr;
+    B.x = st.x.as<float>();
+    B.r = st.r.as<float>();
+    B.Mr = st.Mr.as<float>();
+    B.g = st.g.as<double>();
+    B.lb_kind = o->lb_kind;
+    B.ub_kind = o->ub_kind;
+    B.lb = (float)o->lb;
+    B.ub = (float)o->ub;
+    if (o->lb_kind == AIR_BOUND_VECTOR) {
+        AIR_CHECK(o->lb_vec, AIR_E_ARG, "lbound vector missing");
+        upload_f32(st.lbv, o->lb_vec, n, s);
+        B.lbv = st.lbv.as<float>();
+    }
+    if (o->ub_kind == AIR_BOUND_VECTOR) {
+        AIR_CHECK(o->ub_vec, AIR_E_ARG, "ubound vector missing");
+        upload_f32(st.ubv, o->ub_vec, n, s);
+        B.ubv = st.ubv.as<float>();
+    }
+    B.partial = st.partial.as<double>();
+    B.hist = st.hist.as<double>();
+    B.ctrl = st.ctrl.as<Ctrl>();
+    st.sp.type = o->stoprule;
+    st.sp.kmax = kmax;
+    st.sp.taudelta = o->taudelta;
+    st.kmax = kmax;
+    st.k = 0;
+    // r0 = b - A x0 and the k = 0 check (sirt.m:153-162)
+    if (e->kind == 1) launch_resid(e->A, e->shA, B, st.sp, 0, s);
+    else launch_mf_resid(e->geo, B, st.sp, 0, st.Ax.as<float>(), s);
+    AIR_HIP(hipStreamSynchronize(s));
+    st.prepared = true;
+    st.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st.iter_s = 0;
+}
+
+int read_stop(air_engine* e) {
+    Ctrl c;
+    AIR_HIP(hipMemcpyAsync(&c, e->st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
+    AIR_HIP(hipStreamSynchronize(e->stream));
+    return c.stop;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* air_last_error(void) { return g_last_error.c_str(); }
+
+int air_create(int device, air_engine** out) {
+    return guard([&] {
+        AIR_CHECK(out != nullptr, AIR_E_ARG, "null output handle");
+        int count = 0;
+        AIR_HIP(hipGetDeviceCount(&count));
+        AIR_CHECK(device >= 0 && device < count, AIR_E_ARG, "device index out of range");
+        std::unique_ptr<air_engine> e(new air_engine());
+        e->device = device;
+        AIR_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        AIR_HIP(hipGetDeviceProperties(&prop, device));
+        e->num_cu = prop.multiProcessorCount;
+        AIR_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        *out = e.release();
+    });
+}
+
+void air_destroy(air_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& v : e->ev) (void)hipEventDestroy(v);
+    reset_operator(e);
+    e->st = SirtState();
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int air_synchronize(air_engine* e) {
+    return guard([&] {
+        AIR_CHECK(e, AIR_E_ARG, "null engine");
+        AIR_HIP(hipStreamSynchronize(e->stream));
+    });
+}
+
+int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* colind,
+                       const double* val, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && rowptr && m >= 0 && n >= 0, AIR_E_ARG, "invalid CSR arguments");
+        AIR_CHECK(n <= INT32_MAX, AIR_E_ARG, "n must fit int32 column indices");
+        use_device(e);
+        reset_operator(e);
+        hipStream_t s = e->stream;
+        int64_t nnz = rowptr[m];
+        AIR_CHECK(rowptr[0] == 0 && nnz >= 0, AIR_E_ARG, "rowptr must start at 0");
+        for (int64_t i = 0; i < m; ++i) AIR_CHECK(rowptr[i + 1] >= rowptr[i], AIR_E_ARG, "rowptr must be nondecreasing");
+        for (int64_t q = 0; q < nnz; ++q)
+            AIR_CHECK(colind[q] >= 0 && colind[q] < n, AIR_E_ARG, "column index out of range");
+        DevSparse& A = e->A;
+        A.nrows = m;
+        A.ncols = n;
+        A.nnz = nnz;
+        A.dtype = dtype;
+        (void)dtype_size(dtype);
+        upload(A.ptr, rowptr, m + 1, s);
+        upload(A.idx, colind, nnz, s);
+        DevBuf v64;
+        upload(v64, val, nnz, s);
+        A.val.alloc(dtype_size(dtype) * std::max<int64_t>(nnz, 1));
+        dispatch_dtype(dtype, [&](auto v) {
+            using V = decltype(v);
+            if (nnz > 0) k_narrow<V><<<4096, 256, 0, s>>>(v64.as<double>(), A.val.as<V>(), nnz);
+            return 0;
+        });
+        AIR_HIP(hipGetLastError());
+        AIR_HIP(hipStreamSynchronize(s));
+        finish_matrix(e);
+    });
+}
+
+int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowind,
+                       const double* val, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && colptr && m >= 0 && n >= 0, AIR_E_ARG, "invalid CSC arguments");
+        AIR_CHECK(m <= INT32_MAX, AIR_E_ARG, "m must fit int32 row indices");
+        int64_t nnz = colptr[n];
+        std::vector<int32_t> ri(nnz);
+        for (int64_t q = 0; q < nnz; ++q) {
+            AIR_CHECK(rowind[q] >= 0 && rowind[q] < m, AIR_E_ARG, "row index out of range");
+            ri[q] = (int32_t)rowind[q];
+        }
+        // The CSC of A is the CSR of A': upload it as such; finish_matrix then
+        // builds its transpose, which is the CSR of A.  Swap the two roles.
+        int rc = air_set_matrix_csr(e, n, m, colptr, ri.data(), val, dtype);
+        if (rc != AIR_OK) throw Err{rc, g_last_error};
+        std::swap(e->A, e->AT);
+        std::swap(e->shA, e->shAT);
+        e->m = e->A.nrows;
+        e->n = e->A.ncols;
+    });
+}
+
+static int build_geometry(air_engine* e, int N, int64_t nrays, int edge, int matrix_free, int dtype) {
+    hipStream_t s = e->stream;
+    e->geo.N = N;
+    e->geo.edge_check = edge;
+    e->geo.nrays = nrays;
+    upload(e->geo.rays, e->geo.host_rays.data(), nrays, s);
+    AIR_HIP(hipStreamSynchronize(s));
+    if (matrix_free) {
+        e->kind = 2;
+        e->m = nrays;
+        e->n = (int64_t)N * N;
+    } else {
+        trace_to_csr(e->geo, dtype, e->A, s);
+        finish_matrix(e);
+    }
+    return AIR_OK;
+}
+
+int air_build_paralleltomo(air_engine* e, int32_t N, const double* theta, int32_t nA, int32_t p, double d,
+                           int matrix_free, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && theta && N > 0 && nA >= 0 && p > 0, AIR_E_ARG, "invalid paralleltomo arguments");
+        AIR_CHECK((int64_t)N * N <= INT32_MAX, AIR_E_ARG, "N too large for int32 column indices");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        host_rays_paralleltomo(N, theta, nA, p, d, e->geo.host_rays);
+        build_geometry(e, N, (int64_t)nA * p, 1, matrix_free, dtype);
+    });
+}
+
+int air_build_fanlineartomo(air_engine* e, int32_t N, const double* theta, int32_t nA, int32_t p, double R,
+                            double dw, double sd, int matrix_free, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && theta && N > 0 && nA >= 0 && p > 0, AIR_E_ARG, "invalid fanlineartomo arguments");
+        AIR_CHECK((int64_t)N * N <= INT32_MAX, AIR_E_ARG, "N too large for int32 column indices");
+        // fanlineartomo.m:75-90,112-114
+        double sdN = sd * N, dwN = dw * N, RN = R * N;
+        AIR_CHECK(!(RN < std::sqrt(2.0) / 2 * N), AIR_E_ARG, "R must be greater than half squareroot 2");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        host_rays_fanlineartomo(N, theta, nA, p, RN, dwN, sdN, e->geo.host_rays);
+        build_geometry(e, N, (int64_t)nA * p, 0, matrix_free, dtype);
+    });
+}
+
+int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz) {
+    return guard([&] {
+        need_operator(e);
+        if (m) *m = e->m;
+        if (n) *n = e->n;
+        if (nnz) *nnz = e->kind == 1 ? e->A.nnz : -1;
+    });
+}
+
+int air_get_csr(air_engine* e, int64_t* rowptr, int32_t* colind, double* val) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(e->kind == 1, AIR_E_UNSUPPORTED, "matrix-free operator has no stored CSR");
+        use_device(e);
+        hipStream_t s = e->stream;
+        const DevSparse& A = e->A;
+        if (rowptr) AIR_HIP(hipMemcpyAsync(rowptr, A.ptr.p, 8 * (A.nrows + 1), hipMemcpyDeviceToHost, s));
+        if (colind && A.nnz) AIR_HIP(hipMemcpyAsync(colind, A.idx.p, 4 * A.nnz, hipMemcpyDeviceToHost, s));
+        if (val && A.nnz) {
+            DevBuf tmp;
+            tmp.alloc(8 * A.nnz);
+            dispatch_dtype(A.dtype, [&](auto v) {
+                using V = decltype(v);
+                k_widen<V><<<4096, 256, 0, s>>>(A.val.as<V>(), tmp.as<double>(), A.nnz);
+                return 0;
+            });
+            AIR_HIP(hipGetLastError());
+            AIR_HIP(hipMemcpyAsync(val, tmp.p, 8 * A.nnz, hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+        }
+        AIR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int air_apply(air_engine* e, int transp, const double* in, double* out) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(in && out, AIR_E_ARG, "null vector");
+        AIR_CHECK(transp == 0 || transp == 1, AIR_E_ARG, "transp_flag must be 'size', 'notransp' or 'transp'");
+        use_device(e);
+        hipStream_t s = e->stream;
+        int64_t nin = transp ? e->m : e->n, nout = transp ? e->n : e->m;
+        DevBuf din, dout;
+        upload_f32(din, in, nin, s);
+        dout.alloc(4 * std::max<int64_t>(nout, 1));
+        apply_dev(e, transp, din.as<float>(), dout.as<float>());
+        download_f64(dout.as<float>(), out, nout, s);
+    });
+}
+
+int air_weights(air_engine* e, int method, double* M_out, double* D_out) {
+    return guard([&] {
+        need_operator(e);
+        use_device(e);
+        std::vector<double> M, D;
+        compute_weights(e, method, M, D);
+        if (M_out) std::copy(M.begin(), M.end(), M_out);
+        if (D_out) std::copy(D.begin(), D.end(), D_out);
+    });
+}
+
+int air_spectral_radius(air_engine* e, int method, const double* Mc, const double* Dc, double* rho_out) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(rho_out, AIR_E_ARG, "null output");
+        use_device(e);
+        std::vector<double> M, D;
+        if (method == AIR_CUSTOM) {
+            M.assign(e->m, 1.0);
+            D.assign(e->n, 1.0);
+            if (Mc) std::copy(Mc, Mc + e->m, M.begin());
+            if (Dc) std::copy(Dc, Dc + e->n, D.begin());
+        } else {
+            compute_weights(e, method, M, D);
+        }
+        *rho_out = lanczos_rho(e, M, D);
+    });
+}
+
+int air_calczeta(int32_t kmax, double* z_out) {
+    return guard([&] {
+        AIR_CHECK(kmax > 2, AIR_E_ARG, "The polynomial is not defined for k < 2");
+        AIR_CHECK(z_out, AIR_E_ARG, "null output");
+        for (int k = 2; k < kmax; ++k) z_out[k - 2] = calczeta_one(k);
+    });
+}
+
+int air_sirt_prepare(air_engine* e, const air_sirt_options* opts, const double* b, const double* x0, int32_t kmax) {
+    return guard([&] { sirt_prepare(e, opts, b, x0, kmax); });
+}
+
+int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(e->st.prepared, AIR_E_STATE, "air_sirt_prepare must be called first");
+        use_device(e);
+        SirtState& st = e->st;
+        hipEvent_t a, b;
+        AIR_HIP(hipEventCreate(&a));
+        AIR_HIP(hipEventCreate(&b));
+        auto t0 = std::chrono::steady_clock::now();
+        timing_begin(e, niter);
+        AIR_HIP(hipEventRecord(a, e->stream));
+        for (int i = 0; i < niter && st.k < st.kmax; ++i) enqueue_iteration(e, ++st.k);
+        AIR_HIP(hipEventRecord(b, e->stream));
+        AIR_HIP(hipEventSynchronize(b));
+        timing_end(e);
+        float ms = 0;
+        AIR_HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        st.iter_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (gpu_ms) *gpu_ms = ms;
+    });
+}
+
+int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info* info) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(e->st.prepared, AIR_E_STATE, "air_sirt_prepare must be called first");
+        use_device(e);
+        SirtState& st = e->st;
+        hipStream_t s = e->stream;
+        Ctrl c;
+        AIR_HIP(hipMemcpyAsync(&c, st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        int fin = c.stop ? c.finaliter : st.k;
+        if (x_out) download_f64(st.x.as<float>(), x_out, st.n, s);
+        if (resnorm) {
+            AIR_HIP(hipMemcpyAsync(resnorm, st.hist.p, 8 * (size_t)(fin + 1), hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+        }
+        if (info) {
+            info->stoprule = c.stop ? c.stoprule : 0;
+            info->finaliter = fin;
+            info->relaxpar = st.relax_kind == AIR_RELAX_LINE ? c.lam_cur
+                                                              : (fin > 0 ? st.lam[fin - 1] : NAN);
+            info->rho = st.rho;
+            info->nsaved = 0;
+            info->setup_time = st.setup_s;
+            info->iter_time = st.iter_s;
+            info->timetaken = st.setup_s + st.iter_s;
+        }
+    });
+}
+
+int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const double* x0, const int32_t* K,
+             int32_t nK, double* X_out, int32_t* itersaved, int32_t* n_itersaved, double* resnorm,
+             air_sirt_info* info) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(K != nullptr && nK > 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
+        for (int i = 1; i < nK; ++i) AIR_CHECK(K[i] > K[i - 1], AIR_E_ARG, "Elements of K must be strictly increasing.");
+        const int kmax = K[nK - 1];
+        auto t0 = std::chrono::steady_clock::now();
+        sirt_prepare(e, opts, b, x0, kmax);
+        SirtState& st = e->st;
+        hipStream_t s = e->stream;
+        const int64_t n = st.n;
+        // snapshot slots for K(l) < kmax (sirt.m:247-250)
+        int nsnap = 0;
+        while (nsnap < nK && K[nsnap] < kmax) nsnap++;
+        DevBuf snaps;
+        if (nsnap) snaps.alloc(4 * (size_t)n * nsnap);
+        int l = 0;
+        const int chunk = 32;
+        bool stopped = read_stop(e) != 0;
+        auto ti = std::chrono::steady_clock::now();
+        while (!stopped && st.k < kmax) {
+            int upto = std::min(kmax, st.k + chunk);
+            timing_begin(e, upto - st.k);
+            while (st.k < upto) {
+                enqueue_iteration(e, ++st.k);
+                if (l < nsnap && st.k == K[l]) {
+                    AIR_HIP(hipMemcpyAsync(snaps.as<float>() + (size_t)l * n, st.x.p, 4 * n, hipMemcpyDeviceToDevice, s));
+                    l++;
+                }
+            }
+            stopped = read_stop(e) != 0;
+            timing_end(e);
+        }
+        st.iter_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ti).count();
+        air_sirt_info inf{};
+        int rc = air_sirt_fetch(e, nullptr, resnorm, &inf);
+        if (rc != AIR_OK) throw Err{rc, g_last_error};
+        const int fin = inf.finaliter;
+        int ns = 0;
+        for (int q = 0; q < nK; ++q)
+            if (K[q] < fin) {
+                if (itersaved) itersaved[ns] = K[q];
+                ns++;
+            }
+        int nsaved = fin > 0 ? ns + 1 : 0;
+        if (itersaved) itersaved[ns] = fin;
+        if (n_itersaved) *n_itersaved = ns + 1;
+        if (X_out) {
+            for (int q = 0; q < ns; ++q) download_f64(snaps.as<float>() + (size_t)q * n, X_out + (size_t)q * n, n, s);
+            if (fin > 0) download_f64(st.x.as<float>(), X_out + (size_t)ns * n, n, s);
+        }
+        inf.nsaved = nsaved;
+        inf.timetaken = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (info) *info = inf;
+    });
+}
+
+int air_set_kernel_timing(air_engine* e, int on) {
+    return guard([&] {
+        AIR_CHECK(e, AIR_E_ARG, "null engine");
+        e->timing = on != 0;
+        e->t_rows_ms = e->t_cols_ms = 0;
+        e->launches = 0;
+    });
+}
+
+int air_get_kernel_timing(air_engine* e, double* rows_ms, double* cols_ms, int64_t* launches) {
+    return guard([&] {
+        AIR_CHECK(e, AIR_E_ARG, "null engine");
+        if (rows_ms) *rows_ms = e->t_rows_ms;
+        if (cols_ms) *cols_ms = e->t_cols_ms;
+        if (launches) *launches = e->launches;
+    });
+}
+
+}  // extern "C"

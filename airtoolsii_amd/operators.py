@@ -1,0 +1,123 @@
+"""Engine-backed operators with the AIR Tools II ``afun`` convention.
+
+An :class:`AirOperator` is what ``sirt``/``cimmino``/... accept as ``A``.  It
+is callable exactly like a MATLAB afun handle (auxil/afun_matrix.m:44-53,
+testprobs/paralleltomo.m:150-160):
+
+    A(None, 'size')      -> (m, n)
+    A(v, 'notransp')     -> A @ v   (m-vector, float64)
+    A(w, 'transp')       -> A' @ w  (n-vector, float64)
+
+and it also supports ``A @ v``, ``A.T @ w`` and ``A.shape``.  The matrix (CSR
+and CSC) or the matrix-free ray geometry lives on the GPU; products run as HIP
+kernels.  Host vectors are FP64 at this boundary, as in MATLAB.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import DTYPES, Engine, check, f64, load, ptr
+
+
+class AirOperator:
+    def __init__(self, engine: Engine, dtype: str, matrix_free: bool, geometry: dict | None = None):
+        self.engine = engine
+        self.dtype = dtype
+        self.matrix_free = matrix_free
+        self.geometry = geometry or {}
+        m, n, nnz = engine.size()
+        self.shape = (m, n)
+        self.nnz = nnz if nnz >= 0 else None
+
+    # ---- afun convention
+    def __call__(self, v, transp_flag):
+        if transp_flag == "size":
+            return self.shape
+        if transp_flag == "notransp":
+            return self.matvec(v)
+        if transp_flag == "transp":
+            return self.rmatvec(v)
+        raise ValueError("transp_flag must be 'size', 'notransp' or 'transp'")
+
+    def matvec(self, v):
+        v = f64(v)
+        if v.shape[0] != self.shape[1]:
+            raise ValueError("Incorrect length of input vector")
+        out = np.empty(self.shape[0])
+        check(load().air_apply(self.engine.h, 0, ptr(v), ptr(out)))
+        return out
+
+    def rmatvec(self, w):
+        w = f64(w)
+        if w.shape[0] != self.shape[0]:
+            raise ValueError("Incorrect length of input vector")
+        out = np.empty(self.shape[1])
+        check(load().air_apply(self.engine.h, 1, ptr(w), ptr(out)))
+        return out
+
+    def __matmul__(self, v):
+        return self.matvec(v)
+
+    @property
+    def T(self):
+        return _Transposed(self)
+
+    # ---- stored matrix access
+    def csr_arrays(self):
+        """(rowptr int64, colind int32, values float64 widened from storage)."""
+        if self.matrix_free:
+            raise ValueError("matrix-free operator has no stored CSR")
+        m, n = self.shape
+        rp = np.empty(m + 1, np.int64)
+        ci = np.empty(max(self.nnz, 1), np.int32)
+        v = np.empty(max(self.nnz, 1), np.float64)
+        check(load().air_get_csr(self.engine.h, ptr(rp), ptr(ci), ptr(v)))
+        return rp, ci[: self.nnz], v[: self.nnz]
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, ci, v = self.csr_arrays()
+        return sp.csr_matrix((v, ci, rp), shape=self.shape)
+
+
+class _Transposed:
+    def __init__(self, op):
+        self.op = op
+        self.shape = (op.shape[1], op.shape[0])
+
+    def __matmul__(self, w):
+        return self.op.rmatvec(w)
+
+
+def as_operator(A, dtype: str = "f32", device: int = 0) -> AirOperator:
+    """Wrap ``A`` as an engine operator.  Accepts an :class:`AirOperator`
+    (returned unchanged), a SciPy sparse matrix or a dense 2-D array (uploaded
+    as CSR; the engine builds the CSC on the GPU)."""
+    if isinstance(A, AirOperator):
+        return A
+    if callable(A):
+        raise TypeError("Only engine operators (AirOperator) are accepted as function handles; "
+                        "build A with paralleltomo/fanlineartomo(..., isMatrix=0) or pass a matrix")
+    import scipy.sparse as sp
+    M = sp.csr_matrix(A, dtype=np.float64)
+    M.sum_duplicates()
+    M.sort_indices()
+    e = Engine(device)
+    m, n = M.shape
+    check(load().air_set_matrix_csr(e.h, m, n, ptr(np.ascontiguousarray(M.indptr, np.int64)),
+                                    ptr(np.ascontiguousarray(M.indices, np.int32)),
+                                    ptr(np.ascontiguousarray(M.data, np.float64)), DTYPES[dtype]))
+    return AirOperator(e, dtype, False)
+
+
+def from_matlab_csc(m, n, jc, ir, pr, dtype: str = "f32", device: int = 0) -> AirOperator:
+    """The operand a MEX gateway receives (mxGetJc / mxGetIr / mxGetPr)."""
+    e = Engine(device)
+    check(load().air_set_matrix_csc(e.h, int(m), int(n), ptr(np.ascontiguousarray(jc, np.int64)),
+                                    ptr(np.ascontiguousarray(ir, np.int64)),
+                                    ptr(np.ascontiguousarray(pr, np.float64)), DTYPES[dtype]))
+    return AirOperator(e, dtype, False)
+
+
+__all__ = ["AirOperator", "as_operator", "from_matlab_csc", "_lib"]

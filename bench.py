@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""SIRT throughput on one MI355X: paralleltomo N=512, 360 angles, p=725.
+
+    python bench.py [--steps K] [--warmup W] [--method cimmino|sart|drop|...]
+                    [--dtype f32|bf16|f16] [--matrix-free] [--no-cpu-baseline]
+
+One step is one SIRT iteration x <- P_[0,1](x + lam D A' M (b - A x)) over the
+whole problem (sirt/sirt.m:213-243): a CSC pass (A' M r fused with the update
+and projection) and a CSR pass (residual, M r, FP64 norms and the stop check).
+The matrix is traced on the GPU, the phantom is grains(512, 68 grains, seed 1)
+and b = A x* + 2% Gaussian noise (seed 1); all inputs are resident in HBM
+before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "SIRT iter/s & HBM GB/s at N=512; time to fixed error, learned vs plain λ"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(A_nnz, m, n, s_v, method):
+    """Algorithmic HBM bytes of one launch of each SIRT kernel.
+    rows (CSR): values+indices, rowptr, x once, b, M, r and M r written.
+    cols (CSC): values+indices, colptr, M r once, D (if any), x read+write."""
+    hasM = method != "landweber"
+    hasD = method in ("drop", "sart")
+    rows = A_nnz * (s_v + 4) + 8 * (m + 1) + 4 * n + 4 * m + (4 * m if hasM else 0) + 8 * m
+    cols = A_nnz * (s_v + 4) + 8 * (n + 1) + 4 * m + (4 * n if hasD else 0) + 8 * n
+    return rows, cols
+
+
+def cpu_baseline(N, theta, p, b, method, rho, samples, threads):
+    """The FP64 CPU oracle (a C restatement of sirt.m, OpenMP) on the same
+    problem, timed for a bounded number of iterations on the host cores."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    A = O.paralleltomo(N, theta, p)
+    t_trace = time.perf_counter() - t0
+    opts = {"rho": rho, "lbound": 0, "ubound": 1}
+    O.sirt(method, A, b, 1, None, opts, nthreads=threads)            # warm caches
+    t0 = time.perf_counter()
+    O.sirt(method, A, b, samples, None, opts, nthreads=threads)
+    dt = time.perf_counter() - t0
+    # subtract the per-call setup (weights, transpose) measured with a 1-iteration call
+    t1 = time.perf_counter()
+    O.sirt(method, A, b, 1, None, opts, nthreads=threads)
+    d1 = time.perf_counter() - t1
+    per_iter = max(dt - d1, 1e-9) / max(samples - 1, 1)
+    return {"value": 1.0 / per_iter, "unit": "iter/s", "cores": threads, "kind": "port",
+            "sample": f"{samples} {method} iterations of the FP64 oracle (CSR A.x, CSC A'.y, OpenMP) on the same "
+                      f"{A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}; oracle tracer {t_trace:.1f}s not included"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--method", default="cimmino")
+    ap.add_argument("--dtype", default="f32")
+    ap.add_argument("--N", type=int, default=512)
+    ap.add_argument("--matrix-free", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-samples", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine runs one problem per GPU and no multi-GPU path is "
+              "built (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import airtoolsii_amd as air
+
+    N = args.N
+    theta = np.arange(0.0, 180.0, 0.5)
+    p = int(np.floor(np.sqrt(2) * N + 0.5))
+    if N == 512:
+        p = 725
+    t0 = time.perf_counter()
+    A = air.paralleltomo(N, theta, p, None, 0, 0 if args.matrix_free else 1, dtype=args.dtype)
+    t_trace = time.perf_counter() - t0
+    m, n = A.shape
+    x_true = air.phantomgallery("grains", N, None, 1).ravel(order="F")
+    b0 = A.matvec(x_true)
+    b, _ = air.add_noise(b0, 0.02, seed=1)
+
+    kmax = args.warmup + args.steps
+    opts = {"lbound": 0, "ubound": 1}
+    t0 = time.perf_counter()
+    run = air.SirtRun(args.method, A, b, kmax, None, opts)
+    t_setup = time.perf_counter() - t0
+    if args.warmup:
+        run.iterate(args.warmup)
+    run.kernel_timing(True)
+    t0 = time.perf_counter()
+    gpu_ms = run.iterate(args.steps)
+    wall = time.perf_counter() - t0
+    kt = run.kernel_times()
+    x, info = run.fetch()
+    rel_err = float(np.linalg.norm(x - x_true) / np.linalg.norm(x_true))
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = args.steps / wall
+    s_v = {"f32": 4, "bf16": 2, "f16": 2, "f64": 8}[args.dtype]
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.dtype, "data": "synthetic",
+           "config": {"workload": f"paralleltomo N={N}, theta=0:0.5:179.5 ({len(theta)} angles), p={p}, "
+                                  f"{args.method}, box [0,1], {'matrix-free' if args.matrix_free else 'CSR+CSC'}",
+                      "m": m, "n": n, "nnz": A.nnz, "method": args.method, "phantom": "grains(N, 68, seed 1)",
+                      "noise": "2% Gaussian, seed 1"}}
+    if not args.matrix_free:
+        rows_b, cols_b = kernel_bytes(A.nnz, m, n, s_v, args.method)
+        launches = max(kt["launches"], 1)
+        t_rows = kt["rows_ms"] / launches * 1e-3
+        t_cols = kt["cols_ms"] / launches * 1e-3
+        dom, bytes_dom, t_dom = ("csr_resid", rows_b, t_rows) if t_rows >= t_cols else ("csc_update", cols_b, t_cols)
+        achieved = bytes_dom / t_dom / 1e9
+        it_bytes = rows_b + cols_b
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                           "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(t_dom * 1e6, 2)}
+        out["iteration"] = {"bytes": int(it_bytes), "gbs": round(it_bytes * value / 1e9, 1),
+                            "frac": round(it_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
+                            "csr_resid_us": round(t_rows * 1e6, 2), "csc_update_us": round(t_cols * 1e6, 2)}
+    out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
+    out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
+                     "rho": info["rho"]}
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(N, theta, p, b, args.method, info["rho"], args.cpu_samples, threads)
+        out["vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * airhip.h -- C ABI of the MI355X SIRT engine (libairhip.so).
+ *
+ * Plain C types only: host pointers, sizes, POD structs.  One engine handle
+ * owns one HIP device, one stream and one operator (a stored sparse matrix or
+ * a matrix-free ray geometry).  Calls on one handle are not re-entrant;
+ * separate handles may be used from separate threads.  Every function returns
+ * AIR_OK (0) or a negative AIR_E_* code; air_last_error() gives the message
+ * of the calling thread's last failure.
+ *
+ * Reference interfaces (AIR Tools II, MATLAB) each entry point replaces are
+ * cited per declaration as file:line.
+ */
+#ifndef AIRHIP_H
+#define AIRHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AIR_OK             0
+#define AIR_E_ARG         -1   /* invalid argument (MATLAB error(...) in check_inputs) */
+#define AIR_E_NOMEM       -2   /* device or host allocation failed */
+#define AIR_E_HIP         -3   /* HIP runtime error */
+#define AIR_E_UNSUPPORTED -4   /* option not implemented for this operator */
+#define AIR_E_STATE       -5   /* call out of order (e.g. no operator set) */
+#define AIR_E_GEOMETRY    -6   /* tracer produced a column outside 1..N^2 */
+
+/* storage type of the matrix values; accumulation is FP32 (FP64 for AIR_F64) */
+#define AIR_F32   0
+#define AIR_BF16  1
+#define AIR_F16   2
+#define AIR_F64   3
+
+/* SIRT methods, auxil/get_mfun_dfun.m:47-250 */
+#define AIR_LANDWEBER 0
+#define AIR_CIMMINO   1
+#define AIR_CAV       2
+#define AIR_DROP      3
+#define AIR_SART      4
+#define AIR_CUSTOM    5   /* struct('M',m-vector,'D',n-vector), get_mfun_dfun.m:252-307 */
+
+/* relaxation strategies, auxil/calc_relaxpar.m:105-171 */
+#define AIR_RELAX_DEFAULT 0   /* 1.9/rho */
+#define AIR_RELAX_SCALAR  1
+#define AIR_RELAX_LINE    2
+#define AIR_RELAX_PSI1    3
+#define AIR_RELAX_PSI1MOD 4
+#define AIR_RELAX_PSI2    5
+#define AIR_RELAX_PSI2MOD 6
+#define AIR_RELAX_VECTOR  7   /* caller-supplied per-iteration lambda_k (learned relaxation) */
+
+/* stopping rules, auxil/check_stoprules.m:53-161 (info.stoprule codes match) */
+#define AIR_STOP_NONE 0
+#define AIR_STOP_NCP  1
+#define AIR_STOP_DP   2
+#define AIR_STOP_ME   3
+
+/* bound kinds, auxil/check_inputs.m:108-132 */
+#define AIR_BOUND_NONE   0
+#define AIR_BOUND_SCALAR 1
+#define AIR_BOUND_VECTOR 2
+
+typedef struct air_engine air_engine;
+
+/* options struct of sirt.m:36-83 as parsed by auxil/check_inputs.m:101-260 */
+typedef struct {
+    int           method;        /* AIR_LANDWEBER..AIR_CUSTOM */
+    const double* M;             /* AIR_CUSTOM: m-vector diagonal or NULL (identity) */
+    const double* D;             /* AIR_CUSTOM: n-vector diagonal or NULL (identity) */
+    int           relax_kind;    /* AIR_RELAX_* */
+    double        relaxpar;      /* AIR_RELAX_SCALAR value */
+    const double* relax_vec;     /* AIR_RELAX_VECTOR: kmax values */
+    double        rho;           /* spectral radius; NaN = compute (calc_relaxpar.m:91-103) */
+    int           lb_kind;  double lb;  const double* lb_vec;
+    int           ub_kind;  double ub;  const double* ub_vec;
+    int           stoprule;      /* AIR_STOP_* */
+    double        taudelta;      /* DP / ME */
+    int           res_dims[2];   /* NCP: [p, nA] or [m, 0] */
+    int           ncp_smooth;    /* NCP filter length, default 2 */
+    int           verbose;       /* sirt.m:256-260 print cadence */
+} air_sirt_options;
+
+/* info struct of sirt.m:60-67,272-281 */
+typedef struct {
+    int    stoprule;     /* 0 max iter, 1 NCP, 2 DP, 3 ME */
+    int    finaliter;
+    double relaxpar;     /* relaxpar of the last iteration (scalar strategies) */
+    double rho;
+    int    nsaved;       /* columns written to X */
+    double timetaken;    /* seconds, whole call */
+    double setup_time;   /* seconds: weights + rho + first residual */
+    double iter_time;    /* seconds: the K-loop on the device */
+} air_sirt_info;
+
+/* ---- lifetime -------------------------------------------------------- */
+const char* air_last_error(void);
+int  air_create(int device, air_engine** out);
+void air_destroy(air_engine* e);
+int  air_synchronize(air_engine* e);
+
+/* ---- operators -------------------------------------------------------
+ * A stored matrix: CSR with 0-based int32 column indices (ascending per row). */
+int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr,
+                       const int32_t* colind, const double* val, int dtype);
+/* A MATLAB sparse matrix as handed over by mxGetJc/mxGetIr/mxGetPr (CSC,
+ * 0-based row indices): the operand of auxil/afun_matrix.m:44-53. */
+int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colptr,
+                       const int64_t* rowind, const double* val, int dtype);
+/* testprobs/paralleltomo.m:1-110 (defaults resolved by the caller: p, d).
+ * matrix_free=0: trace on the GPU into CSR (+CSC); 1: keep the geometry
+ * only and project ray by ray on every product (paralleltomo.m:94-96). */
+int air_build_paralleltomo(air_engine* e, int32_t N, const double* theta, int32_t nA,
+                           int32_t p, double d, int matrix_free, int dtype);
+/* testprobs/fanlineartomo.m:1-136; R, dw, sd are in units of N as in the
+ * reference signature (the engine multiplies by N, fanlineartomo.m:75-90). */
+int air_build_fanlineartomo(air_engine* e, int32_t N, const double* theta, int32_t nA,
+                            int32_t p, double R, double dw, double sd,
+                            int matrix_free, int dtype);
+/* A(v,'size') of auxil/check_inputs.m:74 (nnz = -1 for matrix-free) */
+int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz);
+/* copy the stored CSR back to the host (values widened to double) */
+int air_get_csr(air_engine* e, int64_t* rowptr, int32_t* colind, double* val);
+/* the same, with the tracer's FP64 values (only when dtype was AIR_F64) */
+/* afun convention, auxil/afun_matrix.m:47-50 / paralleltomo.m:282-287:
+ * transp=0 -> out = A*in (m), transp=1 -> out = A'*in (n).  Host FP64. */
+int air_apply(air_engine* e, int transp, const double* in, double* out);
+
+/* ---- SIRT building blocks ---------------------------------------------
+ * auxil/get_mfun_dfun.m:47-250: diagonals of M (m) and D (n), FP64. */
+int air_weights(air_engine* e, int method, double* M_out, double* D_out);
+/* largest eigenvalue of D A' M A, calc_relaxpar.m:91-103 (Lanczos) */
+int air_spectral_radius(air_engine* e, int method, const double* Mcustom,
+                        const double* Dcustom, double* rho_out);
+/* calczeta, calc_relaxpar.m:179-244, for k = 2..kmax-1 -> z[k-2] */
+int air_calczeta(int32_t kmax, double* z_out);
+
+/* ---- SIRT driver, sirt/sirt.m:1-281 ------------------------------------
+ * K: nK strictly increasing iteration numbers (kmax = K[nK-1]).
+ * X_out: n x nK column-major FP64 (first info.nsaved columns written).
+ * itersaved: nK+1 ints (info.itersaved of sirt.m:279), returns count in
+ * *n_itersaved.  resnorm (optional, kmax+1 doubles): ||r_k||, k=0..finaliter. */
+int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const double* x0,
+             const int32_t* K, int32_t nK, double* X_out, int32_t* itersaved,
+             int32_t* n_itersaved, double* resnorm, air_sirt_info* info);
+
+/* Device-resident form of the same loop, for callers that time the
+ * iterations alone.  prepare = sirt.m:143-169 (weights, rho, lambda, r0,
+ * the k=0 stop check); iterate enqueues n iterations of sirt.m:199-261 and
+ * waits for them; fetch copies x_k and the info back.  kmax bounds the
+ * total number of iterations (it feeds the stop rule as K(end)). */
+int air_sirt_prepare(air_engine* e, const air_sirt_options* opts, const double* b,
+                     const double* x0, int32_t kmax);
+int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms);
+int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info* info);
+
+/* per-kernel HIP-event timing of the SIRT kernels (0 = off).  Times are
+ * accumulated over iterate() calls while on; reset by enabling again. */
+int air_set_kernel_timing(air_engine* e, int on);
+int air_get_kernel_timing(air_engine* e, double* rows_ms, double* cols_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

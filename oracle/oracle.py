@@ -1,0 +1,389 @@
+"""FP64 CPU oracle for the AIR Tools II SIRT path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it; the product package
+(``airtoolsii_amd``) never does.
+
+It wraps ``build/libair_oracle.so`` (built from ``air_oracle.c`` by the
+Makefile beside it) and restates the MATLAB host logic around the C loop:
+
+* ``check_inputs``      auxil/check_inputs.m:59-259  (SIRT fields)
+* ``calc_relaxpar``     auxil/calc_relaxpar.m:89-171 (rho via a converged
+                        symmetric Lanczos from SciPy instead of eigs/ARPACK
+                        under rng(0), which only a MATLAB run reproduces)
+* ``sirt``              sirt/sirt.m:132-281
+* ``train_relaxpar``    train/train_relaxpar.m:43-271 (SIRT branch)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libair_oracle.so")
+_lib = None
+
+METHODS = {"landweber": 0, "cimmino": 1, "cav": 2, "drop": 3, "sart": 4}
+STOPRULES = {"NONE": 0, "NCP": 1, "DP": 2, "ME": 3}
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+
+
+class _SirtOpts(C.Structure):
+    _fields_ = [("stoprule_type", C.c_int), ("taudelta", C.c_double),
+                ("res_dims", C.c_int * 2), ("ncp_smooth", C.c_int),
+                ("lb_kind", C.c_int), ("lb", C.c_double), ("lb_vec", C.c_void_p),
+                ("ub_kind", C.c_int), ("ub", C.c_double), ("ub_vec", C.c_void_p),
+                ("line_search", C.c_int), ("lam", C.c_void_p), ("nthreads", C.c_int)]
+
+
+class _SirtInfo(C.Structure):
+    _fields_ = [("stoprule", C.c_int), ("finaliter", C.c_int),
+                ("relaxpar", C.c_double), ("nsaved", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.ora_sind.restype = C.c_double; L.ora_sind.argtypes = [C.c_double]
+        L.ora_cosd.restype = C.c_double; L.ora_cosd.argtypes = [C.c_double]
+        L.ora_atand.restype = C.c_double; L.ora_atand.argtypes = [C.c_double]
+        L.ora_round.restype = C.c_double; L.ora_round.argtypes = [C.c_double]
+        L.ora_linspace.argtypes = [C.c_double, C.c_double, C.c_int64, _f64p]
+        L.ora_calczeta.restype = C.c_double; L.ora_calczeta.argtypes = [C.c_int]
+        L.ora_mt_seed.argtypes = [C.c_void_p, C.c_uint32]
+        L.ora_mt_rand.restype = C.c_double; L.ora_mt_rand.argtypes = [C.c_void_p]
+        L.ora_phantom_shepplogan.argtypes = [C.c_int, _f64p]
+        L.ora_phantom_smooth.argtypes = [C.c_int, C.c_int, _f64p]
+        L.ora_phantom_threephases.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, _f64p]
+        L.ora_phantom_grains.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, _f64p]
+        L.ora_paralleltomo.restype = C.c_void_p
+        L.ora_paralleltomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int, C.c_double]
+        L.ora_fanlineartomo.restype = C.c_void_p
+        L.ora_fanlineartomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_double]
+        for f in ("ora_csr_m", "ora_csr_n", "ora_csr_nnz"):
+            getattr(L, f).restype = C.c_int64
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.ora_csr_error.restype = C.c_int; L.ora_csr_error.argtypes = [C.c_void_p]
+        L.ora_csr_copy.argtypes = [C.c_void_p, _i64p, _i32p, _f64p]
+        L.ora_csr_free.argtypes = [C.c_void_p]
+        L.ora_weights.restype = C.c_int
+        L.ora_weights.argtypes = [C.c_int, C.c_int64, C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p]
+        L.ora_spmv.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        L.ora_sirt.restype = C.c_int
+        L.ora_sirt.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _f64p, _i64p, _i32p, _f64p,
+                               C.c_void_p, C.c_void_p, _f64p, _f64p, _i32p, C.c_int,
+                               C.POINTER(_SirtOpts), _f64p, _i32p, _f64p, C.POINTER(_SirtInfo)]
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- built-ins
+def sind(x):
+    return lib().ora_sind(float(x))
+
+
+def cosd(x):
+    return lib().ora_cosd(float(x))
+
+
+def linspace(d1, d2, n):
+    out = np.empty(int(n), np.float64)
+    lib().ora_linspace(float(d1), float(d2), int(n), out)
+    return out
+
+
+def calczeta(k):
+    """calc_relaxpar.m:179-215 for a vector k (error for k(1) < 2, :202-204)."""
+    k = np.atleast_1d(np.asarray(k, dtype=np.int64))
+    if k.size == 0 or k[0] < 2:
+        raise ValueError("The polynomial is not defined for k < 2")
+    return np.array([lib().ora_calczeta(int(v)) for v in k])
+
+
+def mt_rand(seed, count):
+    st = (C.c_uint32 * 625)()
+    lib().ora_mt_seed(C.byref(st), int(seed))
+    return np.array([lib().ora_mt_rand(C.byref(st)) for _ in range(int(count))])
+
+
+# ---------------------------------------------------------------- phantoms
+def phantomgallery(name, N, P1=None, P2=None):
+    """phantomgallery.m:79-151 for shepplogan/smooth/threephases/grains.
+    Returns the N x N image (column-major semantics: use ``im.ravel('F')``)."""
+    L = lib()
+    out = np.zeros(N * N, np.float64)
+    if name == "shepplogan":
+        L.ora_phantom_shepplogan(N, out)
+    elif name == "smooth":
+        L.ora_phantom_smooth(N, 4 if P1 is None else int(P1), out)
+    elif name == "threephases":
+        L.ora_phantom_threephases(N, 100 if P1 is None else int(P1),
+                                  0 if P2 is None else 1, 0 if P2 is None else int(P2), out)
+    elif name == "grains":
+        L.ora_phantom_grains(N, 0 if P1 is None else int(P1),
+                             0 if P2 is None else 1, 0 if P2 is None else int(P2), out)
+    else:
+        raise ValueError("Illegal phantom name")
+    return out.reshape((N, N), order="F")
+
+
+# ---------------------------------------------------------------- tracers
+def _take_csr(h):
+    L = lib()
+    if not h:
+        raise MemoryError("oracle tracer failed")
+    try:
+        if L.ora_csr_error(h):
+            raise IndexError("Index exceeds matrix dimensions (column outside 1..N^2)")
+        m, n, nnz = L.ora_csr_m(h), L.ora_csr_n(h), L.ora_csr_nnz(h)
+        rp = np.empty(m + 1, np.int64); ci = np.empty(nnz, np.int32); v = np.empty(nnz, np.float64)
+        L.ora_csr_copy(h, rp, ci, v)
+    finally:
+        L.ora_csr_free(h)
+    return sp.csr_matrix((v, ci, rp), shape=(m, n))
+
+
+def paralleltomo(N, theta=None, p=None, d=None):
+    """testprobs/paralleltomo.m:62-89 defaults + :112-301 matrix build."""
+    theta = np.arange(0.0, 180.0) if theta is None else np.atleast_1d(np.asarray(theta, np.float64))
+    p = int(lib().ora_round(np.sqrt(2) * N)) if p is None else int(p)
+    d = float(p - 1) if d is None else float(d)
+    return _take_csr(lib().ora_paralleltomo(int(N), np.ascontiguousarray(theta), len(theta), p, d))
+
+
+def fanlineartomo(N, theta=None, p=None, R=2.0, dw=2.5, sd=3.0):
+    """testprobs/fanlineartomo.m:70-114 defaults + :141-358 matrix build."""
+    theta = np.arange(0.0, 359.0, 2.0) if theta is None else np.atleast_1d(np.asarray(theta, np.float64))
+    p = int(lib().ora_round(np.sqrt(2) * N)) if p is None else int(p)
+    Rn, dwn, sdn = R * N, dw * N, sd * N
+    if Rn < np.sqrt(2) / 2 * N:
+        raise ValueError("R must be greater than half squareroot 2")
+    return _take_csr(lib().ora_fanlineartomo(int(N), np.ascontiguousarray(theta), len(theta), p,
+                                             float(Rn), float(dwn), float(sdn)))
+
+
+# ---------------------------------------------------------------- weights
+def _csr_parts(A):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return (np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int32),
+            np.ascontiguousarray(A.data, np.float64))
+
+
+def weights(method, A):
+    """get_mfun_dfun.m matrix branch; returns (M, D) diagonals."""
+    m, n = A.shape
+    rp, ci, v = _csr_parts(A)
+    M = np.empty(m); D = np.empty(n)
+    if lib().ora_weights(METHODS[method], m, n, rp, ci, v, M, D) != 0:
+        raise ValueError("SIRT method not defined.")
+    return M, D
+
+
+def spmv(A, x, nthreads=0):
+    rp, ci, v = _csr_parts(A)
+    y = np.empty(A.shape[0])
+    lib().ora_spmv(A.shape[0], rp, ci, v, np.ascontiguousarray(x, np.float64), y, nthreads)
+    return y
+
+
+def spectral_radius(A, M, D):
+    """Largest eigenvalue of D A' M A (calc_relaxpar.m:91-103), computed as the
+    top eigenvalue of the symmetric D^(1/2) A' M A D^(1/2) with SciPy's Lanczos
+    to tolerance 1e-12 (a library routine, not reference code)."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    A = sp.csr_matrix(A)
+    AT = A.T.tocsr()
+    sD = np.sqrt(D)
+
+    def mv(v):
+        v = np.ravel(v)
+        return sD * (AT @ (M * (A @ (sD * v))))
+
+    op = LinearOperator((A.shape[1], A.shape[1]), matvec=mv, dtype=np.float64)
+    vals = eigsh(op, k=1, which="LA", tol=1e-12, v0=np.ones(A.shape[1]), return_eigenvectors=False)
+    return float(vals[0])
+
+
+# ---------------------------------------------------------------- SIRT
+def relaxpar_vector(relaxpar, rho, kmax):
+    """calc_relaxpar.m:105-171 -> (casel, per-iteration lambda vector or None, relaxpar)."""
+    if relaxpar is None:
+        lam = 1.9 / rho
+        return 1, np.full(kmax, lam), lam
+    if not isinstance(relaxpar, str):
+        lam = float(relaxpar)
+        return 1, np.full(kmax, lam), lam
+    s = relaxpar.lower()
+    if s.startswith("line"):
+        return 2, None, None
+    if s.startswith("psi1"):
+        z = calczeta(np.arange(2, kmax))
+        nu = 2.0 if s.startswith("psi1mod") else 1.0
+        v = np.concatenate([[np.sqrt(2), np.sqrt(2)], nu * 2 * (1 - z)]) / rho
+        return 3, v[:kmax], v
+    if s.startswith("psi2"):
+        kk = np.arange(2, kmax)
+        z = calczeta(kk)
+        nu = 1.5 if s.startswith("psi2mod") else 1.0
+        v = np.concatenate([[np.sqrt(2), np.sqrt(2)], nu * 2 * (1 - z) / ((1 - z ** kk) ** 2)]) / rho
+        return 3, v[:kmax], v
+    raise ValueError("The chosen relaxation strategy is not valid for this method.")
+
+
+def sirt(method, A, b, K, x0=None, options=None, nthreads=0, return_resnorm=False):
+    """sirt/sirt.m:132-281 on a SciPy sparse A (FP64).  ``method`` is a name
+    or a dict with optional 'M'/'D' vectors (custom method)."""
+    options = dict(options or {})
+    A = sp.csr_matrix(A, dtype=np.float64)
+    m, n = A.shape
+    b = np.ascontiguousarray(np.ravel(b), np.float64)
+    if b.shape[0] != m:
+        raise ValueError("The size of A and b do not match.")
+    K = np.atleast_1d(np.asarray(K, np.int64))
+    if K.size == 0:
+        raise ValueError("Max no. of iterations must ALWAYS be specified.")
+    if not np.all(np.diff(K) > 0):
+        raise ValueError("Elements of K must be strictly increasing.")
+    kmax = int(K[-1])
+    x0 = np.zeros(n) if x0 is None else np.ascontiguousarray(np.ravel(x0), np.float64)
+    if x0.shape[0] != n:
+        raise ValueError("The size of x0 does not match the problem.")
+    if method is None:
+        method = "sart"
+    if isinstance(method, str):
+        M, D = weights(method, A)
+    else:
+        M = np.ascontiguousarray(method.get("M"), np.float64) if method.get("M") is not None else np.ones(m)
+        D = np.ascontiguousarray(method.get("D"), np.float64) if method.get("D") is not None else np.ones(n)
+    rho = options.get("rho", np.nan)
+    if isinstance(method, str) and method == "sart":
+        rho = 1.0
+    rp_in = options.get("relaxpar")
+    needs_rho = not (isinstance(rp_in, str) and rp_in.lower().startswith("line"))
+    if np.isnan(rho) and needs_rho:
+        rho = spectral_radius(A, M, D)
+    casel, lam, relaxpar = relaxpar_vector(rp_in, rho, kmax)
+    st = options.get("stoprule", {}) or {}
+    stype = str(st.get("type", "none")).upper()
+    o = _SirtOpts()
+    o.stoprule_type = STOPRULES[stype]
+    o.taudelta = float(st.get("taudelta", np.nan))
+    rd = st.get("res_dims", None)
+    if rd is not None:
+        rd = list(np.atleast_1d(rd))
+        o.res_dims[0] = int(rd[0]); o.res_dims[1] = int(rd[1]) if len(rd) > 1 else 0
+    o.ncp_smooth = int(st.get("ncp_smooth", 2))
+    keep = []
+    for nm, kind, val, vec in (("lbound", "lb_kind", "lb", "lb_vec"), ("ubound", "ub_kind", "ub", "ub_vec")):
+        if nm in options and options[nm] is not None:
+            bv = np.ravel(np.asarray(options[nm], np.float64))
+            if bv.size == 1:
+                setattr(o, kind, 1); setattr(o, val, float(bv[0]))
+            elif bv.size == n:
+                arr = np.ascontiguousarray(bv); keep.append(arr)
+                setattr(o, kind, 2); setattr(o, vec, arr.ctypes.data)
+            else:
+                raise ValueError(f"{nm} must either be scalar or same length as vector of unknowns x.")
+    o.line_search = 1 if casel == 2 else 0
+    if lam is not None:
+        lam = np.ascontiguousarray(np.resize(lam, kmax), np.float64); keep.append(lam)
+        o.lam = lam.ctypes.data
+    o.nthreads = int(nthreads)
+    rp, ci, v = _csr_parts(A)
+    AT = A.T.tocsr(); AT.sort_indices()
+    cp, ri, vc = (np.ascontiguousarray(AT.indptr, np.int64), np.ascontiguousarray(AT.indices, np.int32),
+                  np.ascontiguousarray(AT.data, np.float64))
+    Mc = np.ascontiguousarray(M); Dc = np.ascontiguousarray(D)
+    X = np.zeros(n * len(K)); its = np.zeros(len(K) + 1, np.int32); rn = np.zeros(kmax + 1)
+    info = _SirtInfo()
+    lib().ora_sirt(m, n, rp, ci, v, cp, ri, vc, Mc.ctypes.data, Dc.ctypes.data, b, x0,
+                   np.ascontiguousarray(K, np.int32), len(K), C.byref(o), X, its, rn, C.byref(info))
+    X = X.reshape((len(K), n)).T[:, :info.nsaved]
+    nsv = int(np.sum(K < info.finaliter)) + 1
+    out_info = {"stoprule": info.stoprule, "finaliter": info.finaliter,
+                "relaxpar": relaxpar if info.finaliter == 0 or casel != 1 else float(lam[0]),
+                "rho": rho, "itersaved": its[:nsv].astype(np.int64)}
+    if casel == 2:
+        out_info["relaxpar"] = info.relaxpar
+    if return_resnorm:
+        return X, out_info, {"M": M, "D": D}, rn[:info.finaliter + 1]
+    return X, out_info, {"M": M, "D": D}
+
+
+def train_relaxpar(A, b, x_ex, method, kmax, options=None, run=None):
+    """train/train_relaxpar.m:43-271, SIRT branch.  ``run(A,b,K,x0,opts)``
+    defaults to this oracle's ``sirt`` with the named method."""
+    if run is None:
+        def run(A_, b_, K_, x0_, opts_):
+            X, info, _ = sirt(method, A_, b_, K_, x0_, opts_)
+            return X, info
+    opts = {} if options is None else dict(options)
+    if options is not None:
+        opts["stoprule"] = {"type": "none"}
+        opts.pop("relaxpar", None)
+    fudge = 0.02
+    stepk = 10
+    K = np.arange(1, stepk + 1)
+    k = 0
+    Er = []
+    x0 = None
+    while True:
+        if k + stepk > kmax:
+            K = np.arange(1, kmax - k + 1)
+            k = kmax
+        else:
+            k = k + stepk
+        Xnew, info = run(A, b, K, x0, opts)
+        if k <= stepk:
+            opts["rho"] = info["rho"]
+            relaxparmax = 2.0 / info["rho"]
+        Er.extend(np.sqrt(np.sum((Xnew - x_ex[:, None]) ** 2, axis=0)).tolist())
+        kopt = int(np.argmin(Er)) + 1
+        minE = Er[kopt - 1]
+        if kopt == k and k < kmax:
+            x0 = Xnew[:, -1]
+        else:
+            break
+    meub = (1 + fudge) * minE
+
+    def probe(lam):
+        o2 = dict(opts); o2["relaxpar"] = lam
+        X, _ = run(A, b, np.arange(1, kopt + 1), None, o2)
+        ee = np.sqrt(np.sum((X - x_ex[:, None]) ** 2, axis=0))
+        kk = int(np.sum((ee[:-1] > meub) & (np.diff(ee) < 0))) + 1
+        return kk, ee[kk - 1]
+
+    r = (3 - np.sqrt(5)) / 2
+    alpha, beta = 0.0, relaxparmax
+    alphap = alpha + r * (beta - alpha)
+    betap = alpha + (1 - r) * (beta - alpha)
+    k_a, e_a = probe(alphap)
+    k_b, e_b = probe(betap)
+    while abs(alphap - betap) > relaxparmax * 0.01:
+        if e_a > meub or (not e_b > meub and k_a >= k_b):
+            z = alphap + (1 - r) * abs(beta - alphap)
+            alpha, alphap, betap = alphap, betap, z
+            k_a, e_a = k_b, e_b
+            k_b, e_b = probe(betap)
+        else:
+            z = alpha + r * abs(betap - alpha)
+            beta, betap, alphap = betap, alphap, z
+            k_b, e_b = k_a, e_a
+            k_a, e_a = probe(alphap)
+    return (alphap + betap) / 2

@@ -1,0 +1,199 @@
+"""SIRT on the GPU (K2/K3/K4) against the FP64 oracle: iterates within 1e-5
+relative after K iterations on identical inputs (same A, b, rho), stop rules,
+save lists, bounds, relaxation strategies, weights, rho and train_relaxpar."""
+import numpy as np
+import pytest
+
+import airtoolsii_amd as air
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5          # relative 2-norm error of FP32 iterates vs the FP64 oracle
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    """C1: paralleltomo N=64, 0:179, p=91; Shepp-Logan; 1% seeded noise."""
+    N, theta, p = 64, np.arange(0.0, 180.0), 91
+    Ao = oracle.paralleltomo(N, theta, p)
+    A = air.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    e = np.random.default_rng(1).standard_normal(b.size)
+    b = b + 0.01 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    rho = {mth: (1.0 if mth == "sart" else oracle.spectral_radius(Ao, *oracle.weights(mth, Ao)))
+           for mth in ("landweber", "cimmino", "cav", "drop", "sart")}
+    return dict(A=A, Ao=Ao, x=x, b=b, rho=rho, N=N, theta=theta, p=p)
+
+
+@pytest.mark.parametrize("method", ["landweber", "cimmino", "cav", "drop", "sart"])
+def test_methods_match_oracle_c1(c1, oracle, method):
+    opts = {"rho": c1["rho"][method], "lbound": 0}
+    X, info, _ = air.sirt(method, c1["A"], c1["b"], 50, None, opts)
+    Xo, info_o, _ = oracle.sirt(method, c1["Ao"], c1["b"], 50, None, opts)
+    err = rel(X[:, -1], Xo[:, -1])
+    print(f"{method}: rel err {err:.2e}")
+    assert err <= TOL
+    assert info["finaliter"] == 50 and info["stoprule"] == 0
+    assert abs(info["relaxpar"] - 1.9 / c1["rho"][method]) <= 1e-15 * info["relaxpar"]
+
+
+@pytest.mark.parametrize("method", ["cimmino", "sart", "drop"])
+def test_wrappers_and_save_list(c1, oracle, method):
+    fn = getattr(air, method)
+    K = [5, 17, 40]
+    opts = {"rho": c1["rho"][method], "lbound": 0, "ubound": 1}
+    X, info, _ = fn(c1["A"], c1["b"], K, None, opts)
+    Xo, info_o, _ = oracle.sirt(method, c1["Ao"], c1["b"], K, None, opts)
+    assert X.shape == Xo.shape == (c1["A"].shape[1], 3)
+    assert list(info["itersaved"]) == list(info_o["itersaved"]) == K
+    for j in range(3):
+        assert rel(X[:, j], Xo[:, j]) <= TOL
+    assert X.min() >= 0 and X.max() <= 1
+
+
+@pytest.mark.parametrize("strategy", ["line", "psi1", "psi1mod", "psi2", "psi2mod", 0.8])
+def test_relaxation_strategies(c1, oracle, strategy):
+    method = "sart"
+    opts = {"relaxpar": strategy, "rho": 1.0, "lbound": 0}
+    X, info, _ = air.sirt(method, c1["A"], c1["b"], 30, None, opts)
+    Xo, info_o, _ = oracle.sirt(method, c1["Ao"], c1["b"], 30, None, opts)
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+    if strategy == "line":
+        assert abs(info["relaxpar"] - info_o["relaxpar"]) <= 1e-4 * abs(info_o["relaxpar"])
+
+
+def test_custom_method_and_landweber_identity(c1, oracle):
+    A, Ao, b = c1["A"], c1["Ao"], c1["b"]
+    rn2 = np.asarray(Ao.multiply(Ao).sum(1)).ravel()
+    Mc = np.where(rn2 > 0, 1 / np.where(rn2 > 0, rn2, 1), 0)
+    Dc = np.linspace(0.5, 1.5, A.shape[1])
+    opts = {"rho": oracle.spectral_radius(Ao, Mc, Dc)}
+    X, _, ext = air.sirt({"M": Mc, "D": Dc}, A, b, 25, None, opts)
+    Xo, _, _ = oracle.sirt({"M": Mc, "D": Dc}, Ao, b, 25, None, opts)
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+    assert np.array_equal(ext.M, Mc) and np.array_equal(ext.D, Dc)
+    X1, _, _ = air.sirt({}, A, b, 10, None, {"rho": c1["rho"]["landweber"]})
+    X2, _, _ = air.landweber(A, b, 10, None, {"rho": c1["rho"]["landweber"]})
+    assert np.array_equal(X1, X2)
+
+
+def test_warm_start_and_vector_bounds(c1, oracle):
+    n = c1["A"].shape[1]
+    rng = np.random.default_rng(3)
+    x0 = rng.uniform(0, 0.5, n)
+    lb = np.where(np.arange(n) % 3 == 0, 0.1, -np.inf)
+    ub = np.where(np.arange(n) % 5 == 0, 0.3, np.inf)
+    opts = {"rho": c1["rho"]["cimmino"], "lbound": lb, "ubound": ub}
+    X, _, _ = air.cimmino(c1["A"], c1["b"], 20, x0, opts)
+    Xo, _, _ = oracle.sirt("cimmino", c1["Ao"], c1["b"], 20, x0, opts)
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+    assert np.all(X[::3, -1] >= 0.1 - 1e-7) and np.all(X[::5, -1] <= 0.3 + 1e-7)
+
+
+def test_discrepancy_principle(c1, oracle):
+    opts = {"rho": 1.0, "lbound": 0}
+    _, _, _, rn = oracle.sirt("sart", c1["Ao"], c1["b"], 60, None, opts, return_resnorm=True)
+    k = 23
+    tau = float(np.sqrt(rn[k] * rn[k + 1]))          # between two iterations: robust to FP32 norms
+    o2 = dict(opts, stoprule={"type": "DP", "taudelta": tau})
+    X, info, _ = air.sart(c1["A"], c1["b"], [10, 20, 30, 60], None, o2)
+    Xo, info_o, _ = oracle.sirt("sart", c1["Ao"], c1["b"], [10, 20, 30, 60], None, o2)
+    assert info["stoprule"] == info_o["stoprule"] == 2
+    assert info["finaliter"] == info_o["finaliter"] == k + 1
+    assert list(info["itersaved"]) == list(info_o["itersaved"]) == [10, 20, k + 1]
+    assert X.shape[1] == 3 and rel(X[:, -1], Xo[:, -1]) <= TOL
+    np.testing.assert_allclose(info["resnorm"], rn[: k + 2], rtol=1e-5)
+    # satisfied at x0: empty X, itersaved = [0]
+    X, info, _ = air.sart(c1["A"], c1["b"], 10, None, dict(opts, stoprule={"type": "DP", "taudelta": 1e12}))
+    assert X.shape[1] == 0 and info["finaliter"] == 0 and list(info["itersaved"]) == [0]
+
+
+def test_monotone_error_rule(c1, oracle):
+    opts = {"rho": c1["rho"]["cimmino"]}
+    kmax = 60
+    Xall, _, _ = oracle.sirt("cimmino", c1["Ao"], c1["b"], np.arange(1, kmax + 1), None, opts)
+    R = [c1["b"]] + [c1["b"] - c1["Ao"] @ Xall[:, k] for k in range(kmax)]
+    crit = np.array([(R[k - 1] @ (R[k - 1] + R[k])) / 2 / np.linalg.norm(R[k - 1]) for k in range(1, kmax + 1)])
+    k = 30
+    tau = float(0.5 * (crit[k - 1] + crit[k]))       # between two iterations
+    o2 = dict(opts, stoprule={"type": "ME", "taudelta": tau})
+    X, info, _ = air.cimmino(c1["A"], c1["b"], kmax, None, o2)
+    Xo, info_o, _ = oracle.sirt("cimmino", c1["Ao"], c1["b"], kmax, None, o2)
+    assert info["stoprule"] == info_o["stoprule"] == 3
+    assert info["finaliter"] == info_o["finaliter"]
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+
+
+@pytest.mark.parametrize("method", ["cimmino", "cav", "drop", "sart"])
+def test_weights_match_oracle(c1, oracle, method):
+    Mo, Do = oracle.weights(method, c1["Ao"])
+    A64 = air.paralleltomo(c1["N"], c1["theta"], c1["p"], dtype="f64")
+    M, D = air.weights(method, A64)
+    np.testing.assert_allclose(M, Mo, rtol=1e-12)
+    np.testing.assert_allclose(D, Do, rtol=1e-12)
+    M, D = air.weights(method, c1["A"])
+    np.testing.assert_allclose(M, Mo, rtol=2e-6)
+    np.testing.assert_allclose(D, Do, rtol=2e-6)
+
+
+@pytest.mark.parametrize("method", ["landweber", "cimmino", "drop"])
+def test_spectral_radius(c1, method):
+    rho = air.spectral_radius(method, c1["A"])
+    assert abs(rho - c1["rho"][method]) <= 1e-5 * c1["rho"][method]
+
+
+def test_default_rho_path(c1, oracle):
+    X, info, _ = air.cimmino(c1["A"], c1["b"], 20)
+    assert abs(info["rho"] - c1["rho"]["cimmino"]) <= 1e-5 * c1["rho"]["cimmino"]
+    Xo, _, _ = oracle.sirt("cimmino", c1["Ao"], c1["b"], 20, None, {"rho": c1["rho"]["cimmino"]})
+    assert rel(X[:, -1], Xo[:, -1]) <= 1e-4
+
+
+@pytest.mark.parametrize("method", ["sart", "cimmino"])
+def test_matrix_free_sirt_matches_matrix(c1, oracle, method):
+    Af = air.paralleltomo(c1["N"], c1["theta"], c1["p"], None, 0, 0)
+    opts = {"rho": c1["rho"][method], "lbound": 0}
+    X, info, _ = air.sirt(method, Af, c1["b"], 20, None, opts)
+    Xo, _, _ = oracle.sirt(method, c1["Ao"], c1["b"], 20, None, opts)
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+
+
+@pytest.mark.parametrize("dtype,tol", [("bf16", 2e-2), ("f16", 3e-3), ("f64", 1e-6)])
+def test_reduced_precision_storage(c1, oracle, dtype, tol):
+    A = air.paralleltomo(c1["N"], c1["theta"], c1["p"], dtype=dtype)
+    opts = {"rho": 1.0, "lbound": 0}
+    X, _, _ = air.sart(A, c1["b"], 30, None, opts)
+    Xo, _, _ = oracle.sirt("sart", c1["Ao"], c1["b"], 30, None, opts)
+    err = rel(X[:, -1], Xo[:, -1])
+    print(f"{dtype}: rel err vs FP64 oracle {err:.2e}")
+    assert err <= tol
+
+
+def test_fanlinear_sirt(oracle):
+    N, theta, p = 48, np.arange(0.0, 360.0, 4.0), 68
+    A = air.fanlineartomo(N, theta, p)
+    Ao = oracle.fanlineartomo(N, theta, p)
+    x = oracle.phantomgallery("threephases", N, None, 2).ravel(order="F")
+    b = Ao @ x
+    opts = {"rho": 1.0, "lbound": 0, "ubound": 1}
+    X, _, _ = air.sart(A, b, 40, None, opts)
+    Xo, _, _ = oracle.sirt("sart", Ao, b, 40, None, opts)
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+
+
+def test_train_relaxpar_matches_oracle(oracle):
+    N, theta, p = 32, np.arange(0.0, 180.0, 3.0), 45
+    A = air.paralleltomo(N, theta, p)
+    Ao = oracle.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    e = np.random.default_rng(2).standard_normal(b.size)
+    b = b + 0.03 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    lam = air.train_relaxpar(A, b, x, "sart", 100)
+    lam_o = oracle.train_relaxpar(Ao, b, x, "sart", 100)
+    assert abs(lam - lam_o) <= 0.03 * 2.0

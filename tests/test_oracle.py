@@ -1,0 +1,444 @@
+"""Pins of the FP64 CPU oracle against golden values, closed forms, invariants
+and exact brute force (CPU only)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from geometry_ref import chord_endpoints, chord_length, exact_ray_cells
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ MATLAB built-ins
+def test_mt19937_matches_matlab_documented_rand(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "matlab_rand_default.json")))
+    got = oracle.mt_rand(g["seed"], 9)
+    assert np.max(np.abs(got - np.array(g["values_column_major"]))) < g["tolerance"]
+
+
+@pytest.mark.parametrize("seed", [1, 42, 2**31 - 1])
+def test_mt19937_matches_numpy_legacy_generator(oracle, seed):
+    ref = np.random.RandomState(seed).random_sample(1000)
+    assert np.array_equal(oracle.mt_rand(seed, 1000), ref)
+
+
+def test_sind_cosd_exact_at_right_angles(oracle):
+    for k in range(-8, 9):
+        s, c = oracle.sind(90 * k), oracle.cosd(90 * k)
+        assert (s, c) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)][k % 4]
+    assert math.copysign(1.0, -oracle.sind(0.0)) == -1.0      # -sind(0) is -0
+
+
+def test_sind_cosd_accuracy_and_symmetry(oracle):
+    rng = np.random.default_rng(0)
+    for deg in np.concatenate([rng.uniform(-720, 720, 500), np.arange(0, 360, 0.5)]):
+        s, c = oracle.sind(deg), oracle.cosd(deg)
+        rs, rc = math.sin(math.radians(deg)), math.cos(math.radians(deg))
+        assert abs(s - rs) <= 4e-16 + 1e-13 * abs(rs) and abs(c - rc) <= 4e-16 + 1e-13 * abs(rc)
+        assert oracle.sind(-deg) == -s and oracle.cosd(-deg) == c
+        assert abs(s * s + c * c - 1) < 4e-16
+
+
+def test_linspace_integer_spacing_exact(oracle):
+    for p in (2, 5, 91, 724, 725, 1448):
+        v = oracle.linspace(-(p - 1) / 2, (p - 1) / 2, p)
+        assert np.array_equal(v, np.arange(p) - (p - 1) / 2)
+
+
+# ------------------------------------------------------------------ paralleltomo
+def par_rays(oracle, theta, p, d):
+    x0 = oracle.linspace(-d / 2, d / 2, p)
+    out = []
+    for th in theta:
+        s, c = oracle.sind(th), oracle.cosd(th)
+        for j in range(p):
+            out.append((c * x0[j] - s * 0.0, s * x0[j] + c * 0.0, -s, c))
+    return out
+
+
+def test_parallel_axis_aligned_rays(oracle):
+    N, p = 8, 9
+    A = oracle.paralleltomo(N, [0, 90, 180, 270], p).toarray()
+    # theta = 0: vertical ray at x = -4+j covers image column j; the ray on the right edge is empty
+    for j in range(p):
+        row = A[j]
+        if j == N:
+            assert not row.any()
+        else:
+            assert np.array_equal(np.nonzero(row)[0], j * N + np.arange(N)) and np.all(row[row != 0] == 1.0)
+    # theta = 90: horizontal ray at y = -4+j covers image row N-1-j; the top edge ray is empty
+    for j in range(p):
+        row = A[p + j]
+        if j == N:
+            assert not row.any()
+        else:
+            assert np.array_equal(np.nonzero(row)[0], np.arange(N) * N + (N - 1 - j))
+    # 180 and 270 degrees trace the same lines in the opposite order
+    assert np.array_equal(A[2 * p:3 * p][::-1], A[0:p])
+    assert np.array_equal(A[3 * p:4 * p][::-1], A[p:2 * p])
+
+
+def test_parallel_diagonal_ray_sqrt2_segments(oracle):
+    N = 8
+    A = oracle.paralleltomo(N, [45], 1, 0.0)          # one ray through the origin
+    v = A.toarray()[0]
+    nz = v[v != 0]
+    assert len(nz) == N and np.allclose(nz, math.sqrt(2), rtol=0, atol=1e-12)
+    # direction (-sind 45, cosd 45): the line y = -x, cells (fx, fy = N-1-fx),
+    # column fx*N + (N-1-fy) = fx*(N+1)
+    cols = np.nonzero(v)[0]
+    assert np.array_equal(cols, np.arange(N) * (N + 1))
+
+
+@pytest.mark.parametrize("N,p,d", [(16, 23, None), (15, 20, 25.0), (32, 45, 40.0)])
+def test_parallel_row_sums_are_chord_lengths(oracle, N, p, d):
+    rng = np.random.default_rng(N)
+    theta = np.concatenate([rng.uniform(0, 360, 20), [0, 30, 45, 90, 135, 180]])
+    d = float(p - 1) if d is None else d
+    A = oracle.paralleltomo(N, theta, p, d)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    for r, (x0, y0, a, b) in enumerate(par_rays(oracle, theta, p, d)):
+        edge = (b == 0 and abs(y0 - N / 2) < 1e-15) or (a == 0 and abs(x0 - N / 2) < 1e-15)
+        ref = 0.0 if edge else chord_length(x0, y0, a, b, N / 2)
+        assert abs(rs[r] - ref) <= 1e-12 * max(1.0, ref), (r, rs[r], ref)
+
+
+def test_parallel_nnz_is_cells_crossed(oracle):
+    N, p = 32, 45
+    rng = np.random.default_rng(3)
+    theta = rng.uniform(0.5, 179.5, 30)
+    A = oracle.paralleltomo(N, theta, p, 40.0)
+    nnz = np.diff(A.indptr)
+    for r, (x0, y0, a, b) in enumerate(par_rays(oracle, theta, p, 40.0)):
+        ep = chord_endpoints(x0, y0, a, b, N / 2)
+        if ep is None:
+            assert nnz[r] == 0
+            continue
+        (px, py), (qx, qy) = ep
+        # generic rays only: skip lines through a lattice point (corner), where
+        # two crossings coincide and one cell fewer is entered
+        ys = [y0 + ((k - x0) / a) * b for k in range(-N // 2, N // 2 + 1)]
+        if any(abs(y - round(y)) < 1e-9 and abs(y) <= N / 2 for y in ys):
+            continue
+        inner = lambda u, v: sum(1 for k in range(-N // 2 + 1, N // 2) if min(u, v) < k < max(u, v))
+        assert nnz[r] == 1 + inner(px, qx) + inner(py, qy)
+
+
+@pytest.mark.parametrize("N", [3, 4, 7, 8])
+def test_parallel_matches_exact_rational_bruteforce(oracle, N):
+    rng = np.random.default_rng(N)
+    theta = np.concatenate([rng.uniform(0, 360, 12), [0, 45, 90, 135, 180, 225, 270, 315]])
+    p = 2 * N + 1
+    d = float(N + 2)
+    A = oracle.paralleltomo(N, theta, p, d).tocsr()
+    for r, (x0, y0, a, b) in enumerate(par_rays(oracle, theta, p, d)):
+        edge = (b == 0 and abs(y0 - N / 2) < 1e-15) or (a == 0 and abs(x0 - N / 2) < 1e-15)
+        ref = {} if edge else exact_ray_cells(N, x0, y0, a, b)
+        got = dict(zip(A.indices[A.indptr[r]:A.indptr[r + 1]].tolist(), A.data[A.indptr[r]:A.indptr[r + 1]]))
+        # exact brute force keeps zero-length-free cells; the reference drops points within 1e-10
+        ref = {c: v for c, v in ref.items() if v > 1e-9}
+        got = {c: v for c, v in got.items() if v > 1e-9}
+        assert set(got) == set(ref), (r, sorted(got), sorted(ref))
+        for c in ref:
+            assert abs(got[c] - ref[c]) <= 1e-13, (r, c, got[c], ref[c])
+
+
+def test_parallel_rays_missing_grid_give_empty_rows(oracle):
+    N = 8
+    A = oracle.paralleltomo(N, [0, 37], 5, 40.0)      # spacing 10: only the central ray hits
+    nnz = np.diff(A.indptr)
+    assert nnz[0] == 0 and nnz[1] == 0 and nnz[2] > 0 and nnz[3] == 0 and nnz[4] == 0
+
+
+def test_c1_size_and_nnz(oracle):
+    A = oracle.paralleltomo(64, np.arange(0, 180.0), 91)
+    assert A.shape == (16380, 4096)
+    assert 9.0e5 < A.nnz < 9.8e5
+    assert np.all(np.diff(A.indices[A.indptr[100]:A.indptr[101]]) > 0)
+
+
+# ------------------------------------------------------------------ fanlineartomo
+def fan_rays(oracle, N, theta, p, R, dw, sd):
+    RN, dwN, sdN = R * N, dw * N, sd * N
+    dew = dwN / p
+    if p % 2:
+        h = (p - 1) // 2
+        om = [90 - (180 / math.pi) * math.atan(sdN / (k * dew)) for k in range(1, h + 1)]
+        omega = [-v for v in om[::-1]] + [0.0] + om
+    else:
+        h = p // 2
+        om = [90 - (180 / math.pi) * math.atan(sdN / ((k - 0.5) * dew)) for k in range(1, h + 1)]
+        omega = [-v for v in om[::-1]] + om
+    out = []
+    for th in theta:
+        st, ct = oracle.sind(th), oracle.cosd(th)
+        x0t, y0t = ct * 0.0 + (-st) * RN, st * 0.0 + ct * RN
+        abx, aby = (-x0t) / RN, (-y0t) / RN
+        for w in omega:
+            so, co = oracle.sind(w), oracle.cosd(w)
+            out.append((x0t, y0t, co * abx + (-so) * aby, so * abx + co * aby))
+    return out
+
+
+@pytest.mark.parametrize("N,p", [(16, 23), (15, 22), (24, 34)])
+def test_fan_row_sums_are_chord_lengths_from_source(oracle, N, p):
+    theta = np.arange(0, 360, 17.0)
+    A = oracle.fanlineartomo(N, theta, p, 2.0, 2.5, 3.0)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    # independent geometry: source on a circle of radius R*N, ray j aimed at the
+    # detector element centre deposall_j on a line at distance sd*N from the source
+    dew = 2.5 * N / p
+    dep = (np.arange(1, (p - 1) // 2 + 1) * dew) if p % 2 else ((np.arange(1, p // 2 + 1) - 0.5) * dew)
+    deposall = np.concatenate([-dep[::-1], [0.0], dep]) if p % 2 else np.concatenate([-dep[::-1], dep])
+    r = 0
+    for th in theta:
+        t = math.radians(th)
+        S = np.array([-math.sin(t) * 2 * N, math.cos(t) * 2 * N])
+        u = -S / np.linalg.norm(S)                      # towards the centre
+        v = np.array([-u[1], u[0]])                      # along the detector (+omega side)
+        for dp in deposall:
+            target = S + 3 * N * u + dp * v
+            dvec = (target - S) / np.linalg.norm(target - S)
+            ref = chord_length(S[0], S[1], dvec[0], dvec[1], N / 2)
+            assert abs(rs[r] - ref) <= 1e-9 * max(1.0, ref), (r, rs[r], ref)
+            r += 1
+
+
+@pytest.mark.parametrize("N", [4, 7])
+def test_fan_matches_exact_rational_bruteforce(oracle, N):
+    theta = np.array([0.0, 33.0, 90.0, 151.0, 270.0, 300.0])
+    p = 2 * N + 3
+    A = oracle.fanlineartomo(N, theta, p, 2.0, 2.5, 3.0).tocsr()
+    for r, (x0, y0, a, b) in enumerate(fan_rays(oracle, N, theta, p, 2.0, 2.5, 3.0)):
+        ref = {c: v for c, v in exact_ray_cells(N, x0, y0, a, b).items() if v > 1e-9}
+        got = dict(zip(A.indices[A.indptr[r]:A.indptr[r + 1]].tolist(), A.data[A.indptr[r]:A.indptr[r + 1]]))
+        got = {c: v for c, v in got.items() if v > 1e-9}
+        assert set(got) == set(ref), (r, sorted(got), sorted(ref))
+        for c in ref:
+            assert abs(got[c] - ref[c]) <= 1e-12
+
+
+def test_fan_rejects_source_inside(oracle):
+    with pytest.raises(ValueError):
+        oracle.fanlineartomo(8, [0], 5, 0.5)
+
+
+# ------------------------------------------------------------------ weights
+def small_problem(oracle, N=16, p=23, d=30.0):
+    return oracle.paralleltomo(N, np.arange(0, 180, 7.0), p, d)
+
+
+def test_weights_closed_forms(oracle):
+    A = small_problem(oracle)
+    m, n = A.shape
+    A2 = A.multiply(A)
+    rn2 = np.asarray(A2.sum(1)).ravel()
+    rab = np.asarray(abs(A).sum(1)).ravel()
+    cab = np.asarray(abs(A).sum(0)).ravel()
+    cnz = np.diff(A.tocsc().indptr).astype(float)
+    with np.errstate(divide="ignore"):
+        exp = {
+            "landweber": (np.ones(m), np.ones(n)),
+            "cimmino": ((1 / m) * (1 / rn2), np.ones(n)),
+            "cav": (1 / (A2 @ cnz), np.ones(n)),
+            "drop": (1 / rn2, 1 / cnz),
+            "sart": (1 / rab, 1 / cab),
+        }
+    assert (rn2 == 0).any(), "problem should contain rays that miss the grid"
+    for meth, (Me, De) in exp.items():
+        Me, De = np.where(np.isinf(Me), 0, Me), np.where(np.isinf(De), 0, De)
+        M, D = oracle.weights(meth, A)
+        np.testing.assert_allclose(M, Me, rtol=1e-13, atol=0)
+        np.testing.assert_allclose(D, De, rtol=1e-13, atol=0)
+
+
+# ------------------------------------------------------------------ calczeta / relaxpar
+def test_calczeta_roots(oracle):
+    k = np.arange(2, 200)
+    z = oracle.calczeta(k)
+    assert abs(z[0] - 1 / 3) < 1e-15
+    assert np.all((z > 0) & (z < 1)) and np.all(np.diff(z) > 0)
+    for kk, zz in zip(k, z):
+        res = (2 * kk - 1) * zz ** (kk - 1) - sum(zz ** i for i in range(kk - 1))
+        assert abs(res) < 1e-10 * (2 * kk)
+    with pytest.raises(ValueError):
+        oracle.calczeta([])
+
+
+def test_psi_vectors_in_stable_range(oracle):
+    rho = 3.0
+    for s in ("psi1", "psi1mod", "psi2", "psi2mod"):
+        casel, lam, _ = oracle.relaxpar_vector(s, rho, 30)
+        assert casel == 3 and lam.shape == (30,)
+        assert np.all(lam > 0) and np.allclose(lam[:2], math.sqrt(2) / rho)
+    _, l1, _ = oracle.relaxpar_vector("psi1", rho, 30)
+    _, l1m, _ = oracle.relaxpar_vector("psi1mod", rho, 30)
+    np.testing.assert_allclose(l1m[2:], 2 * l1[2:], rtol=1e-15)
+    assert np.all(l1[2:] * rho < 2)
+
+
+# ------------------------------------------------------------------ SIRT loop
+@pytest.fixture(scope="module")
+def prob(oracle):
+    A = oracle.paralleltomo(24, np.arange(0, 180, 4.0), 34)
+    x = oracle.phantomgallery("shepplogan", 24).ravel(order="F")
+    b = A @ x
+    e = np.random.default_rng(5).standard_normal(b.size)
+    b = b + 0.01 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    return A, x, b
+
+
+def test_landweber_equals_identity_custom_method(oracle, prob):
+    A, x, b = prob
+    X1, i1, _ = oracle.sirt("landweber", A, b, 20)
+    X2, i2, _ = oracle.sirt({}, A, b, 20)
+    assert np.array_equal(X1, X2) and i1["rho"] == i2["rho"]
+
+
+def test_cimmino_equals_scaled_custom_M_under_default_relaxpar(oracle, prob):
+    A, x, b = prob
+    rn2 = np.asarray(A.multiply(A).sum(1)).ravel()
+    Mc = np.where(rn2 > 0, 1 / np.where(rn2 > 0, rn2, 1), 0)
+    X1, i1, _ = oracle.sirt("cimmino", A, b, 30)
+    X2, i2, _ = oracle.sirt({"M": Mc}, A, b, 30)
+    np.testing.assert_allclose(i2["rho"], i1["rho"] * A.shape[0], rtol=1e-9)
+    np.testing.assert_allclose(X1, X2, rtol=1e-8, atol=1e-12)
+
+
+def test_diagonal_system_closed_form(oracle):
+    n = 50
+    a = np.linspace(0.5, 2.0, n)
+    xs = np.cos(np.arange(n))
+    A = sp.diags(a).tocsr()
+    b = a * xs
+    lam = 0.3
+    K = [1, 2, 5, 17]
+    X, info, _ = oracle.sirt("landweber", A, b, K, None, {"relaxpar": lam})
+    for col, k in enumerate(K):
+        np.testing.assert_allclose(X[:, col], xs * (1 - (1 - lam * a * a) ** k), rtol=1e-12, atol=1e-14)
+    # Cimmino on a diagonal system: M = 1/(n a^2) -> x_k = x*(1 - (1 - lam/n)^k)
+    X, _, _ = oracle.sirt("cimmino", A, b, K, None, {"relaxpar": 40.0})
+    for col, k in enumerate(K):
+        np.testing.assert_allclose(X[:, col], xs * (1 - (1 - 40.0 / n) ** k), rtol=1e-12, atol=1e-14)
+    # SART: M = 1/a, D = 1/a -> x_k = x*(1 - (1 - lam)^k), rho = 1
+    X, info, _ = oracle.sirt("sart", A, b, K, None, {"relaxpar": 0.7})
+    assert info["rho"] == 1.0
+    for col, k in enumerate(K):
+        np.testing.assert_allclose(X[:, col], xs * (1 - 0.3 ** k), rtol=1e-12, atol=1e-14)
+
+
+def test_line_search_solves_scaled_identity_in_one_step(oracle):
+    n = 30
+    A = (2.5 * sp.identity(n)).tocsr()
+    xs = np.sin(np.arange(n) + 1.0)
+    X, info, _ = oracle.sirt("landweber", A, 2.5 * xs, 1, None, {"relaxpar": "line"})
+    np.testing.assert_allclose(X[:, 0], xs, rtol=1e-14)
+    np.testing.assert_allclose(info["relaxpar"], 1 / 2.5 ** 2, rtol=1e-14)
+
+
+def test_default_rho_matches_power_iteration(oracle, prob):
+    A, x, b = prob
+    M, D = oracle.weights("drop", A)
+    rho = oracle.spectral_radius(A, M, D)
+    v = np.ones(A.shape[1])
+    for _ in range(3000):
+        w = D * (A.T @ (M * (A @ v)))
+        lam = np.linalg.norm(w) / np.linalg.norm(v)
+        v = w / np.linalg.norm(w)
+    assert abs(rho - lam) < 1e-6 * rho
+
+
+def test_bounds_projection(oracle, prob):
+    A, x, b = prob
+    X, _, _ = oracle.sirt("sart", A, b, 15, None, {"lbound": 0, "ubound": 1})
+    assert X.min() >= 0 and X.max() <= 1
+    n = A.shape[1]
+    lb = np.full(n, -np.inf); lb[::3] = 0.2
+    ub = np.full(n, np.inf); ub[1::3] = 0.1
+    X, _, _ = oracle.sirt("sart", A, b, 15, None, {"lbound": lb, "ubound": ub})
+    assert np.all(X[::3, 0] >= 0.2) and np.all(X[1::3, 0] <= 0.1)
+
+
+def test_save_list_semantics(oracle, prob):
+    A, x, b = prob
+    Xall, _, _ = oracle.sirt("cimmino", A, b, np.arange(1, 21), None, {"rho": 0.01})
+    Xk, info, _ = oracle.sirt("cimmino", A, b, [3, 7, 20], None, {"rho": 0.01})
+    assert np.array_equal(Xk, Xall[:, [2, 6, 19]])
+    assert list(info["itersaved"]) == [3, 7, 20] and info["finaliter"] == 20 and info["stoprule"] == 0
+
+
+def test_discrepancy_principle_stop(oracle, prob):
+    A, x, b = prob
+    Xall, _, _, rn = oracle.sirt("sart", A, b, np.arange(1, 61), None, {}, return_resnorm=True)
+    tau = rn[25] * 1.0000001                            # first k with ||r_k|| <= tau
+    kstop = int(np.argmax(rn <= tau))
+    X, info, _ = oracle.sirt("sart", A, b, [5, 10, 60], None, {"stoprule": {"type": "DP", "taudelta": tau}})
+    assert info["stoprule"] == 2 and info["finaliter"] == kstop
+    assert list(info["itersaved"]) == [k for k in (5, 10) if k < kstop] + [kstop]
+    assert np.array_equal(X[:, -1], Xall[:, kstop - 1])
+    # DP satisfied at x0: no iteration, empty X, itersaved = [0]
+    X, info, _ = oracle.sirt("sart", A, b, 10, None, {"stoprule": {"type": "DP", "taudelta": 1e9}})
+    assert X.shape[1] == 0 and info["finaliter"] == 0 and list(info["itersaved"]) == [0]
+
+
+def test_monotone_error_rule(oracle, prob):
+    A, x, b = prob
+    kmax = 80
+    Xall, _, _ = oracle.sirt("cimmino", A, b, np.arange(1, kmax + 1), None, {"rho": 0.02})
+    R = [b - A @ np.zeros(A.shape[1])] + [b - A @ Xall[:, k] for k in range(kmax)]
+    crit = [(R[k - 1] @ (R[k - 1] + R[k])) / 2 / np.linalg.norm(R[k - 1]) for k in range(1, kmax + 1)]
+    tau = crit[40] * 1.000001
+    kstop = 1 + int(np.argmax(np.array(crit) <= tau))
+    X, info, _ = oracle.sirt("cimmino", A, b, kmax, None, {"rho": 0.02, "stoprule": {"type": "ME", "taudelta": tau}})
+    assert info["stoprule"] == 3 and info["finaliter"] == kstop
+    np.testing.assert_array_equal(X[:, -1], Xall[:, kstop - 1])
+
+
+def test_train_relaxpar_on_synthetic_error_model(oracle):
+    # Error curve e_k(lam) = 1 + 0.5*max(k* - k, 0)^2 + 0.001*max(k - k*, 0) with
+    # k* = ceil(10/lam): every lam reaches the same floor, larger lam sooner, so the
+    # search must end near the top of (0, 2/rho).  Warm starts continue the count.
+    rho = 1.0
+    state = {"k_end": 0}
+
+    def run(A, b, K, x0, opts):
+        lam = opts.get("relaxpar", 1.9 / rho)
+        k0 = 0 if x0 is None else state["k_end"]
+        ks = k0 + np.asarray(K)
+        state["k_end"] = int(ks[-1])
+        kstar = math.ceil(10 / lam)
+        err = 1 + 0.5 * np.maximum(kstar - ks, 0) ** 2 + 0.001 * np.maximum(ks - kstar, 0)
+        return err[None, :], {"rho": rho}
+
+    lam = oracle.train_relaxpar(None, None, np.zeros(1), "sart", 100, None, run=run)
+    assert 0.9 * 2 / rho <= lam < 2 / rho
+
+
+# ------------------------------------------------------------------ phantoms
+def test_phantoms_basic_properties(oracle):
+    sl = oracle.phantomgallery("shepplogan", 64)
+    assert sl.min() == 0 and abs(sl.max() - 1) < 1e-12
+    assert sl[32, 32] > 0 and sl[0, 0] == 0
+    g = oracle.phantomgallery("grains", 64, None, 1)
+    labels = np.unique(g)
+    assert g.max() == 1 and len(labels) <= round(3 * math.sqrt(64))
+    t = oracle.phantomgallery("threephases", 64, None, 2)
+    assert set(np.unique(t)).issubset({0.0, 0.5, 1.0})
+    s = oracle.phantomgallery("smooth", 32)
+    assert abs(s.max() - 1) < 1e-15 and s.min() > 0
+
+
+def test_product_phantoms_match_oracle(oracle):
+    from airtoolsii_amd import phantomgallery
+    for name, args in (("shepplogan", ()), ("smooth", (None,)), ("grains", (None, 1)), ("grains", (30, 7)),
+                       ("threephases", (None, 3))):
+        a = phantomgallery(name, 48, *args)
+        b = oracle.phantomgallery(name, 48, *args)
+        if name == "threephases":
+            assert np.mean(a != b) < 1e-3
+        else:
+            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
