@@ -27,7 +27,8 @@ BOUND_NONE, BOUND_SCALAR, BOUND_VECTOR = 0, 1, 2
 EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "air_set_matrix_csr",
            "air_set_matrix_csc", "air_build_paralleltomo", "air_build_fanlineartomo", "air_size", "air_get_csr",
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
-           "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing"]
+           "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
+           "air_layout_info"]
 
 
 class AirError(RuntimeError):
@@ -44,6 +45,14 @@ class SirtOptions(C.Structure):
                 ("ub_kind", C.c_int), ("ub", C.c_double), ("ub_vec", C.c_void_p),
                 ("stoprule", C.c_int), ("taudelta", C.c_double), ("res_dims", C.c_int * 2),
                 ("ncp_smooth", C.c_int), ("verbose", C.c_int)]
+
+
+class Layout(C.Structure):
+    _fields_ = [("tiled", C.c_int), ("value_bytes", C.c_int), ("m", C.c_int64), ("n", C.c_int64),
+                ("nnz", C.c_int64), ("n_internal", C.c_int64), ("pieces", C.c_int64), ("runs", C.c_int64),
+                ("tiles_fwd", C.c_int64), ("tiles_bwd", C.c_int64), ("staged_rows", C.c_int64),
+                ("lds_bytes", C.c_int64), ("fwd_lanes", C.c_int), ("fwd_loads", C.c_int),
+                ("bwd_lanes", C.c_int), ("bwd_loads", C.c_int)]
 
 
 class SirtInfo(C.Structure):
@@ -89,7 +98,8 @@ def load():
     L.air_sirt_iterate.argtypes = [_P, C.c_int32, _dp]
     L.air_sirt_fetch.argtypes = [_P, _P, _P, C.POINTER(SirtInfo)]
     L.air_set_kernel_timing.argtypes = [_P, C.c_int]
-    L.air_get_kernel_timing.argtypes = [_P, _dp, _dp, C.POINTER(_i64)]
+    L.air_get_kernel_timing.argtypes = [_P, _dp, _dp, _dp, C.POINTER(_i64)]
+    L.air_layout_info.argtypes = [_P, C.POINTER(Layout)]
     for name in EXPORTS:
         if name not in ("air_last_error", "air_destroy"):
             getattr(L, name).restype = C.c_int

@@ -26,9 +26,10 @@ ARCH = os.environ.get("AIR_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = {
-    "air_tracer.hip": ["-ffp-contract=off"],
+    "air_tracer.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
     "air_sort.hip": [],
     "air_sirt.hip": [],
+    "air_tiled.hip": [],
     "air_engine.hip": [],
 }
 
@@ -54,7 +55,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -50,12 +50,13 @@ struct air_engine {
     air::DevSparse A, AT;
     air::Shape shA, shAT;
     air::Geometry geo;
+    air::TiledOp T;            // LDS-tiled layout used by the SIRT loop when ready
     air::SirtState st;
     // per-kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
-    double t_rows_ms = 0, t_cols_ms = 0;
+    double t_rows_ms = 0, t_cols_ms = 0, t_red_ms = 0;
     int64_t launches = 0;
 };
 
@@ -125,6 +126,7 @@ __global__ void k_widen(const V* in, double* out, int64_t n) {
 void reset_operator(air_engine* e) {
     e->A = DevSparse();
     e->AT = DevSparse();
+    e->T = TiledOp();
     e->geo.rays.release();
     e->geo.host_rays.clear();
     e->kind = 0;
@@ -132,14 +134,25 @@ void reset_operator(air_engine* e) {
     e->st.prepared = false;
 }
 
-void finish_matrix(air_engine* e) {
+bool tiled_disabled() {
+    const char* v = std::getenv("AIR_DISABLE_TILED");
+    return v && v[0] && v[0] != '0';
+}
+
+// CSC by transpose, then the LDS-tiled layout of the SIRT loop (image side N
+// for traced geometries, 0 for a general matrix).
+void finish_matrix(air_engine* e, int N) {
     csr_to_csc(e->A, e->AT, e->stream);
     e->shA = default_shape(e->A, e->num_cu);
     e->shAT = default_shape(e->AT, e->num_cu);
     e->kind = 1;
     e->m = e->A.nrows;
     e->n = e->A.ncols;
+    e->T = TiledOp();
+    if (!tiled_disabled()) build_tiled(e->A, N, e->T, e->stream);
 }
+
+bool use_tiled(const air_engine* e) { return e->kind == 1 && e->T.ready; }
 
 // y = A x (trans=0) or y = A' x (trans=1), device FP32 in / out.
 void apply_dev(air_engine* e, int trans, const float* in, float* out) {
@@ -367,10 +380,12 @@ void relax_schedule(int kind, double relaxpar, const double* vec, double rho, in
     }
 }
 
+static constexpr int EV_PER_ITER = 4;
+
 void timing_begin(air_engine* e, int niter) {
     e->ev_used = 0;
     if (!e->timing) return;
-    size_t need = 3 * (size_t)std::max(niter, 1);
+    size_t need = EV_PER_ITER * (size_t)std::max(niter, 1);
     while (e->ev.size() < need) {
         hipEvent_t v;
         AIR_HIP(hipEventCreate(&v));
@@ -380,45 +395,84 @@ void timing_begin(air_engine* e, int niter) {
 
 void record(air_engine* e, int slot) {
     if (!e->timing) return;
-    AIR_HIP(hipEventRecord(e->ev[(size_t)e->ev_used * 3 + slot], e->stream));
+    AIR_HIP(hipEventRecord(e->ev[(size_t)e->ev_used * EV_PER_ITER + slot], e->stream));
 }
 
-// one SIRT iteration k (sirt.m:213-243)
+hipEvent_t mid_event(air_engine* e) {
+    return e->timing ? e->ev[(size_t)e->ev_used * EV_PER_ITER + 2] : nullptr;
+}
+
+// one SIRT iteration k (sirt.m:213-243): the A'(M r) + update pass, then the
+// residual pass; events 0..3 bracket them when kernel timing is on.
 void enqueue_iteration(air_engine* e, int k) {
     SirtState& st = e->st;
     hipStream_t s = e->stream;
     const bool line = st.relax_kind == AIR_RELAX_LINE;
     double lam = line ? 0.0 : st.lam[k - 1];
-    if (e->kind == 1) {
-        record(e, 0);
+    record(e, 0);
+    if (use_tiled(e)) {
+        launch_tiled_update(e->T, st.bufs, lam, line ? 1 : 0, k, s);
+        record(e, 1);
+        launch_tiled_resid(e->T, st.bufs, st.sp, k, s, mid_event(e));
+    } else if (e->kind == 1) {
         if (line) launch_line_update(e->AT, e->shAT, st.bufs, k, s);
         else launch_update(e->AT, e->shAT, st.bufs, lam, k, s);
         record(e, 1);
         launch_resid(e->A, e->shA, st.bufs, st.sp, k, s);
         record(e, 2);
-        if (e->timing) e->ev_used++;
     } else {
-        record(e, 0);
         launch_mf_update(e->geo, e->n, st.bufs, lam, line ? 1 : 0, k, s);
         record(e, 1);
         launch_mf_resid(e->geo, st.bufs, st.sp, k, st.Ax.as<float>(), s);
         record(e, 2);
-        if (e->timing) e->ev_used++;
     }
+    record(e, 3);
+    if (e->timing) e->ev_used++;
 }
 
 // after the stream has drained: sum the per-kernel event intervals
 void timing_end(air_engine* e) {
     if (!e->timing) return;
     for (int i = 0; i < e->ev_used; ++i) {
-        float a = 0, b = 0;
-        AIR_HIP(hipEventElapsedTime(&a, e->ev[3 * i], e->ev[3 * i + 1]));
-        AIR_HIP(hipEventElapsedTime(&b, e->ev[3 * i + 1], e->ev[3 * i + 2]));
+        float a = 0, b = 0, c = 0;
+        hipEvent_t* v = &e->ev[(size_t)EV_PER_ITER * i];
+        AIR_HIP(hipEventElapsedTime(&a, v[0], v[1]));
+        AIR_HIP(hipEventElapsedTime(&b, v[1], v[2]));
+        AIR_HIP(hipEventElapsedTime(&c, v[2], v[3]));
         e->t_cols_ms += a;
         e->t_rows_ms += b;
+        e->t_red_ms += c;
         e->launches += 1;
     }
     e->ev_used = 0;
+}
+
+// length of the x-like vectors (x, D, bounds) of the SIRT loop
+int64_t xlen(const air_engine* e) { return use_tiled(e) ? e->T.n_int : e->n; }
+
+// user-order FP64 n-vector -> FP32 vector of the loop's column order
+void upload_xlike(air_engine* e, DevBuf& d, const double* h, hipStream_t s) {
+    if (!use_tiled(e)) {
+        upload_f32(d, h, e->n, s);
+        return;
+    }
+    DevBuf tmp;
+    upload_f32(tmp, h, e->n, s);
+    d.alloc(4 * (size_t)e->T.n_int);
+    permute_to_internal(e->T, tmp.as<float>(), d.as<float>(), s);
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
+// FP32 vector of the loop's column order -> user-order FP64 host vector
+void download_xlike(air_engine* e, const float* d, double* h, hipStream_t s) {
+    if (!use_tiled(e)) {
+        download_f64(d, h, e->n, s);
+        return;
+    }
+    DevBuf tmp;
+    tmp.alloc(4 * (size_t)std::max<int64_t>(e->n, 1));
+    permute_to_user(e->T, d, tmp.as<float>(), s);
+    download_f64(tmp.as<float>(), h, e->n, s);
 }
 
 void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, const double* x0, int kmax) {
@@ -462,18 +516,22 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     relax_schedule(o->relax_kind, o->relaxpar, o->relax_vec, rho, kmax, st.lam);
     // device vectors
     upload_f32(st.b, b, m, s);
-    if (x0) upload_f32(st.x, x0, n, s);
+    if (x0) upload_xlike(e, st.x, x0, s);
     else {
         std::vector<double> z(n, 0.0);
-        upload_f32(st.x, z.data(), n, s);
+        upload_xlike(e, st.x, z.data(), s);
     }
     if (st.hasM) upload_f32(st.M, M.data(), m, s);
-    if (st.hasD) upload_f32(st.D, D.data(), n, s);
+    if (st.hasD) upload_xlike(e, st.D, D.data(), s);
+    const int64_t nx = xlen(e);
     st.r.alloc(4 * std::max<int64_t>(m, 1));
     st.Mr.alloc(4 * std::max<int64_t>(m, 1));
-    st.g.alloc(8 * std::max<int64_t>(n, 1));
+    st.g.alloc(8 * std::max<int64_t>(nx, 1));
     st.Ax.alloc(4 * std::max<int64_t>(m, 1));
-    st.partial.alloc(8 * 16384);
+    // row-pass reductions: 4 doubles per block; column-pass ones: 1 per block
+    const int64_t row_slots = 4 * std::max<int64_t>(2048, use_tiled(e) ? tiled_reduce_grid(m) : 2048);
+    const int64_t col_slots = std::max<int64_t>(2048, e->T.ntiles_b);
+    st.partial.alloc(8 * (size_t)(row_slots + col_slots));
     st.hist.alloc(8 * (size_t)(kmax + 1));
     st.ctrl.alloc(sizeof(Ctrl));
     AIR_HIP(hipMemsetAsync(st.ctrl.p, 0, sizeof(Ctrl), s));
@@ -493,15 +551,16 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     B.ub = (float)o->ub;
     if (o->lb_kind == AIR_BOUND_VECTOR) {
         AIR_CHECK(o->lb_vec, AIR_E_ARG, "lbound vector missing");
-        upload_f32(st.lbv, o->lb_vec, n, s);
+        upload_xlike(e, st.lbv, o->lb_vec, s);
         B.lbv = st.lbv.as<float>();
     }
     if (o->ub_kind == AIR_BOUND_VECTOR) {
         AIR_CHECK(o->ub_vec, AIR_E_ARG, "ubound vector missing");
-        upload_f32(st.ubv, o->ub_vec, n, s);
+        upload_xlike(e, st.ubv, o->ub_vec, s);
         B.ubv = st.ubv.as<float>();
     }
     B.partial = st.partial.as<double>();
+    B.partial_cols = st.partial.as<double>() + row_slots;
     B.hist = st.hist.as<double>();
     B.ctrl = st.ctrl.as<Ctrl>();
     st.sp.type = o->stoprule;
@@ -510,7 +569,8 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     st.kmax = kmax;
     st.k = 0;
     // r0 = b - A x0 and the k = 0 check (sirt.m:153-162)
-    if (e->kind == 1) launch_resid(e->A, e->shA, B, st.sp, 0, s);
+    if (use_tiled(e)) launch_tiled_resid(e->T, B, st.sp, 0, s, nullptr);
+    else if (e->kind == 1) launch_resid(e->A, e->shA, B, st.sp, 0, s);
     else launch_mf_resid(e->geo, B, st.sp, 0, st.Ax.as<float>(), s);
     AIR_HIP(hipStreamSynchronize(s));
     st.prepared = true;
@@ -567,38 +627,43 @@ int air_synchronize(air_engine* e) {
     });
 }
 
+// Upload a host CSR into e->A (values narrowed on the device); no transpose.
+static void upload_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* colind,
+                       const double* val, int dtype) {
+    AIR_CHECK(e && rowptr && m >= 0 && n >= 0, AIR_E_ARG, "invalid CSR arguments");
+    AIR_CHECK(n <= INT32_MAX, AIR_E_ARG, "n must fit int32 column indices");
+    use_device(e);
+    reset_operator(e);
+    hipStream_t s = e->stream;
+    int64_t nnz = rowptr[m];
+    AIR_CHECK(rowptr[0] == 0 && nnz >= 0, AIR_E_ARG, "rowptr must start at 0");
+    for (int64_t i = 0; i < m; ++i) AIR_CHECK(rowptr[i + 1] >= rowptr[i], AIR_E_ARG, "rowptr must be nondecreasing");
+    for (int64_t q = 0; q < nnz; ++q) AIR_CHECK(colind[q] >= 0 && colind[q] < n, AIR_E_ARG, "column index out of range");
+    DevSparse& A = e->A;
+    A.nrows = m;
+    A.ncols = n;
+    A.nnz = nnz;
+    A.dtype = dtype;
+    (void)dtype_size(dtype);
+    upload(A.ptr, rowptr, m + 1, s);
+    alloc_sparse_arrays(A, nnz, dtype, s);
+    if (nnz > 0) AIR_HIP(hipMemcpyAsync(A.idx.p, colind, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    DevBuf v64;
+    upload(v64, val, nnz, s);
+    dispatch_dtype(dtype, [&](auto v) {
+        using V = decltype(v);
+        if (nnz > 0) k_narrow<V><<<4096, 256, 0, s>>>(v64.as<double>(), A.val.as<V>(), nnz);
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* colind,
                        const double* val, int dtype) {
     return guard([&] {
-        AIR_CHECK(e && rowptr && m >= 0 && n >= 0, AIR_E_ARG, "invalid CSR arguments");
-        AIR_CHECK(n <= INT32_MAX, AIR_E_ARG, "n must fit int32 column indices");
-        use_device(e);
-        reset_operator(e);
-        hipStream_t s = e->stream;
-        int64_t nnz = rowptr[m];
-        AIR_CHECK(rowptr[0] == 0 && nnz >= 0, AIR_E_ARG, "rowptr must start at 0");
-        for (int64_t i = 0; i < m; ++i) AIR_CHECK(rowptr[i + 1] >= rowptr[i], AIR_E_ARG, "rowptr must be nondecreasing");
-        for (int64_t q = 0; q < nnz; ++q)
-            AIR_CHECK(colind[q] >= 0 && colind[q] < n, AIR_E_ARG, "column index out of range");
-        DevSparse& A = e->A;
-        A.nrows = m;
-        A.ncols = n;
-        A.nnz = nnz;
-        A.dtype = dtype;
-        (void)dtype_size(dtype);
-        upload(A.ptr, rowptr, m + 1, s);
-        upload(A.idx, colind, nnz, s);
-        DevBuf v64;
-        upload(v64, val, nnz, s);
-        A.val.alloc(dtype_size(dtype) * std::max<int64_t>(nnz, 1));
-        dispatch_dtype(dtype, [&](auto v) {
-            using V = decltype(v);
-            if (nnz > 0) k_narrow<V><<<4096, 256, 0, s>>>(v64.as<double>(), A.val.as<V>(), nnz);
-            return 0;
-        });
-        AIR_HIP(hipGetLastError());
-        AIR_HIP(hipStreamSynchronize(s));
-        finish_matrix(e);
+        upload_csr(e, m, n, rowptr, colind, val, dtype);
+        finish_matrix(e, 0);
     });
 }
 
@@ -613,14 +678,18 @@ int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colpt
             AIR_CHECK(rowind[q] >= 0 && rowind[q] < m, AIR_E_ARG, "row index out of range");
             ri[q] = (int32_t)rowind[q];
         }
-        // The CSC of A is the CSR of A': upload it as such; finish_matrix then
-        // builds its transpose, which is the CSR of A.  Swap the two roles.
-        int rc = air_set_matrix_csr(e, n, m, colptr, ri.data(), val, dtype);
-        if (rc != AIR_OK) throw Err{rc, g_last_error};
+        // The CSC of A is the CSR of A': upload it as such, transpose it into
+        // the CSR of A, and swap the two roles.
+        upload_csr(e, n, m, colptr, ri.data(), val, dtype);
+        csr_to_csc(e->A, e->AT, e->stream);
         std::swap(e->A, e->AT);
-        std::swap(e->shA, e->shAT);
+        e->shA = default_shape(e->A, e->num_cu);
+        e->shAT = default_shape(e->AT, e->num_cu);
+        e->kind = 1;
         e->m = e->A.nrows;
         e->n = e->A.ncols;
+        e->T = TiledOp();
+        if (!tiled_disabled()) build_tiled(e->A, 0, e->T, e->stream);
     });
 }
 
@@ -637,7 +706,7 @@ static int build_geometry(air_engine* e, int N, int64_t nrays, int edge, int mat
         e->n = (int64_t)N * N;
     } else {
         trace_to_csr(e->geo, dtype, e->A, s);
-        finish_matrix(e);
+        finish_matrix(e, N);
     }
     return AIR_OK;
 }
@@ -798,7 +867,7 @@ int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info*
         AIR_HIP(hipMemcpyAsync(&c, st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
         AIR_HIP(hipStreamSynchronize(s));
         int fin = c.stop ? c.finaliter : st.k;
-        if (x_out) download_f64(st.x.as<float>(), x_out, st.n, s);
+        if (x_out) download_xlike(e, st.x.as<float>(), x_out, s);
         if (resnorm) {
             AIR_HIP(hipMemcpyAsync(resnorm, st.hist.p, 8 * (size_t)(fin + 1), hipMemcpyDeviceToHost, s));
             AIR_HIP(hipStreamSynchronize(s));
@@ -833,8 +902,9 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
         // snapshot slots for K(l) < kmax (sirt.m:247-250)
         int nsnap = 0;
         while (nsnap < nK && K[nsnap] < kmax) nsnap++;
+        const int64_t nx = xlen(e);
         DevBuf snaps;
-        if (nsnap) snaps.alloc(4 * (size_t)n * nsnap);
+        if (nsnap) snaps.alloc(4 * (size_t)nx * nsnap);
         int l = 0;
         const int chunk = 32;
         bool stopped = read_stop(e) != 0;
@@ -845,7 +915,7 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
             while (st.k < upto) {
                 enqueue_iteration(e, ++st.k);
                 if (l < nsnap && st.k == K[l]) {
-                    AIR_HIP(hipMemcpyAsync(snaps.as<float>() + (size_t)l * n, st.x.p, 4 * n, hipMemcpyDeviceToDevice, s));
+                    AIR_HIP(hipMemcpyAsync(snaps.as<float>() + (size_t)l * nx, st.x.p, 4 * nx, hipMemcpyDeviceToDevice, s));
                     l++;
                 }
             }
@@ -867,8 +937,8 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
         if (itersaved) itersaved[ns] = fin;
         if (n_itersaved) *n_itersaved = ns + 1;
         if (X_out) {
-            for (int q = 0; q < ns; ++q) download_f64(snaps.as<float>() + (size_t)q * n, X_out + (size_t)q * n, n, s);
-            if (fin > 0) download_f64(st.x.as<float>(), X_out + (size_t)ns * n, n, s);
+            for (int q = 0; q < ns; ++q) download_xlike(e, snaps.as<float>() + (size_t)q * nx, X_out + (size_t)q * n, s);
+            if (fin > 0) download_xlike(e, st.x.as<float>(), X_out + (size_t)ns * n, s);
         }
         inf.nsaved = nsaved;
         inf.timetaken = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -880,17 +950,43 @@ int air_set_kernel_timing(air_engine* e, int on) {
     return guard([&] {
         AIR_CHECK(e, AIR_E_ARG, "null engine");
         e->timing = on != 0;
-        e->t_rows_ms = e->t_cols_ms = 0;
+        e->t_rows_ms = e->t_cols_ms = e->t_red_ms = 0;
         e->launches = 0;
     });
 }
 
-int air_get_kernel_timing(air_engine* e, double* rows_ms, double* cols_ms, int64_t* launches) {
+int air_get_kernel_timing(air_engine* e, double* cols_ms, double* rows_ms, double* reduce_ms, int64_t* launches) {
     return guard([&] {
         AIR_CHECK(e, AIR_E_ARG, "null engine");
-        if (rows_ms) *rows_ms = e->t_rows_ms;
         if (cols_ms) *cols_ms = e->t_cols_ms;
+        if (rows_ms) *rows_ms = e->t_rows_ms;
+        if (reduce_ms) *reduce_ms = e->t_red_ms;
         if (launches) *launches = e->launches;
+    });
+}
+
+int air_layout_info(air_engine* e, air_layout* out) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(out, AIR_E_ARG, "null output");
+        air_layout L{};
+        L.tiled = use_tiled(e) ? 1 : 0;
+        L.m = e->m;
+        L.n = e->n;
+        L.nnz = e->kind == 1 ? e->A.nnz : -1;
+        L.value_bytes = e->kind == 1 ? (int)dtype_size(e->A.dtype) : 4;
+        if (L.tiled) {
+            const TiledOp& T = e->T;
+            L.n_internal = T.n_int;
+            L.pieces = T.npieces;
+            L.runs = T.nruns;
+            L.tiles_fwd = T.ntiles_f;
+            L.tiles_bwd = T.ntiles_b;
+            L.staged_rows = T.staged_rows;
+            L.lds_bytes = (int64_t)T.lds_b;
+            tiled_shapes(T, &L.fwd_lanes, &L.fwd_loads, &L.bwd_lanes, &L.bwd_loads);
+        }
+        *out = L;
     });
 }
 

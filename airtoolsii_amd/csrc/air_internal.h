@@ -119,6 +119,18 @@ struct DevSparse {
     double avg_len() const { return nrows ? (double)nnz / (double)nrows : 0.0; }
 };
 
+// Zeroed entries kept after nnz in idx/val: the row kernels read rows with
+// 16-byte vector loads from an aligned start and may run past the last entry.
+constexpr int64_t AIR_SPARSE_PAD = 2048;
+
+inline void alloc_sparse_arrays(DevSparse& S, int64_t nnz, int dtype, hipStream_t s) {
+    const size_t vs = dtype_size(dtype);
+    S.idx.alloc(sizeof(int) * (size_t)(nnz + AIR_SPARSE_PAD));
+    S.val.alloc(vs * (size_t)(nnz + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync(S.idx.as<int>() + nnz, 0, sizeof(int) * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)S.val.p + vs * nnz, 0, vs * AIR_SPARSE_PAD, s));
+}
+
 // Ray of a line-model tracer: start point (x0,y0) and direction (a,b).
 struct RayParam {
     double x0, y0, a, b;
@@ -151,6 +163,43 @@ void scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t 
 // CSR -> CSC on the device (rows ascending within each column).
 void csr_to_csc(const DevSparse& A, DevSparse& AT, hipStream_t s);
 
+// ---------------------------------------------------------------- tiled operator (air_tiled.hip)
+// The SIRT hot path's layout of A: columns (pixels) are renumbered tile-major
+// (64x64 tiles made of 16x16 sub-tiles, column-major inside) and A is stored
+// twice with 16-bit local indices:
+//  forward (A x): per 64x64 tile, the row segments ("pieces") of A inside the
+//    tile, each entry (value, local column); the tile of x is staged in LDS and
+//    every piece yields one partial sum; a second kernel adds a row's partials.
+//  backward (A' y): per 16x16 sub-tile, the columns of A with (value, local
+//    row); the sub-tile's distinct rows of y are staged in LDS (as runs of
+//    consecutive rows) and each column is summed with LDS gathers.
+struct TiledOp {
+    bool ready = false;
+    int dtype = AIR_F32;
+    int N = 0;                  // image side for 2-D geometries (0: generic column blocks)
+    int64_t m = 0, n = 0, n_int = 0, nnz = 0;
+    int64_t ntiles_f = 0, ntiles_b = 0, npieces = 0, nruns = 0;
+    int bt_size = 256;          // columns per backward tile
+    int nblk_f = 0;             // forward blocks
+    int maxruns = 0, maxrows = 0;
+    int64_t staged_rows = 0;    // sum over backward tiles of their distinct rows
+    size_t lds_b = 0;           // dynamic LDS bytes of the backward kernel
+    DevBuf perm;                // int32 [n]: user column -> internal column
+    DevBuf fval, fcode, piece_ptr, blk_tile, blk_p0, rp_ptr, rp_idx, partial;
+    DevBuf bval, blr, cell_ptr, run_ptr, run_row, run_off, run_len;
+    int fwd_cfg = 0, bwd_cfg = 0;          // kernel shapes picked by the autotuner
+    float tune_fwd_ms[8] = {}, tune_bwd_ms[8] = {};
+};
+// Build the tiled operator from the CSR of A (N > 0: N x N image columns).
+// Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
+bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s);
+void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStream_t s);
+void autotune_tiled(TiledOp& T, hipStream_t s);
+void tiled_shapes(const TiledOp& T, int* fwd_lanes, int* fwd_loads, int* bwd_lanes, int* bwd_loads);
+void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s);
+void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);
+void permute_to_user(const TiledOp& T, const float* x_int, float* x_user, hipStream_t s);
+
 // ---------------------------------------------------------------- kernels (air_sirt.hip)
 // Per-launch shape of the compressed-row kernels: G lanes per row, grid blocks.
 struct Shape {
@@ -172,8 +221,6 @@ struct Ctrl {
     double lam_cur;      // relaxpar of the last executed iteration
     double rMr;          // r'Mr of the current residual (line search numerator)
     double last_norm;
-    unsigned int cnt_rows;
-    unsigned int cnt_cols;
 };
 
 struct StopParams {
@@ -194,7 +241,8 @@ struct SirtBufs {
     const float* ubv;
     int lb_kind, ub_kind;
     float lb, ub;
-    double* partial;     // per-block partial sums
+    double* partial;     // per-block partial sums of the row pass (4 per block)
+    double* partial_cols;  // per-block partial sums of the column pass (1 per block)
     double* hist;        // ||r_k||, k = 0..kmax
     Ctrl* ctrl;
 };
@@ -205,6 +253,14 @@ void launch_resid(const DevSparse& A, Shape sh, const SirtBufs& B, StopParams sp
 void launch_update(const DevSparse& AT, Shape sh, const SirtBufs& B, double lam, int k, hipStream_t s);
 // line search: g = A' Mr (FP64) with sum D g^2; then lam = r'Mr / sum D g^2 and the update.
 void launch_line_update(const DevSparse& AT, Shape sh, const SirtBufs& B, int k, hipStream_t s);
+// the same two passes on the tiled operator (x, D, bounds in the internal column order)
+void launch_tiled_resid(const TiledOp& T, const SirtBufs& B, StopParams sp, int k, hipStream_t s, hipEvent_t mid);
+void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int line, int k, hipStream_t s);
+void launch_apply_line(int64_t n, const SirtBufs& B, hipStream_t s);
+// single-block reductions of the per-block partials (stop rule / line-search lambda)
+void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s);
+void launch_finalize_line(const SirtBufs& B, int nblocks, hipStream_t s);
+unsigned tiled_reduce_grid(int64_t m);
 // matrix-free iteration pieces
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s);
 void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s);

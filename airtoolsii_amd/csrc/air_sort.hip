@@ -66,8 +66,7 @@ void csr_to_csc(const DevSparse& A, DevSparse& AT, hipStream_t s) {
     AT.nnz = nnz;
     AT.dtype = A.dtype;
     AT.ptr.alloc(sizeof(int64_t) * (n + 1));
-    AT.idx.alloc(sizeof(int) * (size_t)std::max<int64_t>(nnz, 1));
-    AT.val.alloc(dtype_size(A.dtype) * (size_t)std::max<int64_t>(nnz, 1));
+    alloc_sparse_arrays(AT, nnz, A.dtype, s);
     // column counts -> colptr
     DevBuf cnt;
     cnt.alloc(sizeof(int64_t) * (n + 1));

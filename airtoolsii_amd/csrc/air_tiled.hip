@@ -1,0 +1,845 @@
+// air_tiled.hip -- LDS-tiled layout of A for the SIRT hot path on gfx950.
+//
+// The compressed-row kernels gather one vector element per nonzero from L2,
+// and on CDNA4 those divergent 4-byte gathers, not HBM, bound the SpMV.  Here
+// the gathered vector lives in LDS instead:
+//
+//  forward  r = b - A x  (sirt/sirt.m:240): columns are renumbered tile-major
+//    (64x64 pixel tiles); for each tile the nonzeros are grouped by row into
+//    "pieces" (one ray's cells inside the tile), each entry (value, 12-bit
+//    local column).  A block stages its tile of x in LDS (16 KB), lanes groups
+//    sum pieces with LDS gathers, and a second kernel adds the ~10 partial
+//    sums of each row and runs the residual epilogue (M r, FP64 norms, stop).
+//  backward g = A' M r  (sirt/sirt.m:214-215): per 16x16 sub-tile the columns
+//    hold (value, 16-bit local row).  A block stages the sub-tile's distinct
+//    rows of M r in LDS -- for tomography they are runs of consecutive rays,
+//    one per angle -- and each wave sums one column, then applies the update
+//    x = P_C(x + lam D g) (sirt.m:229-237) in the epilogue.
+//
+// Entries are 6 bytes (FP32 value + 16-bit index) instead of 8, and nothing
+// is gathered from global memory inside the loops.  The layout is built once
+// on the device from the CSR by stable radix sorts, so entry order (and the
+// floating-point summation order) is deterministic.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "air_device.h"
+
+namespace air {
+
+static constexpr int FT = 4096;            // columns per forward tile (64 x 64)
+static constexpr int PIECES_PER_BLOCK = 256;
+static constexpr size_t MAX_LDS_B = 64 * 1024;
+
+// 16-bit index loads matching VecOf<V>::VE
+typedef unsigned short us2v __attribute__((ext_vector_type(2)));
+typedef unsigned short us4v __attribute__((ext_vector_type(4)));
+typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+
+template <int VE>
+__device__ __forceinline__ void load_code(const unsigned short* p, unsigned (&out)[VE]) {
+    if constexpr (VE == 2) {
+        us2v w = __builtin_nontemporal_load((const us2v*)p);
+        out[0] = w[0]; out[1] = w[1];
+    } else if constexpr (VE == 4) {
+        us4v w = __builtin_nontemporal_load((const us4v*)p);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i] = w[i];
+    } else {
+        us8v w = __builtin_nontemporal_load((const us8v*)p);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out[i] = w[i];
+    }
+}
+
+// sum over entries [s,e) of val[q] * lds[code[q] & mask]; G lanes, U vector loads per lane per round
+template <typename V, int G, int U>
+__device__ __forceinline__ typename AccOf<V>::T lds_dot(const V* __restrict__ val, const unsigned short* __restrict__ code,
+                                                       const float* lds, int64_t s, int64_t e, int gl, unsigned mask) {
+    using A = typename AccOf<V>::T;
+    constexpr int VE = VecOf<V>::VE, STEP = G * VE;
+    A acc = 0;
+    for (int64_t base = s & ~(int64_t)(VE - 1); base < e; base += (int64_t)STEP * U) {
+        A v[U][VE];
+        unsigned c[U][VE];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t q0 = base + (int64_t)u * STEP + (int64_t)gl * VE;
+            load_vals<V, A>(val + q0, v[u]);
+            load_code<VE>(code + q0, c[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < VE; ++i) {
+                const int64_t q = base + (int64_t)u * STEP + (int64_t)gl * VE + i;
+                if (q >= s && q < e) acc += v[u][i] * (A)lds[c[u][i] & mask];
+            }
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- forward pieces
+template <typename V, int G, int U>
+__global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile, const int* __restrict__ blk_p0,
+                                                const int64_t* __restrict__ piece_ptr, const V* __restrict__ fval,
+                                                const unsigned short* __restrict__ fcode, const float* __restrict__ x,
+                                                float* __restrict__ partial, const Ctrl* ctrl) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ f4 xs4[FT / 4];
+    if (ctrl && load_stop(ctrl)) return;
+    const int t = blk_tile[blockIdx.x], p0 = blk_p0[blockIdx.x], p1 = blk_p0[blockIdx.x + 1];
+    const f4* xt = (const f4*)(x + (int64_t)t * FT);
+    f4 tmp[FT / (4 * BLOCK)];
+#pragma unroll
+    for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
+#pragma unroll
+    for (int i = 0; i < FT / (4 * BLOCK); ++i) xs4[threadIdx.x + BLOCK * i] = tmp[i];
+    __syncthreads();
+    const float* xs = (const float*)xs4;
+    constexpr int GPB = BLOCK / G;
+    const int gl = threadIdx.x % G;
+    int k = p0 + threadIdx.x / G;
+    int64_t s = 0, e = 0;
+    if (k < p1) { s = piece_ptr[k]; e = piece_ptr[k + 1]; }
+    while (k < p1) {
+        const int kn = k + GPB;
+        int64_t sn = 0, en = 0;
+        if (kn < p1) { sn = piece_ptr[kn]; en = piece_ptr[kn + 1]; }
+        auto a = lds_dot<V, G, U>(fval, fcode, xs, s, e, gl, 0x0fffu);
+        a = group_sum<G>(a);
+        if (gl == 0) partial[k] = (float)a;
+        k = kn;
+        s = sn;
+        e = en;
+    }
+}
+
+// y = sum of a row's partials (8 lanes per row)
+__device__ __forceinline__ float row_partials(const int64_t* __restrict__ rp_ptr, const int* __restrict__ rp_idx,
+                                              const float* __restrict__ partial, int64_t row, int gl) {
+    float a = 0.f;
+    for (int64_t q = rp_ptr[row] + gl; q < rp_ptr[row + 1]; q += 8) a += partial[rp_idx[q]];
+    return group_sum<8>(a);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_treduce_plain(const int64_t* __restrict__ rp_ptr, const int* __restrict__ rp_idx,
+                                                         const float* __restrict__ partial, int64_t m, float* __restrict__ y) {
+    const int gl = threadIdx.x & 7;
+    for (int64_t row = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 3; row < m; row += (int64_t)gridDim.x * (BLOCK / 8)) {
+        float a = row_partials(rp_ptr, rp_idx, partial, row, gl);
+        if (gl == 0) y[row] = a;
+    }
+}
+
+// residual epilogue of sirt.m:240-243 with the FP64 reductions of K2
+template <bool ME>
+__global__ __launch_bounds__(BLOCK) void k_treduce(const int64_t* __restrict__ rp_ptr, const int* __restrict__ rp_idx,
+                                                   const float* __restrict__ partial, int64_t m, const float* __restrict__ b,
+                                                   const float* __restrict__ M, float* __restrict__ r,
+                                                   float* __restrict__ Mr, double* red, double* hist, Ctrl* ctrl,
+                                                   StopParams sp, int k) {
+    __shared__ double sh[4][BLOCK / 64];
+    if (load_stop(ctrl)) return;
+    const int gl = threadIdx.x & 7;
+    double v[4] = {0, 0, 0, 0};
+    for (int64_t row = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 3; row < m; row += (int64_t)gridDim.x * (BLOCK / 8)) {
+        float ax = row_partials(rp_ptr, rp_idx, partial, row, gl);
+        if (gl == 0) {
+            float ri = b[row] - ax;
+            float mri = M ? M[row] * ri : ri;
+            if (ME) {
+                double ro = (double)r[row];
+                v[2] += ro * (ro + (double)ri);
+                v[3] += ro * ro;
+            }
+            r[row] = ri;
+            Mr[row] = mri;
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)ri * (double)mri;
+        }
+    }
+    block_reduce<4>(v, sh);
+    store_block_partials<4>(v, red);
+}
+
+// ---------------------------------------------------------------- backward columns
+// Stage the tile's rows of y (runs of consecutive rows) into LDS; returns the LDS row array.
+__device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, const int* __restrict__ run_ptr,
+                                             const int* __restrict__ run_row, const int* __restrict__ run_off,
+                                             const int* __restrict__ run_len, const float* __restrict__ y) {
+    int* roff = (int*)smem;                 // [maxruns + 1]
+    int* rrow = roff + maxruns + 1;         // [maxruns]
+    float* ys = (float*)(rrow + maxruns);
+    const int r0 = run_ptr[t], nr = run_ptr[t + 1] - r0;
+    for (int i = threadIdx.x; i < nr; i += BLOCK) {
+        roff[i] = run_off[r0 + i];
+        rrow[i] = run_row[r0 + i];
+    }
+    if (threadIdx.x == 0) roff[nr] = nr > 0 ? run_off[r0 + nr - 1] + run_len[r0 + nr - 1] : 0;
+    __syncthreads();
+    const int nrows = roff[nr];
+    constexpr int UNR = 8;
+    for (int k0 = threadIdx.x; k0 < nrows; k0 += BLOCK * UNR) {
+        float vv[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int kk = k0 + u * BLOCK;
+            vv[u] = 0.f;
+            if (kk < nrows) {
+                int lo = 0, hi = nr - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (roff[mid] <= kk) lo = mid; else hi = mid - 1;
+                }
+                vv[u] = y[rrow[lo] + (kk - roff[lo])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int kk = k0 + u * BLOCK;
+            if (kk < nrows) ys[kk] = vv[u];
+        }
+    }
+    __syncthreads();
+    return ys;
+}
+
+struct BwdArgs {
+    const int* run_ptr;
+    const int* run_row;
+    const int* run_off;
+    const int* run_len;
+    const int64_t* cell_ptr;
+    const unsigned short* blr;
+    int maxruns;
+    int bt;                   // columns per tile
+};
+
+// MODE 0: g = A'y (FP32 out); 1: SIRT update of x; 2: line search (g in FP64, sum D g^2, lam)
+template <typename V, int MODE, int G, int U>
+__global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__ bval, const float* __restrict__ y,
+                                                float* __restrict__ gout, double* __restrict__ g64,
+                                                const float* __restrict__ D, float* __restrict__ x, double lam,
+                                                int lb_kind, float lb, const float* __restrict__ lbv, int ub_kind,
+                                                float ub, const float* __restrict__ ubv, double* red, Ctrl* ctrl) {
+    extern __shared__ float smem[];
+    __shared__ double sh[1][BLOCK / 64];
+    if (MODE != 0 && load_stop(ctrl)) return;
+    const int t = blockIdx.x;
+    const float* ys = stage_runs(smem, a.maxruns, t, a.run_ptr, a.run_row, a.run_off, a.run_len, y);
+    const int lane = threadIdx.x % G, w = threadIdx.x / G;
+    double dsum[1] = {0};
+    const int64_t c0 = (int64_t)t * a.bt;
+    int c = w;
+    int64_t s = 0, e = 0;
+    float xj = 0.f, dj = 1.f;
+    if (c < a.bt) {
+        s = a.cell_ptr[c0 + c];
+        e = a.cell_ptr[c0 + c + 1];
+        if (MODE == 1) { xj = x[c0 + c]; if (D) dj = D[c0 + c]; }
+    }
+    while (c < a.bt) {
+        const int cn = c + BLOCK / G;
+        int64_t sn = 0, en = 0;
+        float xn = 0.f, dn = 1.f;
+        if (cn < a.bt) {
+            sn = a.cell_ptr[c0 + cn];
+            en = a.cell_ptr[c0 + cn + 1];
+            if (MODE == 1) { xn = x[c0 + cn]; if (D) dn = D[c0 + cn]; }
+        }
+        auto acc = lds_dot<V, G, U>(bval, a.blr, ys, s, e, lane, 0xffffu);
+        acc = group_sum<G>(acc);
+        if (lane == 0) {
+            const int64_t j = c0 + c;
+            if (MODE == 0) {
+                gout[j] = (float)acc;
+            } else if (MODE == 1) {
+                double dg = D ? (double)dj * (double)(float)acc : (double)(float)acc;
+                x[j] = project((double)xj + lam * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
+            } else {
+                double gj = (double)(float)acc;
+                g64[j] = gj;
+                dsum[0] += D ? (double)D[j] * (gj * gj) : gj * gj;
+            }
+        }
+        c = cn;
+        s = sn;
+        e = en;
+        xj = xn;
+        dj = dn;
+    }
+    if (MODE == 1 && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
+    if (MODE == 2) {
+        block_reduce<1>(dsum, sh);
+        store_block_partials<1>(dsum, red);
+    }
+}
+
+// ---------------------------------------------------------------- layout build kernels
+__device__ __host__ __forceinline__ int64_t nested_index(int64_t fx, int64_t fyp, int64_t T1n) {
+    const int64_t t1 = (fx >> 6) * T1n + (fyp >> 6);
+    const int64_t sx = fx & 63, sy = fyp & 63;
+    const int64_t t2 = (sx >> 4) * 4 + (sy >> 4);
+    return t1 * 4096 + t2 * 256 + (sx & 15) * 16 + (sy & 15);
+}
+
+__global__ void k_perm_build(int* perm, int64_t n, int N, int64_t T1n) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        perm[j] = N > 0 ? (int)nested_index(j / N, j % N, T1n) : (int)j;
+}
+
+__global__ void k_rows_of(const int64_t* __restrict__ ptr, int64_t m, int* __restrict__ rowof) {
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    for (int64_t q = ptr[row] + lane; q < ptr[row + 1]; q += 64) rowof[q] = (int)row;
+}
+
+__global__ void k_map_cols(const int* __restrict__ col, const int* __restrict__ perm, int64_t nnz, uint32_t* __restrict__ cp,
+                           uint32_t* __restrict__ ftile, uint32_t* __restrict__ iota) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = (uint32_t)perm[col[q]];
+        cp[q] = c;
+        ftile[q] = c / FT;
+        iota[q] = (uint32_t)q;
+    }
+}
+
+template <typename V>
+__global__ void k_fwd_fill(const uint32_t* __restrict__ order, const uint32_t* __restrict__ ftile_s,
+                           const uint32_t* __restrict__ cp, const int* __restrict__ rowof, const V* __restrict__ val,
+                           int64_t nnz, V* __restrict__ fval, unsigned short* __restrict__ fcode,
+                           uint32_t* __restrict__ flag) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = order[q];
+        const int row = rowof[e];
+        bool start = q == 0;
+        if (!start) start = ftile_s[q] != ftile_s[q - 1] || rowof[order[q - 1]] != row;
+        fval[q] = val[e];
+        fcode[q] = (unsigned short)((cp[e] % FT) | (start ? 0x8000u : 0u));
+        flag[q] = start ? 1u : 0u;
+    }
+}
+
+__global__ void k_pieces(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pidx,
+                         const uint32_t* __restrict__ order, const uint32_t* __restrict__ ftile_s,
+                         const int* __restrict__ rowof, int64_t nnz, int64_t* __restrict__ piece_ptr,
+                         uint32_t* __restrict__ piece_row, int* __restrict__ piece_tile, uint32_t* __restrict__ iota_p) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[q]) continue;
+        const uint32_t p = pidx[q];
+        piece_ptr[p] = q;
+        piece_row[p] = (uint32_t)rowof[order[q]];
+        piece_tile[p] = (int)ftile_s[q];
+        iota_p[p] = p;
+    }
+}
+
+template <typename V>
+__global__ void k_bwd_fill(const uint32_t* __restrict__ order, const int* __restrict__ rowof, const V* __restrict__ val,
+                           const uint32_t* __restrict__ cp_s, int64_t nnz, int bt, V* __restrict__ bval,
+                           uint64_t* __restrict__ key2, uint32_t* __restrict__ iota) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = order[q];
+        bval[q] = val[e];
+        key2[q] = ((uint64_t)(cp_s[q] / (uint32_t)bt) << 32) | (uint32_t)rowof[e];
+        iota[q] = (uint32_t)q;
+    }
+}
+
+__global__ void k_count_u32(const uint32_t* __restrict__ key, int64_t n, int64_t* __restrict__ cnt) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd((unsigned long long*)&cnt[key[q]], 1ull);
+}
+
+__global__ void k_uniq_flags(const uint64_t* __restrict__ key2s, int64_t nnz, uint32_t* __restrict__ flag) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x)
+        flag[q] = (q == 0 || key2s[q] != key2s[q - 1]) ? 1u : 0u;
+}
+
+// unique (tile,row) pairs, each tile's first unique index, and the runs of consecutive rows
+__global__ void k_uniq_list(const uint64_t* __restrict__ key2s, const uint32_t* __restrict__ flag,
+                            const uint32_t* __restrict__ uid, int64_t nnz, uint32_t* __restrict__ urow,
+                            uint32_t* __restrict__ utile, uint32_t* __restrict__ tile_first) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[q]) continue;
+        const uint32_t u = uid[q];
+        const uint32_t tile = (uint32_t)(key2s[q] >> 32);
+        urow[u] = (uint32_t)(key2s[q] & 0xffffffffu);
+        utile[u] = tile;
+        if (q == 0 || (uint32_t)(key2s[q - 1] >> 32) != tile) tile_first[tile] = u;
+    }
+}
+
+__global__ void k_local_rows(const uint64_t* __restrict__ key2s, const uint32_t* __restrict__ order2,
+                             const uint32_t* __restrict__ flag, const uint32_t* __restrict__ uid,
+                             const uint32_t* __restrict__ tile_first, int64_t nnz, unsigned short* __restrict__ blr) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t tile = (uint32_t)(key2s[q] >> 32);
+        const uint32_t u = uid[q] + flag[q] - 1;        // inclusive unique index
+        blr[order2[q]] = (unsigned short)(u - tile_first[tile]);
+    }
+}
+
+__global__ void k_run_flags(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ utile, int64_t nu,
+                            uint32_t* __restrict__ flag) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nu; j += (int64_t)gridDim.x * blockDim.x)
+        flag[j] = (j == 0 || utile[j] != utile[j - 1] || urow[j] != urow[j - 1] + 1) ? 1u : 0u;
+}
+
+__global__ void k_runs(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ utile,
+                       const uint32_t* __restrict__ tile_first, const uint32_t* __restrict__ flag,
+                       const uint32_t* __restrict__ rid, int64_t nu, int* __restrict__ run_row, int* __restrict__ run_off,
+                       int* __restrict__ run_len, int* __restrict__ run_tile) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nu; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = rid[j] + flag[j] - 1;
+        if (flag[j]) {
+            run_row[r] = (int)urow[j];
+            run_off[r] = (int)(j - tile_first[utile[j]]);
+            run_tile[r] = (int)utile[j];
+        }
+        atomicAdd(&run_len[r], 1);
+    }
+}
+
+__global__ void k_scatter_perm(const float* __restrict__ xu, const int* __restrict__ perm, int64_t n, float* __restrict__ xi) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        xi[perm[j]] = xu[j];
+}
+__global__ void k_gather_perm(const float* __restrict__ xi, const int* __restrict__ perm, int64_t n, float* __restrict__ xu) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        xu[j] = xi[perm[j]];
+}
+
+// ---------------------------------------------------------------- host: build
+static unsigned grid_for(int64_t n) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + BLOCK - 1) / BLOCK, 65536));
+}
+static int bits_for(uint64_t v) {
+    int b = 1;
+    while (b < 64 && ((uint64_t)1 << b) <= v) b++;
+    return b;
+}
+
+template <typename K, typename Vv>
+static void sort_pairs(const K* kin, K* kout, const Vv* vin, Vv* vout, int64_t n, int end_bit, hipStream_t s) {
+    size_t tb = 0;
+    AIR_HIP(rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, (size_t)n, 0, end_bit, s));
+    DevBuf tmp;
+    tmp.alloc(tb);
+    AIR_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kin, kout, vin, vout, (size_t)n, 0, end_bit, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
+// exclusive scan of n uint32 flags; returns the total
+static uint64_t scan_flags(const uint32_t* flag, uint32_t* out, int64_t n, hipStream_t s) {
+    size_t tb = 0;
+    AIR_HIP(rocprim::exclusive_scan(nullptr, tb, flag, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s));
+    DevBuf tmp;
+    tmp.alloc(tb);
+    AIR_HIP(rocprim::exclusive_scan(tmp.p, tb, flag, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s));
+    uint32_t last_out = 0, last_flag = 0;
+    AIR_HIP(hipMemcpyAsync(&last_out, out + n - 1, 4, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipMemcpyAsync(&last_flag, flag + n - 1, 4, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+    return (uint64_t)last_out + last_flag;
+}
+
+template <typename T>
+static void pad_alloc(DevBuf& b, int64_t n, hipStream_t s) {
+    b.alloc(sizeof(T) * (size_t)(n + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((T*)b.p + n, 0, sizeof(T) * AIR_SPARSE_PAD, s));
+}
+
+bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
+    T = TiledOp();
+    const int64_t m = A.nrows, n = A.ncols, nnz = A.nnz;
+    if (nnz == 0 || nnz >= (int64_t)UINT32_MAX || m >= (int64_t)INT32_MAX) return false;
+    if (N > 0 && (int64_t)N * N != n) N = 0;
+    T.dtype = A.dtype;
+    T.N = N;
+    T.m = m;
+    T.n = n;
+    T.nnz = nnz;
+    const int64_t T1n = N > 0 ? (N + 63) / 64 : 0;
+    T.n_int = N > 0 ? T1n * T1n * FT : ((n + FT - 1) / FT) * FT;
+    if (T.n_int >= (int64_t)INT32_MAX) return false;
+    T.ntiles_f = T.n_int / FT;
+    // ---- column map
+    T.perm.alloc(4 * (size_t)n);
+    k_perm_build<<<grid_for(n), BLOCK, 0, s>>>(T.perm.as<int>(), n, N, T1n);
+    DevBuf rowof, cp, ftile, iota, ftile_s, order;
+    rowof.alloc(4 * (size_t)nnz);
+    cp.alloc(4 * (size_t)nnz);
+    ftile.alloc(4 * (size_t)nnz);
+    iota.alloc(4 * (size_t)nnz);
+    k_rows_of<<<(unsigned)((m * 64 + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(A.ptr.as<int64_t>(), m, rowof.as<int>());
+    k_map_cols<<<grid_for(nnz), BLOCK, 0, s>>>(A.idx.as<int>(), T.perm.as<int>(), nnz, cp.as<uint32_t>(),
+                                                ftile.as<uint32_t>(), iota.as<uint32_t>());
+    AIR_HIP(hipGetLastError());
+    // ---- forward layout: stable sort by forward tile keeps (row, column) order inside a tile
+    ftile_s.alloc(4 * (size_t)nnz);
+    order.alloc(4 * (size_t)nnz);
+    sort_pairs(ftile.as<uint32_t>(), ftile_s.as<uint32_t>(), iota.as<uint32_t>(), order.as<uint32_t>(), nnz,
+               bits_for((uint64_t)T.ntiles_f), s);
+    ftile.release();
+    pad_alloc<unsigned short>(T.fcode, nnz, s);
+    T.fval.alloc(dtype_size(T.dtype) * (size_t)(nnz + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((char*)T.fval.p + dtype_size(T.dtype) * nnz, 0, dtype_size(T.dtype) * AIR_SPARSE_PAD, s));
+    DevBuf flag, pidx;
+    flag.alloc(4 * (size_t)nnz);
+    pidx.alloc(4 * (size_t)nnz);
+    dispatch_dtype(T.dtype, [&](auto v) {
+        using V = decltype(v);
+        k_fwd_fill<V><<<grid_for(nnz), BLOCK, 0, s>>>(order.as<uint32_t>(), ftile_s.as<uint32_t>(), cp.as<uint32_t>(),
+                                                      rowof.as<int>(), A.val.as<V>(), nnz, T.fval.as<V>(),
+                                                      T.fcode.as<unsigned short>(), flag.as<uint32_t>());
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+    T.npieces = (int64_t)scan_flags(flag.as<uint32_t>(), pidx.as<uint32_t>(), nnz, s);
+    const int64_t P = T.npieces;
+    T.piece_ptr.alloc(8 * (size_t)(P + 1));
+    DevBuf piece_row, piece_tile, iota_p, prow_s;
+    piece_row.alloc(4 * (size_t)P);
+    piece_tile.alloc(4 * (size_t)P);
+    iota_p.alloc(4 * (size_t)P);
+    k_pieces<<<grid_for(nnz), BLOCK, 0, s>>>(flag.as<uint32_t>(), pidx.as<uint32_t>(), order.as<uint32_t>(),
+                                              ftile_s.as<uint32_t>(), rowof.as<int>(), nnz, T.piece_ptr.as<int64_t>(),
+                                              piece_row.as<uint32_t>(), piece_tile.as<int>(), iota_p.as<uint32_t>());
+    AIR_HIP(hipGetLastError());
+    AIR_HIP(hipMemcpyAsync(T.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
+    // forward blocks: PIECES_PER_BLOCK pieces of one tile each
+    {
+        std::vector<int> ptile(P);
+        AIR_HIP(hipMemcpyAsync(ptile.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        std::vector<int> bt, bp;
+        int64_t k = 0;
+        while (k < P) {
+            const int tile = ptile[k];
+            int64_t kend = k;
+            while (kend < P && ptile[kend] == tile) kend++;
+            for (int64_t q = k; q < kend; q += PIECES_PER_BLOCK) {
+                bt.push_back(tile);
+                bp.push_back((int)q);
+            }
+            k = kend;
+        }
+        bp.push_back((int)P);
+        T.nblk_f = (int)bt.size();
+        T.blk_tile.alloc(4 * bt.size());
+        T.blk_p0.alloc(4 * bp.size());
+        AIR_HIP(hipMemcpyAsync(T.blk_tile.p, bt.data(), 4 * bt.size(), hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(T.blk_p0.p, bp.data(), 4 * bp.size(), hipMemcpyHostToDevice, s));
+        AIR_HIP(hipStreamSynchronize(s));
+    }
+    // row -> pieces (stable sort of piece rows keeps tile order)
+    prow_s.alloc(4 * (size_t)P);
+    T.rp_idx.alloc(4 * (size_t)P);
+    sort_pairs(piece_row.as<uint32_t>(), prow_s.as<uint32_t>(), iota_p.as<uint32_t>(), T.rp_idx.as<uint32_t>(), P,
+               bits_for((uint64_t)m), s);
+    {
+        DevBuf cnt;
+        cnt.alloc(8 * (size_t)(m + 1));
+        AIR_HIP(hipMemsetAsync(cnt.p, 0, 8 * (m + 1), s));
+        k_count_u32<<<grid_for(P), BLOCK, 0, s>>>(piece_row.as<uint32_t>(), P, cnt.as<int64_t>());
+        T.rp_ptr.alloc(8 * (size_t)(m + 1));
+        scan_exclusive_i64(cnt.as<int64_t>(), T.rp_ptr.as<int64_t>(), m, s);
+    }
+    T.partial.alloc(4 * (size_t)std::max<int64_t>(P, 1));
+    piece_row.release(); piece_tile.release(); iota_p.release(); prow_s.release();
+    ftile_s.release(); pidx.release();
+    // ---- backward layout: stable sort by internal column (rows ascending inside a column)
+    T.bt_size = 256;
+    DevBuf cp_s;
+    cp_s.alloc(4 * (size_t)nnz);
+    k_map_cols<<<grid_for(nnz), BLOCK, 0, s>>>(A.idx.as<int>(), T.perm.as<int>(), nnz, cp.as<uint32_t>(),
+                                                flag.as<uint32_t>(), iota.as<uint32_t>());
+    sort_pairs(cp.as<uint32_t>(), cp_s.as<uint32_t>(), iota.as<uint32_t>(), order.as<uint32_t>(), nnz,
+               bits_for((uint64_t)T.n_int), s);
+    {
+        DevBuf cnt;
+        cnt.alloc(8 * (size_t)(T.n_int + 1));
+        AIR_HIP(hipMemsetAsync(cnt.p, 0, 8 * (T.n_int + 1), s));
+        k_count_u32<<<grid_for(nnz), BLOCK, 0, s>>>(cp.as<uint32_t>(), nnz, cnt.as<int64_t>());
+        T.cell_ptr.alloc(8 * (size_t)(T.n_int + 1));
+        scan_exclusive_i64(cnt.as<int64_t>(), T.cell_ptr.as<int64_t>(), T.n_int, s);
+    }
+    cp.release();
+    T.bval.alloc(dtype_size(T.dtype) * (size_t)(nnz + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((char*)T.bval.p + dtype_size(T.dtype) * nnz, 0, dtype_size(T.dtype) * AIR_SPARSE_PAD, s));
+    pad_alloc<unsigned short>(T.blr, nnz, s);
+    DevBuf key2, key2s, order2;
+    key2.alloc(8 * (size_t)nnz);
+    dispatch_dtype(T.dtype, [&](auto v) {
+        using V = decltype(v);
+        k_bwd_fill<V><<<grid_for(nnz), BLOCK, 0, s>>>(order.as<uint32_t>(), rowof.as<int>(), A.val.as<V>(),
+                                                      cp_s.as<uint32_t>(), nnz, T.bt_size, T.bval.as<V>(),
+                                                      key2.as<uint64_t>(), iota.as<uint32_t>());
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+    cp_s.release(); rowof.release();
+    T.ntiles_b = T.n_int / T.bt_size;
+    key2s.alloc(8 * (size_t)nnz);
+    sort_pairs(key2.as<uint64_t>(), key2s.as<uint64_t>(), iota.as<uint32_t>(), order.as<uint32_t>(), nnz,
+               32 + bits_for((uint64_t)T.ntiles_b), s);
+    key2.release(); iota.release();
+    DevBuf uid;
+    uid.alloc(4 * (size_t)nnz);
+    k_uniq_flags<<<grid_for(nnz), BLOCK, 0, s>>>(key2s.as<uint64_t>(), nnz, flag.as<uint32_t>());
+    const int64_t NU = (int64_t)scan_flags(flag.as<uint32_t>(), uid.as<uint32_t>(), nnz, s);
+    DevBuf urow, utile, tile_first;
+    urow.alloc(4 * (size_t)NU);
+    utile.alloc(4 * (size_t)NU);
+    tile_first.alloc(4 * (size_t)(T.ntiles_b + 1));
+    AIR_HIP(hipMemsetAsync(tile_first.p, 0, 4 * (T.ntiles_b + 1), s));
+    k_uniq_list<<<grid_for(nnz), BLOCK, 0, s>>>(key2s.as<uint64_t>(), flag.as<uint32_t>(), uid.as<uint32_t>(), nnz,
+                                                 urow.as<uint32_t>(), utile.as<uint32_t>(), tile_first.as<uint32_t>());
+    k_local_rows<<<grid_for(nnz), BLOCK, 0, s>>>(key2s.as<uint64_t>(), order.as<uint32_t>(), flag.as<uint32_t>(),
+                                                  uid.as<uint32_t>(), tile_first.as<uint32_t>(), nnz,
+                                                  T.blr.as<unsigned short>());
+    AIR_HIP(hipGetLastError());
+    key2s.release(); order.release();
+    // runs of consecutive rows inside each tile
+    DevBuf rflag, rid;
+    rflag.alloc(4 * (size_t)NU);
+    rid.alloc(4 * (size_t)NU);
+    k_run_flags<<<grid_for(NU), BLOCK, 0, s>>>(urow.as<uint32_t>(), utile.as<uint32_t>(), NU, rflag.as<uint32_t>());
+    T.nruns = (int64_t)scan_flags(rflag.as<uint32_t>(), rid.as<uint32_t>(), NU, s);
+    const int64_t R = T.nruns;
+    DevBuf run_tile;
+    T.run_row.alloc(4 * (size_t)R);
+    T.run_off.alloc(4 * (size_t)R);
+    T.run_len.alloc(4 * (size_t)R);
+    run_tile.alloc(4 * (size_t)R);
+    AIR_HIP(hipMemsetAsync(T.run_len.p, 0, 4 * R, s));
+    k_runs<<<grid_for(NU), BLOCK, 0, s>>>(urow.as<uint32_t>(), utile.as<uint32_t>(), tile_first.as<uint32_t>(),
+                                           rflag.as<uint32_t>(), rid.as<uint32_t>(), NU, T.run_row.as<int>(),
+                                           T.run_off.as<int>(), T.run_len.as<int>(), run_tile.as<int>());
+    AIR_HIP(hipGetLastError());
+    // per-tile run ranges and LDS need
+    {
+        std::vector<int> rt(R), rl(R), ro(R);
+        AIR_HIP(hipMemcpyAsync(rt.data(), run_tile.p, 4 * R, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipMemcpyAsync(rl.data(), T.run_len.p, 4 * R, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipMemcpyAsync(ro.data(), T.run_off.p, 4 * R, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        std::vector<int> rp(T.ntiles_b + 1, 0);
+        for (int64_t r = 0; r < R; ++r) rp[rt[r] + 1]++;
+        for (int64_t t = 0; t < T.ntiles_b; ++t) rp[t + 1] += rp[t];
+        int maxruns = 0, maxrows = 0;
+        int64_t staged = 0;
+        for (int64_t t = 0; t < T.ntiles_b; ++t) {
+            const int nr = rp[t + 1] - rp[t];
+            maxruns = std::max(maxruns, nr);
+            if (nr > 0) {
+                const int rows_t = ro[rp[t + 1] - 1] + rl[rp[t + 1] - 1];
+                maxrows = std::max(maxrows, rows_t);
+                staged += rows_t;
+            }
+        }
+        T.staged_rows = staged;
+        T.maxruns = maxruns;
+        T.maxrows = maxrows;
+        T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
+        if (T.lds_b > MAX_LDS_B || maxrows > 65535) return false;
+        T.run_ptr.alloc(4 * rp.size());
+        AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
+        AIR_HIP(hipStreamSynchronize(s));
+    }
+    T.ready = true;
+    autotune_tiled(T, s);
+    return true;
+}
+
+// ---------------------------------------------------------------- host: launches
+static BwdArgs bwd_args(const TiledOp& T) {
+    BwdArgs a;
+    a.run_ptr = T.run_ptr.as<int>();
+    a.run_row = T.run_row.as<int>();
+    a.run_off = T.run_off.as<int>();
+    a.run_len = T.run_len.as<int>();
+    a.cell_ptr = T.cell_ptr.as<int64_t>();
+    a.blr = T.blr.as<unsigned short>();
+    a.maxruns = T.maxruns;
+    a.bt = T.bt_size;
+    return a;
+}
+
+// kernel shapes the autotuner chooses from: (lanes per piece/column, vector loads per round)
+static const int FWD_CFG[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {32, 1}};
+static const int BWD_CFG[][2] = {{64, 2}, {64, 1}, {32, 2}, {32, 1}};
+static constexpr int N_FWD_CFG = 5, N_BWD_CFG = 4;
+
+template <typename F>
+static void dispatch_fwd(int cfg, F&& f) {
+    switch (cfg) {
+    case 0: f(std::integral_constant<int, 16>{}, std::integral_constant<int, 2>{}); break;
+    case 1: f(std::integral_constant<int, 16>{}, std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 8>{}, std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 8>{}, std::integral_constant<int, 1>{}); break;
+    default: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 1>{}); break;
+    }
+}
+template <typename F>
+static void dispatch_bwd(int cfg, F&& f) {
+    switch (cfg) {
+    case 0: f(std::integral_constant<int, 64>{}, std::integral_constant<int, 2>{}); break;
+    case 1: f(std::integral_constant<int, 64>{}, std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 1>{}); break;
+    }
+}
+
+static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* ctrl, hipStream_t s, int cfg = -1) {
+    dispatch_dtype(T.dtype, [&](auto v) {
+        using V = decltype(v);
+        dispatch_fwd(cfg < 0 ? T.fwd_cfg : cfg, [&](auto g, auto u) {
+            constexpr int G = decltype(g)::value, U = decltype(u)::value;
+            k_tfwd<V, G, U><<<T.nblk_f, BLOCK, 0, s>>>(T.blk_tile.as<int>(), T.blk_p0.as<int>(),
+                                                       T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
+                                                       T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
+        });
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+}
+
+template <int MODE>
+static void launch_bwd(const TiledOp& T, const float* y, float* gout, const SirtBufs* B, double lam, hipStream_t s,
+                       int cfg = -1) {
+    dispatch_dtype(T.dtype, [&](auto v) {
+        using V = decltype(v);
+        dispatch_bwd(cfg < 0 ? T.bwd_cfg : cfg, [&](auto g, auto u) {
+            constexpr int G = decltype(g)::value, U = decltype(u)::value;
+            if (MODE == 0)
+                k_tbwd<V, 0, G, U><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
+                    bwd_args(T), T.bval.as<V>(), y, gout, nullptr, nullptr, nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f,
+                    nullptr, nullptr, nullptr);
+            else
+                k_tbwd<V, MODE, G, U><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
+                    bwd_args(T), T.bval.as<V>(), y, nullptr, B->g, B->D, B->x, lam, B->lb_kind, B->lb, B->lbv,
+                    B->ub_kind, B->ub, B->ubv, B->partial_cols, B->ctrl);
+        });
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+}
+
+unsigned tiled_reduce_grid(int64_t m) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((m * 8 + BLOCK - 1) / BLOCK, 65535));
+}
+static unsigned reduce_grid(int64_t m) { return tiled_reduce_grid(m); }
+
+void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStream_t s) {
+    launch_fwd_pieces(T, x_int, nullptr, s);
+    k_treduce_plain<<<reduce_grid(T.m), BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(),
+                                                       T.partial.as<float>(), T.m, y);
+    AIR_HIP(hipGetLastError());
+}
+
+void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s) {
+    launch_bwd<0>(T, y, g_int, nullptr, 0.0, s);
+}
+
+void launch_tiled_resid(const TiledOp& T, const SirtBufs& B, StopParams sp, int k, hipStream_t s, hipEvent_t mid) {
+    launch_fwd_pieces(T, B.x, B.ctrl, s);
+    if (mid) AIR_HIP(hipEventRecord(mid, s));
+    if (sp.type == AIR_STOP_ME)
+        k_treduce<true><<<reduce_grid(T.m), BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(),
+                                                           T.partial.as<float>(), T.m, B.b, B.M, B.r, B.Mr, B.partial,
+                                                           B.hist, B.ctrl, sp, k);
+    else
+        k_treduce<false><<<reduce_grid(T.m), BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(),
+                                                            T.partial.as<float>(), T.m, B.b, B.M, B.r, B.Mr, B.partial,
+                                                            B.hist, B.ctrl, sp, k);
+    AIR_HIP(hipGetLastError());
+    launch_finalize_rows(B, (int)reduce_grid(T.m), sp, k, s);
+}
+
+void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int line, int k, hipStream_t s) {
+    (void)k;
+    if (line) {
+        launch_bwd<2>(T, B.Mr, nullptr, &B, 0.0, s);
+        launch_finalize_line(B, (int)T.ntiles_b, s);
+        launch_apply_line(T.n_int, B, s);
+    } else {
+        launch_bwd<1>(T, B.Mr, nullptr, &B, lam, s);
+    }
+}
+
+void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
+    *fl = FWD_CFG[T.fwd_cfg][0];
+    *fu = FWD_CFG[T.fwd_cfg][1];
+    *bl = BWD_CFG[T.bwd_cfg][0];
+    *bu = BWD_CFG[T.bwd_cfg][1];
+}
+
+// Time every kernel shape on this matrix (a few launches each, outside any
+// SIRT run) and keep the fastest; AIR_TILED_CFG="f,b" fixes the choice.
+void autotune_tiled(TiledOp& T, hipStream_t s) {
+    const char* fixed = std::getenv("AIR_TILED_CFG");
+    if (fixed && fixed[0]) {
+        int f = 0, b = 0;
+        if (std::sscanf(fixed, "%d,%d", &f, &b) == 2) {
+            T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG - 1);
+            T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG - 1);
+            return;
+        }
+    }
+    DevBuf x, y, g;
+    x.alloc(4 * (size_t)T.n_int);
+    y.alloc(4 * (size_t)std::max<int64_t>(T.m, 1));
+    g.alloc(4 * (size_t)T.n_int);
+    AIR_HIP(hipMemsetAsync(x.p, 0, 4 * (size_t)T.n_int, s));
+    AIR_HIP(hipMemsetAsync(y.p, 0, 4 * (size_t)T.m, s));
+    hipEvent_t e0, e1;
+    AIR_HIP(hipEventCreate(&e0));
+    AIR_HIP(hipEventCreate(&e1));
+    auto time_it = [&](auto&& launch) {
+        launch();
+        AIR_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 4; ++r) launch();
+        AIR_HIP(hipEventRecord(e1, s));
+        AIR_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        AIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    float best = 1e30f;
+    for (int c = 0; c < N_FWD_CFG; ++c) {
+        float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
+        T.tune_fwd_ms[c] = t / 4;
+        if (t < best) { best = t; T.fwd_cfg = c; }
+    }
+    best = 1e30f;
+    for (int c = 0; c < N_BWD_CFG; ++c) {
+        float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });
+        T.tune_bwd_ms[c] = t / 4;
+        if (t < best) { best = t; T.bwd_cfg = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s) {
+    AIR_HIP(hipMemsetAsync(x_int, 0, 4 * (size_t)T.n_int, s));
+    k_scatter_perm<<<grid_for(T.n), BLOCK, 0, s>>>(x_user, T.perm.as<int>(), T.n, x_int);
+    AIR_HIP(hipGetLastError());
+}
+
+void permute_to_user(const TiledOp& T, const float* x_int, float* x_user, hipStream_t s) {
+    k_gather_perm<<<grid_for(T.n), BLOCK, 0, s>>>(x_int, T.perm.as<int>(), T.n, x_user);
+    AIR_HIP(hipGetLastError());
+}
+
+}  // namespace air

@@ -337,8 +337,7 @@ void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s) {
     AIR_CHECK(herr == 0, AIR_E_GEOMETRY, "Index exceeds matrix dimensions: a ray segment maps outside 1..N^2");
     AIR_CHECK(nnz < (int64_t)INT32_MAX * 64, AIR_E_ARG, "matrix too large");
     A.nnz = nnz;
-    A.idx.alloc(sizeof(int) * (size_t)std::max<int64_t>(nnz, 1));
-    A.val.alloc(dtype_size(dtype) * (size_t)std::max<int64_t>(nnz, 1));
+    alloc_sparse_arrays(A, nnz, dtype, s);
     dispatch_dtype(dtype, [&](auto v) {
         using V = decltype(v);
         if (m > 0)

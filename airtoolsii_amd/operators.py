@@ -63,6 +63,12 @@ class AirOperator:
     def T(self):
         return _Transposed(self)
 
+    def layout(self):
+        """Storage used by the SIRT loop (tiled or compressed rows) and its sizes."""
+        L = _lib.Layout()
+        check(load().air_layout_info(self.engine.h, L))
+        return {f: getattr(L, f) for f, _ in L._fields_}
+
     # ---- stored matrix access
     def csr_arrays(self):
         """(rowptr int64, colind int32, values float64 widened from storage)."""

@@ -295,6 +295,7 @@ class SirtRun:
         check(load().air_set_kernel_timing(self.A.engine.h, 1 if on else 0))
 
     def kernel_times(self):
-        r, c, n = _lib.C.c_double(), _lib.C.c_double(), _lib.C.c_int64()
-        check(load().air_get_kernel_timing(self.A.engine.h, _lib.C.byref(r), _lib.C.byref(c), _lib.C.byref(n)))
-        return {"rows_ms": r.value, "cols_ms": c.value, "launches": n.value}
+        c, r, d, n = _lib.C.c_double(), _lib.C.c_double(), _lib.C.c_double(), _lib.C.c_int64()
+        check(load().air_get_kernel_timing(self.A.engine.h, _lib.C.byref(c), _lib.C.byref(r), _lib.C.byref(d),
+                                           _lib.C.byref(n)))
+        return {"cols_ms": c.value, "rows_ms": r.value, "reduce_ms": d.value, "launches": n.value}

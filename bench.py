@@ -5,8 +5,9 @@
                     [--dtype f32|bf16|f16] [--matrix-free] [--no-cpu-baseline]
 
 One step is one SIRT iteration x <- P_[0,1](x + lam D A' M (b - A x)) over the
-whole problem (sirt/sirt.m:213-243): a CSC pass (A' M r fused with the update
-and projection) and a CSR pass (residual, M r, FP64 norms and the stop check).
+whole problem (sirt/sirt.m:213-243): a column pass (A' M r fused with the update
+and projection) and a row pass (residual, M r, FP64 norms and the stop check),
+on the engine's LDS-tiled layout of A.
 The matrix is traced on the GPU, the phantom is grains(512, 68 grains, seed 1)
 and b = A x* + 2% Gaussian noise (seed 1); all inputs are resident in HBM
 before the timed region.  Prints one JSON line.
@@ -28,15 +29,36 @@ METRIC = "SIRT iter/s & HBM GB/s at N=512; time to fixed error, learned vs plain
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(A_nnz, m, n, s_v, method):
-    """Algorithmic HBM bytes of one launch of each SIRT kernel.
-    rows (CSR): values+indices, rowptr, x once, b, M, r and M r written.
-    cols (CSC): values+indices, colptr, M r once, D (if any), x read+write."""
+def kernel_bytes(L, method, s_v):
+    """Algorithmic HBM bytes of one launch of each SIRT kernel (unique data moved).
+
+    Tiled layout (entries: value + 16-bit local index):
+      cols   A'(M r) + update: entries, column pointers, staging runs, M r once,
+             x read+write, D (DROP/SART);
+      rows   forward pieces: entries, piece pointers, x once, one partial per piece;
+      reduce residual epilogue: row->piece lists, partials, b, M, r and M r.
+    CSR+CSC layout: entries (value + int32 index), pointers, gathered vector
+    once, the epilogue vectors (the reduce figure is 0)."""
+    m, n, nnz = L["m"], L["n"], L["nnz"]
     hasM = method != "landweber"
     hasD = method in ("drop", "sart")
-    rows = A_nnz * (s_v + 4) + 8 * (m + 1) + 4 * n + 4 * m + (4 * m if hasM else 0) + 8 * m
-    cols = A_nnz * (s_v + 4) + 8 * (n + 1) + 4 * m + (4 * n if hasD else 0) + 8 * n
-    return rows, cols
+    if L["tiled"]:
+        ni, P, R = L["n_internal"], L["pieces"], L["runs"]
+        cols = nnz * (s_v + 2) + 8 * (ni + 1) + 12 * R + 4 * (L["tiles_bwd"] + 1) + 4 * m + 8 * ni \
+            + (4 * ni if hasD else 0)
+        rows = nnz * (s_v + 2) + 8 * (P + 1) + 4 * ni + 4 * P
+        red = 8 * (m + 1) + 4 * P + 4 * P + 4 * m + (4 * m if hasM else 0) + 8 * m
+    else:
+        cols = nnz * (s_v + 4) + 8 * (n + 1) + 4 * m + (4 * n if hasD else 0) + 8 * n
+        rows = nnz * (s_v + 4) + 8 * (m + 1) + 4 * n + 4 * m + (4 * m if hasM else 0) + 8 * m
+        red = 0
+    return cols, rows, red
+
+
+def canonical_bytes(nnz, m, n, s_v):
+    """Canonical two-pass bytes of one SIRT iteration with int32 indices and FP32
+    vectors: 2 nnz (s_v + 4) + 4(m+1) + 4(n+1) + 4 (4n + 4m)."""
+    return 2 * nnz * (s_v + 4) + 4 * (m + 1) + 4 * (n + 1) + 4 * (4 * n + 4 * m)
 
 
 def cpu_baseline(N, theta, p, b, method, rho, samples, threads):
@@ -115,23 +137,35 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": args.dtype, "data": "synthetic",
            "config": {"workload": f"paralleltomo N={N}, theta=0:0.5:179.5 ({len(theta)} angles), p={p}, "
-                                  f"{args.method}, box [0,1], {'matrix-free' if args.matrix_free else 'CSR+CSC'}",
+                                  f"{args.method}, box [0,1]" + (", matrix-free" if args.matrix_free else ""),
                       "m": m, "n": n, "nnz": A.nnz, "method": args.method, "phantom": "grains(N, 68, seed 1)",
                       "noise": "2% Gaussian, seed 1"}}
     if not args.matrix_free:
-        rows_b, cols_b = kernel_bytes(A.nnz, m, n, s_v, args.method)
+        L = A.layout()
+        cols_b, rows_b, red_b = kernel_bytes(L, args.method, s_v)
         launches = max(kt["launches"], 1)
-        t_rows = kt["rows_ms"] / launches * 1e-3
         t_cols = kt["cols_ms"] / launches * 1e-3
-        dom, bytes_dom, t_dom = ("csr_resid", rows_b, t_rows) if t_rows >= t_cols else ("csc_update", cols_b, t_cols)
+        t_rows = kt["rows_ms"] / launches * 1e-3
+        t_red = kt["reduce_ms"] / launches * 1e-3
+        names = ("tiled_backward_update", "tiled_forward") if L["tiled"] else ("csc_update", "csr_resid")
+        if t_cols >= t_rows:
+            dom, bytes_dom, t_dom = names[0], cols_b, t_cols
+        else:
+            dom, bytes_dom, t_dom = names[1], rows_b, t_rows
         achieved = bytes_dom / t_dom / 1e9
-        it_bytes = rows_b + cols_b
+        it_bytes = cols_b + rows_b + red_b
+        canon = canonical_bytes(A.nnz, m, n, s_v)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                            "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(t_dom * 1e6, 2)}
-        out["iteration"] = {"bytes": int(it_bytes), "gbs": round(it_bytes * value / 1e9, 1),
+        out["iteration"] = {"layout": "tiled" if L["tiled"] else "csr+csc", "bytes": int(it_bytes),
+                            "gbs": round(it_bytes * value / 1e9, 1),
                             "frac": round(it_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
-                            "csr_resid_us": round(t_rows * 1e6, 2), "csc_update_us": round(t_cols * 1e6, 2)}
+                            "canonical_bytes": int(canon), "canonical_gbs": round(canon * value / 1e9, 1),
+                            "canonical_frac": round(canon * value / 1e9 / HBM_PEAK_GBS, 4),
+                            "cols_us": round(t_cols * 1e6, 2), "rows_us": round(t_rows * 1e6, 2),
+                            "reduce_us": round(t_red * 1e6, 2), "pieces": L["pieces"], "runs": L["runs"],
+                            "shapes": [L["fwd_lanes"], L["fwd_loads"], L["bwd_lanes"], L["bwd_loads"]]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
                      "rho": info["rho"]}

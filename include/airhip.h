@@ -122,7 +122,6 @@ int air_build_fanlineartomo(air_engine* e, int32_t N, const double* theta, int32
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz);
 /* copy the stored CSR back to the host (values widened to double) */
 int air_get_csr(air_engine* e, int64_t* rowptr, int32_t* colind, double* val);
-/* the same, with the tracer's FP64 values (only when dtype was AIR_F64) */
 /* afun convention, auxil/afun_matrix.m:47-50 / paralleltomo.m:282-287:
  * transp=0 -> out = A*in (m), transp=1 -> out = A'*in (n).  Host FP64. */
 int air_apply(air_engine* e, int transp, const double* in, double* out);
@@ -156,9 +155,27 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms);
 int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info* info);
 
 /* per-kernel HIP-event timing of the SIRT kernels (0 = off).  Times are
- * accumulated over iterate() calls while on; reset by enabling again. */
+ * accumulated over iterate() calls while on; reset by enabling again.
+ * cols: the A'(M r) + update pass; rows: the A x pass; reduce: the residual
+ * epilogue kernel of the tiled layout (0 for the compressed-row layout). */
 int air_set_kernel_timing(air_engine* e, int on);
-int air_get_kernel_timing(air_engine* e, double* rows_ms, double* cols_ms, int64_t* launches);
+int air_get_kernel_timing(air_engine* e, double* cols_ms, double* rows_ms, double* reduce_ms, int64_t* launches);
+
+/* Storage of the operator used by the SIRT loop. */
+typedef struct {
+    int     tiled;        /* 1: LDS-tiled layout, 0: CSR + CSC */
+    int     value_bytes;  /* 4 (FP32), 2 (BF16/FP16), 8 (FP64) */
+    int64_t m, n, nnz;
+    int64_t n_internal;   /* padded column count of the tiled layout */
+    int64_t pieces;       /* forward row segments */
+    int64_t runs;         /* backward staging runs */
+    int64_t tiles_fwd, tiles_bwd;
+    int64_t staged_rows;  /* sum over backward tiles of their distinct rows */
+    int64_t lds_bytes;    /* dynamic LDS of the backward kernel */
+    int     fwd_lanes, fwd_loads;   /* autotuned shape: lanes per piece, vector loads per round */
+    int     bwd_lanes, bwd_loads;   /* lanes per column, vector loads per round */
+} air_layout;
+int air_layout_info(air_engine* e, air_layout* out);
 
 #ifdef __cplusplus
 }
