@@ -188,13 +188,16 @@ struct TiledOp {
     DevBuf fval, fcode, piece_ptr, blk_tile, blk_p0, rp_ptr, rp_idx, partial;
     DevBuf bval, blr, cell_ptr, run_ptr, run_row, run_off, run_len;
     int fwd_cfg = 0, bwd_cfg = 0;          // kernel shapes picked by the autotuner
-    float tune_fwd_ms[8] = {}, tune_bwd_ms[8] = {};
+    int ppb = 256;                         // pieces per forward block
+    std::vector<int> piece_tile_h;         // forward tile of every piece (host copy)
+    float tune_fwd_ms[16] = {}, tune_bwd_ms[8] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
 bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s);
 void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStream_t s);
 void autotune_tiled(TiledOp& T, hipStream_t s);
+void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);
 void tiled_shapes(const TiledOp& T, int* fwd_lanes, int* fwd_loads, int* bwd_lanes, int* bwd_loads);
 void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s);
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);

@@ -203,9 +203,19 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
     __shared__ double sh[4][BLOCK / 64];
     if (load_stop(ctrl)) return;
     double v[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += BLOCK)
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const d4* p4 = (const d4*)partial;
+    int b = threadIdx.x;
+    for (; b + 3 * BLOCK < nblocks; b += 4 * BLOCK) {
+        d4 a0 = p4[b], a1 = p4[b + BLOCK], a2 = p4[b + 2 * BLOCK], a3 = p4[b + 3 * BLOCK];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] += partial[(int64_t)b * 4 + q];
+        for (int q = 0; q < 4; ++q) v[q] += ((a0[q] + a1[q]) + (a2[q] + a3[q]));
+    }
+    for (; b < nblocks; b += BLOCK) {
+        d4 a0 = p4[b];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] += a0[q];
+    }
     block_reduce<4>(v, sh);
     if (threadIdx.x == 0) evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], hist);
 }

@@ -34,8 +34,8 @@
 namespace air {
 
 static constexpr int FT = 4096;            // columns per forward tile (64 x 64)
-static constexpr int PIECES_PER_BLOCK = 256;
-static constexpr size_t MAX_LDS_B = 64 * 1024;
+static constexpr size_t MAX_LDS_B = 60 * 1024;
+static constexpr int MAX_BT = 256;         // columns per backward tile
 
 // 16-bit index loads matching VecOf<V>::VE
 typedef unsigned short us2v __attribute__((ext_vector_type(2)));
@@ -58,41 +58,108 @@ __device__ __forceinline__ void load_code(const unsigned short* p, unsigned (&ou
     }
 }
 
-// sum over entries [s,e) of val[q] * lds[code[q] & mask]; G lanes, U vector loads per lane per round
+// LDS slot of a staged element: one pad word per 32 spreads the power-of-two
+// strides of the tile layouts (16, 256 words) over the LDS banks.
+__device__ __forceinline__ unsigned lds_slot(unsigned i) { return i + (i >> 5); }
+
+// One round of a segment: U vector loads of values and 16-bit indices per lane.
 template <typename V, int G, int U>
-__device__ __forceinline__ typename AccOf<V>::T lds_dot(const V* __restrict__ val, const unsigned short* __restrict__ code,
-                                                       const float* lds, int64_t s, int64_t e, int gl, unsigned mask) {
+struct SegRound {
     using A = typename AccOf<V>::T;
-    constexpr int VE = VecOf<V>::VE, STEP = G * VE;
-    A acc = 0;
-    for (int64_t base = s & ~(int64_t)(VE - 1); base < e; base += (int64_t)STEP * U) {
-        A v[U][VE];
-        unsigned c[U][VE];
+    static constexpr int VE = VecOf<V>::VE, STEP = G * VE, SPAN = STEP * U;
+    A v[U][VE];
+    unsigned c[U][VE];
+    __device__ __forceinline__ void load(const V* __restrict__ val, const unsigned short* __restrict__ code,
+                                         int64_t base, int gl) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t q0 = base + (int64_t)u * STEP + (int64_t)gl * VE;
             load_vals<V, A>(val + q0, v[u]);
             load_code<VE>(code + q0, c[u]);
         }
+    }
+    // Branch-free: masked entries read LDS slot 0 and contribute 0, so the
+    // U*VE LDS reads of a round issue back to back.
+    __device__ __forceinline__ A dot(const float* lds, int64_t base, int64_t s, int64_t e, int gl,
+                                     unsigned mask) const {
+        A xv[U][VE];
+        bool ok[U][VE];
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int i = 0; i < VE; ++i) {
                 const int64_t q = base + (int64_t)u * STEP + (int64_t)gl * VE + i;
-                if (q >= s && q < e) acc += v[u][i] * (A)lds[c[u][i] & mask];
+                ok[u][i] = q >= s && q < e;
+                const unsigned idx = ok[u][i] ? (c[u][i] & mask) : 0u;
+                xv[u][i] = (A)lds[lds_slot(idx)];
             }
+        A acc = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < VE; ++i) acc += ok[u][i] ? v[u][i] * xv[u][i] : (A)0;
+        return acc;
     }
-    return acc;
+};
+
+// Software-pipelined walk over segments k = first, first+stride, ... < end:
+// the first rounds of the next DEPTH segments are in flight while segment k
+// is summed, and segment bounds are fetched DEPTH+1 segments ahead.  Past the
+// last segment the prefetch re-reads the current one, so the number of loads
+// in flight stays fixed.  bounds(k, s, e) gives the entry range; done(k, acc)
+// receives the lane-partial sum of segment k.
+template <typename V, int G, int U, int DEPTH, typename Bounds, typename Done>
+__device__ __forceinline__ void pipelined_segments(int first, int end, int stride, const V* __restrict__ val,
+                                                   const unsigned short* __restrict__ code, const float* lds,
+                                                   unsigned mask, int gl, Bounds&& bounds, Done&& done) {
+    using R = SegRound<V, G, U>;
+    constexpr int VE = R::VE;
+    static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
+    int kk[DEPTH + 2];
+    int64_t ss[DEPTH + 2], ee[DEPTH + 2];
+#pragma unroll
+    for (int d = 0; d < DEPTH + 1; ++d) {
+        kk[d] = first + d * stride;
+        ss[d] = ee[d] = 0;
+        if (kk[d] < end) bounds(kk[d], ss[d], ee[d]);
+    }
+    if (kk[0] >= end) return;
+    R buf[DEPTH + 1];
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) buf[d].load(val, code, (kk[d] < end ? ss[d] : ss[0]) & ~(int64_t)(VE - 1), gl);
+    while (kk[0] < end) {
+        kk[DEPTH + 1] = kk[DEPTH] + stride;
+        ss[DEPTH + 1] = ee[DEPTH + 1] = 0;
+        if (kk[DEPTH + 1] < end) bounds(kk[DEPTH + 1], ss[DEPTH + 1], ee[DEPTH + 1]);
+        buf[DEPTH].load(val, code, (kk[DEPTH] < end ? ss[DEPTH] : ss[0]) & ~(int64_t)(VE - 1), gl);
+        const int64_t s = ss[0], e = ee[0];
+        const int64_t b0 = s & ~(int64_t)(VE - 1);
+        auto acc = buf[0].dot(lds, b0, s, e, gl, mask);
+        for (int64_t base = b0 + R::SPAN; base < e; base += R::SPAN) {
+            R more;
+            more.load(val, code, base, gl);
+            acc += more.dot(lds, base, s, e, gl, mask);
+        }
+        done(kk[0], acc);
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) buf[d] = buf[d + 1];
+#pragma unroll
+        for (int d = 0; d < DEPTH + 1; ++d) {
+            kk[d] = kk[d + 1];
+            ss[d] = ss[d + 1];
+            ee[d] = ee[d + 1];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- forward pieces
-template <typename V, int G, int U>
+template <typename V, int G, int U, int DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile, const int* __restrict__ blk_p0,
                                                 const int64_t* __restrict__ piece_ptr, const V* __restrict__ fval,
                                                 const unsigned short* __restrict__ fcode, const float* __restrict__ x,
                                                 float* __restrict__ partial, const Ctrl* ctrl) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    __shared__ f4 xs4[FT / 4];
+    __shared__ float xs[FT + FT / 32];
     if (ctrl && load_stop(ctrl)) return;
     const int t = blk_tile[blockIdx.x], p0 = blk_p0[blockIdx.x], p1 = blk_p0[blockIdx.x + 1];
     const f4* xt = (const f4*)(x + (int64_t)t * FT);
@@ -100,25 +167,21 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile
 #pragma unroll
     for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
 #pragma unroll
-    for (int i = 0; i < FT / (4 * BLOCK); ++i) xs4[threadIdx.x + BLOCK * i] = tmp[i];
+    for (int i = 0; i < FT / (4 * BLOCK); ++i) {
+        const unsigned i0 = 4u * (threadIdx.x + BLOCK * i);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xs[lds_slot(i0 + j)] = tmp[i][j];
+    }
     __syncthreads();
-    const float* xs = (const float*)xs4;
     constexpr int GPB = BLOCK / G;
     const int gl = threadIdx.x % G;
-    int k = p0 + threadIdx.x / G;
-    int64_t s = 0, e = 0;
-    if (k < p1) { s = piece_ptr[k]; e = piece_ptr[k + 1]; }
-    while (k < p1) {
-        const int kn = k + GPB;
-        int64_t sn = 0, en = 0;
-        if (kn < p1) { sn = piece_ptr[kn]; en = piece_ptr[kn + 1]; }
-        auto a = lds_dot<V, G, U>(fval, fcode, xs, s, e, gl, 0x0fffu);
-        a = group_sum<G>(a);
-        if (gl == 0) partial[k] = (float)a;
-        k = kn;
-        s = sn;
-        e = en;
-    }
+    pipelined_segments<V, G, U, DEPTH>(
+        p0 + threadIdx.x / G, p1, GPB, fval, fcode, xs, 0x0fffu, gl,
+        [&](int k, int64_t& s, int64_t& e) { s = piece_ptr[k]; e = piece_ptr[k + 1]; },
+        [&](int k, auto acc) {
+            acc = group_sum<G>(acc);
+            if (gl == 0) partial[k] = (float)acc;
+        });
 }
 
 // y = sum of a row's partials (8 lanes per row)
@@ -204,7 +267,7 @@ __device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, co
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const int kk = k0 + u * BLOCK;
-            if (kk < nrows) ys[kk] = vv[u];
+            if (kk < nrows) ys[lds_slot(kk)] = vv[u];
         }
     }
     __syncthreads();
@@ -223,7 +286,7 @@ struct BwdArgs {
 };
 
 // MODE 0: g = A'y (FP32 out); 1: SIRT update of x; 2: line search (g in FP64, sum D g^2, lam)
-template <typename V, int MODE, int G, int U>
+template <typename V, int MODE, int G, int U, int DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__ bval, const float* __restrict__ y,
                                                 float* __restrict__ gout, double* __restrict__ g64,
                                                 const float* __restrict__ D, float* __restrict__ x, double lam,
@@ -237,44 +300,33 @@ __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__
     const int lane = threadIdx.x % G, w = threadIdx.x / G;
     double dsum[1] = {0};
     const int64_t c0 = (int64_t)t * a.bt;
-    int c = w;
-    int64_t s = 0, e = 0;
-    float xj = 0.f, dj = 1.f;
-    if (c < a.bt) {
-        s = a.cell_ptr[c0 + c];
-        e = a.cell_ptr[c0 + c + 1];
-        if (MODE == 1) { xj = x[c0 + c]; if (D) dj = D[c0 + c]; }
-    }
-    while (c < a.bt) {
-        const int cn = c + BLOCK / G;
-        int64_t sn = 0, en = 0;
-        float xn = 0.f, dn = 1.f;
-        if (cn < a.bt) {
-            sn = a.cell_ptr[c0 + cn];
-            en = a.cell_ptr[c0 + cn + 1];
-            if (MODE == 1) { xn = x[c0 + cn]; if (D) dn = D[c0 + cn]; }
+    // epilogue operands of the tile's columns, staged with the rows of y
+    __shared__ float xt[MAX_BT], dt[MAX_BT];
+    if (MODE == 1) {
+        for (int c = threadIdx.x; c < a.bt; c += BLOCK) {
+            xt[c] = x[c0 + c];
+            dt[c] = D ? D[c0 + c] : 1.f;
         }
-        auto acc = lds_dot<V, G, U>(bval, a.blr, ys, s, e, lane, 0xffffu);
-        acc = group_sum<G>(acc);
-        if (lane == 0) {
+        __syncthreads();
+    }
+    pipelined_segments<V, G, U, DEPTH>(
+        w, a.bt, BLOCK / G, bval, a.blr, ys, 0xffffu, lane,
+        [&](int c, int64_t& s, int64_t& e) { s = a.cell_ptr[c0 + c]; e = a.cell_ptr[c0 + c + 1]; },
+        [&](int c, auto acc) {
+            acc = group_sum<G>(acc);
+            if (lane != 0) return;
             const int64_t j = c0 + c;
             if (MODE == 0) {
                 gout[j] = (float)acc;
             } else if (MODE == 1) {
-                double dg = D ? (double)dj * (double)(float)acc : (double)(float)acc;
-                x[j] = project((double)xj + lam * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
+                double dg = D ? (double)dt[c] * (double)(float)acc : (double)(float)acc;
+                x[j] = project((double)xt[c] + lam * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
             } else {
                 double gj = (double)(float)acc;
                 g64[j] = gj;
                 dsum[0] += D ? (double)D[j] * (gj * gj) : gj * gj;
             }
-        }
-        c = cn;
-        s = sn;
-        e = en;
-        xj = xn;
-        dj = dn;
-    }
+        });
     if (MODE == 1 && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
     if (MODE == 2) {
         block_reduce<1>(dsum, sh);
@@ -419,6 +471,32 @@ __global__ void k_gather_perm(const float* __restrict__ xi, const int* __restric
 }
 
 // ---------------------------------------------------------------- host: build
+// Forward blocks: up to ppb consecutive pieces of one tile each.
+void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s) {
+    const std::vector<int>& ptile = T.piece_tile_h;
+    const int64_t P = (int64_t)ptile.size();
+    std::vector<int> bt, bp;
+    int64_t k = 0;
+    while (k < P) {
+        const int tile = ptile[k];
+        int64_t kend = k;
+        while (kend < P && ptile[kend] == tile) kend++;
+        for (int64_t q = k; q < kend; q += ppb) {
+            bt.push_back(tile);
+            bp.push_back((int)q);
+        }
+        k = kend;
+    }
+    bp.push_back((int)P);
+    T.ppb = ppb;
+    T.nblk_f = (int)bt.size();
+    T.blk_tile.alloc(4 * std::max<size_t>(bt.size(), 1));
+    T.blk_p0.alloc(4 * bp.size());
+    if (!bt.empty()) AIR_HIP(hipMemcpyAsync(T.blk_tile.p, bt.data(), 4 * bt.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(T.blk_p0.p, bp.data(), 4 * bp.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 static unsigned grid_for(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + BLOCK - 1) / BLOCK, 65536));
 }
@@ -516,31 +594,11 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
                                               piece_row.as<uint32_t>(), piece_tile.as<int>(), iota_p.as<uint32_t>());
     AIR_HIP(hipGetLastError());
     AIR_HIP(hipMemcpyAsync(T.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
-    // forward blocks: PIECES_PER_BLOCK pieces of one tile each
-    {
-        std::vector<int> ptile(P);
-        AIR_HIP(hipMemcpyAsync(ptile.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
-        AIR_HIP(hipStreamSynchronize(s));
-        std::vector<int> bt, bp;
-        int64_t k = 0;
-        while (k < P) {
-            const int tile = ptile[k];
-            int64_t kend = k;
-            while (kend < P && ptile[kend] == tile) kend++;
-            for (int64_t q = k; q < kend; q += PIECES_PER_BLOCK) {
-                bt.push_back(tile);
-                bp.push_back((int)q);
-            }
-            k = kend;
-        }
-        bp.push_back((int)P);
-        T.nblk_f = (int)bt.size();
-        T.blk_tile.alloc(4 * bt.size());
-        T.blk_p0.alloc(4 * bp.size());
-        AIR_HIP(hipMemcpyAsync(T.blk_tile.p, bt.data(), 4 * bt.size(), hipMemcpyHostToDevice, s));
-        AIR_HIP(hipMemcpyAsync(T.blk_p0.p, bp.data(), 4 * bp.size(), hipMemcpyHostToDevice, s));
-        AIR_HIP(hipStreamSynchronize(s));
-    }
+    // forward blocks are cut per tile from the piece -> tile map (kept on the host)
+    T.piece_tile_h.resize(P);
+    AIR_HIP(hipMemcpyAsync(T.piece_tile_h.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+    set_fwd_blocks(T, 256, s);
     // row -> pieces (stable sort of piece rows keeps tile order)
     prow_s.alloc(4 * (size_t)P);
     T.rp_idx.alloc(4 * (size_t)P);
@@ -650,7 +708,7 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
         T.staged_rows = staged;
         T.maxruns = maxruns;
         T.maxrows = maxrows;
-        T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
+        T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows + (size_t)maxrows / 32 + 1);
         if (T.lds_b > MAX_LDS_B || maxrows > 65535) return false;
         T.run_ptr.alloc(4 * rp.size());
         AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
@@ -675,37 +733,42 @@ static BwdArgs bwd_args(const TiledOp& T) {
     return a;
 }
 
-// kernel shapes the autotuner chooses from: (lanes per piece/column, vector loads per round)
-static const int FWD_CFG[][2] = {{16, 2}, {16, 1}, {8, 2}, {8, 1}, {32, 1}};
-static const int BWD_CFG[][2] = {{64, 2}, {64, 1}, {32, 2}, {32, 1}};
+// kernel shapes the autotuner chooses from: (lanes per piece/column, vector
+// loads per round, segments prefetched ahead)
+static const int FWD_CFG[][3] = {{16, 2, 1}, {16, 2, 2}, {16, 1, 2}, {8, 2, 2}, {32, 1, 2}};
+static const int BWD_CFG[][3] = {{64, 2, 1}, {64, 2, 2}, {64, 1, 2}, {32, 2, 2}};
 static constexpr int N_FWD_CFG = 5, N_BWD_CFG = 4;
+template <int A, int B, int C> using Shape3 = std::integer_sequence<int, A, B, C>;
 
 template <typename F>
 static void dispatch_fwd(int cfg, F&& f) {
     switch (cfg) {
-    case 0: f(std::integral_constant<int, 16>{}, std::integral_constant<int, 2>{}); break;
-    case 1: f(std::integral_constant<int, 16>{}, std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 8>{}, std::integral_constant<int, 2>{}); break;
-    case 3: f(std::integral_constant<int, 8>{}, std::integral_constant<int, 1>{}); break;
-    default: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 1>{}); break;
+    case 0: f(Shape3<16, 2, 1>{}); break;
+    case 1: f(Shape3<16, 2, 2>{}); break;
+    case 2: f(Shape3<16, 1, 2>{}); break;
+    case 3: f(Shape3<8, 2, 2>{}); break;
+    default: f(Shape3<32, 1, 2>{}); break;
     }
 }
 template <typename F>
 static void dispatch_bwd(int cfg, F&& f) {
     switch (cfg) {
-    case 0: f(std::integral_constant<int, 64>{}, std::integral_constant<int, 2>{}); break;
-    case 1: f(std::integral_constant<int, 64>{}, std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 2>{}); break;
-    default: f(std::integral_constant<int, 32>{}, std::integral_constant<int, 1>{}); break;
+    case 0: f(Shape3<64, 2, 1>{}); break;
+    case 1: f(Shape3<64, 2, 2>{}); break;
+    case 2: f(Shape3<64, 1, 2>{}); break;
+    default: f(Shape3<32, 2, 2>{}); break;
     }
 }
+template <int A, int B, int C> constexpr int sh0(std::integer_sequence<int, A, B, C>) { return A; }
+template <int A, int B, int C> constexpr int sh1(std::integer_sequence<int, A, B, C>) { return B; }
+template <int A, int B, int C> constexpr int sh2(std::integer_sequence<int, A, B, C>) { return C; }
 
 static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* ctrl, hipStream_t s, int cfg = -1) {
     dispatch_dtype(T.dtype, [&](auto v) {
         using V = decltype(v);
-        dispatch_fwd(cfg < 0 ? T.fwd_cfg : cfg, [&](auto g, auto u) {
-            constexpr int G = decltype(g)::value, U = decltype(u)::value;
-            k_tfwd<V, G, U><<<T.nblk_f, BLOCK, 0, s>>>(T.blk_tile.as<int>(), T.blk_p0.as<int>(),
+        dispatch_fwd(cfg < 0 ? T.fwd_cfg : cfg, [&](auto shp) {
+            constexpr int G = sh0(decltype(shp){}), U = sh1(decltype(shp){}), DP = sh2(decltype(shp){});
+            k_tfwd<V, G, U, DP><<<T.nblk_f, BLOCK, 0, s>>>(T.blk_tile.as<int>(), T.blk_p0.as<int>(),
                                                        T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
                                                        T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
         });
@@ -719,14 +782,14 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
                        int cfg = -1) {
     dispatch_dtype(T.dtype, [&](auto v) {
         using V = decltype(v);
-        dispatch_bwd(cfg < 0 ? T.bwd_cfg : cfg, [&](auto g, auto u) {
-            constexpr int G = decltype(g)::value, U = decltype(u)::value;
+        dispatch_bwd(cfg < 0 ? T.bwd_cfg : cfg, [&](auto shp) {
+            constexpr int G = sh0(decltype(shp){}), U = sh1(decltype(shp){}), DP = sh2(decltype(shp){});
             if (MODE == 0)
-                k_tbwd<V, 0, G, U><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
+                k_tbwd<V, 0, G, U, DP><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
                     bwd_args(T), T.bval.as<V>(), y, gout, nullptr, nullptr, nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f,
                     nullptr, nullptr, nullptr);
             else
-                k_tbwd<V, MODE, G, U><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
+                k_tbwd<V, MODE, G, U, DP><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
                     bwd_args(T), T.bval.as<V>(), y, nullptr, B->g, B->D, B->x, lam, B->lb_kind, B->lb, B->lbv,
                     B->ub_kind, B->ub, B->ubv, B->partial_cols, B->ctrl);
         });
@@ -736,7 +799,7 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
 }
 
 unsigned tiled_reduce_grid(int64_t m) {
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((m * 8 + BLOCK - 1) / BLOCK, 65535));
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((m * 8 + BLOCK - 1) / BLOCK, 2048));
 }
 static unsigned reduce_grid(int64_t m) { return tiled_reduce_grid(m); }
 
@@ -777,6 +840,7 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
     }
 }
 
+#include <utility>
 void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
     *fl = FWD_CFG[T.fwd_cfg][0];
     *fu = FWD_CFG[T.fwd_cfg][1];
@@ -816,11 +880,17 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         return ms;
     };
     float best = 1e30f;
-    for (int c = 0; c < N_FWD_CFG; ++c) {
-        float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
-        T.tune_fwd_ms[c] = t / 4;
-        if (t < best) { best = t; T.fwd_cfg = c; }
+    int best_ppb = 256;
+    static const int PPB[] = {256, 512, 1024};
+    for (int pi = 0; pi < 3; ++pi) {
+        set_fwd_blocks(T, PPB[pi], s);
+        for (int c = 0; c < N_FWD_CFG; ++c) {
+            float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
+            T.tune_fwd_ms[3 * c + pi] = t / 4;
+            if (t < best) { best = t; T.fwd_cfg = c; best_ppb = PPB[pi]; }
+        }
     }
+    set_fwd_blocks(T, best_ppb, s);
     best = 1e30f;
     for (int c = 0; c < N_BWD_CFG; ++c) {
         float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });

@@ -67,7 +67,11 @@ class AirOperator:
         """Storage used by the SIRT loop (tiled or compressed rows) and its sizes."""
         L = _lib.Layout()
         check(load().air_layout_info(self.engine.h, L))
-        return {f: getattr(L, f) for f, _ in L._fields_}
+        out = {}
+        for f, _ in L._fields_:
+            v = getattr(L, f)
+            out[f] = [round(float(a), 2) for a in v] if hasattr(v, "__len__") else v
+        return out
 
     # ---- stored matrix access
     def csr_arrays(self):

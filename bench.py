@@ -165,7 +165,9 @@ def main():
                             "canonical_frac": round(canon * value / 1e9 / HBM_PEAK_GBS, 4),
                             "cols_us": round(t_cols * 1e6, 2), "rows_us": round(t_rows * 1e6, 2),
                             "reduce_us": round(t_red * 1e6, 2), "pieces": L["pieces"], "runs": L["runs"],
-                            "shapes": [L["fwd_lanes"], L["fwd_loads"], L["bwd_lanes"], L["bwd_loads"]]}
+                            "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
+                                       L["bwd_loads"]],
+                            "tune_fwd_us": L["tune_fwd_us"][:15], "tune_bwd_us": L["tune_bwd_us"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
                      "rho": info["rho"]}

@@ -174,6 +174,9 @@ typedef struct {
     int64_t lds_bytes;    /* dynamic LDS of the backward kernel */
     int     fwd_lanes, fwd_loads;   /* autotuned shape: lanes per piece, vector loads per round */
     int     bwd_lanes, bwd_loads;   /* lanes per column, vector loads per round */
+    int     fwd_pieces_per_block;
+    float   tune_fwd_us[16];        /* autotuner timings: forward shape c at 256/512/1024 pieces per block -> [3c+i] */
+    float   tune_bwd_us[4];         /* backward shapes */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 
