@@ -985,8 +985,9 @@ int air_layout_info(air_engine* e, air_layout* out) {
             L.staged_rows = T.staged_rows;
             L.lds_bytes = (int64_t)T.lds_b;
             tiled_shapes(T, &L.fwd_lanes, &L.fwd_loads, &L.bwd_lanes, &L.bwd_loads);
-            L.fwd_pieces_per_block = T.ppb;
-            for (int q = 0; q < 16; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
+            L.fwd_stream = tiled_fwd_stream(T) ? 1 : 0;
+            L.fwd_pieces_per_block = L.fwd_stream ? T.wave_target : T.ppb;
+            for (int q = 0; q < 32; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
             for (int q = 0; q < 4; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
         }
         *out = L;

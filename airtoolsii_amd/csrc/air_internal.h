@@ -190,7 +190,11 @@ struct TiledOp {
     int fwd_cfg = 0, bwd_cfg = 0;          // kernel shapes picked by the autotuner
     int ppb = 256;                         // pieces per forward block
     std::vector<int> piece_tile_h;         // forward tile of every piece (host copy)
-    float tune_fwd_ms[16] = {}, tune_bwd_ms[8] = {};
+    std::vector<int64_t> piece_ptr_h;      // first entry of every piece (host copy)
+    // stream form of the forward product: waves own runs of whole pieces
+    DevBuf wblk_tile, wave_p0;
+    int nblk_w = 0, nwaves = 0, wave_target = 4096;
+    float tune_fwd_ms[32] = {}, tune_bwd_ms[8] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
@@ -198,6 +202,8 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s);
 void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStream_t s);
 void autotune_tiled(TiledOp& T, hipStream_t s);
 void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);
+void set_fwd_waves(TiledOp& T, int target_entries, hipStream_t s);
+bool tiled_fwd_stream(const TiledOp& T);
 void tiled_shapes(const TiledOp& T, int* fwd_lanes, int* fwd_loads, int* bwd_lanes, int* bwd_loads);
 void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s);
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);

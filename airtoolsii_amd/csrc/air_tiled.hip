@@ -184,6 +184,155 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile
         });
 }
 
+// ---------------------------------------------------------------- forward stream
+// Blocks of one launch are handed to the XCDs round-robin; this renumbering
+// gives each XCD a contiguous run of logical blocks (consecutive blocks share
+// a tile of x, which then stays in that XCD's L2).
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    constexpr int NXCD = 8;
+    const int nb8 = nblk - nblk % NXCD;
+    return b < nb8 ? (b % NXCD) * (nb8 / NXCD) + b / NXCD : b;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Forward product as a flat stream.  Every wave owns a run of whole pieces of
+// one tile (entries [E0, E1) of the tile-sorted arrays) and walks it in rounds
+// of 512 entries, 8 consecutive entries per lane, so that every vector load is
+// useful and no load depends on data fetched in the same round.  A piece's
+// sum is formed by a segmented reduction: the start flag (bit 15 of the code)
+// splits each lane's 8 products into a head (the piece open on the left), the
+// pieces closed inside the lane, and a tail; a reverse segmented scan over the
+// lanes adds the heads of the following lanes up to the next start, and the
+// piece left open at the end of a round is carried (wave-uniform) into the
+// next round.  PF = 1 issues the loads of the next round before the current
+// one is summed.
+template <typename V, int PF>
+__global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
+                                                     int nwaves, int nblk, const int64_t* __restrict__ piece_ptr,
+                                                     const V* __restrict__ fval, const unsigned short* __restrict__ fcode,
+                                                     const float* __restrict__ x, float* __restrict__ partial,
+                                                     const Ctrl* ctrl) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    using A = typename AccOf<V>::T;
+    constexpr int VE = VecOf<V>::VE, NV = 8 / VE, WPB = BLOCK / 64, RND = 512;
+    __shared__ float xs[FT + FT / 32];
+    if (ctrl && load_stop(ctrl)) return;
+    const int b = xcd_block(blockIdx.x, nblk);
+    const int t = blk_tile[b];
+    {
+        const f4* xt = (const f4*)(x + (int64_t)t * FT);
+        f4 tmp[FT / (4 * BLOCK)];
+#pragma unroll
+        for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
+#pragma unroll
+        for (int i = 0; i < FT / (4 * BLOCK); ++i) {
+            const unsigned i0 = 4u * (threadIdx.x + BLOCK * i);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xs[lds_slot(i0 + j)] = tmp[i][j];
+        }
+    }
+    __syncthreads();
+    const int w = b * WPB + (int)(threadIdx.x >> 6);
+    if (w >= nwaves) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int pw0 = wave_p0[w], pw1 = wave_p0[w + 1];
+    if (pw0 >= pw1) return;
+    const int64_t E0 = piece_ptr[pw0], E1 = piece_ptr[pw1];
+    int64_t base = (E0 & ~(int64_t)7) + 8 * lane;
+    const int64_t stop_at = E1 + 8 * lane;   // base - 8*lane < E1  <=>  base < stop_at
+    int knext = pw0;                         // index of the next piece start
+    int kopen = -1;                          // piece open at the end of the last round
+    A carry = 0;
+    A v[NV][VE];
+    unsigned c[8];
+    auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[8]) {
+#pragma unroll
+        for (int h = 0; h < NV; ++h) load_vals<V, A>(fval + q + VE * h, vv[h]);
+        load_code<8>(fcode + q, cc);
+    };
+    load_round(base, v, c);
+    while (base < stop_at) {
+        A vn[NV][VE];
+        unsigned cn[8];
+        if (PF) load_round(base + RND < stop_at ? base + RND : base, vn, cn);
+        // products and start flags of this lane's 8 entries
+        A pr[8];
+        unsigned fl = 0;
+        const int lo = (int)max<int64_t>(min<int64_t>(E0 - base, 8), -1);
+        const int hi = (int)max<int64_t>(min<int64_t>(E1 - base, 8), -1);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bool ok = i >= lo && i < hi;
+            const float xv = xs[lds_slot(c[i] & 0x0fffu)];
+            pr[i] = ok ? v[i / VE][i % VE] * (A)xv : (A)0;
+            fl |= (ok && (c[i] & 0x8000u)) ? (1u << i) : 0u;
+        }
+        // starts in lower lanes -> piece index of this lane's first start
+        unsigned excl = 0;
+        int total = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t m = __ballot((fl >> i) & 1u);
+            excl = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, excl));
+            total += __popcll(m);
+        }
+        int k = knext + (int)excl;
+        A acc = 0, head = 0;
+        const bool seen = fl != 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if ((fl >> i) & 1u) {
+                if (fl & ((1u << i) - 1u)) {
+                    partial[k] = (float)acc;
+                    ++k;
+                } else {
+                    head = acc;
+                }
+                acc = 0;
+            }
+            acc += pr[i];
+        }
+        if (!seen) head = acc;
+        // reverse segmented scan: r = head of this lane + heads of the next
+        // lanes up to and including the first lane with a start
+        const uint64_t sm = __ballot(seen);
+        A r = head;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const A rr = __shfl_down(r, d);
+            const uint64_t win = (sm >> lane) & ((1ull << d) - 1ull);
+            if (win == 0 && lane + d < 64) r += rr;
+        }
+        const A after = __shfl_down(r, 1);
+        const A close = acc + (lane < 63 ? after : (A)0);
+        if (sm == 0) {
+            carry += __shfl(r, 0);
+        } else {
+            const int lf = 63 - __builtin_clzll(sm);
+            if (lane == 0 && kopen >= 0) partial[kopen] = (float)(carry + r);
+            if (seen && lane != lf) partial[k] = (float)close;
+            carry = __shfl(close, lf);
+            kopen = __builtin_amdgcn_readlane(k, lf);
+        }
+        knext += total;
+        base += RND;
+        if (PF) {
+#pragma unroll
+            for (int h = 0; h < NV; ++h)
+#pragma unroll
+                for (int i = 0; i < VE; ++i) v[h][i] = vn[h][i];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) c[i] = cn[i];
+        } else if (base < stop_at) {
+            load_round(base, v, c);
+        }
+    }
+    if (lane == 0 && kopen >= 0) partial[kopen] = (float)carry;
+}
+
 // y = sum of a row's partials (8 lanes per row)
 __device__ __forceinline__ float row_partials(const int64_t* __restrict__ rp_ptr, const int* __restrict__ rp_idx,
                                               const float* __restrict__ partial, int64_t row, int gl) {
@@ -497,6 +646,43 @@ void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s) {
     AIR_HIP(hipStreamSynchronize(s));
 }
 
+// Stream form: every tile's entries are split into a multiple of 4 waves
+// (one block = 4 waves of one tile) of about target_entries each, cut at
+// piece boundaries.
+void set_fwd_waves(TiledOp& T, int target, hipStream_t s) {
+    constexpr int WPB = BLOCK / 64;
+    const std::vector<int>& ptile = T.piece_tile_h;
+    const std::vector<int64_t>& pp = T.piece_ptr_h;
+    const int64_t P = (int64_t)ptile.size();
+    std::vector<int> bt, wp;
+    int64_t k = 0;
+    while (k < P) {
+        const int tile = ptile[k];
+        int64_t kend = k;
+        while (kend < P && ptile[kend] == tile) kend++;
+        const int64_t ent = pp[kend] - pp[k];
+        const int64_t nb = std::max<int64_t>(1, (ent + (int64_t)WPB * target - 1) / ((int64_t)WPB * target));
+        const int64_t nw = nb * WPB;
+        int64_t q = k;
+        for (int64_t w = 0; w < nw; ++w) {
+            const int64_t goal = pp[k] + ent * w / nw;
+            while (q < kend && pp[q] < goal) q++;
+            wp.push_back((int)q);
+        }
+        for (int64_t b = 0; b < nb; ++b) bt.push_back(tile);
+        k = kend;
+    }
+    wp.push_back((int)P);
+    T.wave_target = target;
+    T.nblk_w = (int)bt.size();
+    T.nwaves = (int)wp.size() - 1;
+    T.wblk_tile.alloc(4 * std::max<size_t>(bt.size(), 1));
+    T.wave_p0.alloc(4 * wp.size());
+    if (!bt.empty()) AIR_HIP(hipMemcpyAsync(T.wblk_tile.p, bt.data(), 4 * bt.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(T.wave_p0.p, wp.data(), 4 * wp.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 static unsigned grid_for(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + BLOCK - 1) / BLOCK, 65536));
 }
@@ -597,8 +783,11 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
     // forward blocks are cut per tile from the piece -> tile map (kept on the host)
     T.piece_tile_h.resize(P);
     AIR_HIP(hipMemcpyAsync(T.piece_tile_h.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
+    T.piece_ptr_h.resize(P + 1);
+    AIR_HIP(hipMemcpyAsync(T.piece_ptr_h.data(), T.piece_ptr.p, 8 * (P + 1), hipMemcpyDeviceToHost, s));
     AIR_HIP(hipStreamSynchronize(s));
     set_fwd_blocks(T, 256, s);
+    set_fwd_waves(T, T.wave_target, s);
     // row -> pieces (stable sort of piece rows keeps tile order)
     prow_s.alloc(4 * (size_t)P);
     T.rp_idx.alloc(4 * (size_t)P);
@@ -764,9 +953,20 @@ template <int A, int B, int C> constexpr int sh1(std::integer_sequence<int, A, B
 template <int A, int B, int C> constexpr int sh2(std::integer_sequence<int, A, B, C>) { return C; }
 
 static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* ctrl, hipStream_t s, int cfg = -1) {
+    if (cfg < 0) cfg = T.fwd_cfg;
     dispatch_dtype(T.dtype, [&](auto v) {
         using V = decltype(v);
-        dispatch_fwd(cfg < 0 ? T.fwd_cfg : cfg, [&](auto shp) {
+        if (cfg >= N_FWD_CFG) {
+            auto launch = [&](auto kern) {
+                kern<<<T.nblk_w, BLOCK, 0, s>>>(T.wblk_tile.as<int>(), T.wave_p0.as<int>(), T.nwaves, T.nblk_w,
+                                                T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
+                                                T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
+            };
+            if (cfg == N_FWD_CFG) launch(k_tfwd_flat<V, 0>);
+            else launch(k_tfwd_flat<V, 1>);
+            return 0;
+        }
+        dispatch_fwd(cfg, [&](auto shp) {
             constexpr int G = sh0(decltype(shp){}), U = sh1(decltype(shp){}), DP = sh2(decltype(shp){});
             k_tfwd<V, G, U, DP><<<T.nblk_f, BLOCK, 0, s>>>(T.blk_tile.as<int>(), T.blk_p0.as<int>(),
                                                        T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
@@ -776,6 +976,8 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
     });
     AIR_HIP(hipGetLastError());
 }
+
+bool tiled_fwd_stream(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG; }
 
 template <int MODE>
 static void launch_bwd(const TiledOp& T, const float* y, float* gout, const SirtBufs* B, double lam, hipStream_t s,
@@ -842,21 +1044,25 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
 
 #include <utility>
 void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
-    *fl = FWD_CFG[T.fwd_cfg][0];
-    *fu = FWD_CFG[T.fwd_cfg][1];
+    // stream form: 64 lanes per wave, 8 entries per lane (prefetch flag in fu)
+    *fl = T.fwd_cfg >= N_FWD_CFG ? 64 : FWD_CFG[T.fwd_cfg][0];
+    *fu = T.fwd_cfg >= N_FWD_CFG ? T.fwd_cfg - N_FWD_CFG : FWD_CFG[T.fwd_cfg][1];
     *bl = BWD_CFG[T.bwd_cfg][0];
     *bu = BWD_CFG[T.bwd_cfg][1];
 }
 
 // Time every kernel shape on this matrix (a few launches each, outside any
-// SIRT run) and keep the fastest; AIR_TILED_CFG="f,b" fixes the choice.
+// SIRT run) and keep the fastest; AIR_TILED_CFG="f,b[,ppb]" fixes the choice.
 void autotune_tiled(TiledOp& T, hipStream_t s) {
     const char* fixed = std::getenv("AIR_TILED_CFG");
     if (fixed && fixed[0]) {
-        int f = 0, b = 0;
-        if (std::sscanf(fixed, "%d,%d", &f, &b) == 2) {
-            T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG - 1);
+        int f = 0, b = 0, par = 0;
+        const int got = std::sscanf(fixed, "%d,%d,%d", &f, &b, &par);
+        if (got >= 2) {
+            T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + 1);
             T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG - 1);
+            if (got == 3 && T.fwd_cfg < N_FWD_CFG) set_fwd_blocks(T, std::min(std::max(par, 64), 4096), s);
+            if (got == 3 && T.fwd_cfg >= N_FWD_CFG) set_fwd_waves(T, std::min(std::max(par, 512), 1 << 20), s);
             return;
         }
     }
@@ -891,6 +1097,19 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         }
     }
     set_fwd_blocks(T, best_ppb, s);
+    // stream form: prefetch off/on at 2048/4096/8192 entries per wave
+    static const int WT[] = {2048, 4096, 8192};
+    int best_wt = T.wave_target;
+    for (int wi = 0; wi < 3; ++wi) {
+        set_fwd_waves(T, WT[wi], s);
+        for (int pf = 0; pf < 2; ++pf) {
+            const int c = N_FWD_CFG + pf;
+            float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
+            T.tune_fwd_ms[3 * N_FWD_CFG + 3 * pf + wi] = t / 4;
+            if (t < best) { best = t; T.fwd_cfg = c; best_wt = WT[wi]; }
+        }
+    }
+    set_fwd_waves(T, best_wt, s);
     best = 1e30f;
     for (int c = 0; c < N_BWD_CFG; ++c) {
         float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });

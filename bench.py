@@ -3,6 +3,8 @@
 
     python bench.py [--steps K] [--warmup W] [--method cimmino|sart|drop|...]
                     [--dtype f32|bf16|f16] [--matrix-free] [--no-cpu-baseline]
+    python bench.py --copy-calibration     (a plain 1 GiB device copy, for
+                                            scaling HBM byte counters)
 
 One step is one SIRT iteration x <- P_[0,1](x + lam D A' M (b - A x)) over the
 whole problem (sirt/sirt.m:213-243): a column pass (A' M r fused with the update
@@ -27,6 +29,7 @@ sys.path.insert(0, HERE)
 
 METRIC = "SIRT iter/s & HBM GB/s at N=512; time to fixed error, learned vs plain λ"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_c2.json")
 
 
 def kernel_bytes(L, method, s_v):
@@ -83,6 +86,34 @@ def cpu_baseline(N, theta, p, b, method, rho, samples, threads):
                       f"{A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}; oracle tracer {t_trace:.1f}s not included"}
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size: the reference point that
+    turns FETCH_SIZE / WRITE_SIZE counter readings into HBM bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}}),
+          flush=True)
+
+
+def measured_traffic(kernel, workload, dtype):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the copy calibration),
+    when it was taken on this workload; otherwise None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("dtype") != dtype:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else int(k["hbm_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,11 +125,15 @@ def main():
     ap.add_argument("--matrix-free", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine runs one problem per GPU and no multi-GPU path is "
               "built (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import airtoolsii_amd as air
 
@@ -156,7 +191,8 @@ def main():
         it_bytes = cols_b + rows_b + red_b
         canon = canonical_bytes(A.nnz, m, n, s_v)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                           "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": measured_traffic(dom, out["config"]["workload"], args.dtype), "kernel": dom,
                            "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(t_dom * 1e6, 2)}
         out["iteration"] = {"layout": "tiled" if L["tiled"] else "csr+csc", "bytes": int(it_bytes),
                             "gbs": round(it_bytes * value / 1e9, 1),
@@ -165,9 +201,10 @@ def main():
                             "canonical_frac": round(canon * value / 1e9 / HBM_PEAK_GBS, 4),
                             "cols_us": round(t_cols * 1e6, 2), "rows_us": round(t_rows * 1e6, 2),
                             "reduce_us": round(t_red * 1e6, 2), "pieces": L["pieces"], "runs": L["runs"],
+                            "fwd_stream": L["fwd_stream"],
                             "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
                                        L["bwd_loads"]],
-                            "tune_fwd_us": L["tune_fwd_us"][:15], "tune_bwd_us": L["tune_bwd_us"]}
+                            "tune_fwd_us": L["tune_fwd_us"][:21], "tune_bwd_us": L["tune_bwd_us"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
                      "rho": info["rho"]}

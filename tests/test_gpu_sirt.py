@@ -197,3 +197,41 @@ def test_train_relaxpar_matches_oracle(oracle):
     lam = air.train_relaxpar(A, b, x, "sart", 100)
     lam_o = oracle.train_relaxpar(Ao, b, x, "sart", 100)
     assert abs(lam - lam_o) <= 0.03 * 2.0
+
+
+# Every forward/backward kernel shape of the tiled layout (the autotuner picks
+# one per matrix): (forward shape, backward shape, pieces per block or entries
+# per wave).  Shapes 0-4 are per-piece lane groups, 5-6 the stream form
+# without/with prefetch.  N=96 gives 4 forward tiles with padded columns.
+@pytest.mark.parametrize("cfg", ["0,0,256", "1,1,512", "2,2,1024", "3,3,256", "4,0,512",
+                                 "5,1,512", "6,2,2048", "5,3,8192", "6,0,100000"])
+def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
+    N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
+    monkeypatch.setenv("AIR_TILED_CFG", cfg)
+    A = air.paralleltomo(N, theta, p)
+    L = A.layout()
+    f, bcfg, par = (int(v) for v in cfg.split(","))
+    assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
+    assert L["fwd_pieces_per_block"] == (max(par, 512) if f >= 5 else par)
+    Ao = oracle.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("threephases", N, None, 3).ravel(order="F")
+    b = Ao @ x
+    opts = {"rho": oracle.spectral_radius(Ao, *oracle.weights("cimmino", Ao)), "lbound": 0}
+    X, _, _ = air.cimmino(A, b, [3, 15], None, opts)
+    Xo, _, _ = oracle.sirt("cimmino", Ao, b, [3, 15], None, opts)
+    for j in range(2):
+        assert rel(X[:, j], Xo[:, j]) <= TOL, (cfg, j, rel(X[:, j], Xo[:, j]))
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
+@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048"])
+def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
+    """Per-piece and stream forward kernels agree on every storage type."""
+    opts = {"rho": c1["rho"]["sart"], "lbound": 0}
+    out = []
+    for f in ("1,1,256", cfg):
+        monkeypatch.setenv("AIR_TILED_CFG", f)
+        A = air.paralleltomo(c1["N"], c1["theta"], c1["p"], dtype=dtype)
+        X, _, _ = air.sart(A, c1["b"], 20, None, opts)
+        out.append(X[:, -1])
+    assert rel(out[1], out[0]) <= (1e-12 if dtype == "f64" else 1e-5)
