@@ -135,8 +135,8 @@ __device__ __forceinline__ int load_stop(const Ctrl* c) {
 }
 
 // ---------------------------------------------------------------- block reduction + last-block finalize
-template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double (*sh)[BLOCK / 64]) {
+template <int NV, int NT = BLOCK>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double (*sh)[NT / 64]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -148,7 +148,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double (*sh)[BLOCK
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double t = 0;
-            for (int w = 0; w < BLOCK / 64; ++w) t += sh[q][w];
+            for (int w = 0; w < NT / 64; ++w) t += sh[q][w];
             v[q] = t;
         }
     }

@@ -988,7 +988,7 @@ int air_layout_info(air_engine* e, air_layout* out) {
             L.fwd_stream = tiled_fwd_stream(T) ? 1 : 0;
             L.fwd_pieces_per_block = L.fwd_stream ? T.wave_target : T.ppb;
             for (int q = 0; q < 32; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
-            for (int q = 0; q < 4; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
+            for (int q = 0; q < 8; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
         }
         *out = L;
     });

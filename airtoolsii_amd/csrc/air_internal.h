@@ -194,6 +194,9 @@ struct TiledOp {
     // stream form of the forward product: waves own runs of whole pieces
     DevBuf wblk_tile, wave_p0;
     int nblk_w = 0, nwaves = 0, wave_target = 4096;
+    // backward codes carry a column-start flag in bit 15 when every tile has
+    // fewer than 32768 distinct rows (required by the stream form)
+    bool bwd_flags = false;
     float tune_fwd_ms[32] = {}, tune_bwd_ms[8] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).

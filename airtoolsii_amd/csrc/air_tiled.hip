@@ -58,9 +58,6 @@ __device__ __forceinline__ void load_code(const unsigned short* p, unsigned (&ou
     }
 }
 
-// LDS slot of a staged element: one pad word per 32 spreads the power-of-two
-// strides of the tile layouts (16, 256 words) over the LDS banks.
-__device__ __forceinline__ unsigned lds_slot(unsigned i) { return i + (i >> 5); }
 
 // One round of a segment: U vector loads of values and 16-bit indices per lane.
 template <typename V, int G, int U>
@@ -91,7 +88,7 @@ struct SegRound {
                 const int64_t q = base + (int64_t)u * STEP + (int64_t)gl * VE + i;
                 ok[u][i] = q >= s && q < e;
                 const unsigned idx = ok[u][i] ? (c[u][i] & mask) : 0u;
-                xv[u][i] = (A)lds[lds_slot(idx)];
+                xv[u][i] = (A)lds[idx];
             }
         A acc = 0;
 #pragma unroll
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile
                                                 const unsigned short* __restrict__ fcode, const float* __restrict__ x,
                                                 float* __restrict__ partial, const Ctrl* ctrl) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    __shared__ float xs[FT + FT / 32];
+    __shared__ __align__(16) float xs[FT];
     if (ctrl && load_stop(ctrl)) return;
     const int t = blk_tile[blockIdx.x], p0 = blk_p0[blockIdx.x], p1 = blk_p0[blockIdx.x + 1];
     const f4* xt = (const f4*)(x + (int64_t)t * FT);
@@ -167,11 +164,7 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile
 #pragma unroll
     for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
 #pragma unroll
-    for (int i = 0; i < FT / (4 * BLOCK); ++i) {
-        const unsigned i0 = 4u * (threadIdx.x + BLOCK * i);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) xs[lds_slot(i0 + j)] = tmp[i][j];
-    }
+    for (int i = 0; i < FT / (4 * BLOCK); ++i) ((f4*)xs)[threadIdx.x + BLOCK * i] = tmp[i];
     __syncthreads();
     constexpr int GPB = BLOCK / G;
     const int gl = threadIdx.x % G;
@@ -198,27 +191,175 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// Forward product as a flat stream.  Every wave owns a run of whole pieces of
-// one tile (entries [E0, E1) of the tile-sorted arrays) and walks it in rounds
-// of 512 entries, 8 consecutive entries per lane, so that every vector load is
-// useful and no load depends on data fetched in the same round.  A piece's
-// sum is formed by a segmented reduction: the start flag (bit 15 of the code)
-// splits each lane's 8 products into a head (the piece open on the left), the
-// pieces closed inside the lane, and a tail; a reverse segmented scan over the
-// lanes adds the heads of the following lanes up to the next start, and the
-// piece left open at the end of a round is carried (wave-uniform) into the
-// next round.  PF = 1 issues the loads of the next round before the current
-// one is summed.
-template <typename V, int PF>
+// v if keep, else +0 (bit mask, no branch; keeps NaN/Inf of v out of masked lanes)
+__device__ __forceinline__ float mask_if(float v, unsigned keep) {
+    return __uint_as_float(__float_as_uint(v) & (0u - keep));
+}
+__device__ __forceinline__ double mask_if(double v, unsigned keep) {
+    return keep ? v : 0.0;
+}
+
+// Segmented sums over one wave's run of whole segments, entries [E0, E1) of
+// (val, code), walked as a flat stream in rounds of 64*EPL entries, EPL
+// consecutive entries per lane, so that every vector load is useful and no
+// load depends on data fetched in the same round.  Every segment is
+// non-empty and its first code carries the start flag (bit 15); the other
+// code bits (& mask) index the LDS vector.  The flag splits each lane's
+// products into a head (the segment open on the left), the segments closed
+// inside the lane and a tail; a reverse segmented scan over the lanes adds
+// the heads of the following lanes up to the next start, and the segment left
+// open at the end of a round is carried (wave-uniform) into the next round.
+// Rounds that lie inside [E0, E1) skip all masking.  close(k, sum) is called
+// once per segment, k counted from k0.  PF = 1 issues the loads of the next
+// round before the current one is summed.
+template <typename V, int EPL, int PF, typename Close>
+__device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V* __restrict__ val,
+                                                const unsigned short* __restrict__ code, const float* lds,
+                                                unsigned mask, int k0, Close&& close) {
+    using A = typename AccOf<V>::T;
+    constexpr int VE = VecOf<V>::VE, NV = EPL / VE, NC = EPL / 8, RND = 64 * EPL;
+    static_assert(EPL == 8 || EPL == 16, "8 or 16 entries per lane");
+    const int lane = (int)(threadIdx.x & 63);
+    int64_t rb = E0 & ~(int64_t)(EPL - 1);   // first entry of the round (wave-uniform)
+    int knext = k0;                          // index of the next segment start
+    int kopen = -1;                          // segment open at the end of the last round
+    A carry = 0;
+    A v[NV][VE];
+    unsigned c[EPL];
+    auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[EPL]) {
+        q += EPL * lane;
+#pragma unroll
+        for (int h = 0; h < NV; ++h) load_vals<V, A>(val + q + VE * h, vv[h]);
+#pragma unroll
+        for (int h = 0; h < NC; ++h) {
+            unsigned t8[8];
+            load_code<8>(code + q + 8 * h, t8);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cc[8 * h + i] = t8[i];
+        }
+    };
+    load_round(rb, v, c);
+    while (rb < E1) {
+        A vn[NV][VE];
+        unsigned cn[EPL];
+        if (PF) load_round(rb + RND < E1 ? rb + RND : rb, vn, cn);
+        A pr[EPL];
+        unsigned fl = 0;
+        if (rb >= E0 && rb + RND <= E1) {
+            // interior round: every entry is valid
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) {
+                pr[i] = v[i / VE][i % VE] * (A)lds[c[i] & mask];
+                fl |= (c[i] >> 15) << i;
+            }
+        } else {
+            // edge round: entries outside [E0, E1) belong to other segments
+            // (or tiles): they read slot 0 and contribute 0
+            const int64_t base = rb + EPL * lane;
+            const int64_t dlo = E0 - base, dhi = E1 - base;
+            const int lo = dlo <= 0 ? 0 : (dlo >= EPL ? EPL : (int)dlo);
+            const int hi = dhi <= 0 ? 0 : (dhi >= EPL ? EPL : (int)dhi);
+            const unsigned ok = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+            float xv[EPL];
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) xv[i] = lds[((ok >> i) & 1u) ? (c[i] & mask) : 0u];
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) {
+                pr[i] = mask_if(v[i / VE][i % VE] * (A)xv[i], (ok >> i) & 1u);
+                fl |= (c[i] >> 15) << i;
+            }
+            fl &= ok;
+        }
+        // head: products before the lane's first start; tail: from its last start
+        const bool seen = fl != 0;
+        const int ff = seen ? __builtin_ctz(fl) : EPL;
+        const int lfp = seen ? 31 - __builtin_clz(fl) : EPL;
+        A head = 0, acc = 0;
+#pragma unroll
+        for (int i = 0; i < EPL; ++i) {
+            head += i < ff ? pr[i] : (A)0;
+            acc += i >= lfp ? pr[i] : (A)0;
+        }
+        // starts in lower lanes -> index of this lane's first start
+        const uint64_t sm = __ballot(seen);
+        const bool multi = __builtin_popcount(fl) > 1;
+        unsigned excl;
+        int total;
+        if (__ballot(multi) == 0) {
+            excl = __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
+            total = __popcll(sm);
+        } else {
+            excl = 0;
+            total = 0;
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) {
+                const uint64_t m = __ballot((fl >> i) & 1u);
+                excl = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, excl));
+                total += __popcll(m);
+            }
+        }
+        int k = knext + (int)excl;
+        if (multi) {
+            // segments that start and end inside this lane
+            A mid = 0;
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) {
+                if (i > ff && i <= lfp && ((fl >> i) & 1u)) {
+                    close(k, mid);
+                    ++k;
+                    mid = 0;
+                }
+                if (i >= ff && i < lfp) mid += pr[i];
+            }
+        }
+        // reverse segmented scan: r = head of this lane + heads of the next
+        // lanes up to and including the first lane with a start
+        A r = head;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const A rr = __shfl_down(r, d);
+            const uint64_t win = (sm >> lane) & ((1ull << d) - 1ull);
+            if (win == 0 && lane + d < 64) r += rr;
+        }
+        const A after = __shfl_down(r, 1);
+        const A cl = acc + (lane < 63 ? after : (A)0);
+        if (sm == 0) {
+            carry += __shfl(r, 0);
+        } else {
+            const int lf = 63 - __builtin_clzll(sm);
+            if (lane == 0 && kopen >= 0) close(kopen, carry + r);
+            if (seen && lane != lf) close(k, cl);
+            carry = __shfl(cl, lf);
+            kopen = __builtin_amdgcn_readlane(k, lf);
+        }
+        knext += total;
+        rb += RND;
+        if (PF) {
+#pragma unroll
+            for (int h = 0; h < NV; ++h)
+#pragma unroll
+                for (int i = 0; i < VE; ++i) v[h][i] = vn[h][i];
+#pragma unroll
+            for (int i = 0; i < EPL; ++i) c[i] = cn[i];
+        } else if (rb < E1) {
+            load_round(rb, v, c);
+        }
+    }
+    if (lane == 0 && kopen >= 0) close(kopen, carry);
+}
+
+// Forward product as a flat stream: every wave owns a run of whole pieces of
+// one tile (a block is 4 waves of one tile, which share the tile of x in
+// LDS) and writes each piece's sum to partial[piece].
+template <typename V, int EPL, int PF>
 __global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
                                                      int nwaves, int nblk, const int64_t* __restrict__ piece_ptr,
                                                      const V* __restrict__ fval, const unsigned short* __restrict__ fcode,
                                                      const float* __restrict__ x, float* __restrict__ partial,
                                                      const Ctrl* ctrl) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    using A = typename AccOf<V>::T;
-    constexpr int VE = VecOf<V>::VE, NV = 8 / VE, WPB = BLOCK / 64, RND = 512;
-    __shared__ float xs[FT + FT / 32];
+    constexpr int WPB = BLOCK / 64;
+    __shared__ __align__(16) float xs[FT];
     if (ctrl && load_stop(ctrl)) return;
     const int b = xcd_block(blockIdx.x, nblk);
     const int t = blk_tile[b];
@@ -228,109 +369,15 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk
 #pragma unroll
         for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
 #pragma unroll
-        for (int i = 0; i < FT / (4 * BLOCK); ++i) {
-            const unsigned i0 = 4u * (threadIdx.x + BLOCK * i);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) xs[lds_slot(i0 + j)] = tmp[i][j];
-        }
+        for (int i = 0; i < FT / (4 * BLOCK); ++i) ((f4*)xs)[threadIdx.x + BLOCK * i] = tmp[i];
     }
     __syncthreads();
     const int w = b * WPB + (int)(threadIdx.x >> 6);
     if (w >= nwaves) return;
-    const int lane = (int)(threadIdx.x & 63);
     const int pw0 = wave_p0[w], pw1 = wave_p0[w + 1];
     if (pw0 >= pw1) return;
-    const int64_t E0 = piece_ptr[pw0], E1 = piece_ptr[pw1];
-    int64_t base = (E0 & ~(int64_t)7) + 8 * lane;
-    const int64_t stop_at = E1 + 8 * lane;   // base - 8*lane < E1  <=>  base < stop_at
-    int knext = pw0;                         // index of the next piece start
-    int kopen = -1;                          // piece open at the end of the last round
-    A carry = 0;
-    A v[NV][VE];
-    unsigned c[8];
-    auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[8]) {
-#pragma unroll
-        for (int h = 0; h < NV; ++h) load_vals<V, A>(fval + q + VE * h, vv[h]);
-        load_code<8>(fcode + q, cc);
-    };
-    load_round(base, v, c);
-    while (base < stop_at) {
-        A vn[NV][VE];
-        unsigned cn[8];
-        if (PF) load_round(base + RND < stop_at ? base + RND : base, vn, cn);
-        // products and start flags of this lane's 8 entries
-        A pr[8];
-        unsigned fl = 0;
-        const int lo = (int)max<int64_t>(min<int64_t>(E0 - base, 8), -1);
-        const int hi = (int)max<int64_t>(min<int64_t>(E1 - base, 8), -1);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const bool ok = i >= lo && i < hi;
-            const float xv = xs[lds_slot(c[i] & 0x0fffu)];
-            pr[i] = ok ? v[i / VE][i % VE] * (A)xv : (A)0;
-            fl |= (ok && (c[i] & 0x8000u)) ? (1u << i) : 0u;
-        }
-        // starts in lower lanes -> piece index of this lane's first start
-        unsigned excl = 0;
-        int total = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const uint64_t m = __ballot((fl >> i) & 1u);
-            excl = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, excl));
-            total += __popcll(m);
-        }
-        int k = knext + (int)excl;
-        A acc = 0, head = 0;
-        const bool seen = fl != 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if ((fl >> i) & 1u) {
-                if (fl & ((1u << i) - 1u)) {
-                    partial[k] = (float)acc;
-                    ++k;
-                } else {
-                    head = acc;
-                }
-                acc = 0;
-            }
-            acc += pr[i];
-        }
-        if (!seen) head = acc;
-        // reverse segmented scan: r = head of this lane + heads of the next
-        // lanes up to and including the first lane with a start
-        const uint64_t sm = __ballot(seen);
-        A r = head;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const A rr = __shfl_down(r, d);
-            const uint64_t win = (sm >> lane) & ((1ull << d) - 1ull);
-            if (win == 0 && lane + d < 64) r += rr;
-        }
-        const A after = __shfl_down(r, 1);
-        const A close = acc + (lane < 63 ? after : (A)0);
-        if (sm == 0) {
-            carry += __shfl(r, 0);
-        } else {
-            const int lf = 63 - __builtin_clzll(sm);
-            if (lane == 0 && kopen >= 0) partial[kopen] = (float)(carry + r);
-            if (seen && lane != lf) partial[k] = (float)close;
-            carry = __shfl(close, lf);
-            kopen = __builtin_amdgcn_readlane(k, lf);
-        }
-        knext += total;
-        base += RND;
-        if (PF) {
-#pragma unroll
-            for (int h = 0; h < NV; ++h)
-#pragma unroll
-                for (int i = 0; i < VE; ++i) v[h][i] = vn[h][i];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) c[i] = cn[i];
-        } else if (base < stop_at) {
-            load_round(base, v, c);
-        }
-    }
-    if (lane == 0 && kopen >= 0) partial[kopen] = (float)carry;
+    stream_segments<V, EPL, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, 0x0fffu, pw0,
+                           [&](int k, auto sum) { partial[k] = (float)sum; });
 }
 
 // y = sum of a row's partials (8 lanes per row)
@@ -383,6 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_treduce(const int64_t* __restrict__ r
 
 // ---------------------------------------------------------------- backward columns
 // Stage the tile's rows of y (runs of consecutive rows) into LDS; returns the LDS row array.
+template <int NT = BLOCK>
 __device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, const int* __restrict__ run_ptr,
                                              const int* __restrict__ run_row, const int* __restrict__ run_off,
                                              const int* __restrict__ run_len, const float* __restrict__ y) {
@@ -390,7 +438,7 @@ __device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, co
     int* rrow = roff + maxruns + 1;         // [maxruns]
     float* ys = (float*)(rrow + maxruns);
     const int r0 = run_ptr[t], nr = run_ptr[t + 1] - r0;
-    for (int i = threadIdx.x; i < nr; i += BLOCK) {
+    for (int i = threadIdx.x; i < nr; i += NT) {
         roff[i] = run_off[r0 + i];
         rrow[i] = run_row[r0 + i];
     }
@@ -398,11 +446,11 @@ __device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, co
     __syncthreads();
     const int nrows = roff[nr];
     constexpr int UNR = 8;
-    for (int k0 = threadIdx.x; k0 < nrows; k0 += BLOCK * UNR) {
+    for (int k0 = threadIdx.x; k0 < nrows; k0 += NT * UNR) {
         float vv[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const int kk = k0 + u * BLOCK;
+            const int kk = k0 + u * NT;
             vv[u] = 0.f;
             if (kk < nrows) {
                 int lo = 0, hi = nr - 1;
@@ -415,8 +463,8 @@ __device__ __forceinline__ float* stage_runs(float* smem, int maxruns, int t, co
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const int kk = k0 + u * BLOCK;
-            if (kk < nrows) ys[lds_slot(kk)] = vv[u];
+            const int kk = k0 + u * NT;
+            if (kk < nrows) ys[kk] = vv[u];
         }
     }
     __syncthreads();
@@ -432,6 +480,7 @@ struct BwdArgs {
     const unsigned short* blr;
     int maxruns;
     int bt;                   // columns per tile
+    unsigned mask;            // local-row bits of a code (bit 15 is the column-start flag when set)
 };
 
 // MODE 0: g = A'y (FP32 out); 1: SIRT update of x; 2: line search (g in FP64, sum D g^2, lam)
@@ -459,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__
         __syncthreads();
     }
     pipelined_segments<V, G, U, DEPTH>(
-        w, a.bt, BLOCK / G, bval, a.blr, ys, 0xffffu, lane,
+        w, a.bt, BLOCK / G, bval, a.blr, ys, a.mask, lane,
         [&](int c, int64_t& s, int64_t& e) { s = a.cell_ptr[c0 + c]; e = a.cell_ptr[c0 + c + 1]; },
         [&](int c, auto acc) {
             acc = group_sum<G>(acc);
@@ -481,6 +530,105 @@ __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__
         block_reduce<1>(dsum, sh);
         store_block_partials<1>(dsum, red);
     }
+}
+
+// Backward product as a flat stream: one block of NW waves per tile.  The
+// tile's column pointers and its list of non-empty columns are staged in LDS
+// with the rows of y; the waves split the tile's entries at column boundaries
+// and sum every column with stream_segments into LDS; the epilogue (MODE as
+// in k_tbwd) then runs one thread per column.
+template <typename V, int MODE, int NW, int EPL, int PF>
+__global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __restrict__ bval,
+                                                       const float* __restrict__ y, float* __restrict__ gout,
+                                                       double* __restrict__ g64, const float* __restrict__ D,
+                                                       float* __restrict__ x, double lam, int lb_kind, float lb,
+                                                       const float* __restrict__ lbv, int ub_kind, float ub,
+                                                       const float* __restrict__ ubv, double* red, Ctrl* ctrl) {
+    constexpr int NT = NW * 64;
+    static_assert(NT >= MAX_BT, "one thread per column in the epilogue");
+    extern __shared__ float smem[];
+    __shared__ int cpl[MAX_BT + 1];          // column pointers relative to the tile's first entry
+    __shared__ short necol[MAX_BT];          // local column of the k-th non-empty column
+    __shared__ float gt[MAX_BT];             // column sums
+    __shared__ int wtot[MAX_BT / 64];
+    __shared__ double sh[1][NW];
+    if (MODE != 0 && load_stop(ctrl)) return;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t c0 = (int64_t)t * MAX_BT;
+    const int64_t Et0 = a.cell_ptr[c0];
+    const int c = (int)threadIdx.x;
+    float xj = 0.f, dj = 1.f;
+    if (c < MAX_BT) cpl[c] = (int)(a.cell_ptr[c0 + c] - Et0);
+    if (c == 0) cpl[MAX_BT] = (int)(a.cell_ptr[c0 + MAX_BT] - Et0);
+    if (c < MAX_BT) {
+        gt[c] = 0.f;
+        if (MODE == 1) {
+            xj = x[c0 + c];
+            if (D) dj = D[c0 + c];
+        }
+    }
+    const float* ys = stage_runs<NT>(smem, a.maxruns, t, a.run_ptr, a.run_row, a.run_off, a.run_len, y);
+    // cpl is visible here (stage_runs synchronizes)
+    bool ne = false;
+    uint64_t nm = 0;
+    if (c < MAX_BT) {
+        ne = cpl[c + 1] > cpl[c];
+        nm = __ballot(ne);
+        if ((c & 63) == 0) wtot[c >> 6] = __popcll(nm);
+    }
+    __syncthreads();
+    int nne = 0, before = 0;
+#pragma unroll
+    for (int u = 0; u < MAX_BT / 64; ++u) {
+        before += u < (c >> 6) ? wtot[u] : 0;
+        nne += wtot[u];
+    }
+    if (ne) necol[before + __builtin_amdgcn_mbcnt_hi((unsigned)(nm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)nm, 0u))] =
+        (short)c;
+    __syncthreads();
+    {
+        const int wv = c >> 6;
+        const int Et = cpl[MAX_BT];
+        auto start_of = [&](int k) { return k < nne ? cpl[necol[k]] : Et; };
+        auto first_at = [&](int64_t goal) {
+            int lo = 0, hi = nne;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (start_of(mid) >= goal) hi = mid; else lo = mid + 1;
+            }
+            return lo;
+        };
+        const int ka = first_at((int64_t)Et * wv / NW), kb = first_at((int64_t)Et * (wv + 1) / NW);
+        if (ka < kb)
+            stream_segments<V, EPL, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.blr, ys, a.mask, ka,
+                                   [&](int k, auto sum) { gt[necol[k]] = (float)sum; });
+    }
+    __syncthreads();
+    double dsum[1] = {0};
+    if (c < MAX_BT) {
+        const float g = gt[c];
+        const int64_t j = c0 + c;
+        if (MODE == 0) {
+            gout[j] = g;
+        } else if (MODE == 1) {
+            const double dg = D ? (double)dj * (double)g : (double)g;
+            x[j] = project((double)xj + lam * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
+        } else {
+            const double gj = (double)g;
+            g64[j] = gj;
+            dsum[0] = D ? (double)D[j] * (gj * gj) : gj * gj;
+        }
+    }
+    if (MODE == 1 && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
+    if (MODE == 2) {
+        block_reduce<1, NT>(dsum, sh);
+        store_block_partials<1>(dsum, red);
+    }
+}
+
+__global__ void k_flag_col_starts(const int64_t* __restrict__ cell_ptr, int64_t ncols, unsigned short* __restrict__ blr) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x)
+        if (cell_ptr[j] < cell_ptr[j + 1]) blr[cell_ptr[j]] |= (unsigned short)0x8000u;
 }
 
 // ---------------------------------------------------------------- layout build kernels
@@ -897,8 +1045,14 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
         T.staged_rows = staged;
         T.maxruns = maxruns;
         T.maxrows = maxrows;
-        T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows + (size_t)maxrows / 32 + 1);
+        T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
         if (T.lds_b > MAX_LDS_B || maxrows > 65535) return false;
+        T.bwd_flags = maxrows <= 32768 && T.bt_size == MAX_BT && T.ntiles_b * MAX_BT == T.n_int;
+        if (T.bwd_flags) {
+            k_flag_col_starts<<<grid_for(T.n_int), BLOCK, 0, s>>>(T.cell_ptr.as<int64_t>(), T.n_int,
+                                                                 T.blr.as<unsigned short>());
+            AIR_HIP(hipGetLastError());
+        }
         T.run_ptr.alloc(4 * rp.size());
         AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
         AIR_HIP(hipStreamSynchronize(s));
@@ -919,6 +1073,7 @@ static BwdArgs bwd_args(const TiledOp& T) {
     a.blr = T.blr.as<unsigned short>();
     a.maxruns = T.maxruns;
     a.bt = T.bt_size;
+    a.mask = T.bwd_flags ? 0x7fffu : 0xffffu;
     return a;
 }
 
@@ -927,6 +1082,8 @@ static BwdArgs bwd_args(const TiledOp& T) {
 static const int FWD_CFG[][3] = {{16, 2, 1}, {16, 2, 2}, {16, 1, 2}, {8, 2, 2}, {32, 1, 2}};
 static const int BWD_CFG[][3] = {{64, 2, 1}, {64, 2, 2}, {64, 1, 2}, {32, 2, 2}};
 static constexpr int N_FWD_CFG = 5, N_BWD_CFG = 4;
+static constexpr int N_FWD_STREAM = 4;     // stream forward: 8/16 entries per lane, without/with prefetch
+static constexpr int N_BWD_STREAM = 3;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch
 template <int A, int B, int C> using Shape3 = std::integer_sequence<int, A, B, C>;
 
 template <typename F>
@@ -962,8 +1119,12 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                                                 T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
                                                 T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
             };
-            if (cfg == N_FWD_CFG) launch(k_tfwd_flat<V, 0>);
-            else launch(k_tfwd_flat<V, 1>);
+            switch (cfg - N_FWD_CFG) {
+            case 0: launch(k_tfwd_flat<V, 8, 0>); break;
+            case 1: launch(k_tfwd_flat<V, 8, 1>); break;
+            case 2: launch(k_tfwd_flat<V, 16, 0>); break;
+            default: launch(k_tfwd_flat<V, 16, 1>); break;
+            }
             return 0;
         }
         dispatch_fwd(cfg, [&](auto shp) {
@@ -982,18 +1143,30 @@ bool tiled_fwd_stream(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG; }
 template <int MODE>
 static void launch_bwd(const TiledOp& T, const float* y, float* gout, const SirtBufs* B, double lam, hipStream_t s,
                        int cfg = -1) {
+    if (cfg < 0) cfg = T.bwd_cfg;
     dispatch_dtype(T.dtype, [&](auto v) {
         using V = decltype(v);
-        dispatch_bwd(cfg < 0 ? T.bwd_cfg : cfg, [&](auto shp) {
-            constexpr int G = sh0(decltype(shp){}), U = sh1(decltype(shp){}), DP = sh2(decltype(shp){});
+        const BwdArgs a = bwd_args(T);
+        auto launch = [&](auto kern, int nt) {
             if (MODE == 0)
-                k_tbwd<V, 0, G, U, DP><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
-                    bwd_args(T), T.bval.as<V>(), y, gout, nullptr, nullptr, nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f,
-                    nullptr, nullptr, nullptr);
+                kern<<<(unsigned)T.ntiles_b, nt, T.lds_b, s>>>(a, T.bval.as<V>(), y, gout, nullptr, nullptr, nullptr,
+                                                               0.0, 0, 0.f, nullptr, 0, 0.f, nullptr, nullptr, nullptr);
             else
-                k_tbwd<V, MODE, G, U, DP><<<(unsigned)T.ntiles_b, BLOCK, T.lds_b, s>>>(
-                    bwd_args(T), T.bval.as<V>(), y, nullptr, B->g, B->D, B->x, lam, B->lb_kind, B->lb, B->lbv,
-                    B->ub_kind, B->ub, B->ubv, B->partial_cols, B->ctrl);
+                kern<<<(unsigned)T.ntiles_b, nt, T.lds_b, s>>>(a, T.bval.as<V>(), y, nullptr, B->g, B->D, B->x, lam,
+                                                               B->lb_kind, B->lb, B->lbv, B->ub_kind, B->ub, B->ubv,
+                                                               B->partial_cols, B->ctrl);
+        };
+        if (cfg >= N_BWD_CFG) {
+            switch (cfg - N_BWD_CFG) {
+            case 0: launch(k_tbwd_flat<V, MODE, 8, 8, 1>, 512); break;
+            case 1: launch(k_tbwd_flat<V, MODE, 8, 16, 0>, 512); break;
+            default: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
+            }
+            return 0;
+        }
+        dispatch_bwd(cfg, [&](auto shp) {
+            constexpr int G = sh0(decltype(shp){}), U = sh1(decltype(shp){}), DP = sh2(decltype(shp){});
+            launch(k_tbwd<V, MODE, G, U, DP>, BLOCK);
         });
         return 0;
     });
@@ -1044,11 +1217,14 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
 
 #include <utility>
 void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
-    // stream form: 64 lanes per wave, 8 entries per lane (prefetch flag in fu)
-    *fl = T.fwd_cfg >= N_FWD_CFG ? 64 : FWD_CFG[T.fwd_cfg][0];
-    *fu = T.fwd_cfg >= N_FWD_CFG ? T.fwd_cfg - N_FWD_CFG : FWD_CFG[T.fwd_cfg][1];
-    *bl = BWD_CFG[T.bwd_cfg][0];
-    *bu = BWD_CFG[T.bwd_cfg][1];
+    // stream forms: entries per lane and the prefetch flag
+    static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}};
+    static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}};
+    const int fs = T.fwd_cfg - N_FWD_CFG, bs = T.bwd_cfg - N_BWD_CFG;
+    *fl = fs >= 0 ? FS[fs][0] : FWD_CFG[T.fwd_cfg][0];
+    *fu = fs >= 0 ? FS[fs][1] : FWD_CFG[T.fwd_cfg][1];
+    *bl = bs >= 0 ? BS[bs][0] : BWD_CFG[T.bwd_cfg][0];
+    *bu = bs >= 0 ? BS[bs][1] : BWD_CFG[T.bwd_cfg][1];
 }
 
 // Time every kernel shape on this matrix (a few launches each, outside any
@@ -1059,8 +1235,8 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         int f = 0, b = 0, par = 0;
         const int got = std::sscanf(fixed, "%d,%d,%d", &f, &b, &par);
         if (got >= 2) {
-            T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + 1);
-            T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG - 1);
+            T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + N_FWD_STREAM - 1);
+            T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0) - 1);
             if (got == 3 && T.fwd_cfg < N_FWD_CFG) set_fwd_blocks(T, std::min(std::max(par, 64), 4096), s);
             if (got == 3 && T.fwd_cfg >= N_FWD_CFG) set_fwd_waves(T, std::min(std::max(par, 512), 1 << 20), s);
             return;
@@ -1097,21 +1273,21 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         }
     }
     set_fwd_blocks(T, best_ppb, s);
-    // stream form: prefetch off/on at 2048/4096/8192 entries per wave
+    // stream forms at 2048/4096/8192 entries per wave
     static const int WT[] = {2048, 4096, 8192};
     int best_wt = T.wave_target;
     for (int wi = 0; wi < 3; ++wi) {
         set_fwd_waves(T, WT[wi], s);
-        for (int pf = 0; pf < 2; ++pf) {
-            const int c = N_FWD_CFG + pf;
+        for (int sv = 0; sv < N_FWD_STREAM; ++sv) {
+            const int c = N_FWD_CFG + sv;
             float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
-            T.tune_fwd_ms[3 * N_FWD_CFG + 3 * pf + wi] = t / 4;
+            T.tune_fwd_ms[3 * N_FWD_CFG + 3 * sv + wi] = t / 4;
             if (t < best) { best = t; T.fwd_cfg = c; best_wt = WT[wi]; }
         }
     }
     set_fwd_waves(T, best_wt, s);
     best = 1e30f;
-    for (int c = 0; c < N_BWD_CFG; ++c) {
+    for (int c = 0; c < N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0); ++c) {
         float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });
         T.tune_bwd_ms[c] = t / 4;
         if (t < best) { best = t; T.bwd_cfg = c; }

@@ -204,7 +204,7 @@ def main():
                             "fwd_stream": L["fwd_stream"],
                             "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
                                        L["bwd_loads"]],
-                            "tune_fwd_us": L["tune_fwd_us"][:21], "tune_bwd_us": L["tune_bwd_us"]}
+                            "tune_fwd_us": L["tune_fwd_us"][:27], "tune_bwd_us": L["tune_bwd_us"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
                      "rho": info["rho"]}

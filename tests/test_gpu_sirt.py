@@ -201,10 +201,13 @@ def test_train_relaxpar_matches_oracle(oracle):
 
 # Every forward/backward kernel shape of the tiled layout (the autotuner picks
 # one per matrix): (forward shape, backward shape, pieces per block or entries
-# per wave).  Shapes 0-4 are per-piece lane groups, 5-6 the stream form
-# without/with prefetch.  N=96 gives 4 forward tiles with padded columns.
+# per wave).  Forward shapes 0-4 are per-piece lane groups, 5-8 the stream
+# form (8 or 16 entries per lane, without/with prefetch); backward shapes 0-3
+# are per-column lane groups, 4-6 the stream form (8 entries with prefetch,
+# 16 without, 16 with).  N=96 gives 4 forward tiles with padded columns.
 @pytest.mark.parametrize("cfg", ["0,0,256", "1,1,512", "2,2,1024", "3,3,256", "4,0,512",
-                                 "5,1,512", "6,2,2048", "5,3,8192", "6,0,100000"])
+                                 "5,1,512", "6,2,2048", "7,3,8192", "8,0,100000",
+                                 "5,4,2048", "7,5,4096", "0,6,256", "8,6,512"])
 def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
     monkeypatch.setenv("AIR_TILED_CFG", cfg)
@@ -212,6 +215,9 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     L = A.layout()
     f, bcfg, par = (int(v) for v in cfg.split(","))
     assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
+    assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16}.get(bcfg, L["bwd_lanes"])
+    if f >= 5:
+        assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1)][f - 5]
     assert L["fwd_pieces_per_block"] == (max(par, 512) if f >= 5 else par)
     Ao = oracle.paralleltomo(N, theta, p)
     x = oracle.phantomgallery("threephases", N, None, 3).ravel(order="F")
@@ -224,7 +230,7 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
-@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048"])
+@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512"])
 def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
     """Per-piece and stream forward kernels agree on every storage type."""
     opts = {"rho": c1["rho"]["sart"], "lbound": 0}
