@@ -64,26 +64,33 @@ def canonical_bytes(nnz, m, n, s_v):
     return 2 * nnz * (s_v + 4) + 4 * (m + 1) + 4 * (n + 1) + 4 * (4 * n + 4 * m)
 
 
-def cpu_baseline(N, theta, p, b, method, rho, samples, threads):
+def cpu_baseline(N, theta, p, b, method, rho, seconds, threads):
     """The FP64 CPU oracle (a C restatement of sirt.m, OpenMP) on the same
-    problem, timed for a bounded number of iterations on the host cores."""
+    problem, timed over a bounded number of iterations (about `seconds` of
+    CPU work) on the host cores."""
     from oracle import oracle as O
     t0 = time.perf_counter()
     A = O.paralleltomo(N, theta, p)
     t_trace = time.perf_counter() - t0
     opts = {"rho": rho, "lbound": 0, "ubound": 1}
-    O.sirt(method, A, b, 1, None, opts, nthreads=threads)            # warm caches
-    t0 = time.perf_counter()
-    O.sirt(method, A, b, samples, None, opts, nthreads=threads)
-    dt = time.perf_counter() - t0
-    # subtract the per-call setup (weights, transpose) measured with a 1-iteration call
-    t1 = time.perf_counter()
-    O.sirt(method, A, b, 1, None, opts, nthreads=threads)
-    d1 = time.perf_counter() - t1
-    per_iter = max(dt - d1, 1e-9) / max(samples - 1, 1)
+
+    def timed(k):
+        t = time.perf_counter()
+        O.sirt(method, A, b, k, None, opts, nthreads=threads)
+        return time.perf_counter() - t
+
+    timed(1)                                   # warm caches
+    # per-call setup (weights, transpose) is measured by a 1-iteration call and subtracted
+    d1 = timed(1)
+    probe = timed(6)
+    per_iter = max(probe - d1, 1e-9) / 5
+    samples = int(min(max(seconds / per_iter, 6), 20000))
+    dt = timed(samples + 1) - d1
+    per_iter = max(dt, 1e-9) / samples
     return {"value": 1.0 / per_iter, "unit": "iter/s", "cores": threads, "kind": "port",
-            "sample": f"{samples} {method} iterations of the FP64 oracle (CSR A.x, CSC A'.y, OpenMP) on the same "
-                      f"{A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}; oracle tracer {t_trace:.1f}s not included"}
+            "sample": f"{samples} {method} iterations of the FP64 oracle (CSR A.x, CSC A'.y, OpenMP, "
+                      f"{threads} threads) on the same {A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}, "
+                      f"{dt:.1f} s; oracle tracer {t_trace:.1f} s and per-call setup not included"}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -124,7 +131,7 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--matrix-free", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-samples", type=int, default=12)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -210,7 +217,7 @@ def main():
                      "rho": info["rho"]}
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(N, theta, p, b, args.method, info["rho"], args.cpu_samples, threads)
+        out["cpu_baseline"] = cpu_baseline(N, theta, p, b, args.method, info["rho"], args.cpu_seconds, threads)
         out["vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
