@@ -213,6 +213,13 @@ def main():
                                        L["bwd_loads"]],
                             "tune_fwd_us": L["tune_fwd_us"][:27], "tune_bwd_us": L["tune_bwd_us"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
+    # the whole C-ABI call as a host caller sees it: b and x0 from host memory,
+    # weights, r0, the K iterations and X back over PCIe (rho given)
+    t0 = time.perf_counter()
+    air.sirt(args.method, A, b, args.steps, None, dict(opts, rho=info["rho"]))
+    e2e = time.perf_counter() - t0
+    out["end_to_end"] = {"iter_per_s": round(args.steps / e2e, 1), "seconds": round(e2e, 4),
+                         "what": "one air_sirt call: host b in, weights and r0, K iterations, x_K out"}
     out["result"] = {"rel_err_to_phantom": round(rel_err, 5), "resnorm_last": float(info["resnorm"][-1]),
                      "rho": info["rho"]}
     if not args.no_cpu_baseline:
