@@ -154,9 +154,10 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double (*sh)[NT / 
     }
 }
 
-// Stop rules of auxil/check_stoprules.m on the device (thread 0 of the last block).
+// Stop rules of auxil/check_stoprules.m on the device (thread 0 of the finalize
+// block).  ncp_sum is the sum over residual columns of the NCP statistic.
 __device__ void evaluate_stop(Ctrl* ctrl, const StopParams sp, int k, double rr, double rMr, double me_num,
-                              double rkm1sq, double* hist) {
+                              double rkm1sq, double ncp_sum, double* hist) {
     double nrk = sqrt(rr);
     hist[k] = nrk;
     ctrl->rMr = rMr;
@@ -172,7 +173,21 @@ __device__ void evaluate_stop(Ctrl* ctrl, const StopParams sp, int k, double rr,
         else if (k >= sp.kmax) { stop = 1; code = 0; }
         break;
     }
-    default:            // NONE :153-161 (NCP is evaluated on the host)
+    case AIR_STOP_NCP: { // :96-151; dk starts as Inf (check_inputs.m:207-209), max ignores NaN
+        const double ncc = ncp_sum / (double)sp.ncp_cols;
+        double mx_old = -INFINITY, mx_new = ncc;
+        for (int q = 0; q < sp.ncp_smooth; ++q) mx_old = fmax(mx_old, sp.ncp_dk[q]);
+        for (int q = 1; q < sp.ncp_smooth; ++q) mx_new = fmax(mx_new, sp.ncp_dk[q]);
+        if (mx_old < mx_new || k >= sp.kmax) {
+            stop = 1;
+            code = (k != sp.kmax) ? 1 : 0;
+        } else {
+            for (int q = 0; q + 1 < sp.ncp_smooth; ++q) sp.ncp_dk[q] = sp.ncp_dk[q + 1];
+            sp.ncp_dk[sp.ncp_smooth - 1] = ncc;
+        }
+        break;
+    }
+    default:            // NONE :153-161
         if (k >= sp.kmax) { stop = 1; code = 0; }
         break;
     }

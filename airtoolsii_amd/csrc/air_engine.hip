@@ -35,6 +35,7 @@ struct SirtState {
     int64_t m = 0, n = 0;
     bool hasM = false, hasD = false;
     DevBuf b, x, r, Mr, M, D, g, lbv, ubv, partial, hist, ctrl, Ax;
+    std::unique_ptr<NcpState> ncp;   // NCP stop rule (FFT plan and buffers), kept across runs
     SirtBufs bufs{};
     double setup_s = 0, iter_s = 0;
 };
@@ -479,8 +480,14 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     need_operator(e);
     AIR_CHECK(o != nullptr && b != nullptr, AIR_E_ARG, "Too few input arguments.");
     AIR_CHECK(kmax >= 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
-    AIR_CHECK(o->stoprule != AIR_STOP_NCP, AIR_E_UNSUPPORTED, "NCP stopping rule is not available on the device");
     AIR_CHECK(o->stoprule >= 0 && o->stoprule <= 3, AIR_E_ARG, "The chosen stopping rule is not valid.");
+    if (o->stoprule == AIR_STOP_NCP) {
+        // check_stoprules.m:99-120: res_dims = [len, ncols] (2-D) or [m, 0] (1-D)
+        const int64_t len = o->res_dims[0], cols = o->res_dims[1] > 0 ? o->res_dims[1] : 1;
+        AIR_CHECK(len > 0 && len * cols == e->m, AIR_E_ARG,
+                  "options.stoprule.res_dims must give the dimensions of the residual (prod(res_dims) == m).");
+        AIR_CHECK(o->ncp_smooth >= 1 && o->ncp_smooth <= 4096, AIR_E_ARG, "ncp_smooth must be a positive integer.");
+    }
     if (o->stoprule == AIR_STOP_DP || o->stoprule == AIR_STOP_ME)
         AIR_CHECK(!std::isnan(o->taudelta), AIR_E_ARG,
                   "For stopping rules DP and ME a scalar taudelta must be given in options.");
@@ -563,9 +570,20 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     B.partial_cols = st.partial.as<double>() + row_slots;
     B.hist = st.hist.as<double>();
     B.ctrl = st.ctrl.as<Ctrl>();
+    st.sp = StopParams{};
     st.sp.type = o->stoprule;
     st.sp.kmax = kmax;
     st.sp.taudelta = o->taudelta;
+    if (o->stoprule == AIR_STOP_NCP) {
+        if (!st.ncp) st.ncp.reset(new NcpState());
+        const int len = o->res_dims[0], cols = o->res_dims[1] > 0 ? o->res_dims[1] : 1;
+        ncp_prepare(*st.ncp, m, len, cols, o->ncp_smooth, s);
+        st.sp.ncp_stat = st.ncp->colstat.as<double>();
+        st.sp.ncp_dk = st.ncp->dk.as<double>();
+        st.sp.ncp_cols = cols;
+        st.sp.ncp_smooth = o->ncp_smooth;
+        B.ncp = st.ncp.get();
+    }
     st.kmax = kmax;
     st.k = 0;
     // r0 = b - A x0 and the k = 0 check (sirt.m:153-162)

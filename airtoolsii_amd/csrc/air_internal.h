@@ -1,6 +1,7 @@
 // air_internal.h -- shared declarations of the MI355X SIRT engine (not part of the C ABI).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 #include <stdint.h>
 #include <string>
 #include <vector>
@@ -239,6 +240,28 @@ struct StopParams {
     int type;            // AIR_STOP_*
     int kmax;
     double taudelta;
+    // NCP (check_stoprules.m:96-151): per-column statistics and the filter window
+    const double* ncp_stat;
+    double* ncp_dk;
+    int ncp_cols;
+    int ncp_smooth;
+};
+
+// NCP state of one SIRT run: a batched real FFT of the residual columns.
+struct NcpState {
+    int64_t m = 0;
+    int len = 0, ncols = 0, smooth = 2;
+    hipfftHandle plan = 0;
+    bool planned = false;
+    DevBuf rd, spec, colstat, dk;
+    NcpState() = default;
+    NcpState(const NcpState&) = delete;
+    NcpState& operator=(const NcpState&) = delete;
+    ~NcpState() { release(); }
+    void release() {
+        if (planned) (void)hipfftDestroy(plan);
+        planned = false;
+    }
 };
 
 struct SirtBufs {
@@ -257,6 +280,7 @@ struct SirtBufs {
     double* partial_cols;  // per-block partial sums of the column pass (1 per block)
     double* hist;        // ||r_k||, k = 0..kmax
     Ctrl* ctrl;
+    NcpState* ncp = nullptr;  // NCP stop rule state (null otherwise)
 };
 
 // r = b - A x; Mr = M r; FP64 reductions and the stop check of iteration k.
@@ -271,6 +295,9 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
 void launch_apply_line(int64_t n, const SirtBufs& B, hipStream_t s);
 // single-block reductions of the per-block partials (stop rule / line-search lambda)
 void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s);
+// NCP: FFT of the residual columns and their statistics (before the finalize)
+void ncp_prepare(NcpState& N, int64_t m, int len, int ncols, int smooth, hipStream_t s);
+void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s);
 void launch_finalize_line(const SirtBufs& B, int nblocks, hipStream_t s);
 unsigned tiled_reduce_grid(int64_t m);
 // matrix-free iteration pieces

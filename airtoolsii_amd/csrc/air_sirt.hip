@@ -217,7 +217,52 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
         for (int q = 0; q < 4; ++q) v[q] += a0[q];
     }
     block_reduce<4>(v, sh);
-    if (threadIdx.x == 0) evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], hist);
+    double c[1] = {0};
+    if (sp.ncp_cols > 0) {
+        __shared__ double shc[1][BLOCK / 64];
+        for (int j = threadIdx.x; j < sp.ncp_cols; j += BLOCK) c[0] += sp.ncp_stat[j];
+        block_reduce<1>(c, shc);
+    }
+    if (threadIdx.x == 0) evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], c[0], hist);
+}
+
+// NCP statistic of one residual column (check_stoprules.m:107-133): from the
+// half spectrum S[0..len/2] of the column, pk = |S[f]|^2 for f = 1..q
+// (q = floor(len/2)), c = cumsum(pk) / sum(pk), and the column's value is
+// the 2-norm of c - (1:q)/q.  One block per column; each thread owns a
+// contiguous run of frequencies, so the cumulative sum is a block scan.
+__global__ __launch_bounds__(BLOCK) void k_ncp_stats(const double2* __restrict__ spec, int len,
+                                                     double* __restrict__ colstat, const Ctrl* ctrl) {
+    __shared__ double sc[BLOCK];
+    __shared__ double sh[1][BLOCK / 64];
+    if (load_stop(ctrl)) return;
+    const int q = len / 2;
+    const double2* S = spec + (int64_t)blockIdx.x * (len / 2 + 1);
+    const int chunk = (q + BLOCK - 1) / BLOCK;
+    const int f0 = 1 + (int)threadIdx.x * chunk, f1 = min(q + 1, f0 + chunk);
+    double part = 0;
+    for (int f = f0; f < f1; ++f) part += S[f].x * S[f].x + S[f].y * S[f].y;
+    sc[threadIdx.x] = part;
+    __syncthreads();
+    for (int d = 1; d < BLOCK; d <<= 1) {       // inclusive scan of the thread sums
+        const double add = threadIdx.x >= (unsigned)d ? sc[threadIdx.x - d] : 0.0;
+        __syncthreads();
+        sc[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const double total = sc[BLOCK - 1];
+    double cum = sc[threadIdx.x] - part, ss[1] = {0};
+    for (int f = f0; f < f1; ++f) {
+        cum += S[f].x * S[f].x + S[f].y * S[f].y;
+        const double dv = cum / total - (double)f / (double)q;
+        ss[0] += dv * dv;
+    }
+    block_reduce<1>(ss, sh);
+    if (threadIdx.x == 0) colstat[blockIdx.x] = sqrt(ss[0]);
+}
+
+__global__ void k_fill_f64(double* p, int n, double v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 // line search: lam = r'Mr / sum_j D_j g_j^2 from the column pass's block partials
@@ -321,7 +366,39 @@ void launch_resid(const DevSparse& A, Shape sh, const SirtBufs& B, StopParams sp
     launch_finalize_rows(B, sh.blocks, sp, k, s);
 }
 
+void ncp_prepare(NcpState& N, int64_t m, int len, int ncols, int smooth, hipStream_t s) {
+    if (!N.planned || N.len != len || N.ncols != ncols) {
+        N.release();
+        int n[1] = {len}, inembed[1] = {len}, onembed[1] = {len / 2 + 1};
+        AIR_CHECK(hipfftPlanMany(&N.plan, 1, n, inembed, 1, len, onembed, 1, len / 2 + 1, HIPFFT_D2Z, ncols) ==
+                      HIPFFT_SUCCESS, AIR_E_HIP, "hipfftPlanMany failed for the NCP residual transform");
+        N.planned = true;
+        N.len = len;
+        N.ncols = ncols;
+    }
+    AIR_CHECK(hipfftSetStream(N.plan, s) == HIPFFT_SUCCESS, AIR_E_HIP, "hipfftSetStream failed");
+    N.smooth = smooth;
+    N.m = m;
+    N.rd.alloc(8 * (size_t)std::max<int64_t>(m, 1));
+    N.spec.alloc(16 * (size_t)ncols * (len / 2 + 1));
+    N.colstat.alloc(8 * (size_t)ncols);
+    N.dk.alloc(8 * (size_t)smooth);
+    k_fill_f64<<<1, 64, 0, s>>>(N.dk.as<double>(), smooth, INFINITY);   // check_inputs.m:209
+    AIR_HIP(hipGetLastError());
+}
+
+void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s) {
+    (void)ctrl;
+    k_f32_to_f64<<<(unsigned)std::min<int64_t>((m + BLOCK - 1) / BLOCK, 4096), BLOCK, 0, s>>>(r, N.rd.as<double>(), m);
+    AIR_HIP(hipGetLastError());
+    AIR_CHECK(hipfftExecD2Z(N.plan, N.rd.as<double>(), (hipfftDoubleComplex*)N.spec.p) == HIPFFT_SUCCESS, AIR_E_HIP,
+              "hipfftExecD2Z failed");
+    k_ncp_stats<<<(unsigned)N.ncols, BLOCK, 0, s>>>((const double2*)N.spec.p, N.len, N.colstat.as<double>(), ctrl);
+    AIR_HIP(hipGetLastError());
+}
+
 void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s) {
+    if (sp.type == AIR_STOP_NCP && B.ncp) ncp_enqueue(*B.ncp, B.r, B.ncp->m, B.ctrl, s);
     k_finalize_rows<<<1, BLOCK, 0, s>>>(B.partial, nblocks, B.hist, B.ctrl, sp, k);
     AIR_HIP(hipGetLastError());
 }

@@ -146,7 +146,14 @@ def _build_options(A, sirt_method, options, kmax):
     if stype == "NCP":
         if "res_dims" not in st:
             raise ValueError("options.stoprule.res_dims must be specified for NCP stopping rule.")
-        raise NotImplementedError("the NCP stopping rule is not available in this engine yet")
+        rd = np.atleast_1d(np.asarray(st["res_dims"])).astype(np.int64).ravel()
+        if rd.size == 1:            # 1-D residual: one column of length m (check_stoprules.m:99-105)
+            o.res_dims[0], o.res_dims[1] = int(m), 0
+        elif rd.size == 2:
+            o.res_dims[0], o.res_dims[1] = int(rd[0]), int(rd[1])
+        else:
+            raise ValueError("NCP stopping rule only implemented for 1D and 2D residuals, the size of which "
+                             "should be given using options.stoprule.res_dims.")
     o.ncp_smooth = int(st.get("ncp_smooth", 2))
     o.verbose = int(opts.get("verbose", 0) or 0)
     return o, keep, code, Mc, Dc, name

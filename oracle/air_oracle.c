@@ -612,7 +612,7 @@ static double vnorm(const double* v, int64_t m)
 }
 
 /* NCP statistic, check_stoprules.m:96-133 (direct DFT of each column) */
-static double ncp_stat(const double* rk, int64_t m, const int* res_dims)
+double ora_ncp_stat(const double* rk, int64_t m, const int* res_dims)
 {
     int p = res_dims[1] > 0 ? res_dims[0] : (int)m;
     int ncols = res_dims[1] > 0 ? res_dims[1] : 1;
@@ -675,7 +675,7 @@ static int check_stop(const ora_sirt_opts* o, const double* rk, int64_t m, doubl
         return 0;
     }
     case 1: { /* NCP :96-151 */
-        double ncc = ncp_stat(rk, m, o->res_dims);
+        double ncc = ora_ncp_stat(rk, m, o->res_dims);
         double mx_old = -INFINITY, mx_new = ncc;
         for (int q = 0; q < ss->ndk; ++q) if (ss->dk[q] > mx_old) mx_old = ss->dk[q];
         for (int q = 1; q < ss->ndk; ++q) if (ss->dk[q] > mx_new) mx_new = ss->dk[q];

@@ -60,6 +60,8 @@ int ora_weights(int method, int64_t m, int64_t n,
 
 /* ---- calczeta, auxil/calc_relaxpar.m:179-244 */
 double ora_calczeta(int k);
+/* NCP statistic of a residual, check_stoprules.m:96-133: res_dims = {p, ncols} or {m, 0} */
+double ora_ncp_stat(const double* rk, int64_t m, const int* res_dims);
 
 /* ---- products */
 void ora_spmv(int64_t m, const int64_t* rowptr, const int32_t* col, const double* val,

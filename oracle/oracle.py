@@ -115,6 +115,17 @@ def calczeta(k):
     return np.array([lib().ora_calczeta(int(v)) for v in k])
 
 
+def ncp_stat(rk, res_dims):
+    """NCP statistic ncc of a residual, check_stoprules.m:96-133 (res_dims as in options.stoprule)."""
+    rk = np.ascontiguousarray(np.ravel(rk), np.float64)
+    rd = list(np.atleast_1d(res_dims))
+    dims = (C.c_int * 2)(int(rd[0]) if len(rd) > 1 else rk.size, int(rd[1]) if len(rd) > 1 else 0)
+    f = lib().ora_ncp_stat
+    f.restype = C.c_double
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    return f(rk.ctypes.data, rk.size, dims)
+
+
 def mt_rand(seed, count):
     st = (C.c_uint32 * 625)()
     lib().ora_mt_seed(C.byref(st), int(seed))

@@ -241,3 +241,24 @@ def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
         X, _, _ = air.sart(A, c1["b"], 20, None, opts)
         out.append(X[:, -1])
     assert rel(out[1], out[0]) <= (1e-12 if dtype == "f64" else 1e-5)
+
+
+@pytest.mark.parametrize("method,smooth,dims", [("cimmino", 2, "2d"), ("sart", 1, "2d"), ("cimmino", 4, "1d")])
+def test_ncp_stopping_rule(c1, oracle, method, smooth, dims):
+    """NCP on the device (hipFFT of the residual columns, check_stoprules.m:96-151)
+    stops at the oracle's iteration with the oracle's iterate."""
+    res_dims = [c1["p"], len(c1["theta"])] if dims == "2d" else [c1["A"].shape[0]]
+    opts = {"rho": c1["rho"][method], "lbound": 0,
+            "stoprule": {"type": "NCP", "res_dims": res_dims, "ncp_smooth": smooth}}
+    X, info, _ = air.sirt(method, c1["A"], c1["b"], [5, 200], None, opts)
+    Xo, info_o, _ = oracle.sirt(method, c1["Ao"], c1["b"], [5, 200], None, opts)
+    print(method, smooth, dims, info["finaliter"], info_o["finaliter"])
+    assert info["stoprule"] == info_o["stoprule"] == 1
+    assert info["finaliter"] == info_o["finaliter"] < 200
+    assert list(info["itersaved"]) == list(info_o["itersaved"])
+    assert rel(X[:, -1], Xo[:, -1]) <= TOL
+
+
+def test_ncp_rejects_bad_res_dims(c1):
+    with pytest.raises(Exception):
+        air.cimmino(c1["A"], c1["b"], 5, None, {"stoprule": {"type": "NCP", "res_dims": [90, 180]}})

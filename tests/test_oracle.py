@@ -442,3 +442,65 @@ def test_product_phantoms_match_oracle(oracle):
             assert np.mean(a != b) < 1e-3
         else:
             np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+
+
+# ------------------------------------------------------------------ NCP statistic
+def _ncp_numpy(r, res_dims):
+    """check_stoprules.m:96-133 with numpy's FFT (an independent transform)."""
+    r = np.ravel(r)
+    if len(res_dims) == 1:
+        R = r[:, None]
+    else:
+        R = r.reshape(res_dims[1], res_dims[0]).T          # column-major reshape(rk, res_dims)
+    q = R.shape[0] // 2
+    PK = np.abs(np.fft.fft(R, axis=0)[: q + 1]) ** 2
+    c = np.cumsum(PK[1:], axis=0) / PK[1:].sum(axis=0)
+    cw = (np.arange(1, q + 1) / q)[:, None]
+    return float(np.mean(np.sqrt(((c - cw) ** 2).sum(axis=0))))
+
+
+@pytest.mark.parametrize("dims", [[91, 7], [64, 5], [725, 3], [1000]])
+def test_ncp_statistic_matches_fft(oracle, dims):
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(int(np.prod(dims))) + np.linspace(0, 3, int(np.prod(dims)))
+    assert abs(oracle.ncp_stat(r, dims) - _ncp_numpy(r, dims)) <= 1e-12
+
+
+@pytest.mark.parametrize("p,f0", [(91, 5), (64, 1), (64, 32), (101, 50)])
+def test_ncp_statistic_single_frequency_closed_form(oracle, p, f0):
+    """A pure cosine at frequency f0 puts all of pk(1:q) at f0, so c = [f >= f0]
+    and ncc = sqrt(sum_{f<f0} (f/q)^2 + sum_{f>=f0} (1 - f/q)^2)."""
+    t = np.arange(p)
+    r = np.cos(2 * np.pi * f0 * t / p)
+    q = p // 2
+    f = np.arange(1, q + 1)
+    expect = math.sqrt(np.sum(np.where(f < f0, (f / q) ** 2, (1 - f / q) ** 2)))
+    assert abs(oracle.ncp_stat(np.tile(r, 3), [p, 3]) - expect) <= 1e-9
+
+
+def test_ncp_stop_index_follows_window_rule(oracle):
+    """The oracle's NCP stop (check_stoprules.m:96-151, dk = Inf(ncp_smooth,1)) against
+    the rule applied to numpy-FFT statistics of the residuals of the same run."""
+    N, theta, p = 24, np.arange(0.0, 180.0, 6.0), 35
+    A = oracle.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = A @ x
+    b = b + 0.05 * np.linalg.norm(b) * np.random.default_rng(3).standard_normal(b.size) / np.sqrt(b.size)
+    rho = oracle.spectral_radius(A, *oracle.weights("cimmino", A))
+    kmax = 60
+    for smooth in (1, 2, 4):
+        st = {"type": "NCP", "res_dims": [p, len(theta)], "ncp_smooth": smooth}
+        X, info, _ = oracle.sirt("cimmino", A, b, kmax, None, {"rho": rho, "stoprule": st})
+        Xall, _, _ = oracle.sirt("cimmino", A, b, list(range(1, kmax + 1)), None, {"rho": rho})
+        dk = [math.inf] * smooth
+        expect = kmax
+        for k in range(0, kmax + 1):
+            xk = np.zeros(A.shape[1]) if k == 0 else Xall[:, k - 1]
+            ncc = _ncp_numpy(b - A @ xk, [p, len(theta)])
+            if max(dk) < max(dk[1:] + [ncc]) or k >= kmax:
+                expect = k
+                break
+            dk = dk[1:] + [ncc]
+        assert info["finaliter"] == expect, (smooth, info["finaliter"], expect)
+        assert info["stoprule"] == (1 if expect < kmax else 0)
+        assert expect < kmax
