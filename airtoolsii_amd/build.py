@@ -30,6 +30,7 @@ SOURCES = {
     "air_sort.hip": [],
     "air_sirt.hip": [],
     "air_tiled.hip": [],
+    "air_mlp.hip": [],
     "air_engine.hip": [],
 }
 

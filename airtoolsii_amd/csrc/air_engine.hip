@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -1012,4 +1013,107 @@ int air_layout_info(air_engine* e, air_layout* out) {
     });
 }
 
+
+// ---- learned relaxation network
+int air_mlp_forward(air_engine* e, const float* params, const float* X, int64_t n, float* out) {
+    return guard([&] {
+        AIR_CHECK(e && params && X && out && n >= 0, AIR_E_ARG, "invalid MLP arguments");
+        use_device(e);
+        hipStream_t s = e->stream;
+        DevBuf W, Xd, O;
+        W.alloc(4 * (size_t)MLP_PARAMS);
+        Xd.alloc(4 * (size_t)std::max<int64_t>(n, 1) * MLP_IN);
+        O.alloc(4 * (size_t)std::max<int64_t>(n, 1) * MLP_OUT);
+        AIR_HIP(hipMemcpyAsync(W.p, params, 4 * (size_t)MLP_PARAMS, hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(Xd.p, X, 4 * (size_t)n * MLP_IN, hipMemcpyHostToDevice, s));
+        mlp_forward(W.as<float>(), Xd.as<float>(), n, O.as<float>(), s);
+        AIR_HIP(hipMemcpyAsync(out, O.p, 4 * (size_t)n * MLP_OUT, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int air_mlp_grad(air_engine* e, const float* params, const float* X, const float* T, int64_t n,
+                 double stop_weight, float* grad, double* loss) {
+    return guard([&] {
+        AIR_CHECK(e && params && X && T && grad && n > 0 && n < (int64_t)1 << 31, AIR_E_ARG,
+                  "invalid MLP arguments");
+        use_device(e);
+        hipStream_t s = e->stream;
+        const unsigned nblk = mlp_grad_blocks(n);
+        DevBuf W, Xd, Td, part, lp, G;
+        W.alloc(4 * (size_t)MLP_PARAMS);
+        Xd.alloc(4 * (size_t)n * MLP_IN);
+        Td.alloc(4 * (size_t)n * MLP_OUT);
+        part.alloc(4 * (size_t)nblk * MLP_PARAMS);
+        lp.alloc(8 * (size_t)nblk);
+        G.alloc(4 * (size_t)MLP_PARAMS);
+        AIR_HIP(hipMemcpyAsync(W.p, params, 4 * (size_t)MLP_PARAMS, hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(Xd.p, X, 4 * (size_t)n * MLP_IN, hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(Td.p, T, 4 * (size_t)n * MLP_OUT, hipMemcpyHostToDevice, s));
+        mlp_grad(W.as<float>(), Xd.as<float>(), Td.as<float>(), nullptr, n, (float)stop_weight, part.as<float>(),
+                 lp.as<double>(), G.as<float>(), s);
+        std::vector<double> l(nblk);
+        AIR_HIP(hipMemcpyAsync(grad, G.p, 4 * (size_t)MLP_PARAMS, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipMemcpyAsync(l.data(), lp.p, 8 * (size_t)nblk, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        if (loss) {
+            double t = 0;
+            for (double v : l) t += v;
+            *loss = t / (double)n;
+        }
+    });
+}
+
+int air_mlp_train(air_engine* e, float* params, const float* X, const float* T, int64_t n, int epochs,
+                  int batch, double lr, double stop_weight, uint64_t seed, double* epoch_loss) {
+    return guard([&] {
+        AIR_CHECK(e && params && X && T && n > 0 && n < (int64_t)1 << 31 && epochs >= 0 && batch > 0, AIR_E_ARG,
+                  "invalid MLP training arguments");
+        use_device(e);
+        hipStream_t s = e->stream;
+        const int64_t nbat = (n + batch - 1) / batch;
+        const unsigned maxblk = mlp_grad_blocks(batch);
+        DevBuf W, M1, M2, Xd, Td, idx, part, lp, G;
+        W.alloc(4 * (size_t)MLP_PARAMS);
+        M1.alloc(4 * (size_t)MLP_PARAMS);
+        M2.alloc(4 * (size_t)MLP_PARAMS);
+        Xd.alloc(4 * (size_t)n * MLP_IN);
+        Td.alloc(4 * (size_t)n * MLP_OUT);
+        idx.alloc(4 * (size_t)n);
+        part.alloc(4 * (size_t)maxblk * MLP_PARAMS);
+        lp.alloc(8 * (size_t)nbat * maxblk);
+        G.alloc(4 * (size_t)MLP_PARAMS);
+        AIR_HIP(hipMemcpyAsync(W.p, params, 4 * (size_t)MLP_PARAMS, hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemsetAsync(M1.p, 0, 4 * (size_t)MLP_PARAMS, s));
+        AIR_HIP(hipMemsetAsync(M2.p, 0, 4 * (size_t)MLP_PARAMS, s));
+        AIR_HIP(hipMemcpyAsync(Xd.p, X, 4 * (size_t)n * MLP_IN, hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(Td.p, T, 4 * (size_t)n * MLP_OUT, hipMemcpyHostToDevice, s));
+        std::vector<int> perm(n);
+        std::vector<double> l((size_t)nbat * maxblk);
+        std::mt19937_64 rng(seed);
+        int step = 0;
+        for (int ep = 0; ep < epochs; ++ep) {
+            for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+            std::shuffle(perm.begin(), perm.end(), rng);
+            AIR_HIP(hipMemcpyAsync(idx.p, perm.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
+            AIR_HIP(hipMemsetAsync(lp.p, 0, 8 * (size_t)nbat * maxblk, s));
+            for (int64_t bi = 0; bi < nbat; ++bi) {
+                const int64_t off = bi * batch, nb = std::min<int64_t>(batch, n - off);
+                mlp_grad(W.as<float>(), Xd.as<float>(), Td.as<float>(), idx.as<int>() + off, nb, (float)stop_weight,
+                         part.as<float>(), lp.as<double>() + bi * maxblk, G.as<float>(), s);
+                mlp_adam(W.as<float>(), M1.as<float>(), M2.as<float>(), G.as<float>(), (float)lr, ++step, s);
+            }
+            AIR_HIP(hipMemcpyAsync(l.data(), lp.p, 8 * l.size(), hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+            if (epoch_loss) {
+                // each minibatch's loss partials sum to nb * (its mean loss)
+                double t = 0;
+                for (double v : l) t += v;
+                epoch_loss[ep] = t / (double)n;
+            }
+        }
+        AIR_HIP(hipMemcpyAsync(params, W.p, 4 * (size_t)MLP_PARAMS, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+    });
+}
 }  // extern "C"

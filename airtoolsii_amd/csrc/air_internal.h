@@ -213,6 +213,15 @@ void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStr
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);
 void permute_to_user(const TiledOp& T, const float* x_int, float* x_user, hipStream_t s);
 
+// ---------------------------------------------------------------- learned relaxation (air_mlp.hip)
+constexpr int MLP_IN = AIR_MLP_IN, MLP_HID = AIR_MLP_HID, MLP_OUT = AIR_MLP_OUT, MLP_PARAMS = AIR_MLP_PARAMS;
+void mlp_forward(const float* W, const float* X, int64_t n, float* out, hipStream_t s);
+unsigned mlp_grad_blocks(int64_t nb);
+// part: mlp_grad_blocks(nb) x MLP_PARAMS floats; loss_part: mlp_grad_blocks(nb) doubles
+void mlp_grad(const float* W, const float* X, const float* T, const int* idx, int64_t nb, float stop_w,
+              float* part, double* loss_part, float* grad, hipStream_t s);
+void mlp_adam(float* W, float* m1, float* m2, const float* grad, float lr, int step, hipStream_t s);
+
 // ---------------------------------------------------------------- kernels (air_sirt.hip)
 // Per-launch shape of the compressed-row kernels: G lanes per row, grid blocks.
 struct Shape {

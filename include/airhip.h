@@ -185,6 +185,29 @@ typedef struct {
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 
+/* ---- learned relaxation network ---------------------------------------
+ * No counterpart in the reference: the learned lambda_k / stop rule that
+ * replaces the fixed relaxation of calc_relaxpar.m:105-171 and the stop
+ * rules of check_stoprules.m when AIR_RELAX_LEARNED is chosen.  A 16-64-64-2
+ * ReLU perceptron; parameters are AIR_MLP_PARAMS floats laid out as
+ * W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (row = output).
+ * Outputs (o0, o1): lambda_k * rho = 2 sigmoid(o0); o1 is the stop logit. */
+#define AIR_MLP_IN     16
+#define AIR_MLP_HID    64
+#define AIR_MLP_OUT    2
+#define AIR_MLP_PARAMS 5378
+/* raw outputs of n samples: X is n x 16 row-major, out n x 2 */
+int air_mlp_forward(air_engine* e, const float* params, const float* X, int64_t n, float* out);
+/* mean loss over n samples and its gradient (BF16 MFMA, FP32 accumulation).
+ * Targets T (n x 2): t0 = lambda* rho in (0, 2) (t0 < 0 drops the term), t1 in {0, 1}.
+ * Loss = mean[(2 sigmoid(o0) - t0)^2 + stop_weight * BCE(o1, t1)]. */
+int air_mlp_grad(air_engine* e, const float* params, const float* X, const float* T, int64_t n,
+                 double stop_weight, float* grad, double* loss);
+/* Adam training in place: `epochs` passes over the n samples in seeded random
+ * minibatches of `batch`; epoch_loss (optional, epochs values) gets the mean loss. */
+int air_mlp_train(air_engine* e, float* params, const float* X, const float* T, int64_t n, int epochs,
+                  int batch, double lr, double stop_weight, uint64_t seed, double* epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -504,3 +504,28 @@ def test_ncp_stop_index_follows_window_rule(oracle):
         assert info["finaliter"] == expect, (smooth, info["finaliter"], expect)
         assert info["stoprule"] == (1 if expect < kmax else 0)
         assert expect < kmax
+
+
+# ------------------------------------------------------------------ learned relaxation network
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The FP64 network reference: analytic gradient against central differences."""
+    from oracle import mlp
+    rng = np.random.default_rng(4)
+    p = mlp.init_params(1).astype(np.float64) + 0.01 * rng.standard_normal(mlp.NPARAM)
+    X = rng.standard_normal((37, mlp.NI))
+    T = np.stack([rng.uniform(0.1, 1.9, 37), (rng.uniform(size=37) > 0.5).astype(float)], 1)
+    T[::5, 0] = -1                                   # dropped lambda terms
+    L, g = mlp.loss_grad(p, X, T, stop_w=0.7)
+    h = 1e-6
+    for j in list(rng.choice(mlp.NPARAM, 60, replace=False)) + [1024, 5376, 5377]:
+        e = np.zeros(mlp.NPARAM); e[j] = h
+        fd = (mlp.loss_grad(p + e, X, T, 0.7)[0] - mlp.loss_grad(p - e, X, T, 0.7)[0]) / (2 * h)
+        assert abs(fd - g[j]) <= 1e-6 * max(1.0, abs(fd)), (j, fd, g[j])
+
+
+def test_mlp_bf16_rounding():
+    from oracle import mlp
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5e-3, 0.0, 3.0e38])
+    r = mlp.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -7      # ties to even
+    assert abs(r[3] - v[3]) <= 2 ** -9 * abs(v[3]) and r[4] == 0.0
