@@ -19,8 +19,10 @@ AIR_OK = 0
 AIR_E_ARG, AIR_E_NOMEM, AIR_E_HIP, AIR_E_UNSUPPORTED, AIR_E_STATE, AIR_E_GEOMETRY = -1, -2, -3, -4, -5, -6
 DTYPES = {"f32": 0, "float32": 0, "bf16": 1, "bfloat16": 1, "f16": 2, "float16": 2, "f64": 3, "float64": 3}
 METHODS = {"landweber": 0, "cimmino": 1, "cav": 2, "drop": 3, "sart": 4, "custom": 5}
-RELAX = {"default": 0, "scalar": 1, "line": 2, "psi1": 3, "psi1mod": 4, "psi2": 5, "psi2mod": 6, "vector": 7}
+RELAX = {"default": 0, "scalar": 1, "line": 2, "psi1": 3, "psi1mod": 4, "psi2": 5, "psi2mod": 6, "vector": 7,
+         "learned": 8, "greedy": 9}
 STOP = {"NONE": 0, "NCP": 1, "DP": 2, "ME": 3}
+STOP_LEARNED = 4
 BOUND_NONE, BOUND_SCALAR, BOUND_VECTOR = 0, 1, 2
 
 # every symbol include/airhip.h declares
@@ -28,7 +30,7 @@ EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "ai
            "air_set_matrix_csc", "air_build_paralleltomo", "air_build_fanlineartomo", "air_size", "air_get_csr",
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
-           "air_layout_info", "air_mlp_forward", "air_mlp_grad", "air_mlp_train"]
+           "air_layout_info", "air_mlp_forward", "air_mlp_grad", "air_mlp_train", "air_sirt_records"]
 
 
 class AirError(RuntimeError):
@@ -44,7 +46,8 @@ class SirtOptions(C.Structure):
                 ("lb_kind", C.c_int), ("lb", C.c_double), ("lb_vec", C.c_void_p),
                 ("ub_kind", C.c_int), ("ub", C.c_double), ("ub_vec", C.c_void_p),
                 ("stoprule", C.c_int), ("taudelta", C.c_double), ("res_dims", C.c_int * 2),
-                ("ncp_smooth", C.c_int), ("verbose", C.c_int)]
+                ("ncp_smooth", C.c_int), ("verbose", C.c_int),
+                ("mlp_params", C.c_void_p), ("noise_norm", C.c_double), ("x_true", C.c_void_p)]
 
 
 class Layout(C.Structure):
@@ -102,6 +105,7 @@ def load():
     L.air_get_kernel_timing.argtypes = [_P, _dp, _dp, _dp, C.POINTER(_i64)]
     L.air_layout_info.argtypes = [_P, C.POINTER(Layout)]
     L.air_mlp_forward.argtypes = [_P, _P, _P, _i64, _P]
+    L.air_sirt_records.argtypes = [_P, _P, _P, _P, _P]
     L.air_mlp_grad.argtypes = [_P, _P, _P, _P, _i64, C.c_double, _P, _dp]
     L.air_mlp_train.argtypes = [_P, _P, _P, _P, _i64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _P]
     for name in EXPORTS:

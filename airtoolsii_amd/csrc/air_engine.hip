@@ -37,6 +37,8 @@ struct SirtState {
     bool hasM = false, hasD = false;
     DevBuf b, x, r, Mr, M, D, g, lbv, ubv, partial, hist, ctrl, Ax;
     std::unique_ptr<NcpState> ncp;   // NCP stop rule (FFT plan and buffers), kept across runs
+    LearnBufs learn;                 // learned / greedy relaxation
+    DevBuf l_mlp, l_xstar, l_state, l_gpart, l_feat, l_target, l_out, l_err;
     SirtBufs bufs{};
     double setup_s = 0, iter_s = 0;
 };
@@ -54,6 +56,12 @@ struct air_engine {
     air::Geometry geo;
     air::TiledOp T;            // LDS-tiled layout used by the SIRT loop when ready
     air::SirtState st;
+    // weights and spectral radius of the built-in methods, kept until the operator changes
+    std::vector<double> wM[AIR_SART + 1], wD[AIR_SART + 1];
+    bool wvalid[AIR_SART + 1] = {};
+    air::DevBuf wMdev[AIR_SART + 1], wDdev[AIR_SART + 1];   // FP32 copies in the loop's order
+    bool wdev[AIR_SART + 1] = {};
+    double wrho[AIR_SART + 1] = {NAN, NAN, NAN, NAN, NAN};
     // per-kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -134,6 +142,15 @@ void reset_operator(air_engine* e) {
     e->kind = 0;
     e->m = e->n = 0;
     e->st.prepared = false;
+    for (int q = 0; q <= AIR_SART; ++q) {
+        e->wvalid[q] = false;
+        e->wdev[q] = false;
+        e->wMdev[q].release();
+        e->wDdev[q].release();
+        e->wrho[q] = NAN;
+        e->wM[q].clear();
+        e->wD[q].clear();
+    }
 }
 
 bool tiled_disabled() {
@@ -226,7 +243,20 @@ void operator_stats(air_engine* e, int method, Stats& S) {
 inline double inf_to_zero(double v) { return std::isinf(v) ? 0.0 : v; }
 
 // get_mfun_dfun.m matrix-branch formulas; identity is returned as all ones.
+void compute_weights_uncached(air_engine* e, int method, std::vector<double>& M, std::vector<double>& D);
+
+// get_mfun_dfun.m weights depend on A alone: computed once per method and operator
 void compute_weights(air_engine* e, int method, std::vector<double>& M, std::vector<double>& D) {
+    AIR_CHECK(method >= AIR_LANDWEBER && method <= AIR_SART, AIR_E_ARG, "SIRT method not defined.");
+    if (!e->wvalid[method]) {
+        compute_weights_uncached(e, method, e->wM[method], e->wD[method]);
+        e->wvalid[method] = true;
+    }
+    M = e->wM[method];
+    D = e->wD[method];
+}
+
+void compute_weights_uncached(air_engine* e, int method, std::vector<double>& M, std::vector<double>& D) {
     const int64_t m = e->m, n = e->n;
     M.assign(m, 1.0);
     D.assign(n, 1.0);
@@ -382,6 +412,9 @@ void relax_schedule(int kind, double relaxpar, const double* vec, double rho, in
     }
 }
 
+// relaxation chosen on the device each iteration (line search, network, greedy target)
+bool device_lambda(int kind) { return kind == AIR_RELAX_LINE || kind == AIR_RELAX_GREEDY; }
+
 static constexpr int EV_PER_ITER = 4;
 
 void timing_begin(air_engine* e, int niter) {
@@ -409,9 +442,13 @@ hipEvent_t mid_event(air_engine* e) {
 void enqueue_iteration(air_engine* e, int k) {
     SirtState& st = e->st;
     hipStream_t s = e->stream;
-    const bool line = st.relax_kind == AIR_RELAX_LINE;
+    const bool line = device_lambda(st.relax_kind);
     double lam = line ? 0.0 : st.lam[k - 1];
     record(e, 0);
+    if (st.relax_kind == AIR_RELAX_LEARNED) {
+        launch_learned_select(st.bufs, k, s);
+        lam = NAN;                  // the update kernels read the network's lambda_k
+    }
     if (use_tiled(e)) {
         launch_tiled_update(e->T, st.bufs, lam, line ? 1 : 0, k, s);
         record(e, 1);
@@ -482,6 +519,12 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     AIR_CHECK(o != nullptr && b != nullptr, AIR_E_ARG, "Too few input arguments.");
     AIR_CHECK(kmax >= 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
     AIR_CHECK(o->stoprule >= 0 && o->stoprule <= 3, AIR_E_ARG, "The chosen stopping rule is not valid.");
+    AIR_CHECK(o->relax_kind >= AIR_RELAX_DEFAULT && o->relax_kind <= AIR_RELAX_GREEDY, AIR_E_ARG,
+              "The chosen relaxation strategy is not valid.");
+    AIR_CHECK(o->relax_kind != AIR_RELAX_LEARNED || o->mlp_params, AIR_E_ARG,
+              "the learned relaxation needs the network parameters (mlp_params)");
+    AIR_CHECK(o->relax_kind != AIR_RELAX_GREEDY || o->x_true, AIR_E_ARG,
+              "the greedy relaxation needs the exact solution (x_true)");
     if (o->stoprule == AIR_STOP_NCP) {
         // check_stoprules.m:99-120: res_dims = [len, ncols] (2-D) or [m, 0] (1-D)
         const int64_t len = o->res_dims[0], cols = o->res_dims[1] > 0 ? o->res_dims[1] : 1;
@@ -518,19 +561,41 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     double rho = o->rho;
     if (o->method == AIR_SART) rho = 1.0;
     // calc_relaxpar.m:91-103 computes rho whenever it is not given
-    if (std::isnan(rho)) rho = lanczos_rho(e, M, D);
+    if (std::isnan(rho)) {
+        if (o->method != AIR_CUSTOM && !std::isnan(e->wrho[o->method])) {
+            rho = e->wrho[o->method];                 // same operator and weights as an earlier call
+        } else {
+            rho = lanczos_rho(e, M, D);
+            if (o->method != AIR_CUSTOM) e->wrho[o->method] = rho;
+        }
+    }
     st.rho = rho;
     st.relax_kind = o->relax_kind;
     relax_schedule(o->relax_kind, o->relaxpar, o->relax_vec, rho, kmax, st.lam);
     // device vectors
     upload_f32(st.b, b, m, s);
     if (x0) upload_xlike(e, st.x, x0, s);
-    else {
-        std::vector<double> z(n, 0.0);
-        upload_xlike(e, st.x, z.data(), s);
+    else {                                            // x0 = 0 (check_inputs.m:93)
+        st.x.alloc(4 * (size_t)std::max<int64_t>(xlen(e), 1));
+        AIR_HIP(hipMemsetAsync(st.x.p, 0, 4 * (size_t)std::max<int64_t>(xlen(e), 1), s));
     }
-    if (st.hasM) upload_f32(st.M, M.data(), m, s);
-    if (st.hasD) upload_xlike(e, st.D, D.data(), s);
+    const float *dMp = nullptr, *dDp = nullptr;
+    if (o->method == AIR_CUSTOM) {
+        if (st.hasM) upload_f32(st.M, M.data(), m, s);
+        if (st.hasD) upload_xlike(e, st.D, D.data(), s);
+        dMp = st.hasM ? st.M.as<float>() : nullptr;
+        dDp = st.hasD ? st.D.as<float>() : nullptr;
+    } else {
+        // device copies of a built-in method's weights are kept with the operator
+        const int q = o->method;
+        if (!e->wdev[q]) {
+            if (st.hasM) upload_f32(e->wMdev[q], M.data(), m, s);
+            if (st.hasD) upload_xlike(e, e->wDdev[q], D.data(), s);
+            e->wdev[q] = true;
+        }
+        dMp = st.hasM ? e->wMdev[q].as<float>() : nullptr;
+        dDp = st.hasD ? e->wDdev[q].as<float>() : nullptr;
+    }
     const int64_t nx = xlen(e);
     st.r.alloc(4 * std::max<int64_t>(m, 1));
     st.Mr.alloc(4 * std::max<int64_t>(m, 1));
@@ -547,8 +612,8 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     SirtBufs& B = st.bufs;
     B = SirtBufs{};
     B.b = st.b.as<float>();
-    B.M = st.hasM ? st.M.as<float>() : nullptr;
-    B.D = st.hasD ? st.D.as<float>() : nullptr;
+    B.M = dMp;
+    B.D = dDp;
     B.x = st.x.as<float>();
     B.r = st.r.as<float>();
     B.Mr = st.Mr.as<float>();
@@ -571,6 +636,40 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     B.partial_cols = st.partial.as<double>() + row_slots;
     B.hist = st.hist.as<double>();
     B.ctrl = st.ctrl.as<Ctrl>();
+    if (o->relax_kind == AIR_RELAX_LEARNED || o->relax_kind == AIR_RELAX_GREEDY) {
+        LearnBufs& L = st.learn;
+        L = LearnBufs{};
+        L.mode = o->relax_kind;
+        L.kmax = kmax;
+        L.nx = nx;
+        L.rho = (float)rho;
+        L.noise = std::isnan(o->noise_norm) ? 0.0 : o->noise_norm;
+        const size_t kk = (size_t)std::max(kmax, 1);
+        st.l_state.alloc(8 * LEARN_STATE);
+        st.l_feat.alloc(4 * kk * LEARN_NF);
+        st.l_target.alloc(4 * kk);
+        st.l_out.alloc(4 * kk * 2);
+        st.l_err.alloc(8 * kk);
+        AIR_HIP(hipMemsetAsync(st.l_state.p, 0, 8 * LEARN_STATE, s));
+        L.state = st.l_state.as<double>();
+        L.feat = st.l_feat.as<float>();
+        L.target = st.l_target.as<float>();
+        L.out = st.l_out.as<float>();
+        L.err = st.l_err.as<double>();
+        if (o->mlp_params) {       // learned run, or a greedy run along the network's path
+            st.l_mlp.alloc(4 * (size_t)MLP_PARAMS);
+            AIR_HIP(hipMemcpyAsync(st.l_mlp.p, o->mlp_params, 4 * (size_t)MLP_PARAMS, hipMemcpyHostToDevice, s));
+            L.mlp = st.l_mlp.as<float>();
+        }
+        if (o->x_true) {   // greedy target, or the error record of a learned run
+            upload_xlike(e, st.l_xstar, o->x_true, s);
+            L.xstar = st.l_xstar.as<float>();
+            L.gblocks = (int)std::min<int64_t>((nx + 255) / 256, 1024);   // 256-thread blocks
+            st.l_gpart.alloc(8 * 3 * (size_t)std::max(L.gblocks, 1));
+            L.gpart = st.l_gpart.as<double>();
+        }
+        B.learn = &st.learn;
+    }
     st.sp = StopParams{};
     st.sp.type = o->stoprule;
     st.sp.kmax = kmax;
@@ -894,14 +993,39 @@ int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info*
         if (info) {
             info->stoprule = c.stop ? c.stoprule : 0;
             info->finaliter = fin;
-            info->relaxpar = st.relax_kind == AIR_RELAX_LINE ? c.lam_cur
-                                                              : (fin > 0 ? st.lam[fin - 1] : NAN);
+            info->relaxpar = (device_lambda(st.relax_kind) || st.relax_kind == AIR_RELAX_LEARNED)
+                                 ? c.lam_cur : (fin > 0 ? st.lam[fin - 1] : NAN);
             info->rho = st.rho;
             info->nsaved = 0;
             info->setup_time = st.setup_s;
             info->iter_time = st.iter_s;
             info->timetaken = st.setup_s + st.iter_s;
         }
+    });
+}
+
+int air_sirt_records(air_engine* e, float* feat, float* target, float* out, double* err) {
+    return guard([&] {
+        AIR_CHECK(e != nullptr, AIR_E_ARG, "null engine");
+        SirtState& st = e->st;
+        AIR_CHECK(st.prepared && st.bufs.learn, AIR_E_STATE,
+                  "no learned or greedy relaxation run to read records from");
+        use_device(e);
+        hipStream_t s = e->stream;
+        Ctrl c;
+        AIR_HIP(hipMemcpyAsync(&c, st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        const size_t rows = (size_t)(c.stop ? c.finaliter : st.k);
+        // a run stopped by the network at state k-1 has computed row k-1 as well
+        const size_t have = std::min<size_t>(rows + (c.stop && c.stoprule == AIR_STOP_LEARNED ? 1 : 0),
+                                             (size_t)st.learn.kmax);
+        if (have) {
+            if (feat) AIR_HIP(hipMemcpyAsync(feat, st.l_feat.p, 4 * have * LEARN_NF, hipMemcpyDeviceToHost, s));
+            if (target) AIR_HIP(hipMemcpyAsync(target, st.l_target.p, 4 * have, hipMemcpyDeviceToHost, s));
+            if (out) AIR_HIP(hipMemcpyAsync(out, st.l_out.p, 8 * have, hipMemcpyDeviceToHost, s));
+            if (err) AIR_HIP(hipMemcpyAsync(err, st.l_err.p, 8 * have, hipMemcpyDeviceToHost, s));
+        }
+        AIR_HIP(hipStreamSynchronize(s));
     });
 }
 

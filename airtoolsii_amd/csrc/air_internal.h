@@ -215,6 +215,7 @@ void permute_to_user(const TiledOp& T, const float* x_int, float* x_user, hipStr
 
 // ---------------------------------------------------------------- learned relaxation (air_mlp.hip)
 constexpr int MLP_IN = AIR_MLP_IN, MLP_HID = AIR_MLP_HID, MLP_OUT = AIR_MLP_OUT, MLP_PARAMS = AIR_MLP_PARAMS;
+constexpr int MLP_LAMBDA_MAX = AIR_MLP_LAMBDA_MAX;
 void mlp_forward(const float* W, const float* X, int64_t n, float* out, hipStream_t s);
 unsigned mlp_grad_blocks(int64_t nb);
 // part: mlp_grad_blocks(nb) x MLP_PARAMS floats; loss_part: mlp_grad_blocks(nb) doubles
@@ -273,6 +274,26 @@ struct NcpState {
     }
 };
 
+// Device state of the learned / greedy relaxation (AIR_RELAX_LEARNED, AIR_RELAX_GREEDY).
+constexpr int LEARN_NF = 16;          // features per iteration
+constexpr int LEARN_STATE = 8;        // doubles of state carried between iterations
+struct LearnBufs {
+    int mode = 0;
+    int kmax = 0;
+    int64_t nx = 0;
+    float rho = 1.f;
+    double noise = 0.0;
+    const float* mlp = nullptr;       // learned: FP32 parameters
+    const float* xstar = nullptr;     // greedy: the exact solution (loop column order)
+    double* state = nullptr;
+    double* gpart = nullptr;          // greedy: 3 partial sums per block
+    int gblocks = 0;
+    float* feat = nullptr;            // kmax x LEARN_NF
+    float* target = nullptr;          // kmax
+    float* out = nullptr;             // kmax x 2
+    double* err = nullptr;            // kmax
+};
+
 struct SirtBufs {
     const float* b;
     const float* M;      // null = identity
@@ -290,6 +311,7 @@ struct SirtBufs {
     double* hist;        // ||r_k||, k = 0..kmax
     Ctrl* ctrl;
     NcpState* ncp = nullptr;  // NCP stop rule state (null otherwise)
+    const LearnBufs* learn = nullptr;  // learned / greedy relaxation (null otherwise)
 };
 
 // r = b - A x; Mr = M r; FP64 reductions and the stop check of iteration k.
@@ -307,7 +329,11 @@ void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, 
 // NCP: FFT of the residual columns and their statistics (before the finalize)
 void ncp_prepare(NcpState& N, int64_t m, int len, int ncols, int smooth, hipStream_t s);
 void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s);
-void launch_finalize_line(const SirtBufs& B, int nblocks, hipStream_t s);
+// lambda of a line-search style iteration k: r'Mr / sum D g^2, or the learned / greedy choice
+void launch_finalize_line(const SirtBufs& B, int nblocks, int k, hipStream_t s);
+// learned relaxation: the network's lambda_k (read by the update kernels when
+// they are given lam = NaN) and stop, from the residual statistics of x_{k-1}
+void launch_learned_select(const SirtBufs& B, int k, hipStream_t s);
 unsigned tiled_reduce_grid(int64_t m);
 // matrix-free iteration pieces
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s);

@@ -104,6 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const int64_t* __restrict__ pt
                                                   int lb_kind, float lb, const float* __restrict__ lbv, int ub_kind,
                                                   float ub, const float* __restrict__ ubv, Ctrl* ctrl) {
     if (load_stop(ctrl)) return;
+    if (isnan(lam)) lam = ctrl->lam_cur;     // NaN: chosen on the device this iteration
     const int gl = threadIdx.x % G;
     for_each_row<G>(ptr, n, [&](int64_t j, int64_t s, int64_t e) {
         auto acc = row_dot<V, G>(idx, val, Mr, s, e, gl);
@@ -147,13 +148,13 @@ __global__ __launch_bounds__(BLOCK) void k_apply(int64_t n, const double* __rest
                                                  float lb, const float* __restrict__ lbv, int ub_kind, float ub,
                                                  const float* __restrict__ ubv, Ctrl* ctrl) {
     if (load_stop(ctrl)) return;
-    double l = use_ctrl_lam ? ctrl->lam_cur : lam;
+    double l = (use_ctrl_lam || isnan(lam)) ? ctrl->lam_cur : lam;
     for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < n; j += (int64_t)gridDim.x * BLOCK) {
         double gj = (double)(float)g[j];
         double dg = D ? (double)D[j] * gj : gj;
         x[j] = project((double)x[j] + l * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
     }
-    if (!use_ctrl_lam && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
+    if (!use_ctrl_lam && !isnan(lam) && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
 }
 
 // matrix-free line search: sum_j D_j g_j^2 over the FP64 accumulators
@@ -403,7 +404,186 @@ void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, 
     AIR_HIP(hipGetLastError());
 }
 
-void launch_finalize_line(const SirtBufs& B, int nblocks, hipStream_t s) {
+// greedy target: partial sums of <d, x* - x>, <d, d> and ||x - x*||^2 with d = D g (the step direction)
+__global__ __launch_bounds__(BLOCK) void k_greedy_dots(int64_t n, const double* __restrict__ g,
+                                                       const float* __restrict__ D, const float* __restrict__ x,
+                                                       const float* __restrict__ xs, double* __restrict__ part,
+                                                       const Ctrl* ctrl) {
+    __shared__ double sh[3][BLOCK / 64];
+    if (load_stop(ctrl)) return;
+    double v[3] = {0, 0, 0};
+    for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < n; j += (int64_t)gridDim.x * BLOCK) {
+        const double gj = g ? (double)(float)g[j] : 0.0;
+        const double d = D ? (double)D[j] * gj : gj;
+        const double e = (double)xs[j] - (double)x[j];
+        v[0] += d * e;
+        v[1] += d * d;
+        v[2] += e * e;
+    }
+    block_reduce<3>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[(int64_t)blockIdx.x * 3 + q] = v[q];
+}
+
+// Relaxation of iteration k (one wave): features of the state x_{k-1}, then
+// lambda_k from the greedy target or the network, and the network's stop.
+// The features use only residual statistics, so a learned lambda_k is known
+// before the column pass and the update stays fused into it:
+//  0 log(|r|/|r0|)   1 log(|r_prev|/|r0|)   2 log(|r|/|r_prev|)   3 log(r'Mr / r0'Mr0)
+//  4 log(r'Mr / r_prev'Mr_prev)   5 lam_prev rho   6 log(|r_prev|/|r_prev2|)   7 k/100
+//  8 log(1+k)/5   9 log(|r|/|e|)   10 log(|r_prev|/|e|)   11 [|r| > |e|]
+//  (9-11 zero when |e| is unknown)   12 lam_prev2 rho   13 feature 4 of the last iteration
+//  14 log(r'Mr / |r|^2) - log(r0'Mr0 / |r0|^2)   15 [k == 1]
+__global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, LearnBufs L, int k) {
+    __shared__ float f[LEARN_NF];
+    __shared__ float h1[MLP_HID], h2[MLP_HID];
+    __shared__ float Ws[MLP_PARAMS];
+    __shared__ double sums[4];
+    if (load_stop(ctrl)) return;
+    const int t = threadIdx.x;
+    // the network's parameters, staged with coalesced loads while wave 0 works
+    if (L.mlp)
+        for (int i = t; i < MLP_PARAMS; i += 256) Ws[i] = L.mlp[i];
+    // wave 0 reduces the greedy sums and computes the features
+    double b0 = 0, b1 = 0, b2 = 0;
+    if (L.xstar && t < 64)
+        for (int i = t; i < L.gblocks; i += 64) {
+            b0 += L.gpart[3 * i];
+            b1 += L.gpart[3 * i + 1];
+            b2 += L.gpart[3 * i + 2];
+        }
+    for (int d = 32; d > 0; d >>= 1) {
+        b0 += __shfl_xor(b0, d);
+        b1 += __shfl_xor(b1, d);
+        b2 += __shfl_xor(b2, d);
+    }
+    if (t == 0) {
+        // state: 0 |r0|  1 r0'Mr0  2 r_prev'Mr_prev  3 unused  4 f4 prev  5 lam_prev  6 lam_prev2
+        double* S = L.state;
+        const double nr = hist[k - 1], nr1 = k >= 2 ? hist[k - 2] : nr, nr2 = k >= 3 ? hist[k - 3] : nr1;
+        const double rMr = ctrl->rMr;
+        if (k == 1) {
+            for (int q = 0; q < LEARN_STATE; ++q) S[q] = 0;
+            S[0] = nr;
+            S[1] = rMr;
+            S[2] = rMr;
+        }
+        auto lg = [](double v) { return v > 0 ? log(v) : -30.0; };
+        const double rho = (double)L.rho;
+        double fv[LEARN_NF];
+        fv[0] = lg(nr / S[0]);
+        fv[1] = lg(nr1 / S[0]);
+        fv[2] = lg(nr / nr1);
+        fv[3] = lg(rMr / S[1]);
+        fv[4] = lg(rMr / S[2]);
+        fv[5] = S[5] * rho;
+        fv[6] = lg(nr1 / nr2);
+        fv[7] = (k - 1) / 100.0;
+        fv[8] = log1p((double)(k - 1)) / 5.0;
+        const bool hasn = L.noise > 0;
+        fv[9] = hasn ? lg(nr / L.noise) : 0.0;
+        fv[10] = hasn ? lg(nr1 / L.noise) : 0.0;
+        fv[11] = hasn ? (nr > L.noise ? 1.0 : 0.0) : 0.0;
+        fv[12] = S[6] * rho;
+        fv[13] = S[4];
+        fv[14] = lg(rMr / (nr * nr)) - lg(S[1] / (S[0] * S[0]));
+        fv[15] = k == 1 ? 1.0 : 0.0;
+        S[2] = rMr;
+        S[4] = fv[4];
+        for (int q = 0; q < LEARN_NF; ++q) {
+            f[q] = (float)fv[q];
+            if (L.feat) L.feat[(int64_t)(k - 1) * LEARN_NF + q] = (float)fv[q];
+        }
+        sums[0] = b0;
+        sums[1] = b1;
+        sums[2] = b2;
+        if (L.xstar && L.err) L.err[k - 1] = sqrt(b2);
+    }
+    __syncthreads();                  // features and staged weights visible to the block
+    // greedy target: lambda* minimises ||x + lam d - x*|| before projection, in the network's range
+    double lr_star = 1.0;
+    if (L.mode == AIR_RELAX_GREEDY && t == 0) {
+        lr_star = sums[1] > 0 ? sums[0] / sums[1] * (double)L.rho : 1.0;
+        lr_star = fmin(fmax(lr_star, 0.02), (double)MLP_LAMBDA_MAX - 0.02);
+        if (L.target) L.target[k - 1] = (float)lr_star;
+    }
+    if (!L.mlp) {
+        if (t == 0) {
+            const double lam = lr_star / (double)L.rho;
+            ctrl->lam_cur = lam;
+            L.state[6] = L.state[5];
+            L.state[5] = lam;
+        }
+        return;                       // uniform across the block (L.mlp is a kernel argument)
+    }
+    // the network: 16 -> 64 -> 64 -> 2, FP32, one hidden unit per lane.  In a
+    // greedy run with parameters the network's lambda drives the iteration and
+    // the greedy target is only recorded (data along the network's own path).
+    const float* W = Ws;
+    if (t < MLP_HID) {
+        float z = W[MLP_HID * MLP_IN + t];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) z += W[t * MLP_IN + i] * f[i];
+        h1[t] = fmaxf(z, 0.f);
+    }
+    __syncthreads();
+    if (t < MLP_HID) {
+        const float* W2 = W + MLP_HID * MLP_IN + MLP_HID;
+        float z = W2[MLP_HID * MLP_HID + t];
+#pragma unroll 16
+        for (int i = 0; i < MLP_HID; ++i) z += W2[t * MLP_HID + i] * h1[i];
+        h2[t] = fmaxf(z, 0.f);
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const float* W3 = W + MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID;
+    float o0 = W3[t] * h2[t], o1 = W3[MLP_HID + t] * h2[t];
+    for (int d = 32; d > 0; d >>= 1) {
+        o0 += __shfl_xor(o0, d);
+        o1 += __shfl_xor(o1, d);
+    }
+    if (t == 0) {
+        o0 += W3[2 * MLP_HID];
+        o1 += W3[2 * MLP_HID + 1];
+        if (L.out) {
+            L.out[2 * (k - 1)] = o0;
+            L.out[2 * (k - 1) + 1] = o1;
+        }
+        const double lam = (double)MLP_LAMBDA_MAX / (1.0 + exp(-(double)o0)) / (double)L.rho;
+        ctrl->lam_cur = lam;
+        L.state[6] = L.state[5];
+        L.state[5] = lam;
+        if (L.mode == AIR_RELAX_LEARNED && o1 > 0.f) {   // stop at the state before this update
+            ctrl->finaliter = k - 1;
+            ctrl->stoprule = AIR_STOP_LEARNED;
+            __hip_atomic_store(&ctrl->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// learned relaxation: lambda_k (and the stop) before the fused column pass of iteration k
+void launch_learned_select(const SirtBufs& B, int k, hipStream_t s) {
+    const LearnBufs& L = *B.learn;
+    if (L.xstar) {             // error record only (no step direction yet)
+        k_greedy_dots<<<L.gblocks, BLOCK, 0, s>>>(L.nx, nullptr, nullptr, B.x, L.xstar, L.gpart, B.ctrl);
+        AIR_HIP(hipGetLastError());
+    }
+    k_select<<<1, 256, 0, s>>>(B.hist, B.ctrl, L, k);
+    AIR_HIP(hipGetLastError());
+}
+
+void launch_finalize_line(const SirtBufs& B, int nblocks, int k, hipStream_t s) {
+    if (B.learn) {
+        const LearnBufs& L = *B.learn;
+        if (L.xstar) {
+            k_greedy_dots<<<L.gblocks, BLOCK, 0, s>>>(L.nx, B.g, B.D, B.x, L.xstar, L.gpart, B.ctrl);
+            AIR_HIP(hipGetLastError());
+        }
+        (void)nblocks;
+        k_select<<<1, 256, 0, s>>>(B.hist, B.ctrl, L, k);
+        AIR_HIP(hipGetLastError());
+        return;
+    }
     k_finalize_line<<<1, BLOCK, 0, s>>>(B.partial_cols, nblocks, B.ctrl);
     AIR_HIP(hipGetLastError());
 }
@@ -435,7 +615,7 @@ void launch_line_update(const DevSparse& AT, Shape sh, const SirtBufs& B, int k,
         return 0;
     });
     AIR_HIP(hipGetLastError());
-    launch_finalize_line(B, sh.blocks, s);
+    launch_finalize_line(B, sh.blocks, k, s);
     k_apply<<<vec_grid(AT.nrows), BLOCK, 0, s>>>(AT.nrows, B.g, B.D, B.x, 0.0, 1, B.lb_kind, B.lb, B.lbv, B.ub_kind,
                                                  B.ub, B.ubv, B.ctrl);
     AIR_HIP(hipGetLastError());
@@ -466,7 +646,7 @@ void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double la
     if (line) {
         unsigned grid = std::min<unsigned>(vec_grid(n), 2048);
         k_gsum<<<grid, BLOCK, 0, s>>>(n, B.g, B.D, B.partial_cols, B.ctrl);
-        launch_finalize_line(B, (int)grid, s);
+        launch_finalize_line(B, (int)grid, k, s);
     }
     k_apply<<<vec_grid(n), BLOCK, 0, s>>>(n, B.g, B.D, B.x, lam, line, B.lb_kind, B.lb, B.lbv, B.ub_kind, B.ub,
                                           B.ubv, B.ctrl);

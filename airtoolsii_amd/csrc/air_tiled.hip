@@ -493,6 +493,7 @@ __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__
     extern __shared__ float smem[];
     __shared__ double sh[1][BLOCK / 64];
     if (MODE != 0 && load_stop(ctrl)) return;
+    if (MODE == 1 && isnan(lam)) lam = ctrl->lam_cur;  // NaN: chosen on the device this iteration
     const int t = blockIdx.x;
     const float* ys = stage_runs(smem, a.maxruns, t, a.run_ptr, a.run_row, a.run_off, a.run_len, y);
     const int lane = threadIdx.x % G, w = threadIdx.x / G;
@@ -553,6 +554,7 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __res
     __shared__ int wtot[MAX_BT / 64];
     __shared__ double sh[1][NW];
     if (MODE != 0 && load_stop(ctrl)) return;
+    if (MODE == 1 && isnan(lam)) lam = ctrl->lam_cur;  // NaN: chosen on the device this iteration
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int64_t c0 = (int64_t)t * MAX_BT;
     const int64_t Et0 = a.cell_ptr[c0];
@@ -1205,10 +1207,9 @@ void launch_tiled_resid(const TiledOp& T, const SirtBufs& B, StopParams sp, int 
 }
 
 void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int line, int k, hipStream_t s) {
-    (void)k;
     if (line) {
         launch_bwd<2>(T, B.Mr, nullptr, &B, 0.0, s);
-        launch_finalize_line(B, (int)T.ntiles_b, s);
+        launch_finalize_line(B, (int)T.ntiles_b, k, s);
         launch_apply_line(T.n_int, B, s);
     } else {
         launch_bwd<1>(T, B.Mr, nullptr, &B, lam, s);

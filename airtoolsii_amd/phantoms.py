@@ -54,18 +54,19 @@ def smooth(N, p=4):
 def threephases(N, p=100, seed=None):
     """phantomgallery.m:257-292 (rand(p,2) fills column-major)."""
     rs = _rng(seed)
-    I, J = np.meshgrid(np.arange(1, N + 1), np.arange(1, N + 1))
     s1 = 0.025 * N
     c1 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
-    im1 = np.zeros((N, N))
-    for i in range(p):
-        im1 += np.exp(-np.abs(I - c1[i, 0]) ** 3 / (2.5 * s1) ** 3 - np.abs(J - c1[i, 1]) ** 3 / s1 ** 3)
+    # each blob is separable in the grid coordinates: exp(-f(I) - g(J)) = exp(-f(I)) exp(-g(J))
+    ax = np.arange(1, N + 1, dtype=np.float64)
+    u = np.exp(-np.abs(ax[None, :] - c1[:, :1]) ** 3 / (2.5 * s1) ** 3)   # p x N, along I (columns)
+    v = np.exp(-np.abs(ax[None, :] - c1[:, 1:2]) ** 3 / s1 ** 3)          # p x N, along J (rows)
+    im1 = v.T @ u
     im1 = np.where(im1 < 0.35, 0.0, 2.0)
     s2 = 0.025 * N
     c2 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
-    im2 = np.zeros((N, N))
-    for i in range(p):
-        im2 += np.exp(-((I - c2[i, 0]) ** 2) / (2 * s2) ** 2 - ((J - c2[i, 1]) ** 2) / s2 ** 2)
+    u = np.exp(-((ax[None, :] - c2[:, :1]) ** 2) / (2 * s2) ** 2)
+    v = np.exp(-((ax[None, :] - c2[:, 1:2]) ** 2) / s2 ** 2)
+    im2 = v.T @ u
     im2 = np.where(im2 < 0.55, 0.0, 1.0)
     im = im1 + im2
     im[im == 3] = 1

@@ -98,11 +98,19 @@ def _relax(relaxpar):
             return (RELAX["psi1mod"] if s[:7] == "psi1mod" else RELAX["psi1"]), math.nan, None
         if s[:4] == "psi2":
             return (RELAX["psi2mod"] if s[:7] == "psi2mod" else RELAX["psi2"]), math.nan, None
+        if s == "learned":          # the network (options['mlp'] parameters); no reference counterpart
+            return RELAX["learned"], math.nan, None
+        if s == "greedy":           # training trajectories (options['x_true']); no reference counterpart
+            return RELAX["greedy"], math.nan, None
         raise ValueError("The chosen relaxation strategy is not valid for this method.")
     arr = np.ravel(np.asarray(relaxpar, dtype=np.float64))
     if arr.size == 1:
         return RELAX["scalar"], float(arr[0]), None
     return RELAX["vector"], math.nan, np.ascontiguousarray(arr)
+
+
+def _lib_mlp_params():
+    return 5378
 
 
 def _build_options(A, sirt_method, options, kmax):
@@ -117,6 +125,21 @@ def _build_options(A, sirt_method, options, kmax):
         keep.append(Dc); o.D = Dc.ctypes.data
     opts = dict(options or {})
     o.relax_kind, o.relaxpar, rv = _relax(opts.get("relaxpar"))
+    o.noise_norm = float(opts.get("noise_norm", 0.0) or 0.0)
+    if o.relax_kind == RELAX["learned"] and opts.get("mlp") is None:
+        raise ValueError("relaxpar 'learned' needs options['mlp'] (the network parameters)")
+    if o.relax_kind in (RELAX["learned"], RELAX["greedy"]) and opts.get("mlp") is not None:
+        pm = np.ascontiguousarray(np.asarray(opts["mlp"], np.float32).ravel())
+        if pm.size != _lib_mlp_params():
+            raise ValueError("options['mlp'] must hold 5378 parameters")
+        keep.append(pm); o.mlp_params = pm.ctypes.data
+    if o.relax_kind == RELAX["greedy"] and opts.get("x_true") is None:
+        raise ValueError("relaxpar 'greedy' needs options['x_true']")
+    if o.relax_kind in (RELAX["greedy"], RELAX["learned"]) and opts.get("x_true") is not None:
+        xt = f64(opts["x_true"])
+        if xt.size != n:
+            raise ValueError("options['x_true'] must have n values")
+        keep.append(xt); o.x_true = xt.ctypes.data
     if rv is not None:
         if rv.size < kmax:
             raise ValueError("a relaxpar vector must hold at least K(end) values")
@@ -297,6 +320,18 @@ class SirtRun:
         return x, {"stoprule": info.stoprule, "finaliter": info.finaliter, "relaxpar": info.relaxpar,
                    "rho": info.rho, "resnorm": rn[: info.finaliter + 1], "setup_time": info.setup_time,
                    "iter_time": info.iter_time}
+
+    def records(self):
+        """Per-iteration records of a learned / greedy run: features of the state
+        before each update (k-1 = 0..), greedy targets lambda* rho, network
+        outputs and ||x_{k-1} - x_true||."""
+        K = self.kmax
+        feat = np.full((K, 16), np.nan, np.float32)
+        target = np.full(K, np.nan, np.float32)
+        out = np.full((K, 2), np.nan, np.float32)
+        err = np.full(K, np.nan)
+        check(load().air_sirt_records(self.A.engine.h, ptr(feat), ptr(target), ptr(out), ptr(err)))
+        return {"features": feat, "target": target, "out": out, "err": err}
 
     def kernel_timing(self, on=True):
         check(load().air_set_kernel_timing(self.A.engine.h, 1 if on else 0))

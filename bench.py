@@ -5,6 +5,10 @@
                     [--dtype f32|bf16|f16] [--matrix-free] [--no-cpu-baseline]
     python bench.py --copy-calibration     (a plain 1 GiB device copy, for
                                             scaling HBM byte counters)
+    python bench.py --study learned [--learn-train T --learn-eval E --learn-K K]
+        learned relaxation at paralleltomo N=256, 180 angles, p=362: greedy
+        trajectories of T seeded problems, network trained on the GPU, then
+        time to the plain method's error on E held-out problems
 
 One step is one SIRT iteration x <- P_[0,1](x + lam D A' M (b - A x)) over the
 whole problem (sirt/sirt.m:213-243): a column pass (A' M r fused with the update
@@ -121,6 +125,40 @@ def measured_traffic(kernel, workload, dtype):
     return None if k is None else int(k["hbm_bytes"])
 
 
+def learned_study(args):
+    """Learned lambda_k / stop against plain SIRT (train_relaxpar lambda + DP)."""
+    import airtoolsii_amd as air
+    from airtoolsii_amd import learned as Lm
+    N, K = 256, args.learn_K
+    theta = np.arange(0.0, 180.0, 1.0)
+    p = 362
+    t0 = time.perf_counter()
+    A = air.paralleltomo(N, theta, p)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    params, hist = Lm.train_on_policy(A, N, range(1000, 1000 + args.learn_train), K, rounds=args.learn_rounds,
+                                      epochs=args.learn_epochs,
+                                      log=lambda d: print(json.dumps({"progress": d}), file=sys.stderr, flush=True))
+    t_train = time.perf_counter() - t0
+    lam = Lm.plain_relaxpar(A, N, 999, 60)
+    t0 = time.perf_counter()
+    rows = Lm.evaluate(A, N, range(100000, 100000 + args.learn_eval), params, lam, 400)
+    t_eval = time.perf_counter() - t0
+    summ = Lm.summarize(rows)
+    out = {"metric": "time to the plain method's error, learned vs plain lambda (median ratio)",
+           "value": round(summ["time_to_plain_error_over_plain_median"], 4), "unit": "ratio",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"paralleltomo N={N}, theta=0:1:179 ({len(theta)} angles), p={p}, SART, box [0,1]",
+                      "train_trajectories": args.learn_train, "trajectory_K": K, "eval_problems": args.learn_eval,
+                      "training_rounds": args.learn_rounds,
+                      "phantoms": list(Lm.KINDS), "noise": "1-5% Gaussian, seeded",
+                      "plain": f"lambda={lam:.4f} from train_relaxpar, DP with tau=1.02"},
+           "summary": summ, "train_loss": [round(float(v), 5) for v in hist],
+           "seconds": {"build": round(t_build, 2), "trajectories_and_training": round(t_train, 1),
+                       "evaluate": round(t_eval, 1)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +171,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--study", choices=["learned"], default=None)
+    ap.add_argument("--learn-train", type=int, default=2000)
+    ap.add_argument("--learn-eval", type=int, default=512)
+    ap.add_argument("--learn-K", type=int, default=150)
+    ap.add_argument("--learn-epochs", type=int, default=40)
+    ap.add_argument("--learn-rounds", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine runs one problem per GPU and no multi-GPU path is "
@@ -140,6 +184,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.study == "learned":
+        learned_study(args)
         return
 
     import airtoolsii_amd as air

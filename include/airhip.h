@@ -49,13 +49,18 @@ extern "C" {
 #define AIR_RELAX_PSI1MOD 4
 #define AIR_RELAX_PSI2    5
 #define AIR_RELAX_PSI2MOD 6
-#define AIR_RELAX_VECTOR  7   /* caller-supplied per-iteration lambda_k (learned relaxation) */
+#define AIR_RELAX_VECTOR  7   /* caller-supplied per-iteration lambda_k */
+/* No counterpart in the reference (learned relaxation, for callers that want
+ * the network's lambda_k and stop index instead of calc_relaxpar/check_stoprules): */
+#define AIR_RELAX_LEARNED 8   /* lambda_k and a stop decision from the network (opts.mlp_params) */
+#define AIR_RELAX_GREEDY  9   /* training trajectories: lambda_k minimising ||x_k - x_true|| along the step */
 
 /* stopping rules, auxil/check_stoprules.m:53-161 (info.stoprule codes match) */
 #define AIR_STOP_NONE 0
 #define AIR_STOP_NCP  1
 #define AIR_STOP_DP   2
 #define AIR_STOP_ME   3
+#define AIR_STOP_LEARNED 4    /* info.stoprule only: the network's stop logit fired (AIR_RELAX_LEARNED) */
 
 /* bound kinds, auxil/check_inputs.m:108-132 */
 #define AIR_BOUND_NONE   0
@@ -80,6 +85,13 @@ typedef struct {
     int           res_dims[2];   /* NCP: [p, nA] or [m, 0] */
     int           ncp_smooth;    /* NCP filter length, default 2 */
     int           verbose;       /* sirt.m:256-260 print cadence */
+    /* learned relaxation (no counterpart in the reference) */
+    const float*  mlp_params;    /* AIR_RELAX_LEARNED: AIR_MLP_PARAMS network parameters; with
+                                    AIR_RELAX_GREEDY optional: the network's lambda drives the run
+                                    and the greedy target is only recorded */
+    double        noise_norm;    /* ||e|| if known (a feature of the network), else <= 0 */
+    const double* x_true;        /* AIR_RELAX_GREEDY: the exact solution (n values); with
+                                    AIR_RELAX_LEARNED optional, for the error record */
 } air_sirt_options;
 
 /* info struct of sirt.m:60-67,272-281 */
@@ -154,6 +166,14 @@ int air_sirt_prepare(air_engine* e, const air_sirt_options* opts, const double* 
 int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms);
 int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info* info);
 
+/* Per-iteration records of the last AIR_RELAX_LEARNED / AIR_RELAX_GREEDY run
+ * (no counterpart in the reference; the training data of the network): for
+ * the update of iteration k = 1..finaliter, row k-1 holds the 16 features of
+ * the state x_{k-1} (feat), the greedy target lambda* rho (target), the raw
+ * network outputs (out, learned runs) and ||x_{k-1} - x_true|| (err, when
+ * x_true was given).  Any pointer may be NULL; rows beyond finaliter are not written. */
+int air_sirt_records(air_engine* e, float* feat, float* target, float* out, double* err);
+
 /* per-kernel HIP-event timing of the SIRT kernels (0 = off).  Times are
  * accumulated over iterate() calls while on; reset by enabling again.
  * cols: the A'(M r) + update pass; rows: the A x pass; reduce: the residual
@@ -191,16 +211,17 @@ int air_layout_info(air_engine* e, air_layout* out);
  * rules of check_stoprules.m when AIR_RELAX_LEARNED is chosen.  A 16-64-64-2
  * ReLU perceptron; parameters are AIR_MLP_PARAMS floats laid out as
  * W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (row = output).
- * Outputs (o0, o1): lambda_k * rho = 2 sigmoid(o0); o1 is the stop logit. */
+ * Outputs (o0, o1): lambda_k * rho = AIR_MLP_LAMBDA_MAX sigmoid(o0); o1 is the stop logit. */
 #define AIR_MLP_IN     16
 #define AIR_MLP_HID    64
 #define AIR_MLP_OUT    2
 #define AIR_MLP_PARAMS 5378
+#define AIR_MLP_LAMBDA_MAX 4      /* lambda_k rho ranges over (0, 4) */
 /* raw outputs of n samples: X is n x 16 row-major, out n x 2 */
 int air_mlp_forward(air_engine* e, const float* params, const float* X, int64_t n, float* out);
 /* mean loss over n samples and its gradient (BF16 MFMA, FP32 accumulation).
- * Targets T (n x 2): t0 = lambda* rho in (0, 2) (t0 < 0 drops the term), t1 in {0, 1}.
- * Loss = mean[(2 sigmoid(o0) - t0)^2 + stop_weight * BCE(o1, t1)]. */
+ * Targets T (n x 2): t0 = lambda* rho in (0, 4) (t0 < 0 drops the term), t1 in {0, 1}.
+ * Loss = mean[(4 sigmoid(o0) - t0)^2 + stop_weight * BCE(o1, t1)]. */
 int air_mlp_grad(air_engine* e, const float* params, const float* X, const float* T, int64_t n,
                  double stop_weight, float* grad, double* loss);
 /* Adam training in place: `epochs` passes over the n samples in seeded random

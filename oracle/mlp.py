@@ -1,12 +1,12 @@
 """FP64 reference of the learned relaxation network (test infrastructure only).
 
 The network maps 16 residual statistics of a SIRT iteration to (o0, o1):
-lambda_k * rho = 2 sigmoid(o0) and a stop logit o1.  Layers 16 -> 64 -> 64 -> 2
+lambda_k * rho = 4 sigmoid(o0) and a stop logit o1.  Layers 16 -> 64 -> 64 -> 2
 with ReLU; parameters are 5378 values laid out as W1[64][16], b1[64],
 W2[64][64], b2[64], W3[2][64], b3[2] (row = output unit).
 
 The loss of n samples with targets (t0, t1) is
-    mean[(2 sigmoid(o0) - t0)^2 * [t0 >= 0] + stop_w * BCE(o1, t1)].
+    mean[(4 sigmoid(o0) - t0)^2 * [t0 >= 0] + stop_w * BCE(o1, t1)].
 
 `emulate_bf16=True` rounds to BF16 at the points where the device kernels feed
 their BF16 matrix units (inputs, weights, activations and back-propagated
@@ -21,6 +21,7 @@ import numpy as np
 
 NI, NH, NO = 16, 64, 2
 NPARAM = NH * NI + NH + NH * NH + NH + NO * NH + NO
+LAMBDA_MAX = 4.0
 
 
 def unpack(p):
@@ -81,12 +82,12 @@ def loss_grad(p, X, T, stop_w=1.0, emulate_bf16=False):
     o, (x, h1, h2) = forward(p, X, emulate_bf16)
     s0, s1 = _sigmoid(o[:, 0]), _sigmoid(o[:, 1])
     keep = T[:, 0] >= 0
-    e = 2 * s0 - T[:, 0]
+    e = LAMBDA_MAX * s0 - T[:, 0]
     z = o[:, 1]
     bce = np.maximum(z, 0) - z * T[:, 1] + np.log1p(np.exp(-np.abs(z)))
     loss = (np.sum(np.where(keep, e * e, 0.0)) + stop_w * np.sum(bce)) / n
     dO = np.zeros((n, NO))
-    dO[:, 0] = np.where(keep, 2 * e * 2 * s0 * (1 - s0), 0.0) / n
+    dO[:, 0] = np.where(keep, 2 * e * LAMBDA_MAX * s0 * (1 - s0), 0.0) / n
     dO[:, 1] = stop_w * (s1 - T[:, 1]) / n
     db3 = dO.sum(0)
     dOr = r(dO)

@@ -238,7 +238,10 @@ def relaxpar_vector(relaxpar, rho, kmax):
         lam = 1.9 / rho
         return 1, np.full(kmax, lam), lam
     if not isinstance(relaxpar, str):
-        lam = float(relaxpar)
+        arr = np.ravel(np.asarray(relaxpar, np.float64))
+        if arr.size > 1:        # a per-iteration sequence, applied as relaxpar(k) like the psi vectors (:160-164)
+            return 3, arr[:kmax], arr
+        lam = float(arr[0])
         return 1, np.full(kmax, lam), lam
     s = relaxpar.lower()
     if s.startswith("line"):

@@ -1,0 +1,90 @@
+"""The learned relaxation inside the SIRT loop: the greedy target and the
+in-loop network against host computations, and what a trained network gains
+over plain SIRT (train_relaxpar lambda + discrepancy principle)."""
+import numpy as np
+import pytest
+
+import airtoolsii_amd as air
+from airtoolsii_amd import learned as Lm
+from airtoolsii_amd.mlp import init_params
+from airtoolsii_amd.sirt import SirtRun
+from oracle import mlp as ref
+
+pytestmark = pytest.mark.gpu
+
+N, P = 48, 68
+THETA = np.arange(0.0, 180.0, 2.0)
+
+
+@pytest.fixture(scope="module")
+def setup(oracle):
+    A = air.paralleltomo(N, THETA, P)
+    Ao = oracle.paralleltomo(N, THETA, P)
+    return A, Ao
+
+
+def test_greedy_target_first_step(setup, oracle):
+    """lambda*_1 rho = <d, x*> / <d, d> with d = D A' M b (x0 = 0), SART (rho = 1)."""
+    A, Ao = setup
+    x, b, nn, _ = Lm.problem(A, N, 11)
+    M, D = oracle.weights("sart", Ao)
+    d = D * (Ao.T @ (M * b))
+    expect = np.clip(d @ x / (d @ d), 0.02, 3.98)
+    run = SirtRun("sart", A, b, 3, None, {"relaxpar": "greedy", "x_true": x, "noise_norm": nn})
+    run.iterate(3)
+    rec = run.records()
+    assert abs(rec["target"][0] - expect) <= 1e-4 * expect
+    assert abs(rec["err"][0] - np.linalg.norm(x)) <= 1e-5 * np.linalg.norm(x)     # x0 = 0
+    assert np.all(np.isfinite(rec["features"][:3]))
+
+
+def test_learned_loop_matches_host_network(setup, oracle):
+    """The in-loop network (FP32) gives the recorded outputs from the recorded
+    features, and the iterates equal SIRT with that lambda sequence."""
+    A, Ao = setup
+    x, b, nn, _ = Lm.problem(A, N, 12)
+    params = init_params(3)
+    params[-2] = 0.3                       # lambda rho = 4 sigmoid(.) near 2.3 at the start
+    params[-1] = -50.0                     # never stop: compare the whole run
+    K = 20
+    run = SirtRun("sart", A, b, K, None, {"relaxpar": "learned", "mlp": params, "noise_norm": nn})
+    run.iterate(K)
+    xk, info = run.fetch()
+    rec = run.records()
+    assert info["finaliter"] == K
+    o_ref, _ = ref.forward(params, rec["features"].astype(np.float64))
+    assert np.max(np.abs(rec["out"] - o_ref)) <= 1e-4 * max(1.0, np.max(np.abs(o_ref)))
+    lam = 4.0 / (1.0 + np.exp(-rec["out"][:, 0].astype(np.float64)))          # rho = 1 (SART)
+    Xo, _, _ = oracle.sirt("sart", Ao, b, K, None, {"relaxpar": lam})
+    assert np.linalg.norm(xk - Xo[:, -1]) <= 1e-5 * np.linalg.norm(Xo[:, -1])
+
+
+def test_learned_stop_and_dp_fallback(setup):
+    A, _ = setup
+    x, b, nn, _ = Lm.problem(A, N, 13)
+    params = init_params(4)
+    params[5248:5376] = 0.0                # W3 = 0: constant outputs from the biases
+    params[-2] = np.log(1.9 / 2.1)         # lambda rho = 4 sigmoid(b3[0]) = 1.9
+    params[-1] = 50.0                      # stop logit always positive: stop at x_0
+    _, info, _ = air.sart(A, b, 30, None, {"relaxpar": "learned", "mlp": params})
+    assert info["finaliter"] == 0 and info["stoprule"] == 4
+    params[-1] = -50.0                     # never: the discrepancy principle stops the run
+    _, info, _ = air.sart(A, b, 200, None, {"relaxpar": "learned", "mlp": params,
+                                            "stoprule": {"type": "DP", "taudelta": 1.5 * nn}})
+    assert info["stoprule"] == 2 and 0 < info["finaliter"] < 200
+
+
+def test_trained_network_gains_over_plain(setup):
+    """Trained on 240 problems (three rounds of dataset aggregation) and judged on
+    32 held-out ones: the learned run reaches the plain method's final error in
+    fewer iterations and usually ends at a lower error."""
+    A, _ = setup
+    params, hist = Lm.train_on_policy(A, N, range(200, 440), 120, rounds=3, epochs=40)
+    assert np.all(np.isfinite(params)) and hist[-1] < hist[0]
+    lam = Lm.plain_relaxpar(A, N, 7, 60)
+    rows = Lm.evaluate(A, N, range(9000, 9032), params, lam, 300)
+    s = Lm.summarize(rows)
+    print(s)
+    assert s["reached_plain_err_frac"] >= 0.75
+    assert s["iters_to_plain_err_over_plain_median"] <= 0.85
+    assert s["learned_err_le_plain_frac"] >= 0.6

@@ -64,8 +64,8 @@ def test_training_fits_a_smooth_target(net):
     L1, _ = net.grad(p, X, T)
     assert L1 < 0.15 * L0 and hist[-1] < hist[0]
     o = net.forward(p, X)
-    lam = 2 / (1 + np.exp(-o[:, 0]))
-    assert np.sqrt(np.mean((lam - t0) ** 2)) < 0.1
+    lam = 4 / (1 + np.exp(-o[:, 0]))
+    assert np.sqrt(np.mean((lam - t0) ** 2)) < 0.3 * np.std(t0)
     assert np.mean((o[:, 1] > 0) == (t1 > 0.5)) > 0.95
     # deterministic: the same seed gives the same weights
     p2, _ = net.train(p0, X, T, epochs=15, batch=1024, lr=3e-3, seed=1)
