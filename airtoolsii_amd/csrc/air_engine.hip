@@ -961,7 +961,13 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         auto t0 = std::chrono::steady_clock::now();
         timing_begin(e, niter);
         AIR_HIP(hipEventRecord(a, e->stream));
-        for (int i = 0; i < niter && st.k < st.kmax; ++i) enqueue_iteration(e, ++st.k);
+        // with a rule that can stop early, check the stop flag every 32 iterations
+        // instead of enqueueing iterations that would only return
+        const bool early = st.sp.type != AIR_STOP_NONE || st.relax_kind == AIR_RELAX_LEARNED;
+        for (int i = 0; i < niter && st.k < st.kmax; ++i) {
+            enqueue_iteration(e, ++st.k);
+            if (early && (i + 1) % 32 == 0 && read_stop(e)) break;
+        }
         AIR_HIP(hipEventRecord(b, e->stream));
         AIR_HIP(hipEventSynchronize(b));
         timing_end(e);
