@@ -9,6 +9,7 @@ Operators follow the afun convention A(v,'notransp'|'transp'|'size').
 The compute runs in libairhip.so (HIP kernels for gfx950, C ABI in
 include/airhip.h); build it with ``python -m airtoolsii_amd.build``.
 """
+from .batch import BatchOperator, batch_sirt, paralleltomo_batch
 from .operators import AirOperator, as_operator, from_matlab_csc
 from .phantoms import add_noise, phantomgallery
 from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, spectral_radius,
@@ -16,6 +17,6 @@ from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt,
 from .testprobs import fanlineartomo, paralleltomo
 from .train import train_relaxpar
 
-__all__ = ["AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
+__all__ = ["BatchOperator", "batch_sirt", "paralleltomo_batch", "AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
            "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "spectral_radius", "weights",
            "fanlineartomo", "paralleltomo", "train_relaxpar"]

@@ -31,6 +31,7 @@ SOURCES = {
     "air_sirt.hip": [],
     "air_tiled.hip": [],
     "air_mlp.hip": [],
+    "air_batch.hip": [],
     "air_engine.hip": [],
 }
 

@@ -1,0 +1,164 @@
+// air_batch.hip -- many independent problems as one block-diagonal operator.
+//
+// Each problem's LDS-tiled layout is built on its own (its columns fill whole
+// forward and backward tiles of its image), then the layouts are concatenated:
+// entry arrays back to back, and every index that points into a shared array
+// (entry offsets, piece numbers, rows, tiles, runs, internal columns) shifted
+// by the problem's offset.  The SIRT kernels then treat the batch as one
+// operator: one launch per pass covers every problem.
+#include "air_internal.h"
+
+namespace air {
+
+namespace {
+
+__global__ void k_shift_i64(const int64_t* __restrict__ src, int64_t n, int64_t off, int64_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] + off;
+}
+__global__ void k_shift_i32(const int* __restrict__ src, int64_t n, int off, int* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] + off;
+}
+__global__ void k_clear_start_flags(unsigned short* __restrict__ code, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        code[i] &= (unsigned short)0x7fffu;
+}
+unsigned grid_of(int64_t n) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+void shift64(const DevBuf& src, int64_t n, int64_t off, DevBuf& dst, int64_t at, hipStream_t s) {
+    if (n <= 0) return;
+    k_shift_i64<<<grid_of(n), 256, 0, s>>>(src.as<int64_t>(), n, off, dst.as<int64_t>() + at);
+    AIR_HIP(hipGetLastError());
+}
+void shift32(const DevBuf& src, int64_t n, int64_t off, DevBuf& dst, int64_t at, hipStream_t s) {
+    if (n <= 0) return;
+    AIR_CHECK(off < INT32_MAX, AIR_E_UNSUPPORTED, "batched operator too large for 32-bit indices");
+    k_shift_i32<<<grid_of(n), 256, 0, s>>>(src.as<int>(), n, (int)off, dst.as<int>() + at);
+    AIR_HIP(hipGetLastError());
+}
+void copy_bytes(const DevBuf& src, size_t bytes, DevBuf& dst, size_t at, hipStream_t s) {
+    if (bytes) AIR_HIP(hipMemcpyAsync((char*)dst.p + at, src.p, bytes, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace
+
+// Concatenate the tiled layouts of independent problems (all of one storage
+// type, each ready) into B; the parts are released one by one.  When some part
+// cannot carry column-start flags in its backward codes, the flags of the
+// others are cleared so that the whole batch uses one code format.
+bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
+    B = TiledOp();
+    AIR_CHECK(!parts.empty(), AIR_E_ARG, "empty batch");
+    const int dt = parts[0].dtype;
+    int64_t m = 0, n = 0, nint = 0, nnz = 0, P = 0, R = 0, tf = 0, tb = 0, staged = 0, rpi = 0;
+    int maxruns = 0, maxrows = 0;
+    bool flags = true;
+    for (auto& T : parts) {
+        AIR_CHECK(T.ready && T.dtype == dt, AIR_E_STATE, "batch parts must be tiled with one storage type");
+        m += T.m;
+        n += T.n;
+        nint += T.n_int;
+        nnz += T.nnz;
+        P += T.npieces;
+        R += T.nruns;
+        tf += T.ntiles_f;
+        tb += T.ntiles_b;
+        staged += T.staged_rows;
+        maxruns = std::max(maxruns, T.maxruns);
+        maxrows = std::max(maxrows, T.maxrows);
+        flags = flags && T.bwd_flags;
+        rpi += T.npieces;
+    }
+    AIR_CHECK(nint < INT32_MAX && m < INT32_MAX && P < INT32_MAX && R < INT32_MAX, AIR_E_UNSUPPORTED,
+              "batched operator too large for 32-bit indices");
+    const size_t vb = dtype_size(dt);
+    B.dtype = dt;
+    B.N = 0;
+    B.m = m;
+    B.n = n;
+    B.n_int = nint;
+    B.nnz = nnz;
+    B.npieces = P;
+    B.nruns = R;
+    B.ntiles_f = tf;
+    B.ntiles_b = tb;
+    B.staged_rows = staged;
+    B.maxruns = maxruns;
+    B.maxrows = maxrows;
+    B.bt_size = parts[0].bt_size;
+    B.bwd_flags = flags;
+    B.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
+    B.perm.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+    B.fval.alloc(vb * (size_t)(nnz + AIR_SPARSE_PAD));
+    B.fcode.alloc(2 * (size_t)(nnz + AIR_SPARSE_PAD));
+    B.bval.alloc(vb * (size_t)(nnz + AIR_SPARSE_PAD));
+    B.blr.alloc(2 * (size_t)(nnz + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((char*)B.fval.p + vb * nnz, 0, vb * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.fcode.p + 2 * nnz, 0, 2 * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.bval.p + vb * nnz, 0, vb * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.blr.p + 2 * nnz, 0, 2 * AIR_SPARSE_PAD, s));
+    B.piece_ptr.alloc(8 * (size_t)(P + 1));
+    B.rp_ptr.alloc(8 * (size_t)(m + 1));
+    B.rp_idx.alloc(4 * (size_t)std::max<int64_t>(P, 1));
+    B.partial.alloc(4 * (size_t)std::max<int64_t>(P, 1));
+    B.cell_ptr.alloc(8 * (size_t)(nint + 1));
+    B.run_ptr.alloc(4 * (size_t)(tb + 1));
+    B.run_row.alloc(4 * (size_t)std::max<int64_t>(R, 1));
+    B.run_off.alloc(4 * (size_t)std::max<int64_t>(R, 1));
+    B.run_len.alloc(4 * (size_t)std::max<int64_t>(R, 1));
+    B.piece_tile_h.reserve((size_t)P);
+    B.piece_ptr_h.reserve((size_t)P + 1);
+    int64_t om = 0, on = 0, oint = 0, oe = 0, oP = 0, oR = 0, otf = 0, otb = 0;
+    for (auto& T : parts) {
+        // forward
+        shift32(T.perm, T.n, oint, B.perm, on, s);
+        copy_bytes(T.fval, vb * (size_t)T.nnz, B.fval, vb * (size_t)oe, s);
+        copy_bytes(T.fcode, 2 * (size_t)T.nnz, B.fcode, 2 * (size_t)oe, s);
+        shift64(T.piece_ptr, T.npieces, oe, B.piece_ptr, oP, s);
+        shift64(T.rp_ptr, T.m, oP, B.rp_ptr, om, s);      // rp_idx positions of a part start at its piece offset
+        shift32(T.rp_idx, T.npieces, oP, B.rp_idx, oP, s);
+        for (int64_t k = 0; k < T.npieces; ++k) {
+            B.piece_tile_h.push_back(T.piece_tile_h[k] + (int)otf);
+            B.piece_ptr_h.push_back(T.piece_ptr_h[k] + oe);
+        }
+        // backward
+        copy_bytes(T.bval, vb * (size_t)T.nnz, B.bval, vb * (size_t)oe, s);
+        copy_bytes(T.blr, 2 * (size_t)T.nnz, B.blr, 2 * (size_t)oe, s);
+        if (!flags && T.bwd_flags && T.nnz > 0) {        // one code format for the whole batch
+            k_clear_start_flags<<<grid_of(T.nnz), 256, 0, s>>>(B.blr.as<unsigned short>() + oe, T.nnz);
+            AIR_HIP(hipGetLastError());
+        }
+        shift64(T.cell_ptr, T.n_int, oe, B.cell_ptr, oint, s);
+        shift32(T.run_ptr, T.ntiles_b, oR, B.run_ptr, otb, s);
+        shift32(T.run_row, T.nruns, om, B.run_row, oR, s);
+        copy_bytes(T.run_off, 4 * (size_t)T.nruns, B.run_off, 4 * (size_t)oR, s);
+        copy_bytes(T.run_len, 4 * (size_t)T.nruns, B.run_len, 4 * (size_t)oR, s);
+        om += T.m;
+        on += T.n;
+        oint += T.n_int;
+        oe += T.nnz;
+        oP += T.npieces;
+        oR += T.nruns;
+        otf += T.ntiles_f;
+        otb += T.ntiles_b;
+        AIR_HIP(hipStreamSynchronize(s));
+        T = TiledOp();                                    // release the part
+    }
+    // closing entries of the pointer arrays
+    AIR_HIP(hipMemcpyAsync(B.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(B.rp_ptr.as<int64_t>() + m, &P, 8, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(B.cell_ptr.as<int64_t>() + nint, &nnz, 8, hipMemcpyHostToDevice, s));
+    const int Ri = (int)R;
+    AIR_HIP(hipMemcpyAsync(B.run_ptr.as<int>() + tb, &Ri, 4, hipMemcpyHostToDevice, s));
+    B.piece_ptr_h.push_back(nnz);
+    AIR_HIP(hipStreamSynchronize(s));
+    set_fwd_blocks(B, 256, s);
+    set_fwd_waves(B, B.wave_target, s);
+    B.ready = true;
+    autotune_tiled(B, s);
+    return true;
+}
+
+}  // namespace air

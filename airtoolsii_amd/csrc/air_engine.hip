@@ -55,6 +55,10 @@ struct air_engine {
     air::Shape shA, shAT;
     air::Geometry geo;
     air::TiledOp T;            // LDS-tiled layout used by the SIRT loop when ready
+    bool no_tune = false;      // skip the kernel-shape autotuner (parts of a batch)
+    // batch of independent problems (block-diagonal operator without CSR)
+    std::vector<int64_t> b_row_off, b_col_off;
+    std::vector<double> b_rho, b_M, b_D;
     air::SirtState st;
     // weights and spectral radius of the built-in methods, kept until the operator changes
     std::vector<double> wM[AIR_SART + 1], wD[AIR_SART + 1];
@@ -142,6 +146,11 @@ void reset_operator(air_engine* e) {
     e->kind = 0;
     e->m = e->n = 0;
     e->st.prepared = false;
+    e->b_row_off.clear();
+    e->b_col_off.clear();
+    e->b_rho.clear();
+    e->b_M.clear();
+    e->b_D.clear();
     for (int q = 0; q <= AIR_SART; ++q) {
         e->wvalid[q] = false;
         e->wdev[q] = false;
@@ -168,7 +177,7 @@ void finish_matrix(air_engine* e, int N) {
     e->m = e->A.nrows;
     e->n = e->A.ncols;
     e->T = TiledOp();
-    if (!tiled_disabled()) build_tiled(e->A, N, e->T, e->stream);
+    if (!tiled_disabled()) build_tiled(e->A, N, e->T, e->stream, !e->no_tune);
 }
 
 bool use_tiled(const air_engine* e) { return e->kind == 1 && e->T.ready; }
@@ -176,6 +185,19 @@ bool use_tiled(const air_engine* e) { return e->kind == 1 && e->T.ready; }
 // y = A x (trans=0) or y = A' x (trans=1), device FP32 in / out.
 void apply_dev(air_engine* e, int trans, const float* in, float* out) {
     hipStream_t s = e->stream;
+    if (e->kind == 1 && e->A.nnz == 0 && e->T.ready) {     // a batch: only the tiled layout exists
+        DevBuf xi;
+        xi.alloc(4 * (size_t)std::max<int64_t>(e->T.n_int, 1));
+        if (trans) {
+            tiled_backward_plain(e->T, in, xi.as<float>(), s);
+            permute_to_user(e->T, xi.as<float>(), out, s);
+        } else {
+            permute_to_internal(e->T, in, xi.as<float>(), s);
+            tiled_forward_plain(e->T, xi.as<float>(), out, s);
+        }
+        AIR_HIP(hipStreamSynchronize(s));
+        return;
+    }
     if (e->kind == 1) {
         if (trans) spmv(e->AT, e->shAT, in, out, s);
         else spmv(e->A, e->shA, in, out, s);
@@ -863,7 +885,7 @@ int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz) {
         need_operator(e);
         if (m) *m = e->m;
         if (n) *n = e->n;
-        if (nnz) *nnz = e->kind == 1 ? e->A.nnz : -1;
+        if (nnz) *nnz = e->kind == 1 ? (e->A.nnz ? e->A.nnz : e->T.nnz) : -1;
     });
 }
 
@@ -1244,6 +1266,65 @@ int air_mlp_train(air_engine* e, float* params, const float* X, const float* T, 
         }
         AIR_HIP(hipMemcpyAsync(params, W.p, 4 * (size_t)MLP_PARAMS, hipMemcpyDeviceToHost, s));
         AIR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// ---- batches of independent problems
+int air_build_paralleltomo_batch(air_engine* e, int32_t nprob, const int32_t* N, const int32_t* nA,
+                                 const double* theta, const int32_t* p, int method, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && nprob > 0 && N && nA && theta && p, AIR_E_ARG, "invalid batch arguments");
+        AIR_CHECK(method >= AIR_LANDWEBER && method <= AIR_SART, AIR_E_ARG, "SIRT method not defined.");
+        (void)dtype_size(dtype);
+        use_device(e);
+        reset_operator(e);
+        std::vector<TiledOp> parts;
+        parts.reserve(nprob);
+        e->b_row_off.assign(1, 0);
+        e->b_col_off.assign(1, 0);
+        int64_t toff = 0;
+        for (int q = 0; q < nprob; ++q) {
+            AIR_CHECK(N[q] > 0 && nA[q] > 0 && p[q] > 0 && (int64_t)N[q] * N[q] <= INT32_MAX, AIR_E_ARG,
+                      "invalid batch problem");
+            air_engine t;                   // this problem alone: CSR, CSC, tiled layout, weights, rho
+            t.device = e->device;
+            t.num_cu = e->num_cu;
+            t.stream = e->stream;
+            t.no_tune = true;
+            host_rays_paralleltomo(N[q], theta + toff, nA[q], p[q], (double)(p[q] - 1), t.geo.host_rays);
+            build_geometry(&t, N[q], (int64_t)nA[q] * p[q], 1, 0, dtype);
+            AIR_CHECK(t.T.ready, AIR_E_UNSUPPORTED, "a batch problem does not fit the tiled layout");
+            std::vector<double> M, D;
+            compute_weights(&t, method, M, D);
+            const double rho = method == AIR_SART ? 1.0 : lanczos_rho(&t, M, D);   // calc_relaxpar.m:91-110
+            const double lam = 1.9 / rho;
+            for (auto& v : M) v *= lam;    // the problem's default lambda, folded into its rows of M
+            e->b_M.insert(e->b_M.end(), M.begin(), M.end());
+            e->b_D.insert(e->b_D.end(), D.begin(), D.end());
+            e->b_rho.push_back(rho);
+            e->b_row_off.push_back(e->b_row_off.back() + t.m);
+            e->b_col_off.push_back(e->b_col_off.back() + t.n);
+            parts.push_back(std::move(t.T));
+            toff += nA[q];
+            t.stream = nullptr;             // owned by e
+        }
+        concat_tiled(parts, e->T, e->stream);
+        e->kind = 1;
+        e->m = e->b_row_off.back();
+        e->n = e->b_col_off.back();
+    });
+}
+
+int air_batch_info(air_engine* e, int64_t* row_off, int64_t* col_off, double* rho, double* M, double* D) {
+    return guard([&] {
+        AIR_CHECK(e && !e->b_rho.empty(), AIR_E_STATE, "the engine holds no batch");
+        const size_t P = e->b_rho.size();
+        if (row_off) std::copy(e->b_row_off.begin(), e->b_row_off.end(), row_off);
+        if (col_off) std::copy(e->b_col_off.begin(), e->b_col_off.end(), col_off);
+        if (rho) std::copy(e->b_rho.begin(), e->b_rho.end(), rho);
+        if (M) std::copy(e->b_M.begin(), e->b_M.end(), M);
+        if (D) std::copy(e->b_D.begin(), e->b_D.end(), D);
+        (void)P;
     });
 }
 }  // extern "C"

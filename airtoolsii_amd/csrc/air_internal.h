@@ -202,7 +202,9 @@ struct TiledOp {
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
-bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s);
+bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune = true);
+// Concatenate independent problems' tiled layouts into one block-diagonal operator (air_batch.hip).
+bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s);
 void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStream_t s);
 void autotune_tiled(TiledOp& T, hipStream_t s);
 void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);

@@ -872,7 +872,7 @@ static void pad_alloc(DevBuf& b, int64_t n, hipStream_t s) {
     AIR_HIP(hipMemsetAsync((T*)b.p + n, 0, sizeof(T) * AIR_SPARSE_PAD, s));
 }
 
-bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
+bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune) {
     T = TiledOp();
     const int64_t m = A.nrows, n = A.ncols, nnz = A.nnz;
     if (nnz == 0 || nnz >= (int64_t)UINT32_MAX || m >= (int64_t)INT32_MAX) return false;
@@ -1060,7 +1060,7 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s) {
         AIR_HIP(hipStreamSynchronize(s));
     }
     T.ready = true;
-    autotune_tiled(T, s);
+    if (tune) autotune_tiled(T, s);
     return true;
 }
 

@@ -205,6 +205,22 @@ typedef struct {
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 
+/* ---- batches of independent problems -----------------------------------
+ * No counterpart in the reference (which solves one system per call): many
+ * paralleltomo problems of different sizes solved together.  The engine
+ * builds each problem (paralleltomo.m:112-301 with d = p-1) and concatenates
+ * them into the block-diagonal operator diag(A_1, ..., A_P), held only in the
+ * tiled layout, so every SIRT pass is one launch over all problems.  Per
+ * problem it computes the weights of `method` (get_mfun_dfun.m) and rho
+ * (calc_relaxpar.m:91-103); air_batch_info returns M with each problem's
+ * default lambda = 1.9/rho_p folded into its rows, so that air_sirt with
+ * AIR_CUSTOM(M, D), relaxpar 1 and rho 1 performs the independent default-
+ * lambda runs.  theta holds the nA[0] + ... + nA[P-1] angles back to back. */
+int air_build_paralleltomo_batch(air_engine* e, int32_t nprob, const int32_t* N, const int32_t* nA,
+                                 const double* theta, const int32_t* p, int method, int dtype);
+/* row_off, col_off: nprob+1 offsets; rho: nprob; M: m values (lambda-scaled); D: n values */
+int air_batch_info(air_engine* e, int64_t* row_off, int64_t* col_off, double* rho, double* M, double* D);
+
 /* ---- learned relaxation network ---------------------------------------
  * No counterpart in the reference: the learned lambda_k / stop rule that
  * replaces the fixed relaxation of calc_relaxpar.m:105-171 and the stop
