@@ -12,11 +12,11 @@ include/airhip.h); build it with ``python -m airtoolsii_amd.build``.
 from .batch import BatchOperator, batch_sirt, paralleltomo_batch
 from .operators import AirOperator, as_operator, from_matlab_csc
 from .phantoms import add_noise, phantomgallery
-from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, spectral_radius,
+from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, sirt_multi, spectral_radius,
                    weights)
 from .testprobs import fanlineartomo, paralleltomo
 from .train import train_relaxpar
 
 __all__ = ["BatchOperator", "batch_sirt", "paralleltomo_batch", "AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
-           "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "spectral_radius", "weights",
+           "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "sirt_multi", "spectral_radius", "weights",
            "fanlineartomo", "paralleltomo", "train_relaxpar"]

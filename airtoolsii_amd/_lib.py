@@ -31,7 +31,7 @@ EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "ai
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
            "air_layout_info", "air_mlp_forward", "air_mlp_grad", "air_mlp_train", "air_sirt_records",
-           "air_build_paralleltomo_batch", "air_batch_info"]
+           "air_build_paralleltomo_batch", "air_batch_info", "air_sirt_multi"]
 
 
 class AirError(RuntimeError):
@@ -109,6 +109,7 @@ def load():
     L.air_sirt_records.argtypes = [_P, _P, _P, _P, _P]
     L.air_build_paralleltomo_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, C.c_int, C.c_int]
     L.air_batch_info.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.air_sirt_multi.argtypes = [_P, C.POINTER(SirtOptions), C.c_int32, _P, _P, C.c_int32, _P, _P, _dp]
     L.air_mlp_grad.argtypes = [_P, _P, _P, _P, _i64, C.c_double, _P, _dp]
     L.air_mlp_train.argtypes = [_P, _P, _P, _P, _i64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _P]
     for name in EXPORTS:

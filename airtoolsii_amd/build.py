@@ -32,6 +32,7 @@ SOURCES = {
     "air_tiled.hip": [],
     "air_mlp.hip": [],
     "air_batch.hip": [],
+    "air_multi.hip": [],
     "air_engine.hip": [],
 }
 

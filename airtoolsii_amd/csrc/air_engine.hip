@@ -1144,8 +1144,8 @@ int air_layout_info(air_engine* e, air_layout* out) {
         L.tiled = use_tiled(e) ? 1 : 0;
         L.m = e->m;
         L.n = e->n;
-        L.nnz = e->kind == 1 ? e->A.nnz : -1;
-        L.value_bytes = e->kind == 1 ? (int)dtype_size(e->A.dtype) : 4;
+        L.nnz = e->kind == 1 ? (e->A.nnz ? e->A.nnz : e->T.nnz) : -1;
+        L.value_bytes = e->kind == 1 ? (int)dtype_size(e->A.nnz ? e->A.dtype : e->T.dtype) : 4;
         if (L.tiled) {
             const TiledOp& T = e->T;
             L.n_internal = T.n_int;
@@ -1325,6 +1325,102 @@ int air_batch_info(air_engine* e, int64_t* row_off, int64_t* col_off, double* rh
         if (M) std::copy(e->b_M.begin(), e->b_M.end(), M);
         if (D) std::copy(e->b_D.begin(), e->b_D.end(), D);
         (void)P;
+    });
+}
+
+// ---- many right-hand sides on one operator
+int air_sirt_multi(air_engine* e, const air_sirt_options* o, int32_t nrhs, const double* B, const double* X0,
+                   int32_t K, double* X_out, double* resnorm, double* gpu_ms) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(e->kind == 1 && e->A.nnz > 0, AIR_E_UNSUPPORTED,
+                  "several right-hand sides need a stored matrix (CSR and CSC)");
+        AIR_CHECK(o && B && X_out && nrhs >= 1 && nrhs <= 64 && K >= 0, AIR_E_ARG,
+                  "invalid arguments (1 <= nrhs <= 64)");
+        AIR_CHECK(o->stoprule == AIR_STOP_NONE, AIR_E_UNSUPPORTED, "several right-hand sides run K iterations");
+        AIR_CHECK(o->relax_kind == AIR_RELAX_DEFAULT || o->relax_kind == AIR_RELAX_SCALAR, AIR_E_UNSUPPORTED,
+                  "several right-hand sides take the default or a scalar relaxation parameter");
+        AIR_CHECK(o->lb_kind != AIR_BOUND_VECTOR && o->ub_kind != AIR_BOUND_VECTOR, AIR_E_UNSUPPORTED,
+                  "several right-hand sides take scalar bounds");
+        use_device(e);
+        hipStream_t s = e->stream;
+        const int64_t m = e->m, n = e->n;
+        std::vector<double> M, D;
+        bool hasM, hasD;
+        if (o->method == AIR_CUSTOM) {
+            M.assign(m, 1.0);
+            D.assign(n, 1.0);
+            if (o->M) std::copy(o->M, o->M + m, M.begin());
+            if (o->D) std::copy(o->D, o->D + n, D.begin());
+            hasM = o->M != nullptr;
+            hasD = o->D != nullptr;
+        } else {
+            compute_weights(e, o->method, M, D);
+            hasM = o->method != AIR_LANDWEBER;
+            hasD = o->method == AIR_DROP || o->method == AIR_SART;
+        }
+        double lam = o->relaxpar;
+        if (o->relax_kind == AIR_RELAX_DEFAULT) {
+            double rho = o->method == AIR_SART ? 1.0 : o->rho;
+            if (std::isnan(rho)) {
+                if (o->method != AIR_CUSTOM && !std::isnan(e->wrho[o->method])) rho = e->wrho[o->method];
+                else {
+                    rho = lanczos_rho(e, M, D);
+                    if (o->method != AIR_CUSTOM) e->wrho[o->method] = rho;
+                }
+            }
+            lam = 1.9 / rho;
+        }
+        // RHS-minor FP32 blocks: element (i, j) at i * 64 + j
+        const size_t L = 64;
+        std::vector<float> hB((size_t)m * L, 0.f), hX((size_t)n * L, 0.f);
+        for (int32_t j = 0; j < nrhs; ++j)
+            for (int64_t i = 0; i < m; ++i) hB[(size_t)i * L + j] = (float)B[(size_t)j * m + i];
+        if (X0)
+            for (int32_t j = 0; j < nrhs; ++j)
+                for (int64_t i = 0; i < n; ++i) hX[(size_t)i * L + j] = (float)X0[(size_t)j * n + i];
+        DevBuf dB, dX, dR, dMR, dM, dD, part, nrm;
+        dB.alloc(4 * (size_t)m * L);
+        dX.alloc(4 * (size_t)n * L);
+        dR.alloc(4 * (size_t)m * L);
+        dMR.alloc(4 * (size_t)m * L);
+        AIR_HIP(hipMemcpyAsync(dB.p, hB.data(), 4 * hB.size(), hipMemcpyHostToDevice, s));
+        AIR_HIP(hipMemcpyAsync(dX.p, hX.data(), 4 * hX.size(), hipMemcpyHostToDevice, s));
+        if (hasM) upload_f32(dM, M.data(), m, s);
+        if (hasD) upload_f32(dD, D.data(), n, s);
+        const unsigned gr = multi_grid(m, e->num_cu), gc = multi_grid(n, e->num_cu);
+        part.alloc(8 * (size_t)gr * L);
+        nrm.alloc(8 * L);
+        const float* Mp = hasM ? dM.as<float>() : nullptr;
+        const float* Dp = hasD ? dD.as<float>() : nullptr;
+        multi_resid(e->A, nrhs, dX.as<float>(), dB.as<float>(), Mp, dR.as<float>(), dMR.as<float>(),
+                    part.as<double>(), gr, s);
+        hipEvent_t a, b;
+        AIR_HIP(hipEventCreate(&a));
+        AIR_HIP(hipEventCreate(&b));
+        AIR_HIP(hipEventRecord(a, s));
+        for (int k = 1; k <= K; ++k) {
+            multi_update(e->AT, nrhs, dMR.as<float>(), Dp, dX.as<float>(), lam, o->lb_kind, (float)o->lb, o->ub_kind,
+                         (float)o->ub, gc, s);
+            multi_resid(e->A, nrhs, dX.as<float>(), dB.as<float>(), Mp, dR.as<float>(), dMR.as<float>(),
+                        part.as<double>(), gr, s);
+        }
+        AIR_HIP(hipEventRecord(b, s));
+        multi_norms(part.as<double>(), (int)gr, nrm.as<double>(), s);
+        AIR_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        AIR_HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (gpu_ms) *gpu_ms = ms;
+        AIR_HIP(hipMemcpyAsync(hX.data(), dX.p, 4 * hX.size(), hipMemcpyDeviceToHost, s));
+        std::vector<double> hn(L);
+        AIR_HIP(hipMemcpyAsync(hn.data(), nrm.p, 8 * L, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        for (int32_t j = 0; j < nrhs; ++j)
+            for (int64_t i = 0; i < n; ++i) X_out[(size_t)j * n + i] = (double)hX[(size_t)i * L + j];
+        if (resnorm)
+            for (int32_t j = 0; j < nrhs; ++j) resnorm[j] = std::sqrt(hn[j]);
     });
 }
 }  // extern "C"

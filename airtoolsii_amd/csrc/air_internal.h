@@ -215,6 +215,14 @@ void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStr
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);
 void permute_to_user(const TiledOp& T, const float* x_int, float* x_user, hipStream_t s);
 
+// ---------------------------------------------------------------- many right-hand sides (air_multi.hip)
+unsigned multi_grid(int64_t rows, int num_cu);
+void multi_resid(const DevSparse& A, int nrhs, const float* X, const float* B, const float* M, float* R, float* MRv,
+                 double* part, unsigned grid, hipStream_t s);
+void multi_update(const DevSparse& AT, int nrhs, const float* MRv, const float* D, float* X, double lam, int lb_kind,
+                  float lb, int ub_kind, float ub, unsigned grid, hipStream_t s);
+void multi_norms(const double* part, int nblk, double* out, hipStream_t s);
+
 // ---------------------------------------------------------------- learned relaxation (air_mlp.hip)
 constexpr int MLP_IN = AIR_MLP_IN, MLP_HID = AIR_MLP_HID, MLP_OUT = AIR_MLP_OUT, MLP_PARAMS = AIR_MLP_PARAMS;
 constexpr int MLP_LAMBDA_MAX = AIR_MLP_LAMBDA_MAX;

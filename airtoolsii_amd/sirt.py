@@ -341,3 +341,29 @@ class SirtRun:
         check(load().air_get_kernel_timing(self.A.engine.h, _lib.C.byref(c), _lib.C.byref(r), _lib.C.byref(d),
                                            _lib.C.byref(n)))
         return {"cols_ms": c.value, "rows_ms": r.value, "reduce_ms": d.value, "launches": n.value}
+
+
+def sirt_multi(sirt_method, A, B, K, X0=None, options=None):
+    """SIRT for the columns of B (m x nrhs, nrhs <= 64) sharing A, every pass
+    reading A once for all of them (SpMM on the device).  Options as for sirt
+    with the stop rule 'none', a default or scalar relaxpar and scalar bounds.
+    Returns (X n x nrhs, info with 'resnorm' per column and 'gpu_ms')."""
+    A = as_operator(A)
+    m, n = A.shape
+    B = np.asarray(B, np.float64)
+    if B.ndim == 1:
+        B = B[:, None]
+    if B.shape[0] != m or not 1 <= B.shape[1] <= 64:
+        raise ValueError("B must be m x nrhs with 1 <= nrhs <= 64")
+    nrhs = B.shape[1]
+    K = int(np.ravel(K)[-1])
+    o, keep, *_ = _build_options(A, sirt_method, options, K)
+    Bf = np.asfortranarray(B)
+    X0f = None if X0 is None else np.asfortranarray(np.asarray(X0, np.float64).reshape(n, nrhs))
+    X = np.zeros((n, nrhs), order="F")
+    rn = np.zeros(nrhs)
+    ms = _lib.C.c_double()
+    check(load().air_sirt_multi(A.engine.h, o, nrhs, _lib.C.c_void_p(Bf.ctypes.data),
+                                None if X0f is None else _lib.C.c_void_p(X0f.ctypes.data), K,
+                                _lib.C.c_void_p(X.ctypes.data), ptr(rn), _lib.C.byref(ms)))
+    return X, {"resnorm": rn, "gpu_ms": ms.value, "finaliter": K}

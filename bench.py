@@ -5,6 +5,9 @@
                     [--dtype f32|bf16|f16] [--matrix-free] [--no-cpu-baseline]
     python bench.py --copy-calibration     (a plain 1 GiB device copy, for
                                             scaling HBM byte counters)
+    python bench.py --study batch [--batch-problems P --batch-sequential]
+        256 uneven paralleltomo problems (N 64..384, 30..360 angles) in one
+        block-diagonal operator, Cimmino K=100; optionally against one by one
     python bench.py --study learned [--learn-train T --learn-eval E --learn-K K]
         learned relaxation at paralleltomo N=256, 180 angles, p=362: greedy
         trajectories of T seeded problems, network trained on the GPU, then
@@ -159,6 +162,81 @@ def learned_study(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_study(args):
+    """256 uneven paralleltomo problems in one block-diagonal operator (Cimmino,
+    K=100, FP32) against solving them one by one."""
+    import airtoolsii_amd as air
+    from airtoolsii_amd import phantoms
+    rng = np.random.default_rng(args.batch_seed)
+    P = args.batch_problems
+    Ns = rng.integers(64, 385, P)
+    nAs = rng.integers(30, 361, P)
+    thetas = [np.arange(nA) * (180.0 / nA) for nA in nAs]
+    kinds = ("shepplogan", "smooth", "threephases", "grains")
+    K = args.batch_K
+    t0 = time.perf_counter()
+    A = air.paralleltomo_batch(Ns, thetas, method="cimmino")
+    t_build = time.perf_counter() - t0
+    L = A.layout()
+    bs, xs_true = [], []
+    for q in range(P):
+        N = int(Ns[q])
+        kind = kinds[int(rng.integers(len(kinds)))]
+        ps = int(rng.integers(1, 2 ** 31 - 1))
+        img = {"shepplogan": lambda: phantoms.shepplogan(N), "smooth": lambda: phantoms.smooth(N, 4),
+               "threephases": lambda: phantoms.threephases(N, 100, ps),
+               "grains": lambda: phantoms.grains(N, None, ps)}[kind]()
+        xs_true.append(np.asarray(img, np.float64).ravel(order="F"))
+    y = A.matvec(np.concatenate(xs_true))
+    for q in range(P):
+        b0 = y[A.row_off[q]:A.row_off[q + 1]]
+        eta = float(rng.uniform(0.005, 0.05))
+        e = np.random.default_rng(int(rng.integers(1, 2 ** 31 - 1))).standard_normal(b0.size)
+        bs.append(b0 + eta * np.linalg.norm(b0) * e / np.linalg.norm(e))
+    opts = {"relaxpar": 1.0, "rho": 1.0}
+    run = air.SirtRun({"M": A.M_scaled}, A, A.join_rows(bs), K, None, opts)
+    run.iterate(2)
+    run.kernel_timing(True)
+    t0 = time.perf_counter()
+    gpu_ms = run.iterate(K - 2)
+    wall = time.perf_counter() - t0
+    kt = run.kernel_times()
+    x, info = run.fetch()
+    xs = A.split_cols(x)
+    errs = [float(np.linalg.norm(xs[q] - xs_true[q]) / np.linalg.norm(xs_true[q])) for q in range(P)]
+    cols_b, rows_b, red_b = kernel_bytes(L, "cimmino", 4)
+    it_s = wall / (K - 2)
+    out = {"metric": "batched SIRT problem-iterations/s (256 uneven problems, one launch per pass)",
+           "value": round(P / it_s, 1), "unit": "problem-iterations/s", "higher_is_better": True, "n_gpus": 1,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{P} paralleltomo problems, N in 64..384, 30..360 angles over [0,180), "
+                                  f"p=round(sqrt(2)N), Cimmino (default lambda per problem), K={K}",
+                      "phantoms": list(kinds), "noise": "0.5-5% Gaussian, seeded", "seed": args.batch_seed,
+                      "m": A.shape[0], "n": A.shape[1], "nnz": L["nnz"]},
+           "batch": {"ms_per_iteration": round(it_s * 1e3, 3), "gpu_ms_per_iteration": round(gpu_ms / (K - 2), 3),
+                     "cols_ms": round(kt["cols_ms"] / max(kt["launches"], 1), 3),
+                     "rows_ms": round(kt["rows_ms"] / max(kt["launches"], 1), 3),
+                     "bytes_per_iteration": int(cols_b + rows_b + red_b),
+                     "gbs": round((cols_b + rows_b + red_b) / it_s / 1e9, 1),
+                     "frac_of_8tbs": round((cols_b + rows_b + red_b) / it_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "build_s": round(t_build, 2), "rel_err_median": round(float(np.median(errs)), 4)}}
+    if args.batch_sequential:
+        # the same problems one by one, each on its own operator (iterations timed on the device)
+        seq_ms = 0.0
+        for q in range(P):
+            p = int(np.floor(np.sqrt(2) * Ns[q] + 0.5))
+            Aq = air.paralleltomo(int(Ns[q]), thetas[q], p)
+            rq = air.SirtRun("cimmino", Aq, bs[q], K, None, {"rho": float(A.rho[q])})
+            rq.iterate(2)
+            t0 = time.perf_counter()
+            rq.iterate(K - 2)
+            seq_ms += (time.perf_counter() - t0) * 1e3
+            del rq, Aq
+        out["sequential"] = {"ms_per_iteration_all_problems": round(seq_ms / (K - 2), 3),
+                             "batched_speedup": round(seq_ms / (K - 2) / (it_s * 1e3), 2)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +249,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--study", choices=["learned"], default=None)
+    ap.add_argument("--study", choices=["learned", "batch"], default=None)
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-K", type=int, default=100)
+    ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--batch-sequential", action="store_true")
     ap.add_argument("--learn-train", type=int, default=2000)
     ap.add_argument("--learn-eval", type=int, default=512)
     ap.add_argument("--learn-K", type=int, default=150)
@@ -187,6 +269,9 @@ def main():
         return
     if args.study == "learned":
         learned_study(args)
+        return
+    if args.study == "batch":
+        batch_study(args)
         return
 
     import airtoolsii_amd as air

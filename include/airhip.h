@@ -221,6 +221,17 @@ int air_build_paralleltomo_batch(air_engine* e, int32_t nprob, const int32_t* N,
 /* row_off, col_off: nprob+1 offsets; rho: nprob; M: m values (lambda-scaled); D: n values */
 int air_batch_info(air_engine* e, int64_t* row_off, int64_t* col_off, double* rho, double* M, double* D);
 
+/* ---- many right-hand sides on one operator --------------------------------
+ * No counterpart in the reference (one sirt call per right-hand side): SIRT
+ * (sirt.m:199-261 with the stop rule 'none') for nrhs <= 64 right-hand sides
+ * sharing a stored A, each pass reading A once for all of them (SpMM).
+ * B: m x nrhs column-major; X0 (optional) and X_out: n x nrhs column-major;
+ * resnorm (optional): ||b_j - A x_j|| after K iterations; gpu_ms: device time
+ * of the K iterations.  Options: method or custom M/D, default or scalar
+ * relaxation, scalar bounds. */
+int air_sirt_multi(air_engine* e, const air_sirt_options* opts, int32_t nrhs, const double* B, const double* X0,
+                   int32_t K, double* X_out, double* resnorm, double* gpu_ms);
+
 /* ---- learned relaxation network ---------------------------------------
  * No counterpart in the reference: the learned lambda_k / stop rule that
  * replaces the fixed relaxation of calc_relaxpar.m:105-171 and the stop
