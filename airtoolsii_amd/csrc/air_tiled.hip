@@ -199,6 +199,22 @@ __device__ __forceinline__ double mask_if(double v, unsigned keep) {
     return keep ? v : 0.0;
 }
 
+// DPP move of v (bound_ctrl: lanes whose source is outside the row read 0;
+// rows outside ROWMASK read 0)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+// one step of the forward segmented scan inside rows of 16 lanes: add the
+// value D lanes down unless a segment starts in lanes (l-D, l]
+template <int D>
+__device__ __forceinline__ float seg_step(float v, uint64_t sm, int lane) {
+    const float t = dpp_f<0x110 + D, 0xf>(v);                 // row_shr:D
+    const int sh = lane - D + 1;
+    const uint64_t win = sh >= 0 ? (sm >> sh) & ((1ull << D) - 1ull) : 1ull;
+    return win == 0 ? v + t : v;
+}
+
 // Segmented sums over one wave's run of whole segments, entries [E0, E1) of
 // (val, code), walked as a flat stream in rounds of 64*EPL entries, EPL
 // consecutive entries per lane, so that every vector load is useful and no
@@ -312,25 +328,60 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
                 if (i >= ff && i < lfp) mid += pr[i];
             }
         }
-        // reverse segmented scan: r = head of this lane + heads of the next
-        // lanes up to and including the first lane with a start
-        A r = head;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const A rr = __shfl_down(r, d);
-            const uint64_t win = (sm >> lane) & ((1ull << d) - 1ull);
-            if (win == 0 && lane + d < 64) r += rr;
-        }
-        const A after = __shfl_down(r, 1);
-        const A cl = acc + (lane < 63 ? after : (A)0);
-        if (sm == 0) {
-            carry += __shfl(r, 0);
+        if constexpr (std::is_same<A, float>::value) {
+            // forward segmented scan with DPP row shifts and row broadcasts:
+            // v(l) = w(l) + v(l-1) unless lane l holds a start, with w the tail
+            // of a lane with a start and the whole lane otherwise (lane 0 adds
+            // the carried segment).  The segment closing at a lane with a start
+            // then sums v(l-1) and that lane's head.
+            float v = seen ? acc : head;
+            if (lane == 0 && !seen) v += carry;
+            v = seg_step<1>(v, sm, lane);
+            v = seg_step<2>(v, sm, lane);
+            v = seg_step<4>(v, sm, lane);
+            v = seg_step<8>(v, sm, lane);
+            {
+                const float t = dpp_f<0x142, 0xa>(v);         // lane 15 of rows 0, 2 -> rows 1, 3
+                const int rs = lane & ~15;
+                const uint64_t win = (sm >> rs) & ((2ull << (lane - rs)) - 1ull);
+                if (((lane >> 4) & 1) && win == 0) v += t;
+            }
+            {
+                const float t = dpp_f<0x143, 0xc>(v);         // lane 31 -> rows 2, 3
+                if (lane >= 32) {
+                    const uint64_t win = (sm >> 32) & ((2ull << (lane - 32)) - 1ull);
+                    if (win == 0) v += t;
+                }
+            }
+            const float prev = dpp_f<0x138, 0xf>(v);          // v(l-1); 0 at lane 0
+            if (seen) {
+                const int kfirst = knext + (int)excl;          // the lane's first start
+                if (kfirst > knext) close(kfirst - 1, prev + head);
+                else if (kopen >= 0) close(kopen, (lane == 0 ? carry : prev) + head);
+            }
+            carry = readlane_f(v, 63);
+            if (sm != 0) kopen = knext + total - 1;
         } else {
-            const int lf = 63 - __builtin_clzll(sm);
-            if (lane == 0 && kopen >= 0) close(kopen, carry + r);
-            if (seen && lane != lf) close(k, cl);
-            carry = __shfl(cl, lf);
-            kopen = __builtin_amdgcn_readlane(k, lf);
+            // reverse segmented scan: r = head of this lane + heads of the next
+            // lanes up to and including the first lane with a start
+            A r = head;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const A rr = __shfl_down(r, d);
+                const uint64_t win = (sm >> lane) & ((1ull << d) - 1ull);
+                if (win == 0 && lane + d < 64) r += rr;
+            }
+            const A after = __shfl_down(r, 1);
+            const A cl = acc + (lane < 63 ? after : (A)0);
+            if (sm == 0) {
+                carry += __shfl(r, 0);
+            } else {
+                const int lf = 63 - __builtin_clzll(sm);
+                if (lane == 0 && kopen >= 0) close(kopen, carry + r);
+                if (seen && lane != lf) close(k, cl);
+                carry = __shfl(cl, lf);
+                kopen = __builtin_amdgcn_readlane(k, lf);
+            }
         }
         knext += total;
         rb += RND;
