@@ -4,8 +4,9 @@
 
 Each HIP source is compiled to its own object (objects are independent: no
 relocatable device code) and linked into ``airtoolsii_amd/lib/libairhip.so``.
-The ray tracer is compiled with ``-ffp-contract=off`` so that its FP64
-arithmetic rounds operation by operation, as the reference tracer does; all
+The ray and circle tracers are compiled with ``-ffp-contract=off`` so that
+their FP64 arithmetic rounds operation by operation, as the reference tracers
+do; all
 HIP code disables packed-FP32 instruction selection.
 """
 from __future__ import annotations
@@ -27,6 +28,7 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = {
     "air_tracer.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
+    "air_circles.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
     "air_sort.hip": [],
     "air_sirt.hip": [],
     "air_tiled.hip": [],

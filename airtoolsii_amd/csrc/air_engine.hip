@@ -880,6 +880,48 @@ int air_build_fanlineartomo(air_engine* e, int32_t N, const double* theta, int32
     });
 }
 
+int air_build_fancurvedtomo(air_engine* e, int32_t N, const double* theta, int32_t nA, int32_t p, double R,
+                            double d, int matrix_free, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && theta && N > 0 && nA >= 0 && p > 0, AIR_E_ARG, "invalid fancurvedtomo arguments");
+        AIR_CHECK((int64_t)N * N <= INT32_MAX, AIR_E_ARG, "N too large for int32 column indices");
+        // fancurvedtomo.m:104-113
+        AIR_CHECK(!(R < std::sqrt(2.0) / 2), AIR_E_ARG, "R must be greater than half squareroot 2");
+        AIR_CHECK(!(d < 0 || d > 180), AIR_E_ARG, "The angle of the source must be in the interval [0 180]");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        host_rays_fancurvedtomo(N, theta, nA, p, R * N, d, e->geo.host_rays);
+        build_geometry(e, N, (int64_t)nA * p, 0, matrix_free, dtype);
+    });
+}
+
+int air_build_seismictomo(air_engine* e, int32_t N, int32_t s, int32_t p, int matrix_free, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && N > 0 && s > 0 && p > 0, AIR_E_ARG, "invalid seismictomo arguments");
+        AIR_CHECK((int64_t)N * N <= INT32_MAX, AIR_E_ARG, "N too large for int32 column indices");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        host_rays_seismictomo(N, s, p, e->geo.host_rays);
+        build_geometry(e, N, (int64_t)s * p, 0, matrix_free, dtype);
+    });
+}
+
+int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int32_t nA, int32_t numCircles,
+                            int matrix_free, int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && angles && N > 0 && nA >= 0 && numCircles > 0, AIR_E_ARG, "invalid sphericaltomo arguments");
+        AIR_CHECK(!matrix_free, AIR_E_UNSUPPORTED,
+                  "sphericaltomo: the matrix-free form is provided for the line-model tracers only");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        circles_to_csr(N, angles, nA, numCircles, dtype, e->A, e->stream);
+        finish_matrix(e, N);
+    });
+}
+
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz) {
     return guard([&] {
         need_operator(e);

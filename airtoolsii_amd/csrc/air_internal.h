@@ -149,6 +149,11 @@ struct Geometry {
 void host_rays_paralleltomo(int N, const double* theta, int nA, int p, double d, std::vector<RayParam>& out);
 void host_rays_fanlineartomo(int N, const double* theta, int nA, int p, double R, double dw, double sd,
                              std::vector<RayParam>& out);
+void host_rays_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d,
+                             std::vector<RayParam>& out);
+void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out);
+// sphericaltomo: CSR of the circle-quadrature matrix (air_circles.hip).
+void circles_to_csr(int N, const double* angles, int nA, int nC, int dtype, DevSparse& A, hipStream_t s);
 // Trace all rays into CSR (values narrowed to dtype).  Returns nnz.
 void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s);
 // Matrix-free products: y = A u (m) or g = A' v (n, FP64 accumulators).

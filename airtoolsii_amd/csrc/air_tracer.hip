@@ -1,5 +1,5 @@
-// air_tracer.hip -- K1: line-model ray tracer for paralleltomo / fanlineartomo
-// geometries on gfx950, emitting CSR that matches MATLAB's sparse() of the
+// air_tracer.hip -- K1: line-model ray tracer for paralleltomo / fanlineartomo /
+// fancurvedtomo / seismictomo geometries on gfx950, emitting CSR that matches MATLAB's sparse() of the
 // reference tracer entry for entry, plus the matrix-free ray-driven projector.
 //
 // Reference algorithm: testprobs/paralleltomo.m:180-301 and
@@ -122,6 +122,79 @@ void host_rays_fanlineartomo(int N, const double* theta, int nA, int p, double R
             out[(size_t)i * p + j] = r;
         }
     }
+}
+
+// fancurvedtomo.m:140-154,210-241: source at Omega(theta)*[0;R]; ray j turns
+// the central direction -xytheta/R by omega_j, omega = linspace(-d/2,d/2,p)
+// (equal angular steps: the curved detector).
+void host_rays_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d,
+                             std::vector<RayParam>& out) {
+    (void)N;
+    std::vector<double> omega(p);
+    const double d1 = -d / 2.0, d2 = d / 2.0;
+    if (p == 1) omega[0] = d2;
+    else {
+        const double n1 = (double)(p - 1);
+        for (int k = 0; k < p; ++k) omega[k] = d1 + ((double)k * (d2 - d1)) / n1;
+        omega[0] = d1;
+        omega[p - 1] = d2;
+    }
+    std::vector<double> so(p), co(p);
+    for (int j = 0; j < p; ++j) sincosd(omega[j], &so[j], &co[j]);
+    out.resize((size_t)nA * p);
+    for (int i = 0; i < nA; ++i) {
+        double st, ct;
+        sincosd(theta[i], &st, &ct);
+        const double x0t = ct * 0.0 + (-st) * R;
+        const double y0t = st * 0.0 + ct * R;
+        const double abx = (-x0t) / R, aby = (-y0t) / R;
+        for (int j = 0; j < p; ++j) {
+            RayParam r;
+            r.x0 = x0t;
+            r.y0 = y0t;
+            r.a = co[j] * abx + (-so[j]) * aby;
+            r.b = so[j] * abx + co[j] * aby;
+            out[(size_t)i * p + j] = r;
+        }
+    }
+}
+
+// linspace(d1,d2,n) as MATLAB evaluates it: d1 + (k*(d2-d1))/(n-1), both end
+// points exact, n = 1 gives d2, n = 0 nothing.
+static void linspace_into(double d1, double d2, int n, double* out) {
+    if (n <= 0) return;
+    if (n == 1) { out[0] = d2; return; }
+    const double n1 = (double)(n - 1);
+    for (int k = 0; k < n; ++k) out[k] = d1 + ((double)k * (d2 - d1)) / n1;
+    out[0] = d1;
+    out[n - 1] = d2;
+}
+
+// seismictomo.m:98-116,187-202: s sources on the right edge x = N/2, p
+// receivers, floor(p/2) on the left edge and ceil(p/2) on the top edge; ray
+// (i,j) starts at source i with direction (xp_j - x0_i, yp_j - y0_i), row i*p + j.
+void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out) {
+    const double N2 = (double)N / 2.0;
+    const double Ns = ((double)N / s) / 2.0;
+    std::vector<double> y0(s);
+    linspace_into(-N2 + Ns, N2 - Ns, s, y0.data());
+    const int p1 = (p + 1) / 2, p2 = p / 2;
+    const double Np1 = ((double)N / p1) / 2.0, Np2 = ((double)N / p2) / 2.0;
+    std::vector<double> xp(p), yp(p);
+    for (int j = 0; j < p2; ++j) xp[j] = -N2;
+    linspace_into(-N2 + Np1, N2 - Np1, p1, xp.data() + p2);
+    linspace_into(-N2 + Np2, N2 - Np2, p2, yp.data());
+    for (int j = p2; j < p; ++j) yp[j] = N2;
+    out.resize((size_t)s * p);
+    for (int i = 0; i < s; ++i)
+        for (int j = 0; j < p; ++j) {
+            RayParam r;
+            r.x0 = N2;
+            r.y0 = y0[i];
+            r.a = xp[j] - N2;
+            r.b = yp[j] - y0[i];
+            out[(size_t)i * p + j] = r;
+        }
 }
 
 // ---------------------------------------------------------------- device ray walk

@@ -95,8 +95,33 @@ def grains(N, numGrains=None, seed=None):
     return im / im.max()
 
 
+def tectonic(N):
+    """phantomgallery.m:422-454: two plates, the right one (0.75) tilted up at its
+    left end, the left one (1) bending down past column 5*round(N/13)."""
+    N = int(N)
+    r = lambda v: int(_matlab_round(v))
+    N5, N13, N7, N20 = r(N / 5), r(N / 13), r(N / 7), r(N / 20)
+    if N5 < 1 or N13 < 1:
+        raise ValueError("tectonic needs N >= 7")
+    x = np.zeros((N, N))
+    c0 = 5 * N13 - 1                                   # 0-based first column of the right plate
+    x[N5 - 1:N5 + N7, c0:] = 0.75
+    steps = (N20 + 1) // 2                             # odd j in 1..N20 lift the plate by one row
+    if N5 - 1 - steps < 0 or 2 * N5 + (7 * N13 + 2) // 2 > N:
+        raise ValueError("tectonic: plate outside the image for this N")
+    for q in range(steps):
+        j = 2 * q + 1
+        x[N5 - 2 - q, c0 + j:] = 0.75
+    x[N5 - 1:2 * N5, :5 * N13] = 1.0
+    shift = 0
+    for j in range(5 * N13, min(12 * N13, N) + 1):
+        shift += j % 2
+        x[N5 - 1 + shift:2 * N5 + shift, j - 1] = 1.0
+    return x
+
+
 def phantomgallery(name, N, P1=None, P2=None, P3=None):
-    """phantomgallery.m:79-151 dispatch (shepplogan, smooth, threephases, grains)."""
+    """phantomgallery.m:79-151 dispatch (shepplogan, smooth, threephases, grains, tectonic)."""
     if name == "shepplogan":
         return shepplogan(N)
     if name == "smooth":
@@ -105,7 +130,9 @@ def phantomgallery(name, N, P1=None, P2=None, P3=None):
         return threephases(N, 100 if P1 is None else int(P1), P2)
     if name == "grains":
         return grains(N, P1, P2)
-    if name in ("binary", "threephasessmooth", "fourphases", "ppower", "tectonic"):
+    if name == "tectonic":
+        return tectonic(N)
+    if name in ("binary", "threephasessmooth", "fourphases", "ppower"):
         raise NotImplementedError(f"phantom '{name}' is not provided by this engine")
     raise ValueError("Illegal phantom name")
 

@@ -1,7 +1,10 @@
 """Tomography test problems of testprobs/ traced on the GPU.
 
-``paralleltomo`` (testprobs/paralleltomo.m) and ``fanlineartomo``
-(testprobs/fanlineartomo.m) keep the reference argument order and defaults.
+``paralleltomo`` (testprobs/paralleltomo.m), ``fanlineartomo``
+(testprobs/fanlineartomo.m), ``fancurvedtomo`` (testprobs/fancurvedtomo.m) and
+``seismictomo`` (testprobs/seismictomo.m) keep the reference argument order
+and defaults; ``sphericaltomo`` (testprobs/sphericaltomo.m) builds its
+circle-quadrature matrix on the GPU as well.
 With ``isMatrix=1`` the system matrix is traced by the HIP ray tracer into
 CSR (and CSC) in HBM; with ``isMatrix=0`` only the ray geometry is kept and
 every product re-traces the rays (the reference's matrix-free mode).
@@ -9,17 +12,19 @@ every product re-traces the rays (the reference's matrix-free mode).
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from ._lib import DTYPES, Engine, check, f64, load, ptr
 from .operators import AirOperator
-from .phantoms import _matlab_round, shepplogan
+from .phantoms import _matlab_round, shepplogan, tectonic
 
 
-def _finish(A, return_all, extra):
+def _finish(A, return_all, extra, phantom=shepplogan):
     if not return_all:
         return A
-    x = shepplogan(A.geometry["N"]).ravel(order="F")
+    x = phantom(A.geometry["N"]).ravel(order="F")
     b = A.matvec(x)
     return (A, b, x) + extra
 
@@ -57,3 +62,52 @@ def fanlineartomo(N, theta=None, p=None, R=None, dw=None, sd=None, isDisp=0, isM
                     {"kind": "fanlinear", "N": N, "theta": theta, "p": p, "R": R, "dw": dw, "sd": sd})
     # fanlineartomo.m:75-90,133-135: dw and sd come back multiplied by N, R in units of N
     return _finish(A, return_all, (theta, p, R, dw * N, sd * N))
+
+
+def fancurvedtomo(N, theta=None, p=None, R=None, d=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0,
+                  return_all=False):
+    """[A,b,x,theta,p,R,d] = fancurvedtomo(N,theta,p,R,d,isDisp,isMatrix),
+    fancurvedtomo.m:69-137 (R in units of N, d in degrees)."""
+    N = int(N)
+    R = 2.0 if R is None else float(R)
+    if R < math.sqrt(2) / 2:
+        raise ValueError("R must be greater than half squareroot 2")
+    # fancurvedtomo.m:79-83: first and last rays touch the corners (-N/2,N/2), (N/2,N/2)
+    d = 2.0 * ((180.0 / math.pi) * math.atan(1.0 / (2.0 * R - 1.0))) if d is None else float(d)
+    p = int(_matlab_round(np.sqrt(2) * N)) if p is None else int(p)
+    theta = np.arange(0.0, 359.0, 2.0) if theta is None or np.size(theta) == 0 else f64(theta)
+    if d < 0 or d > 180:
+        raise ValueError("The angle of the source must be in the interval [0 180]")
+    e = Engine(device)
+    check(load().air_build_fancurvedtomo(e.h, N, ptr(theta), len(theta), p, R, d, 0 if isMatrix else 1,
+                                         DTYPES[dtype]))
+    A = AirOperator(e, dtype, not isMatrix, {"kind": "fancurved", "N": N, "theta": theta, "p": p, "R": R, "d": d})
+    return _finish(A, return_all, (theta, p, R, d))
+
+
+def seismictomo(N, s=None, p=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0, return_all=False):
+    """[A,b,x,s,p] = seismictomo(N,s,p,isDisp,isMatrix), seismictomo.m:57-93
+    (x is the tectonic phantom)."""
+    N = int(N)
+    if s is not None and np.size(s) > 1:
+        raise ValueError("s must be a scaler")
+    s = N if s is None else int(s)
+    p = 2 * N if p is None else int(p)
+    e = Engine(device)
+    check(load().air_build_seismictomo(e.h, N, s, p, 0 if isMatrix else 1, DTYPES[dtype]))
+    A = AirOperator(e, dtype, not isMatrix, {"kind": "seismic", "N": N, "s": s, "p": p})
+    return _finish(A, return_all, (s, p), tectonic)
+
+
+def sphericaltomo(N, angles=None, numCircles=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0,
+                  return_all=False):
+    """[A,b,x,angles,numCircles] = sphericaltomo(N,angles,numCircles,isDisp,isMatrix),
+    sphericaltomo.m:63-105.  Only the matrix form is provided (isMatrix=0 raises)."""
+    N = int(N)
+    angles = np.arange(0.0, 359.0, 2.0) if angles is None or np.size(angles) == 0 else f64(angles)
+    nC = int(_matlab_round(np.sqrt(2) * N)) if numCircles is None else int(numCircles)
+    e = Engine(device)
+    check(load().air_build_sphericaltomo(e.h, N, ptr(angles), len(angles), nC, 0 if isMatrix else 1,
+                                         DTYPES[dtype]))
+    A = AirOperator(e, dtype, False, {"kind": "spherical", "N": N, "angles": angles, "numCircles": nC})
+    return _finish(A, return_all, (angles, nC))

@@ -130,6 +130,20 @@ int air_build_paralleltomo(air_engine* e, int32_t N, const double* theta, int32_
 int air_build_fanlineartomo(air_engine* e, int32_t N, const double* theta, int32_t nA,
                             int32_t p, double R, double dw, double sd,
                             int matrix_free, int dtype);
+/* testprobs/fancurvedtomo.m:1-339; R in units of N (the engine multiplies by
+ * N, fancurvedtomo.m:108), d the angular span in degrees (default resolved by
+ * the caller: 2*atand(1/(2R-1)), fancurvedtomo.m:79-83). */
+int air_build_fancurvedtomo(air_engine* e, int32_t N, const double* theta, int32_t nA,
+                            int32_t p, double R, double d, int matrix_free, int dtype);
+/* testprobs/seismictomo.m:1-287 (defaults resolved by the caller: s = N,
+ * p = 2N): s sources on the right edge, p receivers on the left and top
+ * edges, one row per source-receiver ray (row = source*p + receiver). */
+int air_build_seismictomo(air_engine* e, int32_t N, int32_t s, int32_t p, int matrix_free, int dtype);
+/* testprobs/sphericaltomo.m:1-270 (defaults resolved by the caller: numCircles
+ * = round(sqrt(2) N)): row m*numCircles + n integrates circle n about centre m
+ * (angles in degrees).  matrix_free must be 0 (AIR_E_UNSUPPORTED otherwise). */
+int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int32_t nA,
+                            int32_t numCircles, int matrix_free, int dtype);
 /* A(v,'size') of auxil/check_inputs.m:74 (nnz = -1 for matrix-free) */
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz);
 /* copy the stored CSR back to the host (values widened to double) */

@@ -467,6 +467,151 @@ ora_csr* ora_fanlineartomo(int N, const double* theta, int nA, int p, double R, 
     return A;
 }
 
+/* fancurvedtomo.m:140-339 (R in absolute units, d in degrees): the rays of one
+ * source position are spaced by equal angles, omega = linspace(-d/2,d/2,p)
+ * (:154), each turning the central direction -xytheta/R (:233-241). */
+ora_csr* ora_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d)
+{
+    double* omega = (double*)malloc(sizeof(double) * (p > 0 ? p : 1));
+    ora_linspace(-d / 2, d / 2, p, omega);
+    ora_csr** blk = (ora_csr**)calloc(nA > 0 ? nA : 1, sizeof(ora_csr*));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        ray_ws w; ws_init(&w, N);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int i = 0; i < nA; ++i) {
+            ora_csr* B = csr_new(p, (int64_t)N * N);
+            double st = ora_sind(theta[i]), ct = ora_cosd(theta[i]);
+            double x0t = ct * 0.0 + (-st) * R;         /* Omega_x(theta)*xy0 */
+            double y0t = st * 0.0 + ct * R;            /* Omega_y(theta)*xy0 */
+            double abx = (-x0t) / R, aby = (-y0t) / R; /* -xytheta/R */
+            for (int j = 0; j < p; ++j) {
+                double so = ora_sind(omega[j]), co = ora_cosd(omega[j]);
+                trace_ray(&w, B, x0t, y0t, co * abx + (-so) * aby, so * abx + co * aby, 0);
+                B->rowptr[j + 1] = B->nnz;
+            }
+            blk[i] = B;
+        }
+        ws_free(&w);
+    }
+    ora_csr* A = csr_concat(blk, nA, p, (int64_t)N * N);
+    free(blk); free(omega);
+    return A;
+}
+
+/* seismictomo.m:96-287: sources (N/2, y0_i) on the right edge, receivers
+ * (xp_j, yp_j) on the left edge (first floor(p/2)) and the top edge (the
+ * other ceil(p/2)); ray (i,j) is the line through both, row i*p + j. */
+ora_csr* ora_seismictomo(int N, int s, int p)
+{
+    double N2 = N / 2.0;
+    double Ns = ((double)N / s) / 2;
+    int p1 = (p + 1) / 2, p2 = p / 2;                   /* ceil(p/2), floor(p/2) */
+    double Np1 = ((double)N / p1) / 2, Np2 = ((double)N / p2) / 2;
+    double* y0 = (double*)malloc(sizeof(double) * (s > 0 ? s : 1));
+    double* xp = (double*)malloc(sizeof(double) * (p > 0 ? p : 1));
+    double* yp = (double*)malloc(sizeof(double) * (p > 0 ? p : 1));
+    ora_linspace(-N2 + Ns, N2 - Ns, s, y0);
+    for (int j = 0; j < p2; ++j) xp[j] = -N2;
+    ora_linspace(-N2 + Np1, N2 - Np1, p1, xp + p2);
+    ora_linspace(-N2 + Np2, N2 - Np2, p2, yp);
+    for (int j = p2; j < p; ++j) yp[j] = N2;
+    ora_csr** blk = (ora_csr**)calloc(s > 0 ? s : 1, sizeof(ora_csr*));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        ray_ws w; ws_init(&w, N);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int i = 0; i < s; ++i) {
+            ora_csr* B = csr_new(p, (int64_t)N * N);
+            for (int j = 0; j < p; ++j) {
+                trace_ray(&w, B, N2, y0[i], xp[j] - N2, yp[j] - y0[i], 0);
+                B->rowptr[j + 1] = B->nnz;
+            }
+            blk[i] = B;
+        }
+        ws_free(&w);
+    }
+    ora_csr* A = csr_concat(blk, s, p, (int64_t)N * N);
+    free(blk); free(y0); free(xp); free(yp);
+    return A;
+}
+
+static int cmp_i32(const void* a, const void* b)
+{
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* sphericaltomo.m:108-270: circle centres xi = (cosd a, sind a) on the unit
+ * circle, numCircles concentric circles of radii linspace(0,2,nC+1)(2:end),
+ * each sampled at nPhi+1 points k = t*dPhi (t = 0..nPhi, both ends of the
+ * period kept, :203); every sample is rounded to a pixel (round half away
+ * from zero), samples outside the image are dropped, and each pixel gets
+ * (2 pi r/nPhi) times the number of samples in it (unique + histc, :227-234). */
+ora_csr* ora_sphericaltomo(int N, const double* angles, int nA, int nC)
+{
+    double* lin = (double*)malloc(sizeof(double) * (nC + 1));
+    ora_linspace(0.0, 2.0, nC + 1, lin);
+    double* radii = lin + 1;
+    double centerImg = ceil(N / 2.0);
+    double dx = sqrt(2.0) / N;
+    int64_t* nPhi = (int64_t*)malloc(sizeof(int64_t) * (nC > 0 ? nC : 1));
+    double* dPhi = (double*)malloc(sizeof(double) * (nC > 0 ? nC : 1));
+    int64_t maxpts = 1;
+    for (int n = 0; n < nC; ++n) {
+        nPhi[n] = (int64_t)ceil((4 * M_PI / dx) * radii[n]);
+        dPhi[n] = 2 * M_PI / (double)nPhi[n];
+        if (nPhi[n] + 1 > maxpts) maxpts = nPhi[n] + 1;
+    }
+    ora_csr** blk = (ora_csr**)calloc(nA > 0 ? nA : 1, sizeof(ora_csr*));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        int32_t* J = (int32_t*)malloc(sizeof(int32_t) * maxpts);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int m = 0; m < nA; ++m) {
+            ora_csr* B = csr_new(nC, (int64_t)N * N);
+            double xix = ora_cosd(angles[m]), xiy = ora_sind(angles[m]);
+            for (int n = 0; n < nC; ++n) {
+                int64_t cnt = 0;
+                for (int64_t t = 0; t <= nPhi[n]; ++t) {
+                    double k = (double)t * dPhi[n];
+                    double xx = (xix + radii[n] * cos(k)) / dx + centerImg;
+                    double yy = (xiy + radii[n] * sin(k)) / dx + centerImg;
+                    double col = round(xx), row = round(yy);
+                    if (col > 0 && col <= N && row > 0 && row <= N)
+                        J[cnt++] = (int32_t)((N - row + 1) + (col - 1) * N);
+                }
+                qsort(J, (size_t)cnt, sizeof(int32_t), cmp_i32);
+                double base = 2 * M_PI * radii[n] / (double)nPhi[n];
+                for (int64_t q = 0; q < cnt;) {
+                    int64_t e = q;
+                    while (e < cnt && J[e] == J[q]) ++e;
+                    csr_push(B, J[q] - 1, base * (double)(e - q));
+                    q = e;
+                }
+                B->rowptr[n + 1] = B->nnz;
+            }
+            blk[m] = B;
+        }
+        free(J);
+    }
+    ora_csr* A = csr_concat(blk, nA, nC, (int64_t)N * N);
+    free(blk); free(lin); free(nPhi); free(dPhi);
+    return A;
+}
+
 int64_t ora_csr_m(const ora_csr* A) { return A->m; }
 int64_t ora_csr_n(const ora_csr* A) { return A->n; }
 int64_t ora_csr_nnz(const ora_csr* A) { return A->nnz; }

@@ -41,6 +41,12 @@ ora_csr* ora_paralleltomo(int N, const double* theta, int nA, int p, double d);
 /* testprobs/fanlineartomo.m:141-358  (R, dw, sd already multiplied by N) */
 ora_csr* ora_fanlineartomo(int N, const double* theta, int nA, int p,
                            double R, double dw, double sd);
+/* testprobs/fancurvedtomo.m:140-339  (R already multiplied by N, d in degrees) */
+ora_csr* ora_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d);
+/* testprobs/seismictomo.m:96-287 */
+ora_csr* ora_seismictomo(int N, int s, int p);
+/* testprobs/sphericaltomo.m:108-270 */
+ora_csr* ora_sphericaltomo(int N, const double* angles, int nA, int nC);
 int64_t ora_csr_m(const ora_csr*);
 int64_t ora_csr_n(const ora_csr*);
 int64_t ora_csr_nnz(const ora_csr*);

@@ -17,6 +17,7 @@ Makefile beside it) and restates the MATLAB host logic around the C loop:
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -75,6 +76,12 @@ def lib():
         L.ora_fanlineartomo.restype = C.c_void_p
         L.ora_fanlineartomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int,
                                         C.c_double, C.c_double, C.c_double]
+        L.ora_fancurvedtomo.restype = C.c_void_p
+        L.ora_fancurvedtomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.ora_seismictomo.restype = C.c_void_p
+        L.ora_seismictomo.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.ora_sphericaltomo.restype = C.c_void_p
+        L.ora_sphericaltomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int]
         for f in ("ora_csr_m", "ora_csr_n", "ora_csr_nnz"):
             getattr(L, f).restype = C.c_int64
             getattr(L, f).argtypes = [C.c_void_p]
@@ -186,6 +193,66 @@ def fanlineartomo(N, theta=None, p=None, R=2.0, dw=2.5, sd=3.0):
         raise ValueError("R must be greater than half squareroot 2")
     return _take_csr(lib().ora_fanlineartomo(int(N), np.ascontiguousarray(theta), len(theta), p,
                                              float(Rn), float(dwn), float(sdn)))
+
+
+def fancurvedtomo(N, theta=None, p=None, R=2.0, d=None):
+    """testprobs/fancurvedtomo.m:69-113 defaults + :140-339 matrix build."""
+    theta = np.arange(0.0, 359.0, 2.0) if theta is None else np.atleast_1d(np.asarray(theta, np.float64))
+    p = int(lib().ora_round(np.sqrt(2) * N)) if p is None else int(p)
+    if R < math.sqrt(2) / 2:
+        raise ValueError("R must be greater than half squareroot 2")
+    if d is None:
+        d = 2.0 * ((180.0 / math.pi) * math.atan(1.0 / (2.0 * R - 1.0)))
+    if d < 0 or d > 180:
+        raise ValueError("The angle of the source must be in the interval [0 180]")
+    return _take_csr(lib().ora_fancurvedtomo(int(N), np.ascontiguousarray(theta), len(theta), p,
+                                             float(R * N), float(d)))
+
+
+def seismictomo(N, s=None, p=None):
+    """testprobs/seismictomo.m:65-74 defaults + :96-287 matrix build."""
+    s = int(N) if s is None else int(s)
+    p = 2 * int(N) if p is None else int(p)
+    return _take_csr(lib().ora_seismictomo(int(N), s, p))
+
+
+def sphericaltomo(N, angles=None, numCircles=None):
+    """testprobs/sphericaltomo.m:63-75 defaults + :108-270 matrix build."""
+    angles = np.arange(0.0, 359.0, 2.0) if angles is None else np.atleast_1d(np.asarray(angles, np.float64))
+    nC = int(lib().ora_round(np.sqrt(2) * N)) if numCircles is None else int(numCircles)
+    return _take_csr(lib().ora_sphericaltomo(int(N), np.ascontiguousarray(angles), len(angles), nC))
+
+
+def tectonic(N):
+    """phantomgallery.m:422-454, element by element in 1-based indices."""
+    rnd = lambda v: int(math.floor(v + 0.5))
+    N5, N13, N7, N20 = rnd(N / 5), rnd(N / 13), rnd(N / 7), rnd(N / 20)
+    x = np.zeros((N, N))
+
+    def setv(i, j, v):                      # x(i,j) = v with 1-based (i,j)
+        if not (1 <= i <= N and 1 <= j <= N):
+            raise IndexError("tectonic index outside the image")
+        x[i - 1, j - 1] = v
+
+    for i in range(N5, N5 + N7 + 1):
+        for j in range(5 * N13, N + 1):
+            setv(i, j, 0.75)
+    i = N5
+    for j in range(1, N20 + 1):
+        if j % 2 != 0:
+            i -= 1
+            for jj in range(5 * N13 + j, N + 1):
+                setv(i, jj, 0.75)
+    for i in range(N5, 2 * N5 + 1):
+        for j in range(1, 5 * N13 + 1):
+            setv(i, j, 1.0)
+    vec = list(range(N5, 2 * N5 + 1))
+    for j in range(5 * N13, min(12 * N13, N) + 1):
+        if j % 2 != 0:
+            vec = [v + 1 for v in vec]
+        for v in vec:
+            setv(v, j, 1.0)
+    return x
 
 
 # ---------------------------------------------------------------- weights

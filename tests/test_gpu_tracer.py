@@ -29,24 +29,45 @@ FAN_CASES = {
     "fan_N31_odd_p": dict(N=31, theta=np.arange(0.0, 360.0, 11.0), p=45),
     "fan_c3_angle_subset": dict(N=1024, theta=np.arange(0.0, 360.0, 0.5)[::48], p=1448),
 }
+CURVED_CASES = {
+    "curved_N64_defaults": dict(N=64, theta=np.arange(0.0, 359.0, 2.0), p=90, R=None, d=None),
+    "curved_N31_R1p5_d40": dict(N=31, theta=np.arange(0.0, 360.0, 11.0), p=45, R=1.5, d=40.0),
+    "curved_N16_full_half_plane": dict(N=16, theta=np.arange(0.0, 360.0, 45.0), p=31, R=2.0, d=180.0),
+    "curved_N512_angle_subset": dict(N=512, theta=np.arange(0.0, 360.0, 1.0)[::24], p=725, R=None, d=None),
+}
+SEISMIC_CASES = {
+    "seismic_N64_defaults": dict(N=64, s=None, p=None),
+    "seismic_N33_s7_p9": dict(N=33, s=7, p=9),
+    "seismic_N8_horizontal_rays": dict(N=8, s=8, p=16),
+    "seismic_N16_one_receiver": dict(N=16, s=5, p=1),
+    "seismic_N256_s32_p512": dict(N=256, s=32, p=512),
+}
 
 
 def build_pair(oracle, kind, cfg, dtype):
     if kind == "par":
         A = air.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"], dtype=dtype)
         Ao = oracle.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"])
-    else:
+    elif kind == "fan":
         A = air.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"], dtype=dtype)
         Ao = oracle.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"])
+    elif kind == "seismic":
+        A = air.seismictomo(cfg["N"], cfg["s"], cfg["p"], dtype=dtype)
+        Ao = oracle.seismictomo(cfg["N"], cfg["s"], cfg["p"])
+    else:
+        R = 2.0 if cfg["R"] is None else cfg["R"]
+        A = air.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], cfg["R"], cfg["d"], dtype=dtype)
+        Ao = oracle.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], R, cfg["d"])
     return A, Ao
 
 
-CASES = [("par", k) for k in PAR_CASES] + [("fan", k) for k in FAN_CASES]
+TABLES = {"par": PAR_CASES, "fan": FAN_CASES, "curved": CURVED_CASES, "seismic": SEISMIC_CASES}
+CASES = [(kind, k) for kind, tab in TABLES.items() for k in tab]
 
 
 @pytest.mark.parametrize("kind,name", CASES)
 def test_csr_bit_exact_fp64(oracle, kind, name):
-    cfg = (PAR_CASES if kind == "par" else FAN_CASES)[name]
+    cfg = TABLES[kind][name]
     print(f"case {kind}/{name}", flush=True)
     A, Ao = build_pair(oracle, kind, cfg, "f64")
     rp, ci, v = A.csr_arrays()
@@ -58,9 +79,10 @@ def test_csr_bit_exact_fp64(oracle, kind, name):
 
 
 @pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
-@pytest.mark.parametrize("kind,name", [("par", "c1_N64_180ang_p91"), ("fan", "fan_N64_even_p")])
+@pytest.mark.parametrize("kind,name", [("par", "c1_N64_180ang_p91"), ("fan", "fan_N64_even_p"),
+                                       ("curved", "curved_N64_defaults"), ("seismic", "seismic_N64_defaults")])
 def test_csr_narrowed_values(oracle, kind, name, dtype):
-    cfg = (PAR_CASES if kind == "par" else FAN_CASES)[name]
+    cfg = TABLES[kind][name]
     A, Ao = build_pair(oracle, kind, cfg, dtype)
     rp, ci, v = A.csr_arrays()
     assert np.array_equal(rp, Ao.indptr) and np.array_equal(ci, Ao.indices)
@@ -98,12 +120,19 @@ def test_transpose_and_products(oracle):
         A(x, "bogus")
 
 
-@pytest.mark.parametrize("kind", ["par", "fan"])
+@pytest.mark.parametrize("kind", ["par", "fan", "curved", "seismic"])
 def test_matrix_free_products_match_matrix(oracle, kind):
-    cfg = PAR_CASES["c1_N64_180ang_p91"] if kind == "par" else FAN_CASES["fan_N64_even_p"]
+    cfg = {"par": PAR_CASES["c1_N64_180ang_p91"], "fan": FAN_CASES["fan_N64_even_p"],
+           "curved": CURVED_CASES["curved_N64_defaults"], "seismic": SEISMIC_CASES["seismic_N64_defaults"]}[kind]
     if kind == "par":
         Af = air.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"], 0, 0)
         Ao = oracle.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"])
+    elif kind == "curved":
+        Af = air.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], None, None, 0, 0)
+        Ao = oracle.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"])
+    elif kind == "seismic":
+        Af = air.seismictomo(cfg["N"], cfg["s"], cfg["p"], 0, 0)
+        Ao = oracle.seismictomo(cfg["N"], cfg["s"], cfg["p"])
     else:
         Af = air.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"], None, None, None, 0, 0)
         Ao = oracle.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"])
@@ -127,3 +156,63 @@ def test_matlab_csc_upload_matches_tracer(oracle):
     Ar = Ao.tocsr()
     Ar.sort_indices()
     assert np.array_equal(rp, Ar.indptr) and np.array_equal(ci, Ar.indices) and np.array_equal(v, Ar.data)
+
+
+def test_curved_return_all_and_errors():
+    A, b, x, theta, p, R, d = air.fancurvedtomo(32, return_all=True)
+    assert A.shape == (180 * 45, 32 * 32) and p == 45 and R == 2.0
+    assert abs(d - 2 * np.degrees(np.arctan(1 / 3))) < 1e-12
+    np.testing.assert_allclose(b, A @ x, rtol=1e-6)
+    with pytest.raises(ValueError, match="half squareroot"):
+        air.fancurvedtomo(8, [0.0], 5, 0.6)
+    with pytest.raises(ValueError, match="interval"):
+        air.fancurvedtomo(8, [0.0], 5, 2.0, 200.0)
+
+
+def test_seismic_return_all(oracle):
+    A, b, x, s, p = air.seismictomo(40, return_all=True)
+    assert A.shape == (40 * 80, 1600) and (s, p) == (40, 80)
+    np.testing.assert_array_equal(x, oracle.tectonic(40).ravel(order="F"))
+    np.testing.assert_allclose(b, oracle.seismictomo(40) @ x, rtol=1e-6)
+
+
+SPHERICAL_CASES = {
+    "spherical_N16_nC23": dict(N=16, angles=np.arange(0.0, 360.0, 37.0), nC=23),
+    "spherical_N33_odd": dict(N=33, angles=np.arange(0.0, 360.0, 10.0), nC=20),
+    "spherical_N64_defaults": dict(N=64, angles=None, nC=None),
+    "spherical_N256_angle_subset": dict(N=256, angles=np.arange(0.0, 360.0, 2.0)[::15], nC=None),
+}
+
+
+@pytest.mark.parametrize("name", list(SPHERICAL_CASES))
+def test_spherical_csr_bit_exact_fp64(oracle, name):
+    cfg = SPHERICAL_CASES[name]
+    print(f"case spherical/{name}", flush=True)
+    A = air.sphericaltomo(cfg["N"], cfg["angles"], cfg["nC"], dtype="f64")
+    Ao = oracle.sphericaltomo(cfg["N"], cfg["angles"], cfg["nC"])
+    rp, ci, v = A.csr_arrays()
+    assert A.shape == Ao.shape and A.nnz == Ao.nnz
+    assert np.array_equal(rp, Ao.indptr)
+    assert np.array_equal(ci, Ao.indices)
+    assert np.array_equal(v.view(np.uint64), Ao.data.view(np.uint64))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_spherical_narrowed_values_and_products(oracle, dtype):
+    A = air.sphericaltomo(48, np.arange(0.0, 360.0, 5.0), 30, dtype=dtype)
+    Ao = oracle.sphericaltomo(48, np.arange(0.0, 360.0, 5.0), 30)
+    rp, ci, v = A.csr_arrays()
+    assert np.array_equal(rp, Ao.indptr) and np.array_equal(ci, Ao.indices)
+    ref = Ao.data.astype(np.float32).astype(np.float64) if dtype == "f32" else bf16_round(Ao.data)
+    assert np.array_equal(v, ref)
+    x = np.random.default_rng(2).uniform(0, 1, Ao.shape[1])
+    tol = 1e-6 if dtype == "f32" else 1e-2
+    assert np.linalg.norm(A @ x - Ao @ x) <= tol * np.linalg.norm(Ao @ x)
+
+
+def test_spherical_return_all_and_matrix_free_refused():
+    A, b, x, angles, nC = air.sphericaltomo(24, return_all=True)
+    assert A.shape == (180 * 34, 576) and nC == 34 and len(angles) == 180
+    np.testing.assert_allclose(b, A @ x, rtol=1e-6)
+    with pytest.raises(Exception, match="matrix-free"):
+        air.sphericaltomo(8, [0.0], 4, 0, 0)

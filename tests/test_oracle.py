@@ -227,6 +227,212 @@ def test_fan_rejects_source_inside(oracle):
         oracle.fanlineartomo(8, [0], 5, 0.5)
 
 
+# ------------------------------------------------------------------ fancurvedtomo
+def curved_rays(oracle, N, theta, p, R, d):
+    RN = R * N
+    omega = [d / 2] if p == 1 else [-d / 2 + (k * d) / (p - 1) for k in range(p)]
+    out = []
+    for th in theta:
+        st, ct = oracle.sind(th), oracle.cosd(th)
+        x0t, y0t = ct * 0.0 + (-st) * RN, st * 0.0 + ct * RN
+        abx, aby = (-x0t) / RN, (-y0t) / RN
+        for w in omega:
+            so, co = oracle.sind(w), oracle.cosd(w)
+            out.append((x0t, y0t, co * abx + (-so) * aby, so * abx + co * aby))
+    return out
+
+
+@pytest.mark.parametrize("N,p,R,d", [(16, 23, 2.0, None), (15, 22, 1.5, 40.0), (24, 7, 3.0, 10.0)])
+def test_curved_row_sums_are_chord_lengths_at_equal_angles(oracle, N, p, R, d):
+    theta = np.arange(0, 360, 17.0)
+    A = oracle.fancurvedtomo(N, theta, p, R, d)
+    d = 2 * math.degrees(math.atan(1 / (2 * R - 1))) if d is None else d
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    # independent geometry: source at angle theta on a circle of radius R*N,
+    # rays at equal angular steps over [-d/2, d/2] about the line to the centre
+    r = 0
+    for th in theta:
+        t = math.radians(th)
+        S = np.array([-math.sin(t) * R * N, math.cos(t) * R * N])
+        phi0 = math.atan2(-S[1], -S[0])
+        for w in np.linspace(-d / 2, d / 2, p):
+            phi = phi0 + math.radians(w)
+            ref = chord_length(S[0], S[1], math.cos(phi), math.sin(phi), N / 2)
+            assert abs(rs[r] - ref) <= 1e-9 * max(1.0, ref), (r, rs[r], ref)
+            r += 1
+
+
+def test_curved_default_span_touches_the_top_corners(oracle):
+    # fancurvedtomo.m:25-28: with the default span the first and last rays from
+    # the source at (0, R N) hit the corners (-N/2, N/2) and (N/2, N/2), so they
+    # carry (almost) nothing while their neighbours cross the image
+    N, p = 32, 45
+    A = oracle.fancurvedtomo(N, [0.0], p, 2.0)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    assert rs[0] < 1e-6 and rs[-1] < 1e-6
+    assert rs[1] > 0.1 and rs[-2] > 0.1
+    B = oracle.fancurvedtomo(N, [0.0], p, 2.0, 2 * math.degrees(math.atan(1 / 3)) * 1.02)
+    assert np.asarray(B.sum(axis=1)).ravel()[0] == 0.0      # a wider fan misses the image
+
+
+@pytest.mark.parametrize("N,p", [(13, 17), (12, 16), (13, 16)])
+def test_curved_mirror_symmetry(oracle, N, p):
+    # theta = 0: ray j and ray p-1-j are mirror images in x = 0 (column fx*N+r ->
+    # (N-1-fx)*N+r); N and p are chosen so that no ray runs along the grid line x = 0
+    A = oracle.fancurvedtomo(N, [0.0], p, 2.0).tocsr()
+    for j in range(p):
+        k = p - 1 - j
+        a = dict(zip(A.indices[A.indptr[j]:A.indptr[j + 1]].tolist(), A.data[A.indptr[j]:A.indptr[j + 1]]))
+        b = dict(zip(A.indices[A.indptr[k]:A.indptr[k + 1]].tolist(), A.data[A.indptr[k]:A.indptr[k + 1]]))
+        a = {c: v for c, v in a.items() if v > 1e-9}
+        b = {(N - 1 - c // N) * N + c % N: v for c, v in b.items() if v > 1e-9}
+        assert set(a) == set(b)
+        for c in a:
+            assert abs(a[c] - b[c]) <= 1e-11
+
+
+@pytest.mark.parametrize("N", [4, 7])
+def test_curved_matches_exact_rational_bruteforce(oracle, N):
+    theta = np.array([0.0, 33.0, 90.0, 151.0, 270.0, 300.0])
+    p = 2 * N + 3
+    A = oracle.fancurvedtomo(N, theta, p, 2.0, 35.0).tocsr()
+    for r, (x0, y0, a, b) in enumerate(curved_rays(oracle, N, theta, p, 2.0, 35.0)):
+        ref = {c: v for c, v in exact_ray_cells(N, x0, y0, a, b).items() if v > 1e-9}
+        got = dict(zip(A.indices[A.indptr[r]:A.indptr[r + 1]].tolist(), A.data[A.indptr[r]:A.indptr[r + 1]]))
+        got = {c: v for c, v in got.items() if v > 1e-9}
+        assert set(got) == set(ref), (r, sorted(got), sorted(ref))
+        for c in ref:
+            assert abs(got[c] - ref[c]) <= 1e-12
+
+
+def test_curved_argument_errors(oracle):
+    with pytest.raises(ValueError, match="half squareroot"):
+        oracle.fancurvedtomo(8, [0], 5, 0.5)
+    with pytest.raises(ValueError, match="interval"):
+        oracle.fancurvedtomo(8, [0], 5, 2.0, 190.0)
+    with pytest.raises(ValueError, match="interval"):
+        oracle.fancurvedtomo(8, [0], 5, 2.0, -1.0)
+
+
+# ------------------------------------------------------------------ seismictomo
+def seismic_rays(N, s, p):
+    N2 = N / 2
+    lin = lambda a, b, n: [b] if n == 1 else [a + (k * (b - a)) / (n - 1) for k in range(n)]
+    y0 = lin(-N2 + (N / s) / 2, N2 - (N / s) / 2, s)
+    p1, p2 = (p + 1) // 2, p // 2
+    xp = [-N2] * p2 + (lin(-N2 + (N / p1) / 2, N2 - (N / p1) / 2, p1) if p1 else [])
+    yp = (lin(-N2 + (N / p2) / 2, N2 - (N / p2) / 2, p2) if p2 else []) + [N2] * p1
+    return [(N2, y, xr - N2, yr - y) for y in y0 for xr, yr in zip(xp, yp)]
+
+
+@pytest.mark.parametrize("N,s,p", [(16, 16, 32), (15, 7, 9), (24, 5, 1), (8, 1, 6)])
+def test_seismic_row_sums_are_source_receiver_distances(oracle, N, s, p):
+    # both end points lie on the boundary, so the chord is the whole segment
+    A = oracle.seismictomo(N, s, p)
+    assert A.shape == (s * p, N * N)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    for r, (x0, y0, a, b) in enumerate(seismic_rays(N, s, p)):
+        assert abs(rs[r] - math.hypot(a, b)) <= 1e-9 * math.hypot(a, b), (r, rs[r], math.hypot(a, b))
+
+
+@pytest.mark.parametrize("N", [4, 7])
+def test_seismic_matches_exact_rational_bruteforce(oracle, N):
+    s, p = 3, 5
+    A = oracle.seismictomo(N, s, p).tocsr()
+    for r, (x0, y0, a, b) in enumerate(seismic_rays(N, s, p)):
+        ref = {c: v for c, v in exact_ray_cells(N, x0, y0, a, b).items() if v > 1e-9}
+        got = dict(zip(A.indices[A.indptr[r]:A.indptr[r + 1]].tolist(), A.data[A.indptr[r]:A.indptr[r + 1]]))
+        got = {c: v for c, v in got.items() if v > 1e-9}
+        assert set(got) == set(ref), (r, sorted(got), sorted(ref))
+        for c in ref:
+            assert abs(got[c] - ref[c]) <= 1e-12
+
+
+def test_seismic_horizontal_ray_stays_in_one_image_row(oracle):
+    # N = 8, s = 8, p = 16: source 4 sits at y = -0.5 and left receiver 4 at
+    # y = -0.5, so that ray runs along image row N/2 + 1 (1-based, from the top)
+    N = 8
+    A = oracle.seismictomo(N, 8, 16).tocsr()
+    r = 3 * 16 + 3
+    cols = A.indices[A.indptr[r]:A.indptr[r + 1]]
+    assert len(cols) == N and np.all(cols % N == N // 2)
+    np.testing.assert_array_equal(A.data[A.indptr[r]:A.indptr[r + 1]], np.ones(N))
+
+
+@pytest.mark.parametrize("N", [7, 10, 20, 64, 100])
+def test_tectonic_phantom_matches_oracle_restatement(oracle, N):
+    import airtoolsii_amd.phantoms as P
+    a, b = P.tectonic(N), oracle.tectonic(N)
+    assert np.array_equal(a, b)
+    assert set(np.unique(a)) <= {0.0, 0.75, 1.0}
+
+
+def test_tectonic_phantom_shape_n64(oracle):
+    # phantomgallery.m:425-453 at N = 64: N5 = 13, N13 = 5, N7 = 9, N20 = 3
+    x = oracle.tectonic(64)
+    assert np.all(x[12:22, 24:] >= 0.75)            # right plate, rows 13..22 from column 25
+    assert np.all(x[11, 25:] == 0.75) and x[11, 24] == 0 and np.all(x[10, 27:] == 0.75) and x[10, 26] == 0
+    assert np.all(x[12:26, :25] == 1.0)             # left plate, rows 13..26, columns 1..25
+    assert np.all(x[12 + 18:26 + 18, 59] == 1.0)    # column 60: shifted by the 18 odd j in 25..60
+    assert x[12 + 17, 59] == 0.0 and x[12 + 18, 60] == 0.0
+
+
+# ------------------------------------------------------------------ sphericaltomo
+def arc_inside(cx, cy, rho, lo, hi, samples=200000):
+    """Length of the circle (cx,cy,rho) inside [lo,hi)^2 by dense angular sampling."""
+    phi = (np.arange(samples) + 0.5) * (2 * np.pi / samples)
+    x, y = cx + rho * np.cos(phi), cy + rho * np.sin(phi)
+    inside = (x >= lo) & (x < hi) & (y >= lo) & (y < hi)
+    return 2 * np.pi * rho * inside.mean()
+
+
+@pytest.mark.parametrize("N,nC", [(16, 23), (17, 10), (32, 45)])
+def test_spherical_row_sums_are_arc_lengths_inside_the_image(oracle, N, nC):
+    angles = np.arange(0.0, 360.0, 37.0)
+    A = oracle.sphericaltomo(N, angles, nC)
+    assert A.shape == (len(angles) * nC, N * N)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    # geometry in pixel units: centres on a circle of radius N/sqrt(2) about
+    # pixel ceil(N/2), circle n of radius 2n/nC * N/sqrt(2); a sample lands in
+    # the image when its rounded pixel is in 1..N, i.e. inside [0.5, N+0.5)^2
+    scale = N / math.sqrt(2)
+    c0 = math.ceil(N / 2)
+    r = 0
+    for a in angles:
+        cx, cy = c0 + scale * math.cos(math.radians(a)), c0 + scale * math.sin(math.radians(a))
+        for n in range(1, nC + 1):
+            rho = 2 * n / nC * scale
+            nphi = math.ceil(4 * math.pi * scale * (2 * n / nC))
+            base = 2 * math.pi * (2 * n / nC) / nphi
+            ref = arc_inside(cx, cy, rho, 0.5, N + 0.5) / scale
+            # up to 8 boundary crossings, each off by at most one sample, plus the repeated sample at 2 pi
+            assert abs(rs[r] - ref) <= 9 * base + 1e-3 * ref, (a, n, rs[r], ref)
+            r += 1
+
+
+@pytest.mark.parametrize("N,nC", [(8, 11), (9, 6), (16, 23)])
+def test_spherical_matches_vectorised_sampling(oracle, N, nC):
+    angles = np.array([0.0, 45.0, 90.0, 133.0, 180.0, 270.0, 300.5])
+    A = oracle.sphericaltomo(N, angles, nC).tocsr()
+    radii = np.linspace(0, 2, nC + 1)[1:]
+    dx = np.sqrt(2) / N
+    for m, a in enumerate(angles):
+        xi = np.array([oracle.cosd(a), oracle.sind(a)])
+        for n, rad in enumerate(radii):
+            nphi = int(np.ceil((4 * np.pi / dx) * rad))
+            k = np.arange(nphi + 1) * (2 * np.pi / nphi)
+            xx = (xi[0] + rad * np.cos(k)) / dx + np.ceil(N / 2)
+            yy = (xi[1] + rad * np.sin(k)) / dx + np.ceil(N / 2)
+            col = np.sign(xx) * np.floor(np.abs(xx) + 0.5)
+            row = np.sign(yy) * np.floor(np.abs(yy) + 0.5)
+            keep = (col > 0) & (col <= N) & (row > 0) & (row <= N)
+            J = ((N - row[keep] + 1) + (col[keep] - 1) * N).astype(np.int64) - 1
+            u, c = np.unique(J, return_counts=True)
+            i = m * nC + n
+            np.testing.assert_array_equal(A.indices[A.indptr[i]:A.indptr[i + 1]], u)
+            np.testing.assert_allclose(A.data[A.indptr[i]:A.indptr[i + 1]], 2 * np.pi * rad / nphi * c, rtol=1e-15)
+
+
 # ------------------------------------------------------------------ weights
 def small_problem(oracle, N=16, p=23, d=30.0):
     return oracle.paralleltomo(N, np.arange(0, 180, 7.0), p, d)
