@@ -3,7 +3,7 @@
 Call surface (AIR Tools II, MATLAB -> Python):
     sirt, landweber, cimmino, cav, drop, sart          sirt/*.m
     paralleltomo, fanlineartomo, fancurvedtomo,
-    seismictomo, sphericaltomo                         testprobs/*.m (built on the GPU)
+    seismictomo, sphericaltomo, seismicwavetomo        testprobs/*.m (built on the GPU)
     phantomgallery                                     testprobs/phantomgallery.m
     train_relaxpar                                     train/train_relaxpar.m
 Operators follow the afun convention A(v,'notransp'|'transp'|'size').
@@ -15,9 +15,10 @@ from .operators import AirOperator, as_operator, from_matlab_csc
 from .phantoms import add_noise, phantomgallery
 from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, sirt_multi, spectral_radius,
                    weights)
-from .testprobs import fancurvedtomo, fanlineartomo, paralleltomo, seismictomo, sphericaltomo
+from .testprobs import (fancurvedtomo, fanlineartomo, paralleltomo, seismictomo, seismicwavetomo,
+                        sphericaltomo)
 from .train import train_relaxpar
 
 __all__ = ["BatchOperator", "batch_sirt", "paralleltomo_batch", "AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
            "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "sirt_multi", "spectral_radius", "weights",
-           "fancurvedtomo", "fanlineartomo", "paralleltomo", "seismictomo", "sphericaltomo", "train_relaxpar"]
+           "fancurvedtomo", "fanlineartomo", "paralleltomo", "seismictomo", "seismicwavetomo", "sphericaltomo", "train_relaxpar"]

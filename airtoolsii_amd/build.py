@@ -29,6 +29,7 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno
 SOURCES = {
     "air_tracer.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
     "air_circles.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
+    "air_fresnel.hip": ["-ffp-contract=off"],
     "air_sort.hip": [],
     "air_sirt.hip": [],
     "air_tiled.hip": [],

@@ -922,6 +922,20 @@ int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int3
     });
 }
 
+int air_build_seismicwavetomo(air_engine* e, int32_t N, int32_t s, int32_t p, double omega, int matrix_free,
+                              int dtype) {
+    return guard([&] {
+        AIR_CHECK(e && N > 0 && s > 0 && p > 0, AIR_E_ARG, "invalid seismicwavetomo arguments");
+        AIR_CHECK(!matrix_free, AIR_E_UNSUPPORTED,
+                  "seismicwavetomo: the matrix-free form is provided for the line-model tracers only");
+        use_device(e);
+        reset_operator(e);
+        (void)dtype_size(dtype);
+        fresnel_to_csr(N, s, p, omega, dtype, e->A, e->stream);
+        finish_matrix(e, N);
+    });
+}
+
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz) {
     return guard([&] {
         need_operator(e);

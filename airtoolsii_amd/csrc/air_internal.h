@@ -151,7 +151,13 @@ void host_rays_fanlineartomo(int N, const double* theta, int nA, int p, double R
                              std::vector<RayParam>& out);
 void host_rays_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d,
                              std::vector<RayParam>& out);
+// seismictomo.m:98-116 / seismicwavetomo.m:119-132: source heights y0 (sources
+// at x = N/2), receiver positions (xp, yp) on the left and top edges.
+void seismic_positions(int N, int s, int p, std::vector<double>& y0, std::vector<double>& xp,
+                       std::vector<double>& yp);
 void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out);
+// seismicwavetomo: CSR of the Fresnel-zone sensitivity matrix (air_fresnel.hip).
+void fresnel_to_csr(int N, int s, int p, double omega, int dtype, DevSparse& A, hipStream_t st);
 // sphericaltomo: CSR of the circle-quadrature matrix (air_circles.hip).
 void circles_to_csr(int N, const double* angles, int nA, int nC, int dtype, DevSparse& A, hipStream_t s);
 // Trace all rays into CSR (values narrowed to dtype).  Returns nnz.

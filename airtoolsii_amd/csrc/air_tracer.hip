@@ -173,18 +173,26 @@ static void linspace_into(double d1, double d2, int n, double* out) {
 // seismictomo.m:98-116,187-202: s sources on the right edge x = N/2, p
 // receivers, floor(p/2) on the left edge and ceil(p/2) on the top edge; ray
 // (i,j) starts at source i with direction (xp_j - x0_i, yp_j - y0_i), row i*p + j.
-void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out) {
+void seismic_positions(int N, int s, int p, std::vector<double>& y0, std::vector<double>& xp,
+                       std::vector<double>& yp) {
     const double N2 = (double)N / 2.0;
     const double Ns = ((double)N / s) / 2.0;
-    std::vector<double> y0(s);
+    y0.assign(s, 0.0);
     linspace_into(-N2 + Ns, N2 - Ns, s, y0.data());
     const int p1 = (p + 1) / 2, p2 = p / 2;
     const double Np1 = ((double)N / p1) / 2.0, Np2 = ((double)N / p2) / 2.0;
-    std::vector<double> xp(p), yp(p);
+    xp.assign(p, 0.0);
+    yp.assign(p, 0.0);
     for (int j = 0; j < p2; ++j) xp[j] = -N2;
     linspace_into(-N2 + Np1, N2 - Np1, p1, xp.data() + p2);
     linspace_into(-N2 + Np2, N2 - Np2, p2, yp.data());
     for (int j = p2; j < p; ++j) yp[j] = N2;
+}
+
+void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out) {
+    const double N2 = (double)N / 2.0;
+    std::vector<double> y0, xp, yp;
+    seismic_positions(N, s, p, y0, xp, yp);
     out.resize((size_t)s * p);
     for (int i = 0; i < s; ++i)
         for (int j = 0; j < p; ++j) {

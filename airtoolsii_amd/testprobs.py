@@ -3,8 +3,9 @@
 ``paralleltomo`` (testprobs/paralleltomo.m), ``fanlineartomo``
 (testprobs/fanlineartomo.m), ``fancurvedtomo`` (testprobs/fancurvedtomo.m) and
 ``seismictomo`` (testprobs/seismictomo.m) keep the reference argument order
-and defaults; ``sphericaltomo`` (testprobs/sphericaltomo.m) builds its
-circle-quadrature matrix on the GPU as well.
+and defaults; ``sphericaltomo`` (testprobs/sphericaltomo.m) and
+``seismicwavetomo`` (testprobs/seismicwavetomo.m) build their circle-quadrature
+and Fresnel-zone matrices on the GPU as well (matrix form only).
 With ``isMatrix=1`` the system matrix is traced by the HIP ray tracer into
 CSR (and CSC) in HBM; with ``isMatrix=0`` only the ray geometry is kept and
 every product re-traces the rays (the reference's matrix-free mode).
@@ -111,3 +112,17 @@ def sphericaltomo(N, angles=None, numCircles=None, isDisp=0, isMatrix=1, *, dtyp
                                          DTYPES[dtype]))
     A = AirOperator(e, dtype, False, {"kind": "spherical", "N": N, "angles": angles, "numCircles": nC})
     return _finish(A, return_all, (angles, nC))
+
+
+def seismicwavetomo(N, s=None, p=None, omega=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0,
+                    return_all=False):
+    """[A,b,x,s,p,omega] = seismicwavetomo(N,s,p,omega,isDisp,isMatrix),
+    seismicwavetomo.m:70-110 (x is the tectonic phantom).  Matrix form only."""
+    N = int(N)
+    s = N if s is None else int(s)
+    p = 2 * N if p is None else int(p)
+    omega = 10.0 if omega is None else float(omega)
+    e = Engine(device)
+    check(load().air_build_seismicwavetomo(e.h, N, s, p, omega, 0 if isMatrix else 1, DTYPES[dtype]))
+    A = AirOperator(e, dtype, False, {"kind": "seismicwave", "N": N, "s": s, "p": p, "omega": omega})
+    return _finish(A, return_all, (s, p, omega), tectonic)

@@ -144,6 +144,11 @@ int air_build_seismictomo(air_engine* e, int32_t N, int32_t s, int32_t p, int ma
  * (angles in degrees).  matrix_free must be 0 (AIR_E_UNSUPPORTED otherwise). */
 int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int32_t nA,
                             int32_t numCircles, int matrix_free, int dtype);
+/* testprobs/seismicwavetomo.m:1-288 (defaults resolved by the caller: s = N,
+ * p = 2N, omega = 10): Fresnel-zone sensitivity of each source-receiver
+ * pair, row = source*p + receiver.  matrix_free must be 0. */
+int air_build_seismicwavetomo(air_engine* e, int32_t N, int32_t s, int32_t p, double omega,
+                              int matrix_free, int dtype);
 /* A(v,'size') of auxil/check_inputs.m:74 (nnz = -1 for matrix-free) */
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz);
 /* copy the stored CSR back to the host (values widened to double) */

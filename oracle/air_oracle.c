@@ -612,6 +612,70 @@ ora_csr* ora_sphericaltomo(int N, const double* angles, int nA, int nC)
     return A;
 }
 
+/* seismicwavetomo.m:114-288: row (i,j) holds the Fresnel-zone kernel of the
+ * wave from source i to receiver j at every pixel centre, normalised to sum
+ * |S-R| over the image and cut at 1e-6, pixels in column-major order. */
+ora_csr* ora_seismicwavetomo(int N, int s, int p, double omega)
+{
+    const double tol = 1e-6, alpha = 10;
+    double N2 = N / 2.0;
+    double Ns = ((double)N / s) / 2;
+    int p1 = (p + 1) / 2, p2 = p / 2;
+    double Np1 = ((double)N / p1) / 2, Np2 = ((double)N / p2) / 2;
+    double* y0 = (double*)malloc(sizeof(double) * (s > 0 ? s : 1));
+    double* xp = (double*)malloc(sizeof(double) * (p > 0 ? p : 1));
+    double* yp = (double*)malloc(sizeof(double) * (p > 0 ? p : 1));
+    ora_linspace(-N2 + Ns, N2 - Ns, s, y0);
+    for (int j = 0; j < p2; ++j) xp[j] = -N2;
+    ora_linspace(-N2 + Np1, N2 - Np1, p1, xp + p2);
+    ora_linspace(-N2 + Np2, N2 - Np2, p2, yp);
+    for (int j = p2; j < p; ++j) yp[j] = N2;
+    double w = omega / N;
+    int64_t n2 = (int64_t)N * N;
+    ora_csr** blk = (ora_csr**)calloc(s > 0 ? s : 1, sizeof(ora_csr*));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        double* K = (double*)malloc(sizeof(double) * n2);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int i = 0; i < s; ++i) {
+            ora_csr* B = csr_new(p, n2);
+            double S1 = N2, S2 = y0[i];
+            for (int j = 0; j < p; ++j) {
+                double R1 = xp[j], R2 = yp[j];
+                double dist = sqrt((S1 - R1) * (S1 - R1) + (S2 - R2) * (S2 - R2));
+                double sum = 0;
+                for (int c = 0; c < N; ++c) {            /* column-major: S(:) */
+                    double x = (-N2 + .5) + c;
+                    for (int r = 0; r < N; ++r) {
+                        double y = (N2 - .5) - r;
+                        double tSX = sqrt((x - S1) * (x - S1) + (y - S2) * (y - S2));
+                        double tRX = sqrt((x - R1) * (x - R1) + (y - R2) * (y - R2));
+                        double dt = tSX + tRX - dist;
+                        double a = alpha * dt * w;
+                        double k = cos(2 * M_PI * dt * w) * exp(-(a * a));
+                        K[(int64_t)c * N + r] = k;
+                        sum += k;
+                    }
+                }
+                for (int64_t q = 0; q < n2; ++q) {
+                    double v = dist * K[q] / sum;
+                    if (!(v < tol)) csr_push(B, (int32_t)q, v);
+                }
+                B->rowptr[j + 1] = B->nnz;
+            }
+            blk[i] = B;
+        }
+        free(K);
+    }
+    ora_csr* A = csr_concat(blk, s, p, n2);
+    free(blk); free(y0); free(xp); free(yp);
+    return A;
+}
+
 int64_t ora_csr_m(const ora_csr* A) { return A->m; }
 int64_t ora_csr_n(const ora_csr* A) { return A->n; }
 int64_t ora_csr_nnz(const ora_csr* A) { return A->nnz; }

@@ -47,6 +47,8 @@ ora_csr* ora_fancurvedtomo(int N, const double* theta, int nA, int p, double R, 
 ora_csr* ora_seismictomo(int N, int s, int p);
 /* testprobs/sphericaltomo.m:108-270 */
 ora_csr* ora_sphericaltomo(int N, const double* angles, int nA, int nC);
+/* testprobs/seismicwavetomo.m:114-288 */
+ora_csr* ora_seismicwavetomo(int N, int s, int p, double omega);
 int64_t ora_csr_m(const ora_csr*);
 int64_t ora_csr_n(const ora_csr*);
 int64_t ora_csr_nnz(const ora_csr*);

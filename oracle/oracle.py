@@ -82,6 +82,8 @@ def lib():
         L.ora_seismictomo.argtypes = [C.c_int, C.c_int, C.c_int]
         L.ora_sphericaltomo.restype = C.c_void_p
         L.ora_sphericaltomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int]
+        L.ora_seismicwavetomo.restype = C.c_void_p
+        L.ora_seismicwavetomo.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
         for f in ("ora_csr_m", "ora_csr_n", "ora_csr_nnz"):
             getattr(L, f).restype = C.c_int64
             getattr(L, f).argtypes = [C.c_void_p]
@@ -221,6 +223,13 @@ def sphericaltomo(N, angles=None, numCircles=None):
     angles = np.arange(0.0, 359.0, 2.0) if angles is None else np.atleast_1d(np.asarray(angles, np.float64))
     nC = int(lib().ora_round(np.sqrt(2) * N)) if numCircles is None else int(numCircles)
     return _take_csr(lib().ora_sphericaltomo(int(N), np.ascontiguousarray(angles), len(angles), nC))
+
+
+def seismicwavetomo(N, s=None, p=None, omega=10.0):
+    """testprobs/seismicwavetomo.m:70-83 defaults + :114-288 matrix build."""
+    s = int(N) if s is None else int(s)
+    p = 2 * int(N) if p is None else int(p)
+    return _take_csr(lib().ora_seismicwavetomo(int(N), s, p, float(omega)))
 
 
 def tectonic(N):

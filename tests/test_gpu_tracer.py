@@ -216,3 +216,38 @@ def test_spherical_return_all_and_matrix_free_refused():
     np.testing.assert_allclose(b, A @ x, rtol=1e-6)
     with pytest.raises(Exception, match="matrix-free"):
         air.sphericaltomo(8, [0.0], 4, 0, 0)
+
+
+WAVE_CASES = {
+    "wave_N16_defaults": dict(N=16, s=None, p=None, omega=None),
+    "wave_N33_s7_p9_omega5": dict(N=33, s=7, p=9, omega=5.0),
+    "wave_N64_defaults": dict(N=64, s=None, p=None, omega=None),
+    "wave_N128_s16_p64": dict(N=128, s=16, p=64, omega=20.0),
+}
+
+
+@pytest.mark.parametrize("name", list(WAVE_CASES))
+def test_seismicwave_csr_matches_oracle(oracle, name):
+    """Columns equal and FP64 values within 1e-12 relative: cos and exp on the
+    device may differ from the C library's by an ulp (the threshold 1e-6 is not
+    met within such a margin by any entry of these cases)."""
+    cfg = WAVE_CASES[name]
+    print(f"case seismicwave/{name}", flush=True)
+    A = air.seismicwavetomo(cfg["N"], cfg["s"], cfg["p"], cfg["omega"], dtype="f64")
+    Ao = oracle.seismicwavetomo(cfg["N"], cfg["s"], cfg["p"], 10.0 if cfg["omega"] is None else cfg["omega"])
+    rp, ci, v = A.csr_arrays()
+    assert A.shape == Ao.shape and A.nnz == Ao.nnz
+    assert np.array_equal(rp, Ao.indptr)
+    assert np.array_equal(ci, Ao.indices)
+    np.testing.assert_allclose(v, Ao.data, rtol=1e-12, atol=0)
+
+
+def test_seismicwave_f32_and_return_all(oracle):
+    A, b, x, s, p, omega = air.seismicwavetomo(24, return_all=True)
+    Ao = oracle.seismicwavetomo(24)
+    assert (s, p, omega) == (24, 48, 10.0) and A.shape == Ao.shape
+    rp, ci, v = A.csr_arrays()
+    assert np.array_equal(ci, Ao.indices)
+    np.testing.assert_allclose(v, Ao.data.astype(np.float32), rtol=1e-6)
+    np.testing.assert_array_equal(x, oracle.tectonic(24).ravel(order="F"))
+    np.testing.assert_allclose(b, Ao @ x, rtol=1e-5)

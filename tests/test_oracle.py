@@ -433,6 +433,50 @@ def test_spherical_matches_vectorised_sampling(oracle, N, nC):
             np.testing.assert_allclose(A.data[A.indptr[i]:A.indptr[i + 1]], 2 * np.pi * rad / nphi * c, rtol=1e-15)
 
 
+# ------------------------------------------------------------------ seismicwavetomo
+def fresnel_dense(N, s, p, omega):
+    """The sensitivity kernels of seismicwavetomo.m:188-212 with numpy arrays,
+    laid out as the reference's meshgrid (rows top to bottom, S(:) column-major)."""
+    N2 = N / 2
+    rays = seismic_rays(N, s, p)
+    xr = np.arange(-N2 + .5, N2, 1.0)
+    yr = np.arange(N2 - .5, -N2, -1.0)
+    xx, yy = np.meshgrid(xr, yr)
+    out = []
+    for x0, y0, a, b in rays:
+        R = np.array([x0 + a, y0 + b]); S = np.array([x0, y0])
+        tSX = np.sqrt((xx - S[0]) ** 2 + (yy - S[1]) ** 2)
+        tRX = np.sqrt((xx - R[0]) ** 2 + (yy - R[1]) ** 2)
+        dist = np.sqrt(np.sum((S - R) ** 2))
+        dt = tSX + tRX - dist
+        K = np.cos(2 * np.pi * dt * (omega / N)) * np.exp(-(10 * dt * (omega / N)) ** 2)
+        out.append((dist * K / K.sum()).ravel(order="F"))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("N,s,p,omega", [(12, 12, 24, 10.0), (9, 4, 7, 5.0)])
+def test_seismicwave_matches_dense_kernels(oracle, N, s, p, omega):
+    A = oracle.seismicwavetomo(N, s, p, omega).toarray()
+    D = fresnel_dense(N, s, p, omega)
+    keep = D >= 1e-6
+    assert np.array_equal(A != 0, keep)
+    np.testing.assert_allclose(A[keep], D[keep], rtol=1e-12)
+
+
+def test_seismicwave_peak_on_the_source_receiver_line(oracle):
+    # the kernel is 1 where delta = 0, i.e. on the segment from S to R; the
+    # largest entry of each row sits at a pixel closest to that segment
+    N, s, p = 16, 4, 8
+    A = oracle.seismicwavetomo(N, s, p).toarray()
+    for r, (x0, y0, a, b) in enumerate(seismic_rays(N, s, p)):
+        q = int(np.argmax(A[r]))
+        c, rr = divmod(q, N)
+        px, py = c - N / 2 + 0.5, N / 2 - 0.5 - rr
+        t = np.clip(((px - x0) * a + (py - y0) * b) / (a * a + b * b), 0, 1)
+        d = math.hypot(px - (x0 + t * a), py - (y0 + t * b))
+        assert d <= math.sqrt(0.5) + 1e-12, (r, d)
+
+
 # ------------------------------------------------------------------ weights
 def small_problem(oracle, N=16, p=23, d=30.0):
     return oracle.paralleltomo(N, np.arange(0, 180, 7.0), p, d)
