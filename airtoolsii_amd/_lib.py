@@ -28,7 +28,7 @@ BOUND_NONE, BOUND_SCALAR, BOUND_VECTOR = 0, 1, 2
 # every symbol include/airhip.h declares
 EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "air_set_matrix_csr",
            "air_set_matrix_csc", "air_build_paralleltomo", "air_build_fanlineartomo", "air_build_fancurvedtomo", "air_build_seismictomo",
-           "air_build_sphericaltomo", "air_build_seismicwavetomo",
+           "air_build_sphericaltomo", "air_build_seismicwavetomo", "air_fbp",
            "air_size", "air_get_csr",
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
@@ -98,6 +98,7 @@ def load():
     L.air_build_seismictomo.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int, C.c_int]
     L.air_build_sphericaltomo.argtypes = [_P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int, C.c_int]
     L.air_build_seismicwavetomo.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int, C.c_int]
+    L.air_fbp.argtypes = [_P, _P, C.c_int32, C.c_int, _P]
     L.air_size.argtypes = [_P, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]
     L.air_get_csr.argtypes = [_P, _P, _P, _P]
     L.air_apply.argtypes = [_P, C.c_int, _P, _P]

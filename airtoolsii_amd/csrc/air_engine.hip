@@ -986,6 +986,32 @@ int air_apply(air_engine* e, int transp, const double* in, double* out) {
     });
 }
 
+int air_fbp(air_engine* e, const double* b, int32_t na, int filter, double* x) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(b && x && na > 0, AIR_E_ARG, "Input A, b, theta must me specified");
+        AIR_CHECK(filter >= AIR_FBP_NONE && filter <= AIR_FBP_HANN, AIR_E_ARG, "Iinvalid filter");
+        use_device(e);
+        hipStream_t s = e->stream;
+        DevBuf din, dout;
+        dout.alloc(4 * std::max<int64_t>(e->n, 1));
+        if (filter == AIR_FBP_NONE) {                     // fbp.m:37
+            upload_f32(din, b, e->m, s);
+            apply_dev(e, 1, din.as<float>(), dout.as<float>());
+            download_f64(dout.as<float>(), x, e->n, s);
+            return;
+        }
+        // fbp.m:40-43: reshape(b, nr, na) needs m to be a multiple of na
+        AIR_CHECK(e->m % na == 0, AIR_E_ARG, "fbp: size(A,1) is not a multiple of length(theta)");
+        const int64_t nr = e->m / na;
+        din.alloc(4 * std::max<int64_t>(e->m, 1));
+        fbp_filter_sinogram(b, nr, na, filter, din.as<float>(), s);
+        apply_dev(e, 1, din.as<float>(), dout.as<float>());
+        download_f64(dout.as<float>(), x, e->n, s);
+        for (int64_t j = 0; j < e->n; ++j) x[j] = x[j] * M_PI / (2.0 * na);   // fbp.m:80
+    });
+}
+
 int air_weights(air_engine* e, int method, double* M_out, double* D_out) {
     return guard([&] {
         need_operator(e);

@@ -156,6 +156,10 @@ void host_rays_fancurvedtomo(int N, const double* theta, int nA, int p, double R
 void seismic_positions(int N, int s, int p, std::vector<double>& y0, std::vector<double>& xp,
                        std::vector<double>& yp);
 void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out);
+// fbp.m:45-77 (air_fbp.hip): frequency response (half spectrum) and the
+// filtered sinogram (nr x na, FP32 on the device).
+std::vector<double> fbp_filter_response(int64_t nr, int filter, int64_t* order_out);
+void fbp_filter_sinogram(const double* b_host, int64_t nr, int64_t na, int filter, float* out, hipStream_t s);
 // seismicwavetomo: CSR of the Fresnel-zone sensitivity matrix (air_fresnel.hip).
 void fresnel_to_csr(int N, int s, int p, double omega, int dtype, DevSparse& A, hipStream_t st);
 // sphericaltomo: CSR of the circle-quadrature matrix (air_circles.hip).

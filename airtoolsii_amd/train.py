@@ -1,9 +1,13 @@
-"""train_relaxpar (train/train_relaxpar.m:43-271), SIRT branch, on the engine.
+"""train_relaxpar (train/train_relaxpar.m:43-271) and train_dpme
+(train/train_dpme.m:1-137), SIRT branch, on the engine.
 
-Step 1 runs the method in chunks of 10 iterations with warm starts until the
-error to ``x_ex`` stops decreasing; step 2 runs the augmented golden-section
-search over (0, 2/rho) with the error bound (1+0.02)*minE.  Every probe is a
-full device-resident SIRT run; only the error curves come back to the host.
+train_relaxpar: step 1 runs the method in chunks of 10 iterations with warm
+starts until the error to ``x_ex`` stops decreasing; step 2 runs the augmented
+golden-section search over (0, 2/rho) with the error bound (1+0.02)*minE.
+train_dpme: for s noisy right-hand sides it finds the iteration of smallest
+error in the same chunked way and returns the mean of the tau that makes the
+DP or ME rule stop there.  Every run is a device-resident SIRT run; iterates
+come back to the host only to measure their error.
 """
 from __future__ import annotations
 
@@ -85,3 +89,61 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None):
             k_b, e_b = k_a, e_a
             k_a, e_a = probe(alphap)
     return (alphap + betap) / 2                              # :270
+
+
+def train_dpme(A, b_exact, x_exact, method, type, delta, s, kmax, options=None, *, noise=None, seed=0):
+    """tau = train_dpme(A,b_exact,x_exact,method,type,delta,s,kmax,options).
+
+    The reference draws the noise with MATLAB's randn (train_dpme.m:63-65),
+    which is not reproduced: ``noise`` (m x s) gives the unscaled noise
+    columns, otherwise they come from NumPy's PCG64 with ``seed``.  Each
+    column is scaled to norm ``delta``.  As in the reference, X keeps the
+    starting vector as its first column (:82,99), so the residual at the
+    optimal iteration kopt is that of iterate kopt-1 (:120) and the ME rule
+    also uses iterate kopt-2 (:126)."""
+    name = _method_name(method)
+    kind = str(type).upper()[:2]
+    if kind not in ("DP", "ME"):
+        raise ValueError("The chosen type is invalid")
+    A = as_operator(A)
+    m = A.shape[0]
+    b_exact = np.ravel(np.asarray(b_exact, np.float64))
+    x_exact = np.ravel(np.asarray(x_exact, np.float64))
+    opts = dict(options) if options is not None else {}
+    opts["stoprule"] = {"type": "none"}                      # :45-50
+    E = np.random.default_rng(seed).standard_normal((m, s)) if noise is None else np.asarray(noise, np.float64)
+    if E.shape != (m, s):
+        raise ValueError("noise must be m x s")
+    E = delta * E / np.sqrt(np.sum(E * E, axis=0))[None, :]  # :64
+    kstep = kmax if kmax < 10 else 10                        # :71-75
+    tauh = np.zeros(s)
+    for i in range(s):                                       # :78-134
+        bi = b_exact + E[:, i]
+        cols = [np.zeros(x_exact.shape[0])]
+        x0 = np.zeros(x_exact.shape[0])
+        k = kstep
+        K = np.arange(1, k + 1)
+        Ek = []
+        while True:
+            Xny, _, _ = sirt(name, A, bi, K, x0, opts)
+            Ek.extend(np.sqrt(np.sum((Xny - x_exact[:, None]) ** 2, axis=0)).tolist())
+            cols.extend(Xny.T)
+            I = int(np.argmin(Ek)) + 1
+            if I != k or k >= kmax:
+                kopt = I
+                break
+            if k + kstep > kmax:
+                K = np.arange(1, kmax - k + 1)
+                k = kmax
+            else:
+                k = k + kstep
+            x0 = Xny[:, -1]
+        rk = bi - A.matvec(cols[kopt - 1])                   # X(:,kopt)
+        if kind == "DP":
+            tauh[i] = np.linalg.norm(rk) / delta
+        else:
+            if kopt < 2:
+                raise IndexError("Index in position 2 is invalid: ME needs X(:,kopt-1) with kopt >= 2")
+            rkm1 = bi - A.matvec(cols[kopt - 2])
+            tauh[i] = 0.5 * (rkm1 @ (rkm1 + rk)) / (delta * np.linalg.norm(rkm1))
+    return float(np.mean(tauh))

@@ -149,6 +149,16 @@ int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int3
  * pair, row = source*p + receiver.  matrix_free must be 0. */
 int air_build_seismicwavetomo(air_engine* e, int32_t N, int32_t s, int32_t p, double omega,
                               int matrix_free, int dtype);
+/* ---- filtered back projection (auxil/fbp.m:31-80) ---------------------
+ * x = A' * filter(b) * pi / (2 na), b the column-major sinogram (m/na rays x
+ * na angles).  AIR_FBP_NONE gives the plain back projection x = A' b (:37). */
+#define AIR_FBP_NONE        0
+#define AIR_FBP_RAMLAK      1
+#define AIR_FBP_SHEPPLOGAN  2
+#define AIR_FBP_COSINE      3
+#define AIR_FBP_HAMMING     4
+#define AIR_FBP_HANN        5
+int air_fbp(air_engine* e, const double* b, int32_t na, int filter, double* x);
 /* A(v,'size') of auxil/check_inputs.m:74 (nnz = -1 for matrix-free) */
 int air_size(air_engine* e, int64_t* m, int64_t* n, int64_t* nnz);
 /* copy the stored CSR back to the host (values widened to double) */

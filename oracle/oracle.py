@@ -13,6 +13,11 @@ Makefile beside it) and restates the MATLAB host logic around the C loop:
                         under rng(0), which only a MATLAB run reproduces)
 * ``sirt``              sirt/sirt.m:132-281
 * ``train_relaxpar``    train/train_relaxpar.m:43-271 (SIRT branch)
+* ``train_dpme``        train/train_dpme.m:1-137 (noise columns passed in)
+* ``fbp``               auxil/fbp.m:31-80 (numpy complex FFT)
+* the test problems of testprobs/ (paralleltomo, fanlineartomo,
+  fancurvedtomo, seismictomo, sphericaltomo, seismicwavetomo) and the
+  phantoms of testprobs/phantomgallery.m, in C or numpy
 """
 from __future__ import annotations
 
@@ -414,6 +419,95 @@ def sirt(method, A, b, K, x0=None, options=None, nthreads=0, return_resnorm=Fals
     if return_resnorm:
         return X, out_info, {"M": M, "D": D}, rn[:info.finaliter + 1]
     return X, out_info, {"M": M, "D": D}
+
+
+def train_dpme(A, b_exact, x_exact, method, type, delta, s, kmax, options=None, noise=None, run=None):
+    """train/train_dpme.m:1-137 with the unscaled noise columns given as
+    ``noise`` (m x s; the reference draws them with randn).  ``run(A,b,K,x0,
+    opts)`` defaults to this oracle's ``sirt``; ``resid(x, b)`` is b - A x."""
+    if run is None:
+        def run(A_, b_, K_, x0_, opts_):
+            X, info, _ = sirt(method, A_, b_, K_, x0_, opts_)
+            return X, info
+    if not str(type)[:2].upper() in ("DP", "ME"):
+        raise ValueError("The chosen type is invalid")
+    opts = {} if options is None else dict(options)
+    opts["stoprule"] = {"type": "none"}
+    m = A.shape[0]
+    e = np.array(noise, np.float64).reshape(m, s)
+    e = delta * e / (np.ones((m, 1)) @ np.sqrt(np.sum(e * e, axis=0))[None, :])
+    b = b_exact[:, None] @ np.ones((1, s)) + e
+    tauh = np.zeros(s)
+    kstep = kmax if kmax < 10 else 10
+    for i in range(s):
+        bi = b[:, i]
+        X = np.zeros((x_exact.shape[0], 1))
+        x0 = np.zeros(x_exact.shape[0])
+        k = kstep
+        K = np.arange(1, k + 1)
+        Ek = np.zeros(0)
+        while True:
+            Xny, _ = run(A, bi, K, x0, opts)
+            xd = Xny - x_exact[:, None]
+            Ek = np.concatenate([Ek, np.sqrt(np.sum(xd * xd, axis=0))])
+            X = np.concatenate([X, Xny], axis=1)
+            I = int(np.argmin(Ek)) + 1
+            if I != k or k >= kmax:
+                kopt = I
+                break
+            if k + kstep > kmax:
+                K = np.arange(1, kmax - k + 1)
+                k = kmax
+            else:
+                k = k + kstep
+            x0 = Xny[:, -1]
+        rk = bi - A @ X[:, kopt - 1]
+        if str(type)[:2].upper() == "DP":
+            tauk = np.linalg.norm(rk) / delta
+        else:
+            if kopt - 2 < 0:
+                raise IndexError("Index in position 2 is invalid")
+            rkm1 = bi - A @ X[:, kopt - 2]
+            tauk = 0.5 * (rkm1 @ (rkm1 + rk)) / (delta * np.linalg.norm(rkm1))
+        tauh[i] = tauk
+    return float(np.mean(tauh))
+
+
+def fbp(A, b, theta, filter="ram-lak"):
+    """auxil/fbp.m:31-80 with numpy's complex FFT over the full symmetric filter."""
+    filter = filter.lower()
+    A = sp.csr_matrix(A)
+    if filter == "none":
+        return A.T @ b
+    m = A.shape[0]
+    na = len(np.atleast_1d(theta))
+    nr = m // na
+    if nr * na != m:
+        raise ValueError("reshape: size(A,1) is not a multiple of length(theta)")
+    B = np.reshape(np.asarray(b, np.float64), (nr, na), order="F")
+    order = max(64, 2 ** int(math.ceil(math.log2(2 * nr))))
+    n = np.arange(order // 2 + 1, dtype=np.float64)
+    f = n / n.max()
+    f[0] = 0.4 / order
+    f[-1] = 1 - f[0]
+    w = 2 * np.pi * n / order
+    if filter == "shepp-logan":
+        t = w[1:] / (2 * np.pi)
+        f[1:] = f[1:] * (np.sin(np.pi * t) / (np.pi * t))
+    elif filter == "cosine":
+        f[1:] = f[1:] * np.cos(w[1:] / 2)
+    elif filter == "hamming":
+        f[1:] = f[1:] * (.54 + .46 * np.cos(w[1:]))
+    elif filter == "hann":
+        f[1:] = f[1:] * (1 + np.cos(w[1:])) / 2
+    elif filter != "ram-lak":
+        raise ValueError("Iinvalid filter")
+    f = np.concatenate([f, f[-2:0:-1]])
+    P = np.zeros((len(f), na))
+    P[:nr] = B
+    F = np.fft.fft(P, axis=0) * f[:, None]
+    P = np.real(np.fft.ifft(F, axis=0))[:nr]
+    return A.T @ P.ravel(order="F") * np.pi / (2 * na)
 
 
 def train_relaxpar(A, b, x_ex, method, kmax, options=None, run=None):

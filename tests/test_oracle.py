@@ -668,6 +668,78 @@ def test_train_relaxpar_on_synthetic_error_model(oracle):
     assert 0.9 * 2 / rho <= lam < 2 / rho
 
 
+def test_train_dpme_on_prescribed_iterates(oracle):
+    # A = I and iterates x_k = x* + (k - 7)^2/100 * u (u a unit vector): the error
+    # is smallest at k = 7, found in the first chunk of 10.  X keeps the start
+    # vector as column 1, so the DP residual is that of iterate kopt - 1 = 6 and
+    # ME also uses iterate 5 (train_dpme.m:82,99,120,126).
+    n = 5
+    xs = np.arange(1.0, n + 1)
+    u = np.zeros(n); u[2] = 1.0
+    state = {"k_end": 0}
+
+    def run(A, b, K, x0, opts):
+        k0 = 0 if not np.any(x0) else state["k_end"]
+        ks = k0 + np.asarray(K)
+        state["k_end"] = int(ks[-1])
+        return np.stack([xs + (k - 7) ** 2 / 100 * u for k in ks], axis=1), {}
+
+    noise = np.ones((n, 2))
+    delta = 0.5
+    A = sp.identity(n, format="csr")
+    tau = oracle.train_dpme(A, xs, xs, "sart", "DP", delta, 2, 30, None, noise, run)
+    e = delta * np.ones(n) / math.sqrt(n)
+    r6 = xs + e - (xs + 1 / 100 * u)
+    assert abs(tau - np.linalg.norm(r6) / delta) < 1e-14
+    tau_me = oracle.train_dpme(A, xs, xs, "sart", "ME", delta, 2, 30, None, noise, run)
+    r5 = xs + e - (xs + 4 / 100 * u)
+    assert abs(tau_me - 0.5 * (r5 @ (r5 + r6)) / (delta * np.linalg.norm(r5))) < 1e-14
+
+
+def test_train_dpme_continues_past_a_chunk_and_caps_at_kmax(oracle):
+    # error falling until k = 23: two more chunks of 10 after the first; with
+    # kmax = 15 the search ends at the cap (kopt = 15, iterate 14 used)
+    n = 3
+    xs = np.zeros(n)
+    state = {"k_end": 0}
+
+    def run(A, b, K, x0, opts):
+        k0 = 0 if not np.any(x0) else state["k_end"]
+        ks = k0 + np.asarray(K)
+        state["k_end"] = int(ks[-1])
+        return np.stack([np.full(n, abs(k - 23) / 10 + 0.01) for k in ks], axis=1), {}
+
+    A = sp.identity(n, format="csr")
+    b = np.zeros(n)
+    noise = np.ones((n, 1))
+    tau = oracle.train_dpme(A, b, xs, "sart", "DP", 1.0, 1, 100, None, noise, run)
+    x22 = np.full(n, 1 / 10 + 0.01)
+    assert abs(tau - np.linalg.norm(np.ones(n) / math.sqrt(n) - x22)) < 1e-14
+    tau15 = oracle.train_dpme(A, b, xs, "sart", "DP", 1.0, 1, 15, None, noise, run)
+    x14 = np.full(n, 9 / 10 + 0.01)
+    assert abs(tau15 - np.linalg.norm(np.ones(n) / math.sqrt(n) - x14)) < 1e-14
+
+
+def test_fbp_reconstructs_a_disk(oracle):
+    # the filtered back projection inverts the Radon transform: a disk of
+    # radius N/4 and value 1 comes back near 1 inside and near 0 outside
+    N = 64
+    theta = np.arange(0.0, 180.0)
+    A = oracle.paralleltomo(N, theta, 91)
+    c = (np.arange(N) - (N - 1) / 2)
+    disk = ((c[:, None] ** 2 + c[None, :] ** 2) <= (N / 4) ** 2).astype(float)
+    b = A @ disk.ravel(order="F")
+    for filt in ("ram-lak", "shepp-logan", "cosine", "hamming", "hann"):
+        x = oracle.fbp(A, b, theta, filt).reshape((N, N), order="F")
+        inner = (c[:, None] ** 2 + c[None, :] ** 2) <= (N / 4 - 3) ** 2
+        outer = (c[:, None] ** 2 + c[None, :] ** 2) >= (N / 4 + 3) ** 2
+        assert abs(np.median(x[inner]) - 1) < 0.05, filt
+        assert np.max(np.abs(x[outer])) < 0.1, filt
+    np.testing.assert_allclose(oracle.fbp(A, b, theta, "none"), A.T @ b)
+    with pytest.raises(ValueError):
+        oracle.fbp(A, b, theta, "gaussian")
+
+
 # ------------------------------------------------------------------ phantoms
 def test_phantoms_basic_properties(oracle):
     sl = oracle.phantomgallery("shepplogan", 64)
