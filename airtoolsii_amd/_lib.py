@@ -28,7 +28,7 @@ BOUND_NONE, BOUND_SCALAR, BOUND_VECTOR = 0, 1, 2
 # every symbol include/airhip.h declares
 EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "air_set_matrix_csr",
            "air_set_matrix_csc", "air_build_paralleltomo", "air_build_fanlineartomo", "air_build_fancurvedtomo", "air_build_seismictomo",
-           "air_build_sphericaltomo", "air_build_seismicwavetomo", "air_fbp",
+           "air_build_sphericaltomo", "air_build_seismicwavetomo", "air_fbp", "air_art", "air_cart",
            "air_size", "air_get_csr",
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
@@ -60,6 +60,14 @@ class Layout(C.Structure):
                 ("lds_bytes", C.c_int64), ("fwd_stream", C.c_int), ("fwd_lanes", C.c_int), ("fwd_loads", C.c_int),
                 ("bwd_lanes", C.c_int), ("bwd_loads", C.c_int), ("fwd_pieces_per_block", C.c_int),
                 ("tune_fwd_us", C.c_float * 32), ("tune_bwd_us", C.c_float * 8)]
+
+
+class RowOptions(C.Structure):
+    _fields_ = [("relax_kind", C.c_int), ("relaxpar", C.c_double), ("relax_vec", C.c_void_p),
+                ("lb_kind", C.c_int), ("lb", C.c_double), ("lb_vec", C.c_void_p),
+                ("ub_kind", C.c_int), ("ub", C.c_double), ("ub_vec", C.c_void_p),
+                ("stoprule", C.c_int), ("taudelta", C.c_double), ("damp", C.c_double), ("THR", C.c_double),
+                ("Kbegin", C.c_int), ("Nunflag", C.c_int), ("seed", C.c_uint64)]
 
 
 class SirtInfo(C.Structure):
@@ -99,6 +107,10 @@ def load():
     L.air_build_sphericaltomo.argtypes = [_P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int, C.c_int]
     L.air_build_seismicwavetomo.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int, C.c_int]
     L.air_fbp.argtypes = [_P, _P, C.c_int32, C.c_int, _P]
+    L.air_art.argtypes = [_P, C.c_int, _P, _i64, C.POINTER(RowOptions), _P, _P, _P, C.c_int32, _P, _P,
+                          C.POINTER(C.c_int32), _P, C.POINTER(SirtInfo)]
+    L.air_cart.argtypes = [_P, _P, _i64, C.POINTER(RowOptions), _P, _P, _P, C.c_int32, _P, _P,
+                           C.POINTER(C.c_int32), _P, C.POINTER(SirtInfo)]
     L.air_size.argtypes = [_P, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]
     L.air_get_csr.argtypes = [_P, _P, _P, _P]
     L.air_apply.argtypes = [_P, C.c_int, _P, _P]

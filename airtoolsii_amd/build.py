@@ -31,6 +31,7 @@ SOURCES = {
     "air_circles.hip": ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"],
     "air_fresnel.hip": ["-ffp-contract=off"],
     "air_fbp.hip": [],
+    "air_art.hip": [],
     "air_sort.hip": [],
     "air_sirt.hip": [],
     "air_tiled.hip": [],

@@ -1,0 +1,538 @@
+// air_art.hip -- ART (art/art.m) and CART (cart/cart.m) row and column
+// action sweeps on gfx950.
+//
+// Both methods are sequential by definition: every row (ART) or column
+// (CART) step reads what the previous step wrote, and in tomography
+// consecutive steps share pixels (rays of one angle overlap their neighbours,
+// rays of different angles cross), so there is no reordering that keeps the
+// reference's result.  The GPU form therefore runs each sweep as ONE wave that
+// walks the steps in the reference's order: the 64 lanes gather the step's
+// entries, a butterfly sum gives the inner product in every lane, the lanes
+// scatter the update, and a work-group fence orders the stores before the next
+// step's loads.  x (ART) and the residual (CART) are FP64 as in MATLAB;
+// matrix values are read in their storage type.  Everything around the sweep
+// (row norms, residuals, stop rules, saved iterates) is wide device work.
+//
+// Random choices (randkaczmarz row orders, CART unflagging) cannot follow
+// MATLAB's rand streams, which are not reproduced; they come from a counter-
+// based generator (SplitMix64 of the seed, a stream number, the sweep and the
+// step) that the host, the device and the CPU restatement evaluate alike.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "air_internal.h"
+
+namespace air {
+
+namespace {
+
+constexpr uint64_t RNG_PERM = 1, RNG_SAMPLE = 2, RNG_UNFLAG = 3;
+
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// uniform in [0, 1) for (seed, stream, sweep k, step t)
+__host__ __device__ __forceinline__ double rng_u01(uint64_t seed, uint64_t stream, uint64_t k, uint64_t t) {
+    const uint64_t z = splitmix64(splitmix64(seed ^ stream) + (k << 32) + t);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double clamp_bounds(double v, int64_t j, int lb_kind, double lb, const double* lbv,
+                                               int ub_kind, double ub, const double* ubv) {
+    // art.m:279-284: max with the lower bound, then min with the upper bound
+    if (lb_kind == AIR_BOUND_SCALAR) v = fmax(v, lb);
+    else if (lb_kind == AIR_BOUND_VECTOR) v = fmax(v, lbv[j]);
+    if (ub_kind == AIR_BOUND_SCALAR) v = fmin(v, ub);
+    else if (ub_kind == AIR_BOUND_VECTOR) v = fmin(v, ubv[j]);
+    return v;
+}
+
+__device__ __forceinline__ void order_stores() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+struct Bounds {
+    int lb_kind = AIR_BOUND_NONE, ub_kind = AIR_BOUND_NONE;
+    double lb = 0, ub = 0;
+    const double* lbv = nullptr;
+    const double* ubv = nullptr;
+};
+
+// One ART sweep (art.m:249-285) over rows[0..L): one wave.
+template <typename V>
+__global__ __launch_bounds__(64) void k_art_sweep(const int64_t* __restrict__ ptr, const int* __restrict__ idx,
+                                                  const V* __restrict__ val, const int* __restrict__ rows,
+                                                  const double* __restrict__ lamv, double lam, int64_t L,
+                                                  const double* __restrict__ b, const double* __restrict__ nrm,
+                                                  double* x, Bounds bd, int64_t n, int project_all_first) {
+    const int lane = threadIdx.x;
+    const bool bounded = bd.lb_kind != AIR_BOUND_NONE || bd.ub_kind != AIR_BOUND_NONE;
+    for (int64_t t = 0; t < L; ++t) {
+        const int r = rows[t];
+        const int64_t s = ptr[r], e = ptr[r + 1];
+        double dot = 0;
+        for (int64_t q = s + lane; q < e; q += 64) dot += to_f64(val[q]) * x[idx[q]];
+        dot = wave_sum(dot);
+        const double lk = lamv ? lamv[t] : lam;
+        const double coef = (lk * (b[r] - dot)) / nrm[r];
+        for (int64_t q = s + lane; q < e; q += 64) {
+            const int j = idx[q];
+            double v = x[j] + coef * to_f64(val[q]);
+            if (bounded) v = clamp_bounds(v, j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+            x[j] = v;
+        }
+        order_stores();
+        if (t == 0 && project_all_first && bounded) {
+            // the reference projects the whole vector after every row; entries
+            // off the row keep their projected value afterwards, so after the
+            // first row of the run only the row's own entries can change
+            for (int64_t j = lane; j < n; j += 64)
+                x[j] = clamp_bounds(x[j], j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+            order_stores();
+        }
+    }
+}
+
+// max |x| over the whole vector, NaN ignored (MATLAB max)
+__device__ double wave_absmax(const double* x, int64_t n, int64_t* arg) {
+    const int lane = threadIdx.x;
+    double best = -1.0;
+    int64_t bi = -1;
+    for (int64_t j = lane; j < n; j += 64) {
+        const double a = fabs(x[j]);
+        if (a > best) { best = a; bi = j; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int64_t oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi >= 0 && (bi < 0 || oi < bi))) { best = ob; bi = oi; }
+    }
+    *arg = bi;
+    return bi < 0 ? NAN : best;
+}
+
+struct CartParams {
+    double lam, thr;
+    int k, kbegin, nunflag;
+    uint64_t seed;
+};
+
+// One CART sweep (cart.m:226-287) over cols[0..L): one wave.  CSC of A.
+template <typename V>
+__global__ __launch_bounds__(64) void k_cart_sweep(const int64_t* __restrict__ cptr, const int* __restrict__ ridx,
+                                                   const V* __restrict__ val, const int* __restrict__ cols,
+                                                   int64_t L, const double* __restrict__ nrm, double* x, double* r,
+                                                   unsigned char* flag, int* nflag, Bounds bd, int64_t n,
+                                                   CartParams cp) {
+    const int lane = threadIdx.x;
+    const bool track = cp.k >= cp.kbegin;
+    int64_t arg = -1;
+    double mm = track ? wave_absmax(x, n, &arg) : NAN;
+    for (int64_t t = 0; t < L; ++t) {
+        const int j = cols[t];
+        if (flag[j]) {
+            const int64_t s = cptr[j], e = cptr[j + 1];
+            double dot = 0;
+            for (int64_t q = s + lane; q < e; q += 64) dot += to_f64(val[q]) * r[ridx[q]];
+            dot = wave_sum(dot);
+            const double delta = dot / nrm[j];
+            double od = cp.lam * delta;
+            const double xkj = x[j];
+            if (bd.lb_kind != AIR_BOUND_NONE) {                   // cart.m:251-259
+                const double lbj = bd.lb_kind == AIR_BOUND_SCALAR ? bd.lb : bd.lbv[j];
+                if (od < lbj - xkj) od = lbj - xkj;
+            }
+            if (bd.ub_kind != AIR_BOUND_NONE) {                   // :261-267
+                const double ubj = bd.ub_kind == AIR_BOUND_SCALAR ? bd.ub : bd.ubv[j];
+                if (od > ubj - xkj) od = ubj - xkj;
+            }
+            const double xn = xkj + od;
+            if (lane == 0) {
+                x[j] = xn;
+                if (track && fabs(od) < cp.thr * mm) {            // :273-276
+                    flag[j] = 0;
+                    nflag[j] = 1;
+                }
+            }
+            for (int64_t q = s + lane; q < e; q += 64) {          // :279
+                const int i = ridx[q];
+                r[i] = r[i] - od * to_f64(val[q]);
+            }
+            order_stores();
+            if (track) {                                          // keep mm = max(abs(xk)) for the next step
+                const double an = fabs(xn);
+                if (an >= mm || isnan(mm)) { if (!isnan(an)) { mm = an; arg = j; } }
+                else if (j == arg) { mm = wave_absmax(x, n, &arg); }
+            }
+        } else {                                                  // :281-285
+            if (lane == 0) {
+                const int draw =
+                    1 + (int)floor(rng_u01(cp.seed, RNG_UNFLAG, (uint64_t)cp.k, (uint64_t)j) * cp.nunflag);
+                if (nflag[j] < draw) nflag[j] = nflag[j] + 1;
+                else flag[j] = 1;
+            }
+            order_stores();                                       // a custom order may visit j again
+        }
+    }
+}
+
+// r = b - A x (FP64), one wave per row
+template <typename V>
+__global__ __launch_bounds__(256) void k_resid64(const int64_t* __restrict__ ptr, const int* __restrict__ idx,
+                                                 const V* __restrict__ val, int64_t m, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ r) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < m; i += (int64_t)gridDim.x * 4) {
+        double acc = 0;
+        for (int64_t q = ptr[i] + lane; q < ptr[i + 1]; q += 64) acc += to_f64(val[q]) * x[idx[q]];
+        acc = wave_sum(acc);
+        if (lane == 0) r[i] = b[i] - acc;
+    }
+}
+
+// squared 2-norms of the compressed rows (rows of A from CSR, columns from CSC)
+template <typename V>
+__global__ __launch_bounds__(256) void k_sqnorm(const int64_t* __restrict__ ptr, const V* __restrict__ val,
+                                                int64_t rows, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < rows; i += (int64_t)gridDim.x * 4) {
+        double acc = 0;
+        for (int64_t q = ptr[i] + lane; q < ptr[i + 1]; q += 64) {
+            const double v = to_f64(val[q]);
+            acc += v * v;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) out[i] = acc;
+    }
+}
+
+// out[0] = r'r, out[1] = p'(p + r) (the monotone-error numerator with p = r_{k-1})
+__global__ __launch_bounds__(1024) void k_stop_dots(const double* __restrict__ r, const double* __restrict__ p,
+                                                    int64_t m, double* __restrict__ out) {
+    __shared__ double sh[2][16];
+    double a = 0, c = 0;
+    for (int64_t i = threadIdx.x; i < m; i += 1024) {
+        a += r[i] * r[i];
+        if (p) c += p[i] * (p[i] + r[i]);
+    }
+    a = wave_sum(a);
+    c = wave_sum(c);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][w] = a; sh[1][w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sc = 0;
+        for (int q = 0; q < 16; ++q) { sa += sh[0][q]; sc += sh[1][q]; }
+        out[0] = sa;
+        out[1] = sc;
+    }
+}
+
+unsigned rows_grid(int64_t rows) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 3) / 4, 65536));
+}
+
+// ---------------------------------------------------------------- host side
+struct StopState {
+    int type = AIR_STOP_NONE;
+    double taudelta = NAN;
+    int kmax = 0;
+    bool have_prev = false;     // ME: r_{k-1} exists (it is NaN at k = 0)
+};
+
+// check_stoprules.m:53-72,153-161 (DP and NONE); returns the info code or -1
+int stop_code(const StopState& S, int k, double rr) {
+    const double nrk = std::sqrt(rr);
+    if (S.type == AIR_STOP_DP) {
+        if (nrk <= S.taudelta || k >= S.kmax) return k != S.kmax ? 2 : 0;
+        return -1;
+    }
+    return k >= S.kmax ? 0 : -1;
+}
+
+void check_order(const int32_t* order, int64_t norder, int64_t limit, const char* what) {
+    for (int64_t q = 0; q < norder; ++q)
+        AIR_CHECK(order[q] >= 1 && order[q] <= limit, AIR_E_ARG,
+                  std::string("Index exceeds matrix dimensions: custom ") + what + " order entry out of range");
+}
+
+Bounds make_bounds(const air_row_options* o, int64_t n, DevBuf& lbd, DevBuf& ubd, hipStream_t s) {
+    Bounds bd;
+    bd.lb_kind = o->lb_kind;
+    bd.ub_kind = o->ub_kind;
+    bd.lb = o->lb;
+    bd.ub = o->ub;
+    if (o->lb_kind == AIR_BOUND_VECTOR) {
+        AIR_CHECK(o->lb_vec, AIR_E_ARG, "lbound vector missing");
+        lbd.alloc(8 * (size_t)n);
+        AIR_HIP(hipMemcpyAsync(lbd.p, o->lb_vec, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+        bd.lbv = lbd.as<double>();
+    }
+    if (o->ub_kind == AIR_BOUND_VECTOR) {
+        AIR_CHECK(o->ub_vec, AIR_E_ARG, "ubound vector missing");
+        ubd.alloc(8 * (size_t)n);
+        AIR_HIP(hipMemcpyAsync(ubd.p, o->ub_vec, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+        bd.ubv = ubd.as<double>();
+    }
+    return bd;
+}
+
+}  // namespace
+
+// Shared driver of both methods: sweeps until a stop rule fires, saving the
+// iterates of K (art.m:227-330, cart.m:211-323).
+void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_method, const int32_t* order,
+                    int64_t norder, const air_row_options* o, const double* b_h, const double* x0_h,
+                    const int32_t* K_in, int32_t nK, double* X_out, int32_t* itersaved, int32_t* n_itersaved,
+                    double* resnorm, air_sirt_info* info, hipStream_t s) {
+    auto t0 = std::chrono::steady_clock::now();
+    AIR_CHECK(o && b_h, AIR_E_ARG, "null options or right-hand side");
+    AIR_CHECK(K_in && nK > 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
+    for (int q = 1; q < nK; ++q) AIR_CHECK(K_in[q] > K_in[q - 1], AIR_E_ARG, "Elements of K must be strictly increasing.");
+    AIR_CHECK(o->damp >= 0, AIR_E_ARG, "Damping must be positive");
+    AIR_CHECK(o->stoprule == AIR_STOP_NONE || o->stoprule == AIR_STOP_DP || o->stoprule == AIR_STOP_ME,
+              AIR_E_UNSUPPORTED, "ART/CART: the stopping rule must be 'none', 'DP' or 'ME'");
+    const int64_t m = A.nrows, n = A.ncols;
+    std::vector<int32_t> K(K_in, K_in + nK);
+    const bool sym = !cart && art_method == AIR_ART_SYMKACZMARZ;
+    if (sym) {                                                      // art.m:123-135
+        for (int q = 0; q < nK; ++q)
+            AIR_CHECK(K[q] % 2 == 0, AIR_E_ARG, "For symkaczmarz only even iteration numbers can be requested.");
+        for (auto& k : K) k /= 2;
+    }
+    const int kmax = K[nK - 1];
+
+    // norms of the rows (ART) or columns (CART), art.m:161-181 / cart.m:154-174
+    const DevSparse& S = cart ? AT : A;
+    const int64_t L0 = S.nrows;
+    DevBuf dnrm;
+    dnrm.alloc(8 * (size_t)std::max<int64_t>(L0, 1));
+    dispatch_dtype(S.dtype, [&](auto v) {
+        using V = decltype(v);
+        if (L0) k_sqnorm<V><<<rows_grid(L0), 256, 0, s>>>(S.ptr.as<int64_t>(), S.val.as<V>(), L0, dnrm.as<double>());
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+    std::vector<double> nrm((size_t)L0);
+    AIR_HIP(hipMemcpyAsync(nrm.data(), dnrm.p, 8 * (size_t)L0, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+
+    // the step order: nonzero rows/columns, or the caller's order (1-based)
+    std::vector<int> I;
+    const bool custom = cart ? (order != nullptr) : (art_method == AIR_ART_CUSTOM);
+    if (custom) {
+        AIR_CHECK(order && norder >= 0, AIR_E_ARG, "custom order missing");
+        check_order(order, norder, L0, cart ? "column" : "row");
+        for (int64_t q = 0; q < norder; ++q)
+            if (nrm[order[q] - 1] > 0) I.push_back(order[q] - 1);
+    } else {
+        AIR_CHECK(cart || art_method == AIR_ART_KACZMARZ || art_method == AIR_ART_SYMKACZMARZ ||
+                      art_method == AIR_ART_RANDKACZMARZ,
+                  AIR_E_ARG, "Unknown ART method specified");
+        for (int64_t q = 0; q < L0; ++q)
+            if (nrm[q] > 0) I.push_back((int)q);
+    }
+    if (sym) {
+        const size_t h = I.size();
+        for (size_t q = 0; q < h; ++q) I.push_back(I[h - 1 - q]);
+    }
+    // damping, art.m:206 / cart.m:193
+    double mx = 0;
+    for (double v : nrm) mx = std::max(mx, v);
+    for (double& v : nrm) v = v + o->damp * mx;
+    AIR_HIP(hipMemcpyAsync(dnrm.p, nrm.data(), 8 * (size_t)L0, hipMemcpyHostToDevice, s));
+    const bool rnd = !cart && art_method == AIR_ART_RANDKACZMARZ;
+    std::vector<double> cumul;
+    bool fast = true;
+    if (rnd) {                                                      // art.m:209-219
+        double tot = 0;
+        for (double v : nrm) tot += v;
+        cumul.resize((size_t)L0);
+        double c = 0, dev = 0;
+        for (int64_t q = 0; q < L0; ++q) {
+            c += nrm[q] / tot;
+            cumul[q] = c;
+            dev = std::max(dev, std::fabs(c - (double)(q + 1) / (double)L0));
+        }
+        fast = dev < 0.05;
+    }
+
+    // relaxation, calc_relaxpar.m:54-87
+    double lam = cart ? 0.25 : 1.0;
+    if (o->relax_kind == AIR_RELAX_SCALAR) lam = o->relaxpar;
+    else if (o->relax_kind == AIR_RELAX_VECTOR) {
+        AIR_CHECK(!cart, AIR_E_ARG, "For CART methods the relaxpar must be a scalar.");
+        AIR_CHECK(o->relax_vec, AIR_E_ARG, "relaxation values missing");
+    } else
+        AIR_CHECK(o->relax_kind == AIR_RELAX_DEFAULT, AIR_E_ARG,
+                  "For ART methods the relaxpar must be a scalar or function handle.");
+
+    // device state
+    DevBuf db, dx, dr, dp, dlbv, dubv, dsteps, dlam, ddots, dflag, dnflag;
+    db.alloc(8 * (size_t)std::max<int64_t>(m, 1));
+    dx.alloc(8 * (size_t)std::max<int64_t>(n, 1));
+    dr.alloc(8 * (size_t)std::max<int64_t>(m, 1));
+    ddots.alloc(16);
+    AIR_HIP(hipMemcpyAsync(db.p, b_h, 8 * (size_t)m, hipMemcpyHostToDevice, s));
+    if (x0_h) AIR_HIP(hipMemcpyAsync(dx.p, x0_h, 8 * (size_t)n, hipMemcpyHostToDevice, s));
+    else AIR_HIP(hipMemsetAsync(dx.p, 0, 8 * (size_t)n, s));
+    const Bounds bd = make_bounds(o, n, dlbv, dubv, s);
+    const size_t L = I.size();
+    dsteps.alloc(4 * std::max<size_t>(L, 1));
+    if (!rnd) AIR_HIP(hipMemcpyAsync(dsteps.p, I.data(), 4 * L, hipMemcpyHostToDevice, s));
+    const bool lam_vec = !cart && o->relax_kind == AIR_RELAX_VECTOR;
+    if (lam_vec) dlam.alloc(8 * std::max<size_t>(L, 1));
+    if (cart) {
+        dflag.alloc((size_t)std::max<int64_t>(n, 1));
+        dnflag.alloc(4 * (size_t)std::max<int64_t>(n, 1));
+        AIR_HIP(hipMemsetAsync(dflag.p, 1, (size_t)n, s));
+        AIR_HIP(hipMemsetAsync(dnflag.p, 0, 4 * (size_t)n, s));
+    }
+    auto residual = [&]() {
+        dispatch_dtype(A.dtype, [&](auto v) {
+            using V = decltype(v);
+            if (m) k_resid64<V><<<rows_grid(m), 256, 0, s>>>(A.ptr.as<int64_t>(), A.idx.as<int>(), A.val.as<V>(), m,
+                                                          dx.as<double>(), db.as<double>(), dr.as<double>());
+            return 0;
+        });
+        AIR_HIP(hipGetLastError());
+    };
+    StopState ss;
+    ss.type = o->stoprule;
+    ss.taudelta = o->taudelta;
+    ss.kmax = kmax;
+    if (ss.type == AIR_STOP_ME) dp.alloc(8 * (size_t)std::max<int64_t>(m, 1));
+    // evaluates the stop rule at sweep k on the residual in dr; returns the code or -1
+    auto check = [&](int k, double* nrk_out) {
+        k_stop_dots<<<1, 1024, 0, s>>>(dr.as<double>(), ss.have_prev ? dp.as<double>() : nullptr, m,
+                                         ddots.as<double>());
+        AIR_HIP(hipGetLastError());
+        double d[2];
+        double pp = 0;
+        AIR_HIP(hipMemcpyAsync(d, ddots.p, 16, hipMemcpyDeviceToHost, s));
+        if (ss.type == AIR_STOP_ME && ss.have_prev) {
+            k_stop_dots<<<1, 1024, 0, s>>>(dp.as<double>(), nullptr, m, ddots.as<double>());
+            AIR_HIP(hipGetLastError());
+            AIR_HIP(hipMemcpyAsync(&pp, ddots.p, 8, hipMemcpyDeviceToHost, s));
+        }
+        AIR_HIP(hipStreamSynchronize(s));
+        *nrk_out = std::sqrt(d[0]);
+        int code = -1;
+        if (ss.type == AIR_STOP_ME) {                              // check_stoprules.m:74-94
+            const double dme = ss.have_prev ? d[1] / 2 : NAN;
+            const double nr1 = ss.have_prev ? std::sqrt(pp) : NAN;
+            if (dme / nr1 <= ss.taudelta) code = 3;
+            else if (k >= kmax) code = 0;
+            else {
+                AIR_HIP(hipMemcpyAsync(dp.p, dr.p, 8 * (size_t)m, hipMemcpyDeviceToDevice, s));
+                ss.have_prev = true;
+            }
+        } else {
+            code = stop_code(ss, k, d[0]);
+        }
+        return code;
+    };
+
+    residual();                                                     // art.m:147 / cart.m:139
+    double nrk = 0;
+    int code = check(0, &nrk);
+    if (resnorm) resnorm[0] = nrk;
+    int k = 0, l = 0, fin = 0;
+    double lam_last = lam;
+    std::vector<int> steps(L);
+    std::vector<double> lams(lam_vec ? L : 0);
+    const int nunflag = o->Nunflag >= 1 ? o->Nunflag : std::max(1, (int)std::lround(kmax / 4.0));
+    auto ti = std::chrono::steady_clock::now();
+    while (code < 0) {
+        ++k;
+        if (rnd) {                                                  // art.m:242-262
+            std::vector<int> perm(I);
+            for (int64_t t = (int64_t)L - 1; t >= 1; --t) {
+                const int64_t j = (int64_t)std::floor(rng_u01(o->seed, RNG_PERM, (uint64_t)k, (uint64_t)t) * (t + 1));
+                std::swap(perm[t], perm[j]);
+            }
+            for (size_t t = 0; t < L; ++t) {
+                if (fast) steps[t] = perm[t];
+                else {
+                    const double u = rng_u01(o->seed, RNG_SAMPLE, (uint64_t)k, (uint64_t)t);
+                    const int64_t ri = std::lower_bound(cumul.begin(), cumul.end(), u) - cumul.begin();
+                    steps[t] = (int)std::min<int64_t>(ri, L0 - 1);   // sum(cumul<rand)+1, kept in range
+                }
+                if (lam_vec) lams[t] = o->relax_vec[(size_t)(k - 1) * m + perm[t]];
+            }
+            AIR_HIP(hipMemcpyAsync(dsteps.p, steps.data(), 4 * L, hipMemcpyHostToDevice, s));
+        } else if (lam_vec) {
+            for (size_t t = 0; t < L; ++t) lams[t] = o->relax_vec[(size_t)(k - 1) * m + I[t]];   // relaxpar((k-1)*m+i)
+        }
+        if (lam_vec && L) {
+            AIR_HIP(hipMemcpyAsync(dlam.p, lams.data(), 8 * L, hipMemcpyHostToDevice, s));
+            lam_last = lams[L - 1];
+        }
+        if (cart) {
+            CartParams cp{lam, o->THR, k, o->Kbegin, nunflag, o->seed};
+            dispatch_dtype(AT.dtype, [&](auto v) {
+                using V = decltype(v);
+                if (L) k_cart_sweep<V><<<1, 64, 0, s>>>(AT.ptr.as<int64_t>(), AT.idx.as<int>(), AT.val.as<V>(),
+                                                        dsteps.as<int>(), (int64_t)L, dnrm.as<double>(),
+                                                        dx.as<double>(), dr.as<double>(), dflag.as<unsigned char>(),
+                                                        dnflag.as<int>(), bd, n, cp);
+                return 0;
+            });
+        } else {
+            dispatch_dtype(A.dtype, [&](auto v) {
+                using V = decltype(v);
+                if (L) k_art_sweep<V><<<1, 64, 0, s>>>(A.ptr.as<int64_t>(), A.idx.as<int>(), A.val.as<V>(),
+                                                       dsteps.as<int>(), lam_vec ? dlam.as<double>() : nullptr, lam,
+                                                       (int64_t)L, db.as<double>(), dnrm.as<double>(),
+                                                       dx.as<double>(), bd, n, k == 1 ? 1 : 0);
+                return 0;
+            });
+            residual();                                             // art.m:288
+        }
+        AIR_HIP(hipGetLastError());
+        code = check(k, &nrk);
+        if (resnorm) resnorm[k] = nrk;
+        if ((l < nK && k == K[l]) || code >= 0) {                 // art.m:295-298
+            if (X_out) AIR_HIP(hipMemcpyAsync(X_out + (size_t)l * n, dx.p, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+            l++;
+        }
+    }
+    AIR_HIP(hipStreamSynchronize(s));
+    fin = k;
+    std::vector<int32_t> Ks(K);
+    if (sym) {                                                      // art.m:321-324
+        fin *= 2;
+        for (auto& q : Ks) q *= 2;
+    }
+    int ns = 0;                                                     // art.m:327
+    for (int q = 0; q < nK; ++q)
+        if (Ks[q] < fin) {
+            if (itersaved) itersaved[ns] = Ks[q];
+            ns++;
+        }
+    if (itersaved) itersaved[ns] = fin;
+    if (n_itersaved) *n_itersaved = ns + 1;
+    if (info) {
+        air_sirt_info inf{};
+        inf.stoprule = code;
+        inf.finaliter = fin;
+        inf.relaxpar = lam_vec ? lam_last : lam;
+        inf.rho = NAN;
+        inf.nsaved = l;
+        inf.iter_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - ti).count();
+        inf.timetaken = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        inf.setup_time = inf.timetaken - inf.iter_time;
+        *info = inf;
+    }
+}
+
+}  // namespace air

@@ -1199,6 +1199,36 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
     });
 }
 
+static void need_stored(air_engine* e, const char* what) {
+    AIR_CHECK(e->kind == 1 && e->A.nnz >= 0 && e->A.ptr.p && e->AT.ptr.p, AIR_E_UNSUPPORTED,
+              std::string(what) + " needs a stored matrix (CSR and CSC); matrix-free and batched operators are "
+                                  "not supported");
+}
+
+int air_art(air_engine* e, int method, const int32_t* order, int64_t norder, const air_row_options* opts,
+            const double* b, const double* x0, const int32_t* K, int32_t nK, double* X_out, int32_t* itersaved,
+            int32_t* n_itersaved, double* resnorm, air_sirt_info* info) {
+    return guard([&] {
+        need_operator(e);
+        need_stored(e, "art");
+        use_device(e);
+        row_action_run(e->A, e->AT, false, method, order, norder, opts, b, x0, K, nK, X_out, itersaved, n_itersaved,
+                       resnorm, info, e->stream);
+    });
+}
+
+int air_cart(air_engine* e, const int32_t* order, int64_t norder, const air_row_options* opts, const double* b,
+             const double* x0, const int32_t* K, int32_t nK, double* X_out, int32_t* itersaved,
+             int32_t* n_itersaved, double* resnorm, air_sirt_info* info) {
+    return guard([&] {
+        need_operator(e);
+        need_stored(e, "cart");
+        use_device(e);
+        row_action_run(e->A, e->AT, true, 0, order, norder, opts, b, x0, K, nK, X_out, itersaved, n_itersaved,
+                       resnorm, info, e->stream);
+    });
+}
+
 int air_set_kernel_timing(air_engine* e, int on) {
     return guard([&] {
         AIR_CHECK(e, AIR_E_ARG, "null engine");

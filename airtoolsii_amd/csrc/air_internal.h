@@ -160,6 +160,11 @@ void host_rays_seismictomo(int N, int s, int p, std::vector<RayParam>& out);
 // filtered sinogram (nr x na, FP32 on the device).
 std::vector<double> fbp_filter_response(int64_t nr, int filter, int64_t* order_out);
 void fbp_filter_sinogram(const double* b_host, int64_t nr, int64_t na, int filter, float* out, hipStream_t s);
+// ART / CART sweeps (air_art.hip): A is the CSR, AT the CSC of the operator.
+void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_method, const int32_t* order,
+                    int64_t norder, const air_row_options* o, const double* b_h, const double* x0_h,
+                    const int32_t* K_in, int32_t nK, double* X_out, int32_t* itersaved, int32_t* n_itersaved,
+                    double* resnorm, air_sirt_info* info, hipStream_t s);
 // seismicwavetomo: CSR of the Fresnel-zone sensitivity matrix (air_fresnel.hip).
 void fresnel_to_csr(int N, int s, int p, double omega, int dtype, DevSparse& A, hipStream_t st);
 // sphericaltomo: CSR of the circle-quadrature matrix (air_circles.hip).

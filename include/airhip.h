@@ -234,6 +234,41 @@ typedef struct {
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 
+/* ---- ART and CART: art/art.m:1-330, cart/cart.m:1-323 ----------------------
+ * Row action (Kaczmarz) and column action sweeps over a stored matrix, in the
+ * reference's step order.  Shared options (check_inputs.m:101-250): */
+#define AIR_ART_KACZMARZ     0
+#define AIR_ART_SYMKACZMARZ  1   /* top -> bottom -> top, one sweep counts as 2 iterations */
+#define AIR_ART_RANDKACZMARZ 2
+#define AIR_ART_CUSTOM       3   /* row order given by the caller (1-based), art.m:198-203 */
+typedef struct {
+    int           relax_kind;    /* AIR_RELAX_DEFAULT (ART 1, CART 0.25), AIR_RELAX_SCALAR, or
+                                    AIR_RELAX_VECTOR (ART only: the function handle's values,
+                                    relax_vec[(k-1)*m + i-1] for sweep k and row i, art.m:273-275) */
+    double        relaxpar;
+    const double* relax_vec;
+    int           lb_kind;  double lb;  const double* lb_vec;
+    int           ub_kind;  double ub;  const double* ub_vec;
+    int           stoprule;      /* AIR_STOP_NONE, AIR_STOP_DP or AIR_STOP_ME */
+    double        taudelta;
+    double        damp;          /* >= 0, default 0 */
+    double        THR;           /* CART flagging threshold, default 1e-4 */
+    int           Kbegin;        /* CART: flagging from this sweep on, default 10 */
+    int           Nunflag;       /* CART unflagging parameter, <= 0: round(kmax/4) */
+    uint64_t      seed;          /* randkaczmarz orders and CART unflagging draws (SplitMix64 stream;
+                                    MATLAB's rand is not reproduced) */
+} air_row_options;
+/* method: AIR_ART_*; order/norder: AIR_ART_CUSTOM row order.  K: nK strictly
+ * increasing iteration numbers (even for symkaczmarz).  X_out: n x nK FP64;
+ * itersaved: nK+1 ints; resnorm (optional): ||r_k|| for k = 0..sweeps. */
+int air_art(air_engine* e, int method, const int32_t* order, int64_t norder, const air_row_options* opts,
+            const double* b, const double* x0, const int32_t* K, int32_t nK, double* X_out,
+            int32_t* itersaved, int32_t* n_itersaved, double* resnorm, air_sirt_info* info);
+/* columnaction (order NULL) or a custom column order (1-based), cart.m:177-190 */
+int air_cart(air_engine* e, const int32_t* order, int64_t norder, const air_row_options* opts,
+             const double* b, const double* x0, const int32_t* K, int32_t nK, double* X_out,
+             int32_t* itersaved, int32_t* n_itersaved, double* resnorm, air_sirt_info* info);
+
 /* ---- batches of independent problems -----------------------------------
  * No counterpart in the reference (which solves one system per call): many
  * paralleltomo problems of different sizes solved together.  The engine

@@ -676,6 +676,82 @@ ora_csr* ora_seismicwavetomo(int N, int s, int p, double omega)
     return A;
 }
 
+/* ---------------------------------------------------------------- ART / CART */
+uint64_t ora_splitmix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double ora_rng_u01(uint64_t seed, uint64_t stream, uint64_t k, uint64_t t)
+{
+    uint64_t z = ora_splitmix64(ora_splitmix64(seed ^ stream) + (k << 32) + t);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void ora_project_all(double* x, int64_t n, int lbk, double lb, const double* lbv,
+                            int ubk, double ub, const double* ubv)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        double v = x[j];
+        if (lbk == 1) v = fmax(v, lb); else if (lbk == 2) v = fmax(v, lbv[j]);
+        if (ubk == 1) v = fmin(v, ub); else if (ubk == 2) v = fmin(v, ubv[j]);
+        x[j] = v;
+    }
+}
+
+/* art.m:249-285, literally: every row step updates x along the row and then
+ * projects the whole vector (max with lbound, min with ubound).  rows are
+ * 0-based; lamv (or the scalar lam) per step. */
+void ora_art_sweep(const int64_t* rp, const int32_t* ci, const double* v, const int32_t* rows, int64_t L,
+                   const double* lamv, double lam, const double* b, const double* nrm, double* x, int64_t n,
+                   int lbk, double lb, const double* lbv, int ubk, double ub, const double* ubv)
+{
+    for (int64_t t = 0; t < L; ++t) {
+        int r = rows[t];
+        double dot = 0;
+        for (int64_t q = rp[r]; q < rp[r + 1]; ++q) dot += v[q] * x[ci[q]];
+        double lk = lamv ? lamv[t] : lam;
+        double coef = (lk * (b[r] - dot)) / nrm[r];
+        for (int64_t q = rp[r]; q < rp[r + 1]; ++q) x[ci[q]] = x[ci[q]] + coef * v[q];
+        if (lbk || ubk) ora_project_all(x, n, lbk, lb, lbv, ubk, ub, ubv);
+    }
+}
+
+/* cart.m:226-287, literally (mm = max(abs(xk)) recomputed at every step,
+ * NaN ignored); cp/ri/v the CSC of A; flag/nflag per column. */
+void ora_cart_sweep(const int64_t* cp, const int32_t* ri, const double* v, const int32_t* cols, int64_t L,
+                    const double* nrm, double* x, double* r, int64_t n, unsigned char* flag, int32_t* nflag,
+                    int lbk, double lb, const double* lbv, int ubk, double ub, const double* ubv,
+                    double lam, double thr, int k, int kbegin, int nunflag, uint64_t seed)
+{
+    for (int64_t t = 0; t < L; ++t) {
+        int j = cols[t];
+        double mm = NAN;
+        for (int64_t q = 0; q < n; ++q) {
+            double a = fabs(x[q]);
+            if (!isnan(a) && (isnan(mm) || a > mm)) mm = a;
+        }
+        if (flag[j]) {
+            double dot = 0;
+            for (int64_t q = cp[j]; q < cp[j + 1]; ++q) dot += v[q] * r[ri[q]];
+            double delta = dot / nrm[j];
+            double od = lam * delta;
+            double xkj = x[j];
+            if (lbk) { double lbj = lbk == 1 ? lb : lbv[j]; if (od < lbj - xkj) od = lbj - xkj; }
+            if (ubk) { double ubj = ubk == 1 ? ub : ubv[j]; if (od > ubj - xkj) od = ubj - xkj; }
+            x[j] = xkj + od;
+            if (k >= kbegin && fabs(od) < thr * mm) { flag[j] = 0; nflag[j] = 1; }
+            for (int64_t q = cp[j]; q < cp[j + 1]; ++q) r[ri[q]] = r[ri[q]] - od * v[q];
+        } else {
+            int draw = 1 + (int)floor(ora_rng_u01(seed, 3, (uint64_t)k, (uint64_t)j) * nunflag);
+            if (nflag[j] < draw) nflag[j] = nflag[j] + 1;
+            else flag[j] = 1;
+        }
+    }
+}
+
 int64_t ora_csr_m(const ora_csr* A) { return A->m; }
 int64_t ora_csr_n(const ora_csr* A) { return A->n; }
 int64_t ora_csr_nnz(const ora_csr* A) { return A->nnz; }

@@ -49,6 +49,17 @@ ora_csr* ora_seismictomo(int N, int s, int p);
 ora_csr* ora_sphericaltomo(int N, const double* angles, int nA, int nC);
 /* testprobs/seismicwavetomo.m:114-288 */
 ora_csr* ora_seismicwavetomo(int N, int s, int p, double omega);
+/* art/art.m:249-285 and cart/cart.m:226-287 sweeps, and the counter-based
+ * random stream shared with the engine (SplitMix64) */
+uint64_t ora_splitmix64(uint64_t z);
+double ora_rng_u01(uint64_t seed, uint64_t stream, uint64_t k, uint64_t t);
+void ora_art_sweep(const int64_t* rp, const int32_t* ci, const double* v, const int32_t* rows, int64_t L,
+                   const double* lamv, double lam, const double* b, const double* nrm, double* x, int64_t n,
+                   int lbk, double lb, const double* lbv, int ubk, double ub, const double* ubv);
+void ora_cart_sweep(const int64_t* cp, const int32_t* ri, const double* v, const int32_t* cols, int64_t L,
+                    const double* nrm, double* x, double* r, int64_t n, unsigned char* flag, int32_t* nflag,
+                    int lbk, double lb, const double* lbv, int ubk, double ub, const double* ubv,
+                    double lam, double thr, int k, int kbegin, int nunflag, uint64_t seed);
 int64_t ora_csr_m(const ora_csr*);
 int64_t ora_csr_n(const ora_csr*);
 int64_t ora_csr_nnz(const ora_csr*);

@@ -15,6 +15,7 @@ Makefile beside it) and restates the MATLAB host logic around the C loop:
 * ``train_relaxpar``    train/train_relaxpar.m:43-271 (SIRT branch)
 * ``train_dpme``        train/train_dpme.m:1-137 (noise columns passed in)
 * ``fbp``               auxil/fbp.m:31-80 (numpy complex FFT)
+* ``art``, ``cart``     art/art.m:227-330, cart/cart.m:211-323 (sweeps in C)
 * the test problems of testprobs/ (paralleltomo, fanlineartomo,
   fancurvedtomo, seismictomo, sphericaltomo, seismicwavetomo) and the
   phantoms of testprobs/phantomgallery.m, in C or numpy
@@ -88,6 +89,18 @@ def lib():
         L.ora_sphericaltomo.restype = C.c_void_p
         L.ora_sphericaltomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int]
         L.ora_seismicwavetomo.restype = C.c_void_p
+        L.ora_splitmix64.restype = C.c_uint64
+        L.ora_splitmix64.argtypes = [C.c_uint64]
+        L.ora_rng_u01.restype = C.c_double
+        L.ora_rng_u01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.ora_art_sweep.restype = None
+        L.ora_art_sweep.argtypes = ([C.c_void_p] * 4 + [C.c_int64, C.c_void_p, C.c_double] + [C.c_void_p] * 3
+                                    + [C.c_int64, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_double, C.c_void_p])
+        L.ora_cart_sweep.restype = None
+        L.ora_cart_sweep.argtypes = ([C.c_void_p] * 4 + [C.c_int64] + [C.c_void_p] * 3 + [C.c_int64]
+                                     + [C.c_void_p] * 2 + [C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_double,
+                                                           C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                                           C.c_int, C.c_uint64])
         L.ora_seismicwavetomo.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
         for f in ("ora_csr_m", "ora_csr_n", "ora_csr_nnz"):
             getattr(L, f).restype = C.c_int64
@@ -471,6 +484,166 @@ def train_dpme(A, b_exact, x_exact, method, type, delta, s, kmax, options=None, 
             tauk = 0.5 * (rkm1 @ (rkm1 + rk)) / (delta * np.linalg.norm(rkm1))
         tauh[i] = tauk
     return float(np.mean(tauh))
+
+
+# ---------------------------------------------------------------- ART / CART
+def _bounds(opts, n):
+    out = []
+    for key in ("lbound", "ubound"):
+        v = opts.get(key)
+        if v is None or np.size(v) == 0:
+            out += [0, 0.0, None]
+        elif np.size(v) == 1:
+            out += [1, float(np.ravel(v)[0]), None]
+        else:
+            out += [2, 0.0, np.ascontiguousarray(np.ravel(v), np.float64)]
+    return out
+
+
+def _stop(opts, k, kmax, r, rprev):
+    """check_stoprules.m:53-94,153-161 for DP, ME, NONE -> code or None."""
+    st = opts.get("stoprule") or {}
+    t = str(st.get("type", "none")).upper()
+    nrk = np.linalg.norm(r)
+    if t == "DP":
+        if nrk <= st["taudelta"] or k >= kmax:
+            return 2 if k != kmax else 0
+        return None
+    if t == "ME":
+        if rprev is not None and (rprev @ (rprev + r) / 2) / np.linalg.norm(rprev) <= st["taudelta"]:
+            return 3
+        if k >= kmax:
+            return 0
+        return None
+    return 0 if k >= kmax else None
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def art(art_method, A, b, K, x0=None, options=None):
+    """art/art.m:1-330 with the inner row loop in C (ora_art_sweep).  The
+    random choices of randkaczmarz use the SplitMix64 stream (options['seed'])."""
+    opts = dict(options or {})
+    A = sp.csr_matrix(A, dtype=np.float64)
+    A.sort_indices()
+    m, n = A.shape
+    K = np.atleast_1d(np.asarray(K, np.int64))
+    method = "kaczmarz" if art_method is None else art_method
+    sym = isinstance(method, str) and method.lower().startswith("sym")
+    if sym:
+        if np.any(K % 2):
+            raise ValueError("For symkaczmarz only even iteration numbers can be requested.")
+        K = K // 2
+    kmax = int(K[-1])
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    nrm = np.asarray(A.multiply(A).sum(axis=1)).ravel()
+    if isinstance(method, str):
+        I = np.nonzero(nrm > 0)[0]
+        if method.lower() == "symkaczmarz":
+            I = np.concatenate([I, I[::-1]])
+        elif method.lower() not in ("kaczmarz", "randkaczmarz"):
+            raise ValueError("Unknown ART method specified")
+    else:
+        I = np.ravel(np.asarray(method, np.int64)) - 1
+        I = I[nrm[I] > 0]
+    nrm = nrm + float(opts.get("damp", 0.0)) * nrm.max()
+    rnd = isinstance(method, str) and method.lower() == "randkaczmarz"
+    seed = int(opts.get("seed", 0))
+    L = lib()
+    if rnd:
+        cumul = np.cumsum(nrm / nrm.sum())
+        fast = np.max(np.abs(cumul - np.arange(1, m + 1) / m)) < 0.05
+    rp = opts.get("relaxpar")
+    lam = 1.0 if rp is None else (float(rp) if not callable(rp) else 0.0)
+    lbk, lb, lbv, ubk, ub, ubv = _bounds(opts, n)
+    rk = b - A @ x
+    code = _stop(opts, 0, kmax, rk, None)
+    rprev = rk if code is None else None
+    X, k = [], 0
+    ip, ic, iv = (np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int32),
+                  np.ascontiguousarray(A.data))
+    li = 0
+    while code is None:
+        k += 1
+        if rnd:
+            perm = I.copy()
+            for t in range(len(perm) - 1, 0, -1):
+                j = int(L.ora_rng_u01(seed, 1, k, t) * (t + 1))
+                perm[t], perm[j] = perm[j], perm[t]
+            if fast:
+                steps = perm
+            else:
+                u = np.array([L.ora_rng_u01(seed, 2, k, t) for t in range(len(perm))])
+                steps = np.minimum(np.array([np.sum(cumul < uu) for uu in u]), m - 1)
+            ivals = perm
+        else:
+            steps = ivals = I
+        lamv = None
+        if callable(rp):
+            lamv = np.ascontiguousarray([float(rp((k - 1) * m + int(i) + 1)) for i in ivals], np.float64)
+        st32 = np.ascontiguousarray(steps, np.int32)
+        L.ora_art_sweep(ip.ctypes.data, ic.ctypes.data, iv.ctypes.data, st32.ctypes.data, len(st32), _ptr(lamv), lam,
+                        b.ctypes.data, nrm.ctypes.data, x.ctypes.data, n, lbk, lb, _ptr(lbv), ubk, ub, _ptr(ubv))
+        rk = b - A @ x
+        code = _stop(opts, k, kmax, rk, rprev)
+        if code is None and str((opts.get("stoprule") or {}).get("type", "none")).upper() == "ME":
+            rprev = rk
+        if (li < len(K) and k == K[li]) or code is not None:
+            X.append(x.copy())
+            li += 1
+    fin = 2 * k if sym else k
+    return np.array(X).T, {"stoprule": code, "finaliter": fin}
+
+
+def cart(cart_method, A, b, K, x0=None, options=None):
+    """cart/cart.m:1-323 with the inner column loop in C (ora_cart_sweep)."""
+    opts = dict(options or {})
+    A = sp.csc_matrix(A, dtype=np.float64)
+    A.sort_indices()
+    m, n = A.shape
+    K = np.atleast_1d(np.asarray(K, np.int64))
+    kmax = int(K[-1])
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    nrm = np.asarray(A.multiply(A).sum(axis=0)).ravel()
+    if cart_method is None or (isinstance(cart_method, str) and cart_method.lower() == "columnaction"):
+        J = np.nonzero(nrm > 0)[0]
+    elif isinstance(cart_method, str):
+        raise ValueError("Unknown CART method specified")
+    else:
+        J = np.ravel(np.asarray(cart_method, np.int64)) - 1
+        J = J[nrm[J] > 0]
+    nrm = nrm + float(opts.get("damp", 0.0)) * nrm.max()
+    lam = float(opts.get("relaxpar", 0.25))
+    thr = float(opts.get("THR", 1e-4))
+    kbegin = int(opts.get("Kbegin", 10))
+    nunflag = int(opts.get("Nunflag", 0) or 0) or max(1, int(math.floor(kmax / 4 + 0.5)))
+    seed = int(opts.get("seed", 0))
+    lbk, lb, lbv, ubk, ub, ubv = _bounds(opts, n)
+    flag = np.ones(n, np.uint8)
+    nflag = np.zeros(n, np.int32)
+    r = b - A @ x
+    code = _stop(opts, 0, kmax, r, None)
+    rprev = r.copy() if code is None else None
+    cp, ri, cv = (np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int32),
+                  np.ascontiguousarray(A.data))
+    J32 = np.ascontiguousarray(J, np.int32)
+    X, k, li = [], 0, 0
+    while code is None:
+        k += 1
+        lib().ora_cart_sweep(cp.ctypes.data, ri.ctypes.data, cv.ctypes.data, J32.ctypes.data, len(J32),
+                             nrm.ctypes.data, x.ctypes.data, r.ctypes.data, n, flag.ctypes.data, nflag.ctypes.data,
+                             lbk, lb, _ptr(lbv), ubk, ub, _ptr(ubv), lam, thr, k, kbegin, nunflag, seed)
+        code = _stop(opts, k, kmax, r, rprev)
+        if code is None and str((opts.get("stoprule") or {}).get("type", "none")).upper() == "ME":
+            rprev = r.copy()
+        if (li < len(K) and k == K[li]) or code is not None:
+            X.append(x.copy())
+            li += 1
+    return np.array(X).T, {"stoprule": code, "finaliter": k, "flags": flag.copy()}
 
 
 def fbp(A, b, theta, filter="ram-lak"):

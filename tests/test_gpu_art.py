@@ -1,0 +1,98 @@
+"""ART (art/art.m) and CART (cart/cart.m) on the engine against the FP64
+oracle's literal sweeps: same step orders, same random streams, same stop
+rules.  With FP64 storage the iterates agree to rounding; with FP32 storage
+they agree with the oracle run on the FP32-rounded matrix."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import airtoolsii_amd as air
+
+pytestmark = pytest.mark.gpu
+
+N, THETA, P = 24, np.arange(0.0, 180.0, 6.0), 34
+
+
+@pytest.fixture(scope="module")
+def prob(oracle):
+    Ao = oracle.paralleltomo(N, THETA, P)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    e = np.random.default_rng(1).standard_normal(b.size)
+    bn = b + 0.02 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    A64 = air.paralleltomo(N, THETA, P, dtype="f64")
+    A32 = air.paralleltomo(N, THETA, P, dtype="f32")
+    Ao32 = Ao.copy()
+    Ao32.data = Ao32.data.astype(np.float32).astype(np.float64)
+    return {"Ao": Ao, "Ao32": Ao32, "A64": A64, "A32": A32, "x": x, "b": bn}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+ART_CASES = {
+    "kaczmarz_plain": ("kaczmarz", [3, 10], {}),
+    "kaczmarz_bounds_damp": ("kaczmarz", [5], {"lbound": 0.0, "ubound": 1.0, "damp": 0.05, "relaxpar": 0.6}),
+    "symkaczmarz": ("symkaczmarz", [4, 10], {"lbound": 0.0}),
+    "randkaczmarz": ("randkaczmarz", [2, 6], {"seed": 17}),
+    "custom_order": (list(range(1020, 0, -3)), [4], {"relaxpar": 1.2}),
+    "relax_function": ("kaczmarz", [4], {"relaxpar": lambda j: 0.5 + 0.4 * np.cos(np.asarray(j) / 500.0)}),
+    "dp_stop": ("kaczmarz", [50], {"stoprule": {"type": "DP", "taudelta": 1.5}}),
+    "me_stop": ("kaczmarz", [50], {"stoprule": {"type": "ME", "taudelta": 1.2}}),
+}
+
+
+@pytest.mark.parametrize("name", list(ART_CASES))
+def test_art_matches_oracle_fp64(oracle, prob, name):
+    meth, K, opts = ART_CASES[name]
+    print(f"art case {name}", flush=True)
+    X, info = air.art(meth, prob["A64"], prob["b"], K, None, opts)
+    Xo, io = oracle.art(meth, prob["Ao"], prob["b"], K, None, opts)
+    assert info["finaliter"] == io["finaliter"] and info["stoprule"] == io["stoprule"]
+    assert X.shape == Xo.shape
+    for q in range(X.shape[1]):
+        assert rel(X[:, q], Xo[:, q]) <= 1e-10, (q, rel(X[:, q], Xo[:, q]))
+
+
+def test_art_fp32_storage_matches_oracle_on_rounded_matrix(oracle, prob):
+    X, info = air.kaczmarz(prob["A32"], prob["b"], [5], None, {"lbound": 0.0})
+    Xo, _ = oracle.art("kaczmarz", prob["Ao32"], prob["b"], [5], None, {"lbound": 0.0})
+    assert rel(X[:, -1], Xo[:, -1]) <= 1e-10
+    Xf, _ = oracle.art("kaczmarz", prob["Ao"], prob["b"], [5], None, {"lbound": 0.0})
+    assert rel(X[:, -1], Xf[:, -1]) <= 1e-5
+
+
+CART_CASES = {
+    "columnaction_default": ("columnaction", [3, 6], {}),
+    "flagging": ("columnaction", [2, 8], {"relaxpar": 1.0, "Kbegin": 3, "THR": 1e-3, "Nunflag": 3, "seed": 4}),
+    "bounds_custom_order": (list(range(576, 0, -1)), [5], {"lbound": 0.0, "ubound": np.full(576, 0.9),
+                                                          "relaxpar": 0.8}),
+    "me_stop": ("columnaction", [300], {"relaxpar": 1.0, "stoprule": {"type": "ME", "taudelta": 1.2}}),
+}
+
+
+@pytest.mark.parametrize("name", list(CART_CASES))
+def test_cart_matches_oracle_fp64(oracle, prob, name):
+    meth, K, opts = CART_CASES[name]
+    print(f"cart case {name}", flush=True)
+    X, info = air.cart(meth, prob["A64"], prob["b"], K, None, opts)
+    Xo, io = oracle.cart(meth, prob["Ao"], prob["b"], K, None, opts)
+    assert info["finaliter"] == io["finaliter"] and info["stoprule"] == io["stoprule"]
+    for q in range(X.shape[1]):
+        assert rel(X[:, q], Xo[:, q]) <= 1e-10, (q, rel(X[:, q], Xo[:, q]))
+
+
+def test_art_info_and_errors(prob):
+    X, info = air.symkaczmarz(prob["A32"], prob["b"], [2, 6])
+    assert info["finaliter"] == 6 and list(info["itersaved"]) == [2, 6] and X.shape[1] == 2
+    assert len(info["resnorm"]) == 4 and np.all(np.diff(info["resnorm"]) < 0)
+    with pytest.raises(ValueError, match="even"):
+        air.symkaczmarz(prob["A32"], prob["b"], [3])
+    with pytest.raises(ValueError, match="Unknown ART"):
+        air.art("bogus", prob["A32"], prob["b"], 2)
+    with pytest.raises(ValueError, match="scalar"):
+        air.cart("columnaction", prob["A32"], prob["b"], 2, None, {"relaxpar": lambda j: 1.0})
+    Af = air.paralleltomo(N, THETA, P, None, 0, 0)
+    with pytest.raises(Exception, match="stored matrix"):
+        air.kaczmarz(Af, prob["b"], 1)

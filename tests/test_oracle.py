@@ -740,6 +740,109 @@ def test_fbp_reconstructs_a_disk(oracle):
         oracle.fbp(A, b, theta, "gaussian")
 
 
+# ------------------------------------------------------------------ ART / CART
+def test_splitmix64_known_answers(oracle):
+    # the first outputs of SplitMix64 from state 0 (state advanced by the golden gamma)
+    g = 0x9E3779B97F4A7C15
+    L = oracle.lib()
+    assert L.ora_splitmix64(0) == 0xE220A8397B1DCDAF
+    assert L.ora_splitmix64(g) == 0x6E789E6AA1B965F4
+    assert L.ora_splitmix64((2 * g) % 2 ** 64) == 0x06C45D188009454F
+
+
+def _small_system(seed=1, m=30, n=20):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=0.3, random_state=seed, format="csr") + sp.eye(m, n, format="csr")
+    x = rng.uniform(0, 1, n)
+    return A.tocsr(), x, A @ x
+
+
+def test_kaczmarz_step_projects_onto_the_row_hyperplane(oracle):
+    A, x, b = _small_system()
+    for i in (0, 7, 29):
+        X, _ = oracle.art([i + 1], A, b + 0.3, 1, None, {"relaxpar": 1.0})
+        assert abs(A[i] @ X[:, 0] - (b[i] + 0.3)) < 1e-13
+
+
+def test_kaczmarz_converges_on_a_consistent_system(oracle):
+    A, x, b = _small_system()
+    for meth in ("kaczmarz", "randkaczmarz"):
+        X, info = oracle.art(meth, A, b, 300, None, {"seed": 3})
+        assert np.linalg.norm(X[:, -1] - x) < 1e-9 * np.linalg.norm(x), meth
+    X, info = oracle.art("symkaczmarz", A, b, 300)
+    assert info["finaliter"] == 300 and np.linalg.norm(X[:, -1] - x) < 1e-9
+
+
+def test_symkaczmarz_is_kaczmarz_over_the_mirrored_order(oracle):
+    A, x, b = _small_system()
+    Xs, _ = oracle.art("symkaczmarz", A, b, [4, 8])
+    order = list(range(1, 31)) + list(range(30, 0, -1))
+    Xc, _ = oracle.art(order, A, b, [2, 4])
+    np.testing.assert_array_equal(Xs, Xc)
+
+
+def test_kaczmarz_bounds_match_a_literal_loop(oracle):
+    A, x, b = _small_system(4)
+    b = b + np.random.default_rng(5).standard_normal(b.size) * 0.1
+    opts = {"lbound": 0.2, "ubound": np.full(20, 0.8), "relaxpar": 0.7, "damp": 0.1}
+    X, _ = oracle.art("kaczmarz", A, b, 3, np.full(20, 2.0), opts)
+    xx = np.full(20, 2.0)
+    D = A.toarray()
+    nrm = (D * D).sum(1)
+    nrm = nrm + 0.1 * nrm.max()
+    for _ in range(3):
+        for i in range(30):
+            xx = xx + (0.7 * (b[i] - D[i] @ xx) / nrm[i]) * D[i]
+            xx = np.minimum(np.maximum(xx, 0.2), 0.8)
+    np.testing.assert_allclose(X[:, -1], xx, rtol=1e-13, atol=1e-14)
+
+
+def test_randkaczmarz_sweeps_are_permutations(oracle):
+    A, x, b = _small_system()
+    L = oracle.lib()
+    for k in (1, 2, 3):
+        perm = list(range(30))
+        for t in range(29, 0, -1):
+            j = int(L.ora_rng_u01(9, 1, k, t) * (t + 1))
+            perm[t], perm[j] = perm[j], perm[t]
+        assert sorted(perm) == list(range(30)) and perm != list(range(30))
+    X1, _ = oracle.art("randkaczmarz", A, b, 2, None, {"seed": 9})
+    X2, _ = oracle.art("randkaczmarz", A, b, 2, None, {"seed": 10})
+    assert not np.array_equal(X1, X2)
+
+
+def test_cart_column_step_zeroes_the_column_gradient(oracle):
+    # with relaxpar 1 and no bounds a column step is exact coordinate descent:
+    # afterwards a_j' r = 0, and the residual kept by the sweep is b - A x
+    A, x, b = _small_system()
+    Ac = A.tocsc()
+    for j in (0, 5, 19):
+        X, _ = oracle.cart([j + 1], A, b, 1, None, {"relaxpar": 1.0})
+        r = b - A @ X[:, 0]
+        assert abs(Ac[:, j].toarray().ravel() @ r) < 1e-12
+    X, info = oracle.cart("columnaction", A, b, 200, None, {"relaxpar": 1.0, "Kbegin": 10 ** 6})
+    assert np.linalg.norm(X[:, -1] - x) < 1e-9
+
+
+def test_cart_flagging_freezes_converged_columns(oracle):
+    # a huge THR flags every column at sweep Kbegin; flagged columns are skipped
+    # until their unflag draw succeeds, so x stays put in the sweep after
+    A, x, b = _small_system()
+    X, info = oracle.cart("columnaction", A, b, [3, 4], None,
+                          {"relaxpar": 1.0, "Kbegin": 3, "THR": 1e9, "Nunflag": 50})
+    assert not info["flags"].any() or info["flags"].sum() < 5
+    np.testing.assert_array_equal(X[:, 0], X[:, 1])
+
+
+def test_art_stop_rules(oracle):
+    A, x, b = _small_system()
+    X, info = oracle.art("kaczmarz", A, b, 100, None, {"stoprule": {"type": "DP", "taudelta": 1e-3}})
+    r = [np.linalg.norm(b - A @ oracle.art("kaczmarz", A, b, k)[0][:, 0]) for k in range(1, info["finaliter"] + 1)]
+    assert info["stoprule"] == 2 and r[-1] <= 1e-3 < r[-2]
+    X, info = oracle.cart("columnaction", A, b, 500, None, {"stoprule": {"type": "ME", "taudelta": 1e-2}})
+    assert info["stoprule"] == 3 and info["finaliter"] < 500
+
+
 # ------------------------------------------------------------------ phantoms
 def test_phantoms_basic_properties(oracle):
     sl = oracle.phantomgallery("shepplogan", 64)
