@@ -29,6 +29,7 @@ BOUND_NONE, BOUND_SCALAR, BOUND_VECTOR = 0, 1, 2
 EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "air_set_matrix_csr",
            "air_set_matrix_csc", "air_build_paralleltomo", "air_build_fanlineartomo", "air_build_fancurvedtomo", "air_build_seismictomo",
            "air_build_sphericaltomo", "air_build_seismicwavetomo", "air_fbp", "air_art", "air_cart",
+           "air_set_graph", "air_graph_launches",
            "air_size", "air_get_csr",
            "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
@@ -123,6 +124,8 @@ def load():
     L.air_sirt_iterate.argtypes = [_P, C.c_int32, _dp]
     L.air_sirt_fetch.argtypes = [_P, _P, _P, C.POINTER(SirtInfo)]
     L.air_set_kernel_timing.argtypes = [_P, C.c_int]
+    L.air_set_graph.argtypes = [_P, C.c_int32]
+    L.air_graph_launches.argtypes = [_P, C.POINTER(_i64)]
     L.air_get_kernel_timing.argtypes = [_P, _dp, _dp, _dp, C.POINTER(_i64)]
     L.air_layout_info.argtypes = [_P, C.POINTER(Layout)]
     L.air_mlp_forward.argtypes = [_P, _P, _P, _i64, _P]

@@ -39,6 +39,7 @@ struct SirtState {
     std::unique_ptr<NcpState> ncp;   // NCP stop rule (FFT plan and buffers), kept across runs
     LearnBufs learn;                 // learned / greedy relaxation
     DevBuf l_mlp, l_xstar, l_state, l_gpart, l_feat, l_target, l_out, l_err;
+    DevBuf lamdev;                   // graph mode: the lambda schedule on the device
     SirtBufs bufs{};
     double setup_s = 0, iter_s = 0;
 };
@@ -66,6 +67,11 @@ struct air_engine {
     air::DevBuf wMdev[AIR_SART + 1], wDdev[AIR_SART + 1];   // FP32 copies in the loop's order
     bool wdev[AIR_SART + 1] = {};
     double wrho[AIR_SART + 1] = {NAN, NAN, NAN, NAN, NAN};
+    // graph mode: iterations per captured HIP graph (0 = direct launches) and
+    // the instantiated graph of the current prepared run
+    int graph_iters = 0;
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_launches = 0;   // graphs launched since the engine was created
     // per-kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -138,6 +144,8 @@ __global__ void k_widen(const V* in, double* out, int64_t n) {
 }
 
 void reset_operator(air_engine* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
     e->A = DevSparse();
     e->AT = DevSparse();
     e->T = TiledOp();
@@ -461,9 +469,30 @@ hipEvent_t mid_event(air_engine* e) {
 
 // one SIRT iteration k (sirt.m:213-243): the A'(M r) + update pass, then the
 // residual pass; events 0..3 bracket them when kernel timing is on.
-void enqueue_iteration(air_engine* e, int k) {
+// Graph mode applies to the tiled layout with a host lambda schedule and the
+// DP/ME/none stop rules: every kernel of such an iteration takes its
+// iteration number and lambda from the device (ctrl->kcur, ctrl->lam_cur), so
+// one captured run of iterations can be replayed for any k.
+bool graph_mode(const air_engine* e) {
+    const SirtState& st = e->st;
+    return e->graph_iters > 0 && !e->timing && use_tiled(e) && !device_lambda(st.relax_kind) &&
+           st.relax_kind != AIR_RELAX_LEARNED && st.relax_kind != AIR_RELAX_GREEDY && st.sp.type != AIR_STOP_NCP &&
+           (!st.prepared || st.bufs.lamv != nullptr);
+}
+
+void drop_graph(air_engine* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+}
+
+void enqueue_iteration(air_engine* e, int k, bool devk = false) {
     SirtState& st = e->st;
     hipStream_t s = e->stream;
+    if (devk) {                     // k and lambda from the device (graph mode)
+        launch_tiled_update(e->T, st.bufs, NAN, 0, -1, s);
+        launch_tiled_resid(e->T, st.bufs, st.sp, -1, s, nullptr);
+        return;
+    }
     const bool line = device_lambda(st.relax_kind);
     double lam = line ? 0.0 : st.lam[k - 1];
     record(e, 0);
@@ -489,6 +518,41 @@ void enqueue_iteration(air_engine* e, int k) {
     }
     record(e, 3);
     if (e->timing) e->ev_used++;
+}
+
+// Enqueue iterations st.k+1 .. st.k+count.  In graph mode whole runs of
+// graph_iters iterations are one hipGraphLaunch of a graph captured once per
+// prepared run; the rest are launched directly in the same device-k form.
+void enqueue_iterations(air_engine* e, int count) {
+    SirtState& st = e->st;
+    hipStream_t s = e->stream;
+    if (!graph_mode(e)) {
+        for (int i = 0; i < count; ++i) enqueue_iteration(e, ++st.k);
+        return;
+    }
+    const int G = e->graph_iters;
+    AIR_CHECK(st.bufs.lamv != nullptr, AIR_E_STATE, "graph mode changed after air_sirt_prepare");
+    while (count > 0) {
+        if (count >= G) {
+            if (!e->gexec) {
+                hipGraph_t g = nullptr;
+                AIR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < G; ++i) enqueue_iteration(e, 0, true);
+                AIR_HIP(hipStreamEndCapture(s, &g));
+                const hipError_t ie = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                AIR_HIP(ie);
+            }
+            AIR_HIP(hipGraphLaunch(e->gexec, s));
+            e->graph_launches++;
+            st.k += G;
+            count -= G;
+        } else {
+            enqueue_iteration(e, 0, true);
+            ++st.k;
+            --count;
+        }
+    }
 }
 
 // after the stream has drained: sum the per-kernel event intervals
@@ -708,6 +772,12 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     }
     st.kmax = kmax;
     st.k = 0;
+    drop_graph(e);
+    if (graph_mode(e) && kmax > 0) {
+        st.lamdev.alloc(8 * (size_t)kmax);
+        AIR_HIP(hipMemcpyAsync(st.lamdev.p, st.lam.data(), 8 * (size_t)kmax, hipMemcpyHostToDevice, s));
+        B.lamv = st.lamdev.as<double>();
+    }
     // r0 = b - A x0 and the k = 0 check (sirt.m:153-162)
     if (use_tiled(e)) launch_tiled_resid(e->T, B, st.sp, 0, s, nullptr);
     else if (e->kind == 1) launch_resid(e->A, e->shA, B, st.sp, 0, s);
@@ -1068,9 +1138,13 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         // with a rule that can stop early, check the stop flag every 32 iterations
         // instead of enqueueing iterations that would only return
         const bool early = st.sp.type != AIR_STOP_NONE || st.relax_kind == AIR_RELAX_LEARNED;
-        for (int i = 0; i < niter && st.k < st.kmax; ++i) {
-            enqueue_iteration(e, ++st.k);
-            if (early && (i + 1) % 32 == 0 && read_stop(e)) break;
+        const int chunk = graph_mode(e) ? e->graph_iters : 32;
+        int left = std::min(niter, st.kmax - st.k);
+        while (left > 0) {
+            const int c = early ? std::min(left, chunk) : left;
+            enqueue_iterations(e, c);
+            left -= c;
+            if (early && left > 0 && read_stop(e)) break;
         }
         AIR_HIP(hipEventRecord(b, e->stream));
         AIR_HIP(hipEventSynchronize(b));
@@ -1159,14 +1233,15 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
         DevBuf snaps;
         if (nsnap) snaps.alloc(4 * (size_t)nx * nsnap);
         int l = 0;
-        const int chunk = 32;
+        const int chunk = graph_mode(e) ? e->graph_iters : 32;
         bool stopped = read_stop(e) != 0;
         auto ti = std::chrono::steady_clock::now();
         while (!stopped && st.k < kmax) {
             int upto = std::min(kmax, st.k + chunk);
             timing_begin(e, upto - st.k);
             while (st.k < upto) {
-                enqueue_iteration(e, ++st.k);
+                const int seg = (l < nsnap && K[l] <= upto ? K[l] : upto) - st.k;   // up to the next saved k
+                enqueue_iterations(e, seg);
                 if (l < nsnap && st.k == K[l]) {
                     AIR_HIP(hipMemcpyAsync(snaps.as<float>() + (size_t)l * nx, st.x.p, 4 * nx, hipMemcpyDeviceToDevice, s));
                     l++;
@@ -1235,6 +1310,24 @@ int air_set_kernel_timing(air_engine* e, int on) {
         e->timing = on != 0;
         e->t_rows_ms = e->t_cols_ms = e->t_red_ms = 0;
         e->launches = 0;
+    });
+}
+
+int air_set_graph(air_engine* e, int32_t iters_per_graph) {
+    return guard([&] {
+        AIR_CHECK(e, AIR_E_ARG, "null engine");
+        AIR_CHECK(iters_per_graph >= 0 && iters_per_graph <= 4096, AIR_E_ARG, "iterations per graph must be 0..4096");
+        use_device(e);
+        drop_graph(e);
+        e->graph_iters = iters_per_graph;
+        e->st.prepared = false;       // the next run is prepared for the new launch mode
+    });
+}
+
+int air_graph_launches(air_engine* e, int64_t* n) {
+    return guard([&] {
+        AIR_CHECK(e && n, AIR_E_ARG, "null argument");
+        *n = e->graph_launches;
     });
 }
 

@@ -270,8 +270,8 @@ struct Ctrl {
     int stop;
     int finaliter;
     int stoprule;
-    int pad0;
-    double lam_cur;      // relaxpar of the last executed iteration
+    int kcur;            // last iteration whose residual and stop check are done
+    double lam_cur;      // relaxpar of the last executed iteration (graph mode: of the next one)
     double rMr;          // r'Mr of the current residual (line search numerator)
     double last_norm;
 };
@@ -341,6 +341,7 @@ struct SirtBufs {
     double* hist;        // ||r_k||, k = 0..kmax
     Ctrl* ctrl;
     NcpState* ncp = nullptr;  // NCP stop rule state (null otherwise)
+    const double* lamv = nullptr;  // graph mode: lambda_1..lambda_kmax on the device (null otherwise)
     const LearnBufs* learn = nullptr;  // learned / greedy relaxation (null otherwise)
 };
 

@@ -199,10 +199,13 @@ __global__ __launch_bounds__(BLOCK) void k_resid_vec(int64_t m, const float* __r
 
 // Reduce the row pass's block partials in a fixed order and evaluate the stop
 // rule of iteration k (auxil/check_stoprules.m) on the device.
+// k < 0 (graph mode): the iteration number comes from ctrl->kcur, and the
+// next iteration's lambda is staged in ctrl->lam_cur from lamv.
 __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restrict__ partial, int nblocks, double* hist,
-                                                         Ctrl* ctrl, StopParams sp, int k) {
+                                                         Ctrl* ctrl, StopParams sp, int k, const double* lamv) {
     __shared__ double sh[4][BLOCK / 64];
     if (load_stop(ctrl)) return;
+    if (k < 0) k = ctrl->kcur + 1;
     double v[4] = {0, 0, 0, 0};
     typedef double d4 __attribute__((ext_vector_type(4)));
     const d4* p4 = (const d4*)partial;
@@ -224,7 +227,11 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
         for (int j = threadIdx.x; j < sp.ncp_cols; j += BLOCK) c[0] += sp.ncp_stat[j];
         block_reduce<1>(c, shc);
     }
-    if (threadIdx.x == 0) evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], c[0], hist);
+    if (threadIdx.x == 0) {
+        evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], c[0], hist);
+        ctrl->kcur = k;
+        if (lamv && k < sp.kmax) ctrl->lam_cur = lamv[k];
+    }
 }
 
 // NCP statistic of one residual column (check_stoprules.m:107-133): from the
@@ -400,7 +407,7 @@ void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipSt
 
 void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s) {
     if (sp.type == AIR_STOP_NCP && B.ncp) ncp_enqueue(*B.ncp, B.r, B.ncp->m, B.ctrl, s);
-    k_finalize_rows<<<1, BLOCK, 0, s>>>(B.partial, nblocks, B.hist, B.ctrl, sp, k);
+    k_finalize_rows<<<1, BLOCK, 0, s>>>(B.partial, nblocks, B.hist, B.ctrl, sp, k, B.lamv);
     AIR_HIP(hipGetLastError());
 }
 

@@ -56,6 +56,12 @@ class AirOperator:
         check(load().air_apply(self.engine.h, 1, ptr(w), ptr(out)))
         return out
 
+    def graph_launches(self):
+        """HIP graphs launched on this operator's engine (graph mode of the SIRT loop)."""
+        n = _lib.C.c_int64()
+        check(load().air_graph_launches(self.engine.h, _lib.C.byref(n)))
+        return n.value
+
     def __matmul__(self, v):
         return self.matvec(v)
 

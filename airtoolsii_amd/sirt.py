@@ -9,6 +9,8 @@ sirt/cav.m:122, sirt/drop.m:122, sirt/landweber.m:119, sirt/sart.m:129.
 ``lbound``, ``ubound``, ``relaxpar`` (scalar, 'line', 'psi1', 'psi1mod',
 'psi2', 'psi2mod', or a per-iteration vector), ``rho``, ``stoprule``
 (dict with ``type``, ``taudelta``, ``res_dims``, ``ncp_smooth``), ``verbose``.
+Engine-only keys: ``graph`` (iterations per captured HIP graph, 0 = direct
+launches), and ``mlp``/``noise_norm``/``x_true`` for the learned relaxation.
 
 The whole K-loop runs on the GPU (libairhip.so); this module only validates
 arguments and converts results, raising ``ValueError`` with the reference's
@@ -207,6 +209,7 @@ def sirt(sirt_method, A, b, K, x0=None, options=None):
     opts = dict(options or {})
     o, keep, code, Mc, Dc, name = _build_options(A, sirt_method, opts, kmax)
     A = as_operator(A)
+    check(load().air_set_graph(A.engine.h, int(opts.get("graph", 0) or 0)))
     K32 = np.ascontiguousarray(K, np.int32)
     X = np.zeros((len(K), n))
     its = np.zeros(len(K) + 1, np.int32)
@@ -305,6 +308,7 @@ class SirtRun:
         self._o, self._keep, *_ = _build_options(self.A, sirt_method, options, self.kmax)
         self._b = f64(b)
         self._x0 = None if x0 is None else f64(x0)
+        check(load().air_set_graph(self.A.engine.h, int(dict(options or {}).get("graph", 0) or 0)))
         check(load().air_sirt_prepare(self.A.engine.h, self._o, ptr(self._b), ptr(self._x0), self.kmax))
 
     def iterate(self, niter):

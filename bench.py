@@ -10,6 +10,8 @@
         block-diagonal operator, Cimmino K=100; optionally against one by one
     python bench.py --study spmm [--spmm-K K]
         64 right-hand sides on the N=512, 360-angle operator (SpMM passes)
+    python bench.py --study graph [--graph-iters G]
+        direct launches against graph mode at N = 64 .. 512
     python bench.py --study learned [--learn-train T --learn-eval E --learn-K K]
         learned relaxation at paralleltomo N=256, 180 angles, p=362: greedy
         trajectories of T seeded problems, network trained on the GPU, then
@@ -271,6 +273,41 @@ def spmm_study(args):
     print(json.dumps(out), flush=True)
 
 
+def graph_study(args):
+    """SIRT iterations per second with direct kernel launches against graph mode
+    (G iterations captured once into a HIP graph and replayed), from the
+    C1-sized problem up to the N=512 operator and the 256-problem batch."""
+    import airtoolsii_amd as air
+    G = args.graph_iters
+    cases = [(64, np.arange(0.0, 180.0), 2000), (128, np.arange(0.0, 180.0), 1000),
+             (256, np.arange(0.0, 180.0), 500), (512, np.arange(0.0, 180.0, 0.5), 200)]
+    rows = []
+    for N, theta, K in cases:
+        p = int(np.floor(np.sqrt(2) * N + 0.5))
+        A = air.paralleltomo(N, theta, p)
+        x = air.phantomgallery("shepplogan", N).ravel(order="F")
+        b, _ = air.add_noise(A.matvec(x), 0.01, seed=1)
+        row = {"workload": f"paralleltomo N={N}, {len(theta)} angles, p={p}, cimmino, lbound 0", "K": K}
+        xs = {}
+        for g in (0, G):
+            run = air.SirtRun("cimmino", A, b, K + G, None, {"lbound": 0.0, "graph": g})
+            run.iterate(G)                                  # weights cached, graph captured
+            t0 = time.perf_counter()
+            gpu_ms = run.iterate(K)
+            wall = time.perf_counter() - t0
+            xs[g] = run.fetch()[0]
+            row["direct" if g == 0 else "graph"] = {"iter_per_s": round(K / wall, 1),
+                                                     "us_per_iter": round(wall / K * 1e6, 2),
+                                                     "gpu_us_per_iter": round(gpu_ms / K * 1e3, 2)}
+        row["speedup"] = round(row["direct"]["us_per_iter"] / row["graph"]["us_per_iter"], 3)
+        row["identical_iterates"] = bool(np.array_equal(xs[0], xs[G]))
+        rows.append(row)
+        del A
+    out = {"metric": "SIRT iter/s, direct launches vs graph mode", "unit": "iter/s", "n_gpus": 1,
+           "dtype": "f32", "data": "synthetic", "iters_per_graph": G, "cases": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,7 +320,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--study", choices=["learned", "batch", "spmm"], default=None)
+    ap.add_argument("--study", choices=["learned", "batch", "spmm", "graph"], default=None)
+    ap.add_argument("--graph-iters", type=int, default=32)
     ap.add_argument("--spmm-K", type=int, default=50)
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-K", type=int, default=100)
@@ -310,6 +348,9 @@ def main():
         return
     if args.study == "spmm":
         spmm_study(args)
+        return
+    if args.study == "graph":
+        graph_study(args)
         return
 
     import airtoolsii_amd as air

@@ -210,6 +210,18 @@ int air_sirt_records(air_engine* e, float* feat, float* target, float* out, doub
 int air_set_kernel_timing(air_engine* e, int on);
 int air_get_kernel_timing(air_engine* e, double* cols_ms, double* rows_ms, double* reduce_ms, int64_t* launches);
 
+/* Graph mode (no counterpart in the reference, which has no launch
+ * mechanism to speak of): iterations of the device-resident loop are captured
+ * once per prepared run into a HIP graph of iters_per_graph iterations and
+ * replayed, which removes the per-kernel launch cost for small problems and
+ * batches.  It applies to the tiled layout with a host lambda schedule
+ * (default, scalar, psi, vector) and the stop rules none/DP/ME; other runs
+ * launch directly.  0 (the default) turns it off.  Takes effect at the next
+ * air_sirt / air_sirt_prepare. */
+int air_set_graph(air_engine* e, int32_t iters_per_graph);
+/* number of graph launches on this engine so far */
+int air_graph_launches(air_engine* e, int64_t* n);
+
 /* Storage of the operator used by the SIRT loop. */
 typedef struct {
     int     tiled;        /* 1: LDS-tiled layout, 0: CSR + CSC */
