@@ -156,8 +156,8 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double (*sh)[NT / 
 
 // Stop rules of auxil/check_stoprules.m on the device (thread 0 of the finalize
 // block).  ncp_sum is the sum over residual columns of the NCP statistic.
-__device__ void evaluate_stop(Ctrl* ctrl, const StopParams sp, int k, double rr, double rMr, double me_num,
-                              double rkm1sq, double ncp_sum, double* hist) {
+__device__ int evaluate_stop(Ctrl* ctrl, const StopParams sp, int k, double rr, double rMr, double me_num,
+                             double rkm1sq, double ncp_sum, double* hist) {
     double nrk = sqrt(rr);
     hist[k] = nrk;
     ctrl->rMr = rMr;
@@ -196,6 +196,7 @@ __device__ void evaluate_stop(Ctrl* ctrl, const StopParams sp, int k, double rr,
         ctrl->stoprule = code;
         __hip_atomic_store(&ctrl->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    return stop;
 }
 
 // Block totals (valid in thread 0 after block_reduce) -> partial[blockIdx.x*NV + q].

@@ -497,7 +497,7 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
     double lam = line ? 0.0 : st.lam[k - 1];
     record(e, 0);
     if (st.relax_kind == AIR_RELAX_LEARNED) {
-        launch_learned_select(st.bufs, k, s);
+        if (!fused_select(st.bufs)) launch_learned_select(st.bufs, k, s);
         lam = NAN;                  // the update kernels read the network's lambda_k
     }
     if (use_tiled(e)) {

@@ -365,6 +365,8 @@ void launch_finalize_line(const SirtBufs& B, int nblocks, int k, hipStream_t s);
 // learned relaxation: the network's lambda_k (read by the update kernels when
 // they are given lam = NaN) and stop, from the residual statistics of x_{k-1}
 void launch_learned_select(const SirtBufs& B, int k, hipStream_t s);
+// true when the finalize kernel of the row pass computes the learned lambda (no separate kernel)
+bool fused_select(const SirtBufs& B);
 unsigned tiled_reduce_grid(int64_t m);
 // matrix-free iteration pieces
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s);

@@ -199,13 +199,43 @@ __global__ __launch_bounds__(BLOCK) void k_resid_vec(int64_t m, const float* __r
 
 // Reduce the row pass's block partials in a fixed order and evaluate the stop
 // rule of iteration k (auxil/check_stoprules.m) on the device.
+// inputs of the learned selection for iteration k: |r_{k-1}|, |r_{k-2}|,
+// |r_{k-3}|, r'Mr of x_{k-1}, the carried state and the stop flag
+struct SelIn {
+    double nr, nr1, nr2, rMr;
+    double S[LEARN_STATE];
+    int stop;
+};
+__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* f,
+                                             float* h1, float* h2, float* Ws, double* sums, float* part,
+                                             bool staged);
+__device__ __forceinline__ void stage_weights(const float* __restrict__ mlp, float* Ws);
+constexpr int NW_LANE_F = (MLP_PARAMS + 255) / 256;
+__device__ __forceinline__ void load_weights(const float* __restrict__ mlp, float (&w)[NW_LANE_F]);
+__device__ __forceinline__ void store_weights(const float (&w)[NW_LANE_F], float* Ws);
+
 // k < 0 (graph mode): the iteration number comes from ctrl->kcur, and the
-// next iteration's lambda is staged in ctrl->lam_cur from lamv.
+// next iteration's lambda is staged in ctrl->lam_cur from lamv.  sel: the
+// learned relaxation's lambda_{k+1} and stop are computed here, at the end of
+// iteration k, instead of in a kernel of their own before the next update.
 __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restrict__ partial, int nblocks, double* hist,
-                                                         Ctrl* ctrl, StopParams sp, int k, const double* lamv) {
+                                                         Ctrl* ctrl, StopParams sp, int k, const double* lamv,
+                                                         LearnBufs Lsel, int sel) {
     __shared__ double sh[4][BLOCK / 64];
+    __shared__ float Ws[MLP_PARAMS];
+    __shared__ SelIn in;
+    __shared__ double hprev[2];
     if (load_stop(ctrl)) return;
     if (k < 0) k = ctrl->kcur + 1;
+    const bool do_sel = sel && k < sp.kmax;
+    float wreg[NW_LANE_F];
+    double spre = 0.0;
+    if (do_sel) {                                    // issued now, stored after the reduction below
+        load_weights(Lsel.mlp, wreg);
+        if (threadIdx.x < LEARN_STATE) spre = Lsel.state[threadIdx.x];
+        else if (threadIdx.x == 64) spre = k >= 1 ? hist[k - 1] : 0.0;
+        else if (threadIdx.x == 65) spre = k >= 2 ? hist[k - 2] : 0.0;
+    }
     double v[4] = {0, 0, 0, 0};
     typedef double d4 __attribute__((ext_vector_type(4)));
     const d4* p4 = (const d4*)partial;
@@ -220,6 +250,12 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] += a0[q];
     }
+    if (do_sel) {
+        store_weights(wreg, Ws);
+        if (threadIdx.x < LEARN_STATE) in.S[threadIdx.x] = spre;
+        else if (threadIdx.x == 64) hprev[0] = spre;
+        else if (threadIdx.x == 65) hprev[1] = spre;
+    }
     block_reduce<4>(v, sh);
     double c[1] = {0};
     if (sp.ncp_cols > 0) {
@@ -228,9 +264,24 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
         block_reduce<1>(c, shc);
     }
     if (threadIdx.x == 0) {
-        evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], c[0], hist);
+        const int st = evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], c[0], hist);
         ctrl->kcur = k;
         if (lamv && k < sp.kmax) ctrl->lam_cur = lamv[k];
+        if (do_sel) {                                // the selection for iteration k + 1
+            in.stop = st;
+            in.nr = sqrt(v[0]);                      // hist[k], as evaluate_stop stores it
+            in.rMr = v[1];
+            in.nr1 = k >= 1 ? hprev[0] : in.nr;
+            in.nr2 = k >= 2 ? hprev[1] : in.nr1;
+        }
+    }
+    if (do_sel) {                                    // uniform: kernel arguments and k
+        __shared__ float f[LEARN_NF];
+        __shared__ float h1[MLP_HID], h2[MLP_HID];
+        __shared__ float part[4 * MLP_HID];
+        __shared__ double sums[4];
+        __syncthreads();                             // in, Ws and hprev visible to the block
+        select_block(in, ctrl, Lsel, k + 1, f, h1, h2, Ws, sums, part, true);
     }
 }
 
@@ -405,9 +456,16 @@ void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipSt
     AIR_HIP(hipGetLastError());
 }
 
+// A learned run without the error record selects lambda_{k+1} inside the finalize of iteration k.
+bool fused_select(const SirtBufs& B) {
+    return B.learn && B.learn->mode == AIR_RELAX_LEARNED && B.learn->mlp && !B.learn->xstar;
+}
+
 void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s) {
     if (sp.type == AIR_STOP_NCP && B.ncp) ncp_enqueue(*B.ncp, B.r, B.ncp->m, B.ctrl, s);
-    k_finalize_rows<<<1, BLOCK, 0, s>>>(B.partial, nblocks, B.hist, B.ctrl, sp, k, B.lamv);
+    const bool sel = fused_select(B);
+    k_finalize_rows<<<1, BLOCK, 0, s>>>(B.partial, nblocks, B.hist, B.ctrl, sp, k, B.lamv,
+                                        sel ? *B.learn : LearnBufs{}, sel ? 1 : 0);
     AIR_HIP(hipGetLastError());
 }
 
@@ -441,19 +499,60 @@ __global__ __launch_bounds__(BLOCK) void k_greedy_dots(int64_t n, const double* 
 //  8 log(1+k)/5   9 log(|r|/|e|)   10 log(|r_prev|/|e|)   11 [|r| > |e|]
 //  (9-11 zero when |e| is unknown)   12 lam_prev2 rho   13 feature 4 of the last iteration
 //  14 log(r'Mr / |r|^2) - log(r0'Mr0 / |r0|^2)   15 [k == 1]
-__global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, LearnBufs L, int k) {
-    __shared__ float f[LEARN_NF];
-    __shared__ float h1[MLP_HID], h2[MLP_HID];
-    __shared__ float Ws[MLP_PARAMS];
-    __shared__ double sums[4];
-    if (load_stop(ctrl)) return;
+// The features of the state x_{k-1}, the network and lambda_k (and the stop)
+// for one block of 256 threads; shared arrays come from the caller.
+constexpr int P_W2 = MLP_HID * MLP_IN + MLP_HID;
+
+// The network's parameters staged input-major (W1 and W2 transposed), so that
+// the 64 lanes computing one layer's outputs read 64 consecutive LDS words.
+// A lane's share of the parameters: all loads are issued before the first
+// LDS store, so the block waits for one memory round trip instead of one per
+// step; the finalize kernel issues them before its reduction and stores them
+// after it.
+__device__ __forceinline__ void load_weights(const float* __restrict__ mlp, float (&w)[NW_LANE_F]) {
+#pragma unroll
+    for (int j = 0; j < NW_LANE_F; ++j) {
+        const int q = (int)threadIdx.x + 256 * j;
+        w[j] = q < MLP_PARAMS ? mlp[q] : 0.f;
+    }
+}
+
+__device__ __forceinline__ void store_weights(const float (&w)[NW_LANE_F], float* Ws) {
+#pragma unroll
+    for (int j = 0; j < NW_LANE_F; ++j) {
+        const int q = (int)threadIdx.x + 256 * j;
+        if (q >= MLP_PARAMS) continue;
+        int d = q;
+        if (q < MLP_HID * MLP_IN) d = (q % MLP_IN) * MLP_HID + q / MLP_IN;
+        else if (q >= P_W2 && q < P_W2 + MLP_HID * MLP_HID) {
+            const int r = q - P_W2;
+            d = P_W2 + (r % MLP_HID) * MLP_HID + r / MLP_HID;
+        }
+        Ws[d] = w[j];
+    }
+}
+
+__device__ __forceinline__ void stage_weights(const float* __restrict__ mlp, float* Ws) {
+    float w[NW_LANE_F];
+    load_weights(mlp, w);
+    store_weights(w, Ws);
+}
+
+__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* f,
+                                             float* h1, float* h2, float* Ws, double* sums, float* part,
+                                             bool staged) {
+    (void)f; (void)h1; (void)h2; (void)sums; (void)part;
+    if (in.stop) return;
+    if (L.mlp && !staged) {           // uniform (kernel argument): the whole block stages
+        stage_weights(L.mlp, Ws);
+        __syncthreads();
+    }
+    // from here on wave 0 alone: the features, the greedy sums and the network
+    // are wave-synchronous (lane broadcasts, no block barriers)
+    if (threadIdx.x >= 64) return;
     const int t = threadIdx.x;
-    // the network's parameters, staged with coalesced loads while wave 0 works
-    if (L.mlp)
-        for (int i = t; i < MLP_PARAMS; i += 256) Ws[i] = L.mlp[i];
-    // wave 0 reduces the greedy sums and computes the features
     double b0 = 0, b1 = 0, b2 = 0;
-    if (L.xstar && t < 64)
+    if (L.xstar)
         for (int i = t; i < L.gblocks; i += 64) {
             b0 += L.gpart[3 * i];
             b1 += L.gpart[3 * i + 1];
@@ -464,53 +563,61 @@ __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, 
         b1 += __shfl_xor(b1, d);
         b2 += __shfl_xor(b2, d);
     }
-    if (t == 0) {
-        // state: 0 |r0|  1 r0'Mr0  2 r_prev'Mr_prev  3 unused  4 f4 prev  5 lam_prev  6 lam_prev2
-        double* S = L.state;
-        const double nr = hist[k - 1], nr1 = k >= 2 ? hist[k - 2] : nr, nr2 = k >= 3 ? hist[k - 3] : nr1;
-        const double rMr = ctrl->rMr;
-        if (k == 1) {
-            for (int q = 0; q < LEARN_STATE; ++q) S[q] = 0;
-            S[0] = nr;
-            S[1] = rMr;
-            S[2] = rMr;
-        }
-        auto lg = [](double v) { return v > 0 ? log(v) : -30.0; };
-        const double rho = (double)L.rho;
-        double fv[LEARN_NF];
-        fv[0] = lg(nr / S[0]);
-        fv[1] = lg(nr1 / S[0]);
-        fv[2] = lg(nr / nr1);
-        fv[3] = lg(rMr / S[1]);
-        fv[4] = lg(rMr / S[2]);
-        fv[5] = S[5] * rho;
-        fv[6] = lg(nr1 / nr2);
-        fv[7] = (k - 1) / 100.0;
-        fv[8] = log1p((double)(k - 1)) / 5.0;
-        const bool hasn = L.noise > 0;
-        fv[9] = hasn ? lg(nr / L.noise) : 0.0;
-        fv[10] = hasn ? lg(nr1 / L.noise) : 0.0;
-        fv[11] = hasn ? (nr > L.noise ? 1.0 : 0.0) : 0.0;
-        fv[12] = S[6] * rho;
-        fv[13] = S[4];
-        fv[14] = lg(rMr / (nr * nr)) - lg(S[1] / (S[0] * S[0]));
-        fv[15] = k == 1 ? 1.0 : 0.0;
-        S[2] = rMr;
-        S[4] = fv[4];
-        for (int q = 0; q < LEARN_NF; ++q) {
-            f[q] = (float)fv[q];
-            if (L.feat) L.feat[(int64_t)(k - 1) * LEARN_NF + q] = (float)fv[q];
-        }
-        sums[0] = b0;
-        sums[1] = b1;
-        sums[2] = b2;
-        if (L.xstar && L.err) L.err[k - 1] = sqrt(b2);
+    // state: 0 |r0|  1 r0'Mr0  2 r_prev'Mr_prev  3 unused  4 f4 prev  5 lam_prev  6 lam_prev2
+    const double nr = in.nr, nr1 = in.nr1, nr2 = in.nr2;
+    const double rMr = in.rMr;
+    const bool first = k == 1;
+    const double* S = in.S;
+    const double S0 = first ? nr : S[0], S1 = first ? rMr : S[1], S2 = first ? rMr : S[2];
+    // the features are FP32 network inputs, so FP32 logs suffice; ratios are
+    // clamped to [1e-30, 1e30] (non-positive ones give -30)
+    auto lg = [](double v) { return v > 1e-30 ? (double)__logf((float)fmin(v, 1e30)) : -30.0; };
+    // lanes pick their own arguments, then all of them make the same two log
+    // calls (a switch around the calls would run 16 logs one after another)
+    const double rho = (double)L.rho;
+    const bool hasn = L.noise > 0;
+    double an = 1.0, ad = 1.0, cn = 1.0, cd = 1.0, lin = 0.0;
+    int kind = 1;                     // 0: lg(an/ad) - lg(cn/cd), 1: lin, 2: lg(lin) / 5
+    switch (t) {
+    case 0: kind = 0; an = nr; ad = S0; break;
+    case 1: kind = 0; an = nr1; ad = S0; break;
+    case 2: kind = 0; an = nr; ad = nr1; break;
+    case 3: kind = 0; an = rMr; ad = S1; break;
+    case 4: kind = 0; an = rMr; ad = S2; break;
+    case 5: lin = (first ? 0.0 : S[5]) * rho; break;
+    case 6: kind = 0; an = nr1; ad = nr2; break;
+    case 7: lin = (k - 1) / 100.0; break;
+    case 8: kind = 2; lin = (double)k; break;                          // log1p(k - 1) / 5
+    case 9: if (hasn) { kind = 0; an = nr; ad = L.noise; } break;
+    case 10: if (hasn) { kind = 0; an = nr1; ad = L.noise; } break;
+    case 11: lin = hasn ? (nr > L.noise ? 1.0 : 0.0) : 0.0; break;
+    case 12: lin = (first ? 0.0 : S[6]) * rho; break;
+    case 13: lin = first ? 0.0 : S[4]; break;
+    case 14: kind = 0; an = rMr; ad = nr * nr; cn = S1; cd = S0 * S0; break;
+    case 15: lin = first ? 1.0 : 0.0; break;
+    default: break;                   // lanes 16..63 carry no feature
     }
-    __syncthreads();                  // features and staged weights visible to the block
+    const double a = an / ad, c = cn / cd;
+    const double la = lg(kind == 2 ? lin : a), lc = lg(c);
+    const double v = kind == 1 ? lin : (kind == 2 ? la / 5.0 : la - lc);
+    const float fv = (float)v;
+    if (t < LEARN_NF && L.feat) L.feat[(int64_t)(k - 1) * LEARN_NF + t] = fv;
+    const double f4 = __shfl(v, 4, 64);
+    if (t == 0) {
+        if (L.xstar && L.err) L.err[k - 1] = sqrt(b2);
+        double* Sw = L.state;         // the state of the next call (every lane has read in.S)
+        if (first) {
+            for (int q = 0; q < LEARN_STATE; ++q) Sw[q] = 0;
+            Sw[0] = nr;
+            Sw[1] = rMr;
+        }
+        Sw[4] = f4;
+        Sw[2] = rMr;
+    }
     // greedy target: lambda* minimises ||x + lam d - x*|| before projection, in the network's range
     double lr_star = 1.0;
     if (L.mode == AIR_RELAX_GREEDY && t == 0) {
-        lr_star = sums[1] > 0 ? sums[0] / sums[1] * (double)L.rho : 1.0;
+        lr_star = b1 > 0 ? b0 / b1 * (double)L.rho : 1.0;
         lr_star = fmin(fmax(lr_star, 0.02), (double)MLP_LAMBDA_MAX - 0.02);
         if (L.target) L.target[k - 1] = (float)lr_star;
     }
@@ -518,33 +625,29 @@ __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, 
         if (t == 0) {
             const double lam = lr_star / (double)L.rho;
             ctrl->lam_cur = lam;
-            L.state[6] = L.state[5];
+            L.state[6] = first ? 0.0 : in.S[5];
             L.state[5] = lam;
         }
-        return;                       // uniform across the block (L.mlp is a kernel argument)
+        return;
     }
-    // the network: 16 -> 64 -> 64 -> 2, FP32, one hidden unit per lane.  In a
-    // greedy run with parameters the network's lambda drives the iteration and
-    // the greedy target is only recorded (data along the network's own path).
+    // the network: 16 -> 64 -> 64 -> 2, FP32, lane o computes hidden unit o of
+    // each layer; the previous layer's values are broadcast lane by lane.  In
+    // a greedy run with parameters the network's lambda drives the iteration
+    // and the greedy target is only recorded (data along the network's path).
     const float* W = Ws;
-    if (t < MLP_HID) {
-        float z = W[MLP_HID * MLP_IN + t];
+    float z = W[MLP_HID * MLP_IN + t];
 #pragma unroll
-        for (int i = 0; i < MLP_IN; ++i) z += W[t * MLP_IN + i] * f[i];
-        h1[t] = fmaxf(z, 0.f);
-    }
-    __syncthreads();
-    if (t < MLP_HID) {
-        const float* W2 = W + MLP_HID * MLP_IN + MLP_HID;
-        float z = W2[MLP_HID * MLP_HID + t];
+    for (int i = 0; i < MLP_IN; ++i)
+        z += W[i * MLP_HID + t] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fv), i));
+    const float hv1 = fmaxf(z, 0.f);
+    const float* W2 = W + P_W2;
+    z = W2[MLP_HID * MLP_HID + t];
 #pragma unroll 16
-        for (int i = 0; i < MLP_HID; ++i) z += W2[t * MLP_HID + i] * h1[i];
-        h2[t] = fmaxf(z, 0.f);
-    }
-    __syncthreads();
-    if (t >= 64) return;
-    const float* W3 = W + MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID;
-    float o0 = W3[t] * h2[t], o1 = W3[MLP_HID + t] * h2[t];
+    for (int i = 0; i < MLP_HID; ++i)
+        z += W2[i * MLP_HID + t] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hv1), i));
+    const float hv2 = fmaxf(z, 0.f);
+    const float* W3 = W2 + MLP_HID * MLP_HID + MLP_HID;
+    float o0 = W3[t] * hv2, o1 = W3[MLP_HID + t] * hv2;
     for (int d = 32; d > 0; d >>= 1) {
         o0 += __shfl_xor(o0, d);
         o1 += __shfl_xor(o1, d);
@@ -556,9 +659,9 @@ __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, 
             L.out[2 * (k - 1)] = o0;
             L.out[2 * (k - 1) + 1] = o1;
         }
-        const double lam = (double)MLP_LAMBDA_MAX / (1.0 + exp(-(double)o0)) / (double)L.rho;
+        const double lam = (double)MLP_LAMBDA_MAX / (1.0 + (double)expf(-o0)) / (double)L.rho;
         ctrl->lam_cur = lam;
-        L.state[6] = L.state[5];
+        L.state[6] = first ? 0.0 : in.S[5];
         L.state[5] = lam;
         if (L.mode == AIR_RELAX_LEARNED && o1 > 0.f) {   // stop at the state before this update
             ctrl->finaliter = k - 1;
@@ -566,6 +669,25 @@ __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, 
             __hip_atomic_store(&ctrl->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, LearnBufs L, int k) {
+    __shared__ float f[LEARN_NF];
+    __shared__ float h1[MLP_HID], h2[MLP_HID];
+    __shared__ float Ws[MLP_PARAMS];
+    __shared__ double sums[4];
+    __shared__ float part[4 * MLP_HID];
+    __shared__ SelIn in;
+    if (threadIdx.x == 0) {
+        in.stop = load_stop(ctrl);
+        in.nr = hist[k - 1];
+        in.nr1 = k >= 2 ? hist[k - 2] : in.nr;
+        in.nr2 = k >= 3 ? hist[k - 3] : in.nr1;
+        in.rMr = ctrl->rMr;
+        for (int q = 0; q < LEARN_STATE; ++q) in.S[q] = L.state[q];
+    }
+    __syncthreads();
+    select_block(in, ctrl, L, k, f, h1, h2, Ws, sums, part, false);
 }
 
 // learned relaxation: lambda_k (and the stop) before the fused column pass of iteration k

@@ -1,7 +1,7 @@
 """The learned relaxation network on the engine (hand-written HIP kernels).
 
 A 16-64-64-2 ReLU perceptron maps residual statistics of a SIRT iteration to
-(o0, o1): lambda_k * rho = 4 sigmoid(o0) and a stop logit o1.  Parameters are
+(o0, o1): lambda_k * rho = 8 sigmoid(o0) and a stop logit o1.  Parameters are
 AIR_MLP_PARAMS (5378) float32 values laid out as W1[64][16], b1[64],
 W2[64][64], b2[64], W3[2][64], b3[2].  Training runs on the GPU: BF16 MFMA
 forward/backward with FP32 accumulation and Adam on FP32 master weights.
@@ -15,6 +15,7 @@ import numpy as np
 from ._lib import Engine, check, load, ptr
 
 NIN, NHID, NOUT, NPARAM = 16, 64, 2, 5378
+LAMBDA_MAX = 8.0          # lambda_k rho = LAMBDA_MAX sigmoid(o0) (AIR_MLP_LAMBDA_MAX)
 
 
 def _f32(a, shape=None):
@@ -25,13 +26,16 @@ def _f32(a, shape=None):
 
 
 def init_params(seed=0):
-    """He-style uniform initialisation with zero biases (float32)."""
+    """He-style uniform initialisation (float32); zero biases except the lambda
+    output's, which starts the network at lambda rho = 1.9 (the default of
+    calc_relaxpar.m:110)."""
     rng = np.random.default_rng(seed)
     W1 = rng.uniform(-1, 1, (NHID, NIN)) * np.sqrt(6.0 / NIN)
     W2 = rng.uniform(-1, 1, (NHID, NHID)) * np.sqrt(6.0 / NHID)
     W3 = rng.uniform(-1, 1, (NOUT, NHID)) * np.sqrt(3.0 / NHID)
+    b3 = np.array([np.log(1.9 / (LAMBDA_MAX - 1.9)), 0.0])
     return np.concatenate([W1.ravel(), np.zeros(NHID), W2.ravel(), np.zeros(NHID), W3.ravel(),
-                           np.zeros(NOUT)]).astype(np.float32)
+                           b3]).astype(np.float32)
 
 
 class MLP:

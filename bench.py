@@ -98,10 +98,25 @@ def cpu_baseline(N, theta, p, b, method, rho, seconds, threads):
     samples = int(min(max(seconds / per_iter, 6), 20000))
     dt = timed(samples + 1) - d1
     per_iter = max(dt, 1e-9) / samples
+    # one core as well: a few iterations (the setup call measured at one thread too)
+    t1 = time.perf_counter()
+    O.sirt(method, A, b, 1, None, opts, nthreads=1)
+    d1s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    O.sirt(method, A, b, 4, None, opts, nthreads=1)
+    dt1 = time.perf_counter() - t1 - d1s
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": 1.0 / per_iter, "unit": "iter/s", "cores": threads, "kind": "port",
             "sample": f"{samples} {method} iterations of the FP64 oracle (CSR A.x, CSC A'.y, OpenMP, "
                       f"{threads} threads) on the same {A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}, "
-                      f"{dt:.1f} s; oracle tracer {t_trace:.1f} s and per-call setup not included"}
+                      f"{dt:.1f} s; oracle tracer {t_trace:.1f} s and per-call setup not included",
+            "one_core_value": round(3.0 / max(dt1, 1e-9), 3), "one_core_sample": "3 iterations at 1 thread",
+            "cpu_model": cpu, "host_cpus": os.cpu_count()}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):

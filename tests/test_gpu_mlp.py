@@ -58,13 +58,14 @@ def test_training_fits_a_smooth_target(net):
     t1 = (X[:, 1] + 0.5 * X[:, 2] > 0.3).astype(np.float32)
     T = np.stack([t0, t1], 1).astype(np.float32)
     p0 = ref.init_params(5)
+    p0[-2] = np.log(1.0 / (ref.LAMBDA_MAX - 1.0))     # start at lambda rho = 1, the targets' centre
     L0, _ = net.grad(p0, X, T)
     p, hist = net.train(p0, X, T, epochs=15, batch=1024, lr=3e-3, seed=1)
     assert hist.shape == (15,) and np.all(np.isfinite(hist))
     L1, _ = net.grad(p, X, T)
     assert L1 < 0.15 * L0 and hist[-1] < hist[0]
     o = net.forward(p, X)
-    lam = 4 / (1 + np.exp(-o[:, 0]))
+    lam = ref.LAMBDA_MAX / (1 + np.exp(-o[:, 0]))
     assert np.sqrt(np.mean((lam - t0) ** 2)) < 0.3 * np.std(t0)
     assert np.mean((o[:, 1] > 0) == (t1 > 0.5)) > 0.95
     # deterministic: the same seed gives the same weights
