@@ -6,7 +6,7 @@ trajectories the engine generates for seeded phantoms:
 
 * along a trajectory every step uses the greedy relaxation lambda*_k, the
   value that minimises ||x_{k-1} + lambda d_k - x_true|| before projection
-  (kept in (0.02, 7.98)/rho); the target of the lambda output is lambda*_k rho;
+  (kept in (0.02, 11.98)/rho); the target of the lambda output is lambda*_k rho;
 * the stop target of a state is 1 once its error is within `stop_tol` of the
   smallest error on the trajectory.
 

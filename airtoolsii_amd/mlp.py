@@ -1,7 +1,7 @@
 """The learned relaxation network on the engine (hand-written HIP kernels).
 
 A 16-64-64-2 ReLU perceptron maps residual statistics of a SIRT iteration to
-(o0, o1): lambda_k * rho = 8 sigmoid(o0) and a stop logit o1.  Parameters are
+(o0, o1): lambda_k * rho = 12 sigmoid(o0) and a stop logit o1.  Parameters are
 AIR_MLP_PARAMS (5378) float32 values laid out as W1[64][16], b1[64],
 W2[64][64], b2[64], W3[2][64], b3[2].  Training runs on the GPU: BF16 MFMA
 forward/backward with FP32 accumulation and Adam on FP32 master weights.
@@ -15,7 +15,7 @@ import numpy as np
 from ._lib import Engine, check, load, ptr
 
 NIN, NHID, NOUT, NPARAM = 16, 64, 2, 5378
-LAMBDA_MAX = 8.0          # lambda_k rho = LAMBDA_MAX sigmoid(o0) (AIR_MLP_LAMBDA_MAX)
+LAMBDA_MAX = 12.0         # lambda_k rho = LAMBDA_MAX sigmoid(o0) (AIR_MLP_LAMBDA_MAX)
 
 
 def _f32(a, shape=None):

@@ -319,12 +319,12 @@ int air_sirt_multi(air_engine* e, const air_sirt_options* opts, int32_t nrhs, co
 #define AIR_MLP_HID    64
 #define AIR_MLP_OUT    2
 #define AIR_MLP_PARAMS 5378
-#define AIR_MLP_LAMBDA_MAX 8      /* lambda_k rho ranges over (0, 8) */
+#define AIR_MLP_LAMBDA_MAX 12     /* lambda_k rho ranges over (0, 12) */
 /* raw outputs of n samples: X is n x 16 row-major, out n x 2 */
 int air_mlp_forward(air_engine* e, const float* params, const float* X, int64_t n, float* out);
 /* mean loss over n samples and its gradient (BF16 MFMA, FP32 accumulation).
- * Targets T (n x 2): t0 = lambda* rho in (0, 8) (t0 < 0 drops the term), t1 in {0, 1}.
- * Loss = mean[(8 sigmoid(o0) - t0)^2 + stop_weight * BCE(o1, t1)]. */
+ * Targets T (n x 2): t0 = lambda* rho in (0, 12) (t0 < 0 drops the term), t1 in {0, 1}.
+ * Loss = mean[(12 sigmoid(o0) - t0)^2 + stop_weight * BCE(o1, t1)]. */
 int air_mlp_grad(air_engine* e, const float* params, const float* X, const float* T, int64_t n,
                  double stop_weight, float* grad, double* loss);
 /* Adam training in place: `epochs` passes over the n samples in seeded random

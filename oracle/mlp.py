@@ -21,7 +21,7 @@ import numpy as np
 
 NI, NH, NO = 16, 64, 2
 NPARAM = NH * NI + NH + NH * NH + NH + NO * NH + NO
-LAMBDA_MAX = 8.0
+LAMBDA_MAX = 12.0
 
 
 def unpack(p):

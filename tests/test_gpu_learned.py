@@ -29,7 +29,7 @@ def test_greedy_target_first_step(setup, oracle):
     x, b, nn, _ = Lm.problem(A, N, 11)
     M, D = oracle.weights("sart", Ao)
     d = D * (Ao.T @ (M * b))
-    expect = np.clip(d @ x / (d @ d), 0.02, 7.98)
+    expect = np.clip(d @ x / (d @ d), 0.02, 11.98)
     run = SirtRun("sart", A, b, 3, None, {"relaxpar": "greedy", "x_true": x, "noise_norm": nn})
     run.iterate(3)
     rec = run.records()
@@ -44,7 +44,7 @@ def test_learned_loop_matches_host_network(setup, oracle):
     A, Ao = setup
     x, b, nn, _ = Lm.problem(A, N, 12)
     params = init_params(3)
-    params[-2] = -1.0                      # lambda rho = 8 sigmoid(.) near 2.2 at the start
+    params[-2] = -1.5                      # lambda rho = 12 sigmoid(.) near 2.2 at the start
     params[-1] = -50.0                     # never stop: compare the whole run
     K = 20
     run = SirtRun("sart", A, b, K, None, {"relaxpar": "learned", "mlp": params, "noise_norm": nn})
@@ -88,7 +88,7 @@ def test_learned_stop_and_dp_fallback(setup):
     x, b, nn, _ = Lm.problem(A, N, 13)
     params = init_params(4)
     params[5248:5376] = 0.0                # W3 = 0: constant outputs from the biases
-    params[-2] = np.log(1.9 / 6.1)         # lambda rho = 8 sigmoid(b3[0]) = 1.9
+    params[-2] = np.log(1.9 / 10.1)        # lambda rho = 12 sigmoid(b3[0]) = 1.9
     params[-1] = 50.0                      # stop logit always positive: stop at x_0
     _, info, _ = air.sart(A, b, 30, None, {"relaxpar": "learned", "mlp": params})
     assert info["finaliter"] == 0 and info["stoprule"] == 4
