@@ -206,8 +206,7 @@ struct SelIn {
     double S[LEARN_STATE];
     int stop;
 };
-__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* f,
-                                             float* h1, float* h2, float* Ws, double* sums, float* part,
+__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* Ws,
                                              bool staged);
 __device__ __forceinline__ void stage_weights(const float* __restrict__ mlp, float* Ws);
 constexpr int NW_LANE_F = (MLP_PARAMS + 255) / 256;
@@ -276,12 +275,8 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_rows(const double* __restric
         }
     }
     if (do_sel) {                                    // uniform: kernel arguments and k
-        __shared__ float f[LEARN_NF];
-        __shared__ float h1[MLP_HID], h2[MLP_HID];
-        __shared__ float part[4 * MLP_HID];
-        __shared__ double sums[4];
         __syncthreads();                             // in, Ws and hprev visible to the block
-        select_block(in, ctrl, Lsel, k + 1, f, h1, h2, Ws, sums, part, true);
+        select_block(in, ctrl, Lsel, k + 1, Ws, true);
     }
 }
 
@@ -538,10 +533,8 @@ __device__ __forceinline__ void stage_weights(const float* __restrict__ mlp, flo
     store_weights(w, Ws);
 }
 
-__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* f,
-                                             float* h1, float* h2, float* Ws, double* sums, float* part,
+__device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const LearnBufs& L, int k, float* Ws,
                                              bool staged) {
-    (void)f; (void)h1; (void)h2; (void)sums; (void)part;
     if (in.stop) return;
     if (L.mlp && !staged) {           // uniform (kernel argument): the whole block stages
         stage_weights(L.mlp, Ws);
@@ -672,11 +665,7 @@ __device__ __forceinline__ void select_block(const SelIn& in, Ctrl* ctrl, const 
 }
 
 __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, LearnBufs L, int k) {
-    __shared__ float f[LEARN_NF];
-    __shared__ float h1[MLP_HID], h2[MLP_HID];
     __shared__ float Ws[MLP_PARAMS];
-    __shared__ double sums[4];
-    __shared__ float part[4 * MLP_HID];
     __shared__ SelIn in;
     if (threadIdx.x == 0) {
         in.stop = load_stop(ctrl);
@@ -687,7 +676,7 @@ __global__ __launch_bounds__(256) void k_select(const double* hist, Ctrl* ctrl, 
         for (int q = 0; q < LEARN_STATE; ++q) in.S[q] = L.state[q];
     }
     __syncthreads();
-    select_block(in, ctrl, L, k, f, h1, h2, Ws, sums, part, false);
+    select_block(in, ctrl, L, k, Ws, false);
 }
 
 // learned relaxation: lambda_k (and the stop) before the fused column pass of iteration k
