@@ -18,6 +18,7 @@
 // based generator (SplitMix64 of the seed, a stream number, the sweep and the
 // step) that the host, the device and the CPU restatement evaluate alike.
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -67,39 +68,107 @@ struct Bounds {
     const double* ubv = nullptr;
 };
 
-// One ART sweep (art.m:249-285) over rows[0..L): one wave.
-template <typename V>
-__global__ __launch_bounds__(64) void k_art_sweep(const int64_t* __restrict__ ptr, const int* __restrict__ idx,
-                                                  const V* __restrict__ val, const int* __restrict__ rows,
-                                                  const double* __restrict__ lamv, double lam, int64_t L,
-                                                  const double* __restrict__ b, const double* __restrict__ nrm,
-                                                  double* x, Bounds bd, int64_t n, int project_all_first) {
+// What one ART step needs besides the row's entries, gathered for a whole
+// sweep by a parallel kernel so that the sweep loads it one step ahead.
+struct ArtStep {
+    int64_t s, e;       // the row's entries [s, e)
+    double b, nrm, lam; // b(ri), normAi(ri) (damped), relaxpar of the step
+};
+
+__global__ void k_art_steps(const int* __restrict__ rows, int64_t L, const int64_t* __restrict__ ptr,
+                            const double* __restrict__ b, const double* __restrict__ nrm,
+                            const double* __restrict__ lamv, double lam, ArtStep* __restrict__ out) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < L; t += (int64_t)gridDim.x * blockDim.x) {
+        const int r = rows[t];
+        ArtStep a;
+        a.s = ptr[r];
+        a.e = ptr[r + 1];
+        a.b = b[r];
+        a.nrm = nrm[r];
+        a.lam = lamv ? lamv[t] : lam;
+        out[t] = a;
+    }
+}
+
+// One ART sweep (art.m:249-285) over the steps st[0..L): one wave.  The next
+// step's record and its first PC*64 entries are loaded while the current
+// step runs; x lives in LDS (copied in and out) when it fits, else in global
+// memory with a work-group fence between steps.
+template <typename V, bool LDS, int PC>
+__global__ __launch_bounds__(64) void k_art_sweep(const ArtStep* __restrict__ st, int64_t L,
+                                                  const int* __restrict__ idx, const V* __restrict__ val,
+                                                  double* xg, Bounds bd, int64_t n, int project_all_first) {
+    extern __shared__ double xlds[];
+    double* x = LDS ? xlds : xg;
     const int lane = threadIdx.x;
     const bool bounded = bd.lb_kind != AIR_BOUND_NONE || bd.ub_kind != AIR_BOUND_NONE;
+    if (LDS) {
+        for (int64_t j = lane; j < n; j += 64) x[j] = xg[j];
+        __syncthreads();
+    }
+    if (L <= 0) return;
+    ArtStep cur = st[0];
+    int cc[PC];
+    double aa[PC];
+#pragma unroll
+    for (int u = 0; u < PC; ++u) {
+        const int64_t q = cur.s + 64 * u + lane;
+        cc[u] = q < cur.e ? idx[q] : 0;
+        aa[u] = q < cur.e ? to_f64(val[q]) : 0.0;
+    }
     for (int64_t t = 0; t < L; ++t) {
-        const int r = rows[t];
-        const int64_t s = ptr[r], e = ptr[r + 1];
+        ArtStep nxt = cur;
+        int nc[PC];
+        double na[PC];
+        if (t + 1 < L) nxt = st[t + 1];
+#pragma unroll
+        for (int u = 0; u < PC; ++u) {
+            const int64_t q = nxt.s + 64 * u + lane;
+            const bool ok = t + 1 < L && q < nxt.e;
+            nc[u] = ok ? idx[q] : 0;
+            na[u] = ok ? to_f64(val[q]) : 0.0;
+        }
         double dot = 0;
-        for (int64_t q = s + lane; q < e; q += 64) dot += to_f64(val[q]) * x[idx[q]];
+#pragma unroll
+        for (int u = 0; u < PC; ++u)
+            if (cur.s + 64 * u + lane < cur.e) dot += aa[u] * x[cc[u]];
+        for (int64_t q = cur.s + 64 * PC + lane; q < cur.e; q += 64) dot += to_f64(val[q]) * x[idx[q]];
         dot = wave_sum(dot);
-        const double lk = lamv ? lamv[t] : lam;
-        const double coef = (lk * (b[r] - dot)) / nrm[r];
-        for (int64_t q = s + lane; q < e; q += 64) {
+        const double coef = (cur.lam * (cur.b - dot)) / cur.nrm;
+#pragma unroll
+        for (int u = 0; u < PC; ++u)
+            if (cur.s + 64 * u + lane < cur.e) {
+                const int j = cc[u];
+                double v = x[j] + coef * aa[u];
+                if (bounded) v = clamp_bounds(v, j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+                x[j] = v;
+            }
+        for (int64_t q = cur.s + 64 * PC + lane; q < cur.e; q += 64) {
             const int j = idx[q];
             double v = x[j] + coef * to_f64(val[q]);
             if (bounded) v = clamp_bounds(v, j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
             x[j] = v;
         }
-        order_stores();
+        if (LDS) __syncthreads();
+        else order_stores();
         if (t == 0 && project_all_first && bounded) {
             // the reference projects the whole vector after every row; entries
             // off the row keep their projected value afterwards, so after the
             // first row of the run only the row's own entries can change
             for (int64_t j = lane; j < n; j += 64)
                 x[j] = clamp_bounds(x[j], j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
-            order_stores();
+            if (LDS) __syncthreads();
+            else order_stores();
+        }
+        cur = nxt;
+#pragma unroll
+        for (int u = 0; u < PC; ++u) {
+            cc[u] = nc[u];
+            aa[u] = na[u];
         }
     }
+    if (LDS)
+        for (int64_t j = lane; j < n; j += 64) xg[j] = x[j];
 }
 
 // max |x| over the whole vector, NaN ignored (MATLAB max)
@@ -183,6 +252,35 @@ __global__ __launch_bounds__(64) void k_cart_sweep(const int64_t* __restrict__ c
             order_stores();                                       // a custom order may visit j again
         }
     }
+}
+
+template <typename V, bool LDS, int PC>
+void art_sweep_launch(const DevSparse& A, const ArtStep* st, int64_t L, double* x, Bounds bd, int64_t n, int paf,
+                      hipStream_t s) {
+    const size_t lds = LDS ? 8 * (size_t)n : 0;
+    if (LDS && lds > 64 * 1024)
+        AIR_HIP(hipFuncSetAttribute((const void*)k_art_sweep<V, LDS, PC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    k_art_sweep<V, LDS, PC><<<1, 64, lds, s>>>(st, L, A.idx.as<int>(), A.val.as<V>(), x, bd, n, paf);
+    AIR_HIP(hipGetLastError());
+}
+
+// pc: entries prefetched per step in units of 64 (2, 4 or 8)
+void launch_art_sweep(const DevSparse& A, const ArtStep* st, int64_t L, double* x, Bounds bd, int64_t n, int paf,
+                      bool lds, int pc, hipStream_t s) {
+    dispatch_dtype(A.dtype, [&](auto v) {
+        using V = decltype(v);
+        if (lds) {
+            if (pc <= 2) art_sweep_launch<V, true, 2>(A, st, L, x, bd, n, paf, s);
+            else if (pc <= 4) art_sweep_launch<V, true, 4>(A, st, L, x, bd, n, paf, s);
+            else art_sweep_launch<V, true, 8>(A, st, L, x, bd, n, paf, s);
+        } else {
+            if (pc <= 2) art_sweep_launch<V, false, 2>(A, st, L, x, bd, n, paf, s);
+            else if (pc <= 4) art_sweep_launch<V, false, 4>(A, st, L, x, bd, n, paf, s);
+            else art_sweep_launch<V, false, 8>(A, st, L, x, bd, n, paf, s);
+        }
+        return 0;
+    });
 }
 
 // r = b - A x (FP64), one wave per row
@@ -391,6 +489,25 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
     if (!rnd) AIR_HIP(hipMemcpyAsync(dsteps.p, I.data(), 4 * L, hipMemcpyHostToDevice, s));
     const bool lam_vec = !cart && o->relax_kind == AIR_RELAX_VECTOR;
     if (lam_vec) dlam.alloc(8 * std::max<size_t>(L, 1));
+    // ART: per-step records, x in LDS when it fits one work-group's LDS
+    // (AIR_ART_LDS=0 keeps it in global memory), entries prefetched per step
+    DevBuf dinfo;
+    bool use_lds = false;
+    int pc = 2;
+    if (!cart) {
+        dinfo.alloc(sizeof(ArtStep) * std::max<size_t>(L, 1));
+        int dev = 0, lim = 0;
+        AIR_HIP(hipGetDevice(&dev));
+        AIR_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const char* env = std::getenv("AIR_ART_LDS");
+        use_lds = 8 * n <= (int64_t)lim && !(env && env[0] == '0');
+        std::vector<int64_t> hp((size_t)m + 1);
+        AIR_HIP(hipMemcpyAsync(hp.data(), A.ptr.p, 8 * ((size_t)m + 1), hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        int64_t maxlen = 0;
+        for (int64_t q = 0; q < m; ++q) maxlen = std::max(maxlen, hp[q + 1] - hp[q]);
+        pc = maxlen <= 128 ? 2 : (maxlen <= 256 ? 4 : 8);
+    }
     if (cart) {
         dflag.alloc((size_t)std::max<int64_t>(n, 1));
         dnflag.alloc(4 * (size_t)std::max<int64_t>(n, 1));
@@ -488,14 +605,14 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
                 return 0;
             });
         } else {
-            dispatch_dtype(A.dtype, [&](auto v) {
-                using V = decltype(v);
-                if (L) k_art_sweep<V><<<1, 64, 0, s>>>(A.ptr.as<int64_t>(), A.idx.as<int>(), A.val.as<V>(),
-                                                       dsteps.as<int>(), lam_vec ? dlam.as<double>() : nullptr, lam,
-                                                       (int64_t)L, db.as<double>(), dnrm.as<double>(),
-                                                       dx.as<double>(), bd, n, k == 1 ? 1 : 0);
-                return 0;
-            });
+            if (L) {
+                k_art_steps<<<(unsigned)std::min<int64_t>(((int64_t)L + 255) / 256, 4096), 256, 0, s>>>(
+                    dsteps.as<int>(), (int64_t)L, A.ptr.as<int64_t>(), db.as<double>(), dnrm.as<double>(),
+                    lam_vec ? dlam.as<double>() : nullptr, lam, dinfo.as<ArtStep>());
+                AIR_HIP(hipGetLastError());
+                launch_art_sweep(A, dinfo.as<ArtStep>(), (int64_t)L, dx.as<double>(), bd, n, k == 1 ? 1 : 0,
+                                 use_lds, pc, s);
+            }
             residual();                                             // art.m:288
         }
         AIR_HIP(hipGetLastError());

@@ -63,6 +63,33 @@ def test_art_fp32_storage_matches_oracle_on_rounded_matrix(oracle, prob):
     assert rel(X[:, -1], Xf[:, -1]) <= 1e-5
 
 
+@pytest.mark.parametrize("lds", ["0", "1"])
+@pytest.mark.parametrize("name", ["kaczmarz_bounds_damp", "randkaczmarz", "relax_function"])
+def test_art_global_and_lds_forms(oracle, prob, monkeypatch, lds, name):
+    """x in LDS (copied in and out of one work-group) or in global memory with
+    a fence per step: the same steps, so the same iterates."""
+    monkeypatch.setenv("AIR_ART_LDS", lds)
+    meth, K, opts = ART_CASES[name]
+    X, info = air.art(meth, prob["A64"], prob["b"], K, None, opts)
+    Xo, io = oracle.art(meth, prob["Ao"], prob["b"], K, None, opts)
+    assert info["finaliter"] == io["finaliter"]
+    for q in range(X.shape[1]):
+        assert rel(X[:, q], Xo[:, q]) <= 1e-10
+
+
+def test_art_image_beyond_64k_lds(oracle):
+    # N = 100: x takes 80 KB, above 64 KB of LDS (dynamic LDS attribute, or the
+    # global form where the device allows less); rows up to 200 entries
+    N, theta, p = 100, np.arange(0.0, 180.0, 10.0), 141
+    A = air.paralleltomo(N, theta, p, dtype="f64")
+    Ao = oracle.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    X, info = air.kaczmarz(A, b, [3], None, {"lbound": 0.0})
+    Xo, _ = oracle.art("kaczmarz", Ao, b, [3], None, {"lbound": 0.0})
+    assert rel(X[:, -1], Xo[:, -1]) <= 1e-10
+
+
 CART_CASES = {
     "columnaction_default": ("columnaction", [3, 6], {}),
     "flagging": ("columnaction", [2, 8], {"relaxpar": 1.0, "Kbegin": 3, "THR": 1e-3, "Nunflag": 3, "seed": 4}),
