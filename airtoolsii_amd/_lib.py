@@ -68,7 +68,8 @@ class RowOptions(C.Structure):
                 ("lb_kind", C.c_int), ("lb", C.c_double), ("lb_vec", C.c_void_p),
                 ("ub_kind", C.c_int), ("ub", C.c_double), ("ub_vec", C.c_void_p),
                 ("stoprule", C.c_int), ("taudelta", C.c_double), ("damp", C.c_double), ("THR", C.c_double),
-                ("Kbegin", C.c_int), ("Nunflag", C.c_int), ("seed", C.c_uint64)]
+                ("Kbegin", C.c_int), ("Nunflag", C.c_int), ("seed", C.c_uint64), ("res_dims", C.c_int * 2),
+                ("ncp_smooth", C.c_int)]
 
 
 class SirtInfo(C.Structure):

@@ -10,7 +10,8 @@
 Each sweep runs on the GPU in the reference's step order (the methods are
 sequential: every step reads what the one before wrote).  Options follow
 check_inputs.m: relaxpar (scalar; ART also a function of the step index
-(k-1)*m + i), lbound, ubound, stoprule ('none', 'DP', 'ME' with taudelta),
+(k-1)*m + i), lbound, ubound, stoprule ('none', 'NCP' with res_dims and
+ncp_smooth, 'DP' and 'ME' with taudelta),
 damp, and for CART THR, Kbegin, Nunflag.  ``seed`` (no counterpart in the
 reference) fixes the random row orders of randkaczmarz and CART's unflagging
 draws, which come from a SplitMix64 stream instead of MATLAB's rand.
@@ -26,10 +27,11 @@ import numpy as np
 from . import _lib
 from ._lib import RowOptions, SirtInfo, check, f64, load, ptr
 from .operators import as_operator
+from .sirt import ncp_options
 
 ART_METHODS = {"kaczmarz": 0, "symkaczmarz": 1, "randkaczmarz": 2}
 ART_CUSTOM = 3
-STOPS = {"NONE": 0, "DP": 2, "ME": 3}
+STOPS = {"NONE": 0, "NCP": 1, "DP": 2, "ME": 3}
 
 
 def _prepare(A, b, K, x0, options):
@@ -95,7 +97,7 @@ def _options(A, opts, cart, sweeps):
     st = opts.get("stoprule") or {}
     stype = str(st.get("type", "none")).upper()
     if stype not in STOPS:
-        raise ValueError("ART/CART on the engine support the stopping rules 'none', 'DP' and 'ME'.")
+        raise ValueError("The chosen stopping rule is not valid.")
     o.stoprule = STOPS[stype]
     o.taudelta = math.nan
     if stype in ("DP", "ME"):
@@ -103,6 +105,7 @@ def _options(A, opts, cart, sweeps):
         if td is None or np.size(td) != 1:
             raise ValueError("For stopping rules DP and ME a scalar taudelta must be given in options.")
         o.taudelta = float(np.ravel(td)[0])
+    ncp_options(st, stype, m, o)
     o.damp = float(opts.get("damp", 0.0))
     if o.damp < 0:
         raise ValueError("Damping must be positive")

@@ -397,9 +397,19 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
     AIR_CHECK(K_in && nK > 0, AIR_E_ARG, "Max no. of iterations must ALWAYS be specified.");
     for (int q = 1; q < nK; ++q) AIR_CHECK(K_in[q] > K_in[q - 1], AIR_E_ARG, "Elements of K must be strictly increasing.");
     AIR_CHECK(o->damp >= 0, AIR_E_ARG, "Damping must be positive");
-    AIR_CHECK(o->stoprule == AIR_STOP_NONE || o->stoprule == AIR_STOP_DP || o->stoprule == AIR_STOP_ME,
-              AIR_E_UNSUPPORTED, "ART/CART: the stopping rule must be 'none', 'DP' or 'ME'");
+    AIR_CHECK(o->stoprule >= AIR_STOP_NONE && o->stoprule <= AIR_STOP_ME, AIR_E_ARG,
+              "The chosen stopping rule is not valid.");
     const int64_t m = A.nrows, n = A.ncols;
+    if (o->stoprule == AIR_STOP_NCP) {
+        // check_stoprules.m:99-120: res_dims = [len, ncols] (2-D) or [m, 0] (1-D)
+        const int64_t len = o->res_dims[0], cols = o->res_dims[1] > 0 ? o->res_dims[1] : 1;
+        AIR_CHECK(len > 0 && len * cols == m, AIR_E_ARG,
+                  "options.stoprule.res_dims must give the dimensions of the residual (prod(res_dims) == m).");
+        AIR_CHECK(o->ncp_smooth >= 1 && o->ncp_smooth <= 4096, AIR_E_ARG, "ncp_smooth must be a positive integer.");
+    }
+    if (o->stoprule == AIR_STOP_DP || o->stoprule == AIR_STOP_ME)
+        AIR_CHECK(!std::isnan(o->taudelta), AIR_E_ARG,
+                  "For stopping rules DP and ME a scalar taudelta must be given in options.");
     std::vector<int32_t> K(K_in, K_in + nK);
     const bool sym = !cart && art_method == AIR_ART_SYMKACZMARZ;
     if (sym) {                                                      // art.m:123-135
@@ -528,6 +538,20 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
     ss.taudelta = o->taudelta;
     ss.kmax = kmax;
     if (ss.type == AIR_STOP_ME) dp.alloc(8 * (size_t)std::max<int64_t>(m, 1));
+    // NCP: the per-column statistics come from the SIRT engine's transform
+    // (FP64 residual copy, batched D2Z, one block per column); the filter
+    // window dk lives on the host, check_stoprules.m:96-151
+    NcpState ncp;
+    DevBuf dctrl;
+    std::vector<double> dk, colstat;
+    if (ss.type == AIR_STOP_NCP) {
+        const int len = o->res_dims[0], cols = o->res_dims[1] > 0 ? o->res_dims[1] : 1;
+        ncp_prepare(ncp, m, len, cols, o->ncp_smooth, s);
+        dctrl.alloc(sizeof(Ctrl));
+        AIR_HIP(hipMemsetAsync(dctrl.p, 0, sizeof(Ctrl), s));        // stop = 0: the statistics kernel runs
+        dk.assign((size_t)o->ncp_smooth, INFINITY);                  // check_inputs.m:207-209
+        colstat.resize((size_t)cols);
+    }
     // evaluates the stop rule at sweep k on the residual in dr; returns the code or -1
     auto check = [&](int k, double* nrk_out) {
         k_stop_dots<<<1, 1024, 0, s>>>(dr.as<double>(), ss.have_prev ? dp.as<double>() : nullptr, m,
@@ -541,10 +565,26 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
             AIR_HIP(hipGetLastError());
             AIR_HIP(hipMemcpyAsync(&pp, ddots.p, 8, hipMemcpyDeviceToHost, s));
         }
+        if (ss.type == AIR_STOP_NCP) {
+            ncp_enqueue_f64(ncp, dr.as<double>(), m, (const Ctrl*)dctrl.p, s);
+            AIR_HIP(hipMemcpyAsync(colstat.data(), ncp.colstat.p, 8 * colstat.size(), hipMemcpyDeviceToHost, s));
+        }
         AIR_HIP(hipStreamSynchronize(s));
         *nrk_out = std::sqrt(d[0]);
         int code = -1;
-        if (ss.type == AIR_STOP_ME) {                              // check_stoprules.m:74-94
+        if (ss.type == AIR_STOP_NCP) {                             // check_stoprules.m:129-151
+            double ncc = 0;                                        // mean over the columns
+            for (double v : colstat) ncc += v;
+            ncc /= (double)colstat.size();
+            double mx_old = -INFINITY, mx_new = ncc;               // max ignores NaN, as fmax does
+            for (size_t q = 0; q < dk.size(); ++q) mx_old = std::fmax(mx_old, dk[q]);
+            for (size_t q = 1; q < dk.size(); ++q) mx_new = std::fmax(mx_new, dk[q]);
+            if (mx_old < mx_new || k >= kmax) code = k != kmax ? 1 : 0;
+            else {
+                dk.erase(dk.begin());
+                dk.push_back(ncc);
+            }
+        } else if (ss.type == AIR_STOP_ME) {                       // check_stoprules.m:74-94
             const double dme = ss.have_prev ? d[1] / 2 : NAN;
             const double nr1 = ss.have_prev ? std::sqrt(pp) : NAN;
             if (dme / nr1 <= ss.taudelta) code = 3;

@@ -360,6 +360,8 @@ void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, 
 // NCP: FFT of the residual columns and their statistics (before the finalize)
 void ncp_prepare(NcpState& N, int64_t m, int len, int ncols, int smooth, hipStream_t s);
 void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s);
+// the same on an FP64 residual (the row-action methods keep theirs in FP64)
+void ncp_enqueue_f64(NcpState& N, const double* r, int64_t m, const Ctrl* ctrl, hipStream_t s);
 // lambda of a line-search style iteration k: r'Mr / sum D g^2, or the learned / greedy choice
 void launch_finalize_line(const SirtBufs& B, int nblocks, int k, hipStream_t s);
 // learned relaxation: the network's lambda_k (read by the update kernels when

@@ -441,14 +441,23 @@ void ncp_prepare(NcpState& N, int64_t m, int len, int ncols, int smooth, hipStre
     AIR_HIP(hipGetLastError());
 }
 
-void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s) {
-    (void)ctrl;
-    k_f32_to_f64<<<(unsigned)std::min<int64_t>((m + BLOCK - 1) / BLOCK, 4096), BLOCK, 0, s>>>(r, N.rd.as<double>(), m);
-    AIR_HIP(hipGetLastError());
+// transform of the residual copy in N.rd and the per-column statistics
+static void ncp_stats_of_rd(NcpState& N, const Ctrl* ctrl, hipStream_t s) {
     AIR_CHECK(hipfftExecD2Z(N.plan, N.rd.as<double>(), (hipfftDoubleComplex*)N.spec.p) == HIPFFT_SUCCESS, AIR_E_HIP,
               "hipfftExecD2Z failed");
     k_ncp_stats<<<(unsigned)N.ncols, BLOCK, 0, s>>>((const double2*)N.spec.p, N.len, N.colstat.as<double>(), ctrl);
     AIR_HIP(hipGetLastError());
+}
+
+void ncp_enqueue(NcpState& N, const float* r, int64_t m, const Ctrl* ctrl, hipStream_t s) {
+    k_f32_to_f64<<<(unsigned)std::min<int64_t>((m + BLOCK - 1) / BLOCK, 4096), BLOCK, 0, s>>>(r, N.rd.as<double>(), m);
+    AIR_HIP(hipGetLastError());
+    ncp_stats_of_rd(N, ctrl, s);
+}
+
+void ncp_enqueue_f64(NcpState& N, const double* r, int64_t m, const Ctrl* ctrl, hipStream_t s) {
+    AIR_HIP(hipMemcpyAsync(N.rd.p, r, 8 * (size_t)m, hipMemcpyDeviceToDevice, s));
+    ncp_stats_of_rd(N, ctrl, s);
 }
 
 // A learned run without the error record selects lambda_{k+1} inside the finalize of iteration k.

@@ -168,6 +168,14 @@ def _build_options(A, sirt_method, options, kmax):
         if td is None or np.size(td) != 1:
             raise ValueError("For stopping rules DP and ME a scalar taudelta must be given in options.")
         o.taudelta = float(np.ravel(td)[0])
+    ncp_options(st, stype, m, o)
+    o.verbose = int(opts.get("verbose", 0) or 0)
+    return o, keep, code, Mc, Dc, name
+
+
+def ncp_options(st, stype, m, o):
+    """res_dims and ncp_smooth of the NCP rule into an options struct
+    (check_inputs.m:171-180, check_stoprules.m:99-120)."""
     if stype == "NCP":
         if "res_dims" not in st:
             raise ValueError("options.stoprule.res_dims must be specified for NCP stopping rule.")
@@ -180,8 +188,6 @@ def _build_options(A, sirt_method, options, kmax):
             raise ValueError("NCP stopping rule only implemented for 1D and 2D residuals, the size of which "
                              "should be given using options.stoprule.res_dims.")
     o.ncp_smooth = int(st.get("ncp_smooth", 2))
-    o.verbose = int(opts.get("verbose", 0) or 0)
-    return o, keep, code, Mc, Dc, name
 
 
 def sirt(sirt_method, A, b, K, x0=None, options=None):

@@ -261,7 +261,7 @@ typedef struct {
     const double* relax_vec;
     int           lb_kind;  double lb;  const double* lb_vec;
     int           ub_kind;  double ub;  const double* ub_vec;
-    int           stoprule;      /* AIR_STOP_NONE, AIR_STOP_DP or AIR_STOP_ME */
+    int           stoprule;      /* AIR_STOP_NONE, AIR_STOP_NCP, AIR_STOP_DP or AIR_STOP_ME */
     double        taudelta;
     double        damp;          /* >= 0, default 0 */
     double        THR;           /* CART flagging threshold, default 1e-4 */
@@ -269,6 +269,8 @@ typedef struct {
     int           Nunflag;       /* CART unflagging parameter, <= 0: round(kmax/4) */
     uint64_t      seed;          /* randkaczmarz orders and CART unflagging draws (SplitMix64 stream;
                                     MATLAB's rand is not reproduced) */
+    int           res_dims[2];   /* NCP: [p, nA] or [m, 0], as in air_sirt_options */
+    int           ncp_smooth;    /* NCP filter length, default 2 */
 } air_row_options;
 /* method: AIR_ART_*; order/norder: AIR_ART_CUSTOM row order.  K: nK strictly
  * increasing iteration numbers (even for symkaczmarz).  X_out: n x nK FP64;

@@ -500,11 +500,25 @@ def _bounds(opts, n):
     return out
 
 
-def _stop(opts, k, kmax, r, rprev):
-    """check_stoprules.m:53-94,153-161 for DP, ME, NONE -> code or None."""
+def _nanmax(v):
+    """MATLAB's max: NaN entries are ignored unless every entry is NaN."""
+    v = np.asarray(v, np.float64)
+    return np.nan if np.all(np.isnan(v)) else np.nanmax(v)
+
+
+def _stop(opts, k, kmax, r, rprev, dk=None):
+    """check_stoprules.m:53-161 for NCP, DP, ME, NONE -> code or None.  ``dk``
+    is the NCP filter window (a list, updated in place), Inf at the start
+    (check_inputs.m:207-209)."""
     st = opts.get("stoprule") or {}
     t = str(st.get("type", "none")).upper()
     nrk = np.linalg.norm(r)
+    if t == "NCP":
+        ncc = ncp_stat(r, st["res_dims"])
+        if _nanmax(dk) < _nanmax(list(dk[1:]) + [ncc]) or k >= kmax:
+            return 1 if k != kmax else 0
+        dk[:] = list(dk[1:]) + [ncc]
+        return None
     if t == "DP":
         if nrk <= st["taudelta"] or k >= kmax:
             return 2 if k != kmax else 0
@@ -560,7 +574,8 @@ def art(art_method, A, b, K, x0=None, options=None):
     lam = 1.0 if rp is None else (float(rp) if not callable(rp) else 0.0)
     lbk, lb, lbv, ubk, ub, ubv = _bounds(opts, n)
     rk = b - A @ x
-    code = _stop(opts, 0, kmax, rk, None)
+    dk = [np.inf] * int((opts.get("stoprule") or {}).get("ncp_smooth", 2))
+    code = _stop(opts, 0, kmax, rk, None, dk)
     rprev = rk if code is None else None
     X, k = [], 0
     ip, ic, iv = (np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int32),
@@ -588,7 +603,7 @@ def art(art_method, A, b, K, x0=None, options=None):
         L.ora_art_sweep(ip.ctypes.data, ic.ctypes.data, iv.ctypes.data, st32.ctypes.data, len(st32), _ptr(lamv), lam,
                         b.ctypes.data, nrm.ctypes.data, x.ctypes.data, n, lbk, lb, _ptr(lbv), ubk, ub, _ptr(ubv))
         rk = b - A @ x
-        code = _stop(opts, k, kmax, rk, rprev)
+        code = _stop(opts, k, kmax, rk, rprev, dk)
         if code is None and str((opts.get("stoprule") or {}).get("type", "none")).upper() == "ME":
             rprev = rk
         if (li < len(K) and k == K[li]) or code is not None:
@@ -626,7 +641,8 @@ def cart(cart_method, A, b, K, x0=None, options=None):
     flag = np.ones(n, np.uint8)
     nflag = np.zeros(n, np.int32)
     r = b - A @ x
-    code = _stop(opts, 0, kmax, r, None)
+    dk = [np.inf] * int((opts.get("stoprule") or {}).get("ncp_smooth", 2))
+    code = _stop(opts, 0, kmax, r, None, dk)
     rprev = r.copy() if code is None else None
     cp, ri, cv = (np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int32),
                   np.ascontiguousarray(A.data))
@@ -637,7 +653,7 @@ def cart(cart_method, A, b, K, x0=None, options=None):
         lib().ora_cart_sweep(cp.ctypes.data, ri.ctypes.data, cv.ctypes.data, J32.ctypes.data, len(J32),
                              nrm.ctypes.data, x.ctypes.data, r.ctypes.data, n, flag.ctypes.data, nflag.ctypes.data,
                              lbk, lb, _ptr(lbv), ubk, ub, _ptr(ubv), lam, thr, k, kbegin, nunflag, seed)
-        code = _stop(opts, k, kmax, r, rprev)
+        code = _stop(opts, k, kmax, r, rprev, dk)
         if code is None and str((opts.get("stoprule") or {}).get("type", "none")).upper() == "ME":
             rprev = r.copy()
         if (li < len(K) and k == K[li]) or code is not None:

@@ -7,6 +7,7 @@ import pytest
 import scipy.sparse as sp
 
 import airtoolsii_amd as air
+from airtoolsii_amd._lib import AirError
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +41,10 @@ ART_CASES = {
     "relax_function": ("kaczmarz", [4], {"relaxpar": lambda j: 0.5 + 0.4 * np.cos(np.asarray(j) / 500.0)}),
     "dp_stop": ("kaczmarz", [50], {"stoprule": {"type": "DP", "taudelta": 1.5}}),
     "me_stop": ("kaczmarz", [50], {"stoprule": {"type": "ME", "taudelta": 1.2}}),
+    "ncp_stop_2d": ("kaczmarz", [5, 50], {"stoprule": {"type": "NCP", "res_dims": [P, len(THETA)]}}),
+    "ncp_stop_1d": ("kaczmarz", [50], {"stoprule": {"type": "NCP", "res_dims": [1020], "ncp_smooth": 1}}),
+    "ncp_stop_rand": ("randkaczmarz", [50], {"seed": 3, "stoprule": {"type": "NCP", "res_dims": [P, len(THETA)],
+                                                                     "ncp_smooth": 3}}),
 }
 
 
@@ -96,6 +101,7 @@ CART_CASES = {
     "bounds_custom_order": (list(range(576, 0, -1)), [5], {"lbound": 0.0, "ubound": np.full(576, 0.9),
                                                           "relaxpar": 0.8}),
     "me_stop": ("columnaction", [300], {"relaxpar": 1.0, "stoprule": {"type": "ME", "taudelta": 1.2}}),
+    "ncp_stop": ("columnaction", [300], {"stoprule": {"type": "NCP", "res_dims": [P, len(THETA)], "ncp_smooth": 4}}),
 }
 
 
@@ -118,6 +124,26 @@ def test_art_info_and_errors(prob):
         air.symkaczmarz(prob["A32"], prob["b"], [3])
     with pytest.raises(ValueError, match="Unknown ART"):
         air.art("bogus", prob["A32"], prob["b"], 2)
+    with pytest.raises(ValueError, match="res_dims must be specified"):
+        air.kaczmarz(prob["A32"], prob["b"], 2, None, {"stoprule": {"type": "NCP"}})
+    with pytest.raises(AirError, match="prod\\(res_dims\\) == m"):
+        air.columnaction(prob["A32"], prob["b"], 2, None, {"stoprule": {"type": "NCP", "res_dims": [P, 7]}})
+
+
+def test_art_ncp_stop_is_where_the_statistic_first_rises(oracle, prob):
+    """The engine's NCP decisions against the oracle's statistic of the
+    engine's own iterates: with smooth 2 the run stops at the first sweep whose
+    ncc exceeds both of the two before it."""
+    dims = [P, len(THETA)]
+    X, info = air.kaczmarz(prob["A64"], prob["b"], np.arange(1, 51), None,
+                           {"stoprule": {"type": "NCP", "res_dims": dims}})
+    k = info["finaliter"]
+    assert info["stoprule"] == 1 and 2 <= k < 50 and X.shape[1] == k
+    R = [prob["b"]] + [prob["b"] - prob["Ao"] @ X[:, j] for j in range(k)]
+    ncc = [oracle.ncp_stat(r, dims) for r in R]
+    padded = [np.inf, np.inf] + ncc                  # dk starts as Inf
+    assert ncc[k] > max(padded[k:k + 2])
+    assert all(ncc[j] <= max(padded[j:j + 2]) for j in range(k))
     with pytest.raises(ValueError, match="scalar"):
         air.cart("columnaction", prob["A32"], prob["b"], 2, None, {"relaxpar": lambda j: 1.0})
     Af = air.paralleltomo(N, THETA, P, None, 0, 0)

@@ -891,6 +891,37 @@ def test_ncp_statistic_matches_fft(oracle, dims):
     assert abs(oracle.ncp_stat(r, dims) - _ncp_numpy(r, dims)) <= 1e-12
 
 
+def _ncp_first_rise(nccs, smooth):
+    """The NCP window rule (check_stoprules.m:135-151) in closed form:
+    max(dk) < max([dk(2:end); ncc]) holds exactly when ncc exceeds every entry
+    of dk, so the run stops at the first k with ncc_k > max(ncc_{k-smooth..k-1}),
+    the entries before k = 0 being Inf (check_inputs.m:207-209)."""
+    for k, v in enumerate(nccs):
+        if v > max(np.inf if j < 0 else nccs[j] for j in range(k - smooth, k)):
+            return k
+    return None
+
+
+@pytest.mark.parametrize("meth,smooth,two_d", [("kaczmarz", 2, True), ("kaczmarz", 1, False),
+                                               ("randkaczmarz", 3, True), ("columnaction", 3, True)])
+def test_row_action_ncp_stop_matches_closed_form(oracle, meth, smooth, two_d):
+    N, theta, p = 16, np.arange(0.0, 180.0, 10.0), 23
+    A = oracle.paralleltomo(N, theta, p)
+    b = A @ oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    e = np.random.default_rng(2).standard_normal(b.size)
+    b = b + 0.02 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    dims = [p, len(theta)] if two_d else [A.shape[0]]
+    run = oracle.cart if meth == "columnaction" else oracle.art
+    kmax = 60
+    X, _ = run(meth, A, b, np.arange(1, kmax + 1), None, {"seed": 5})
+    nccs = [_ncp_numpy(b, dims)] + [_ncp_numpy(b - A @ X[:, k], dims) for k in range(kmax)]
+    want = _ncp_first_rise(nccs, smooth)
+    assert want is not None and 1 <= want < kmax
+    _, info = run(meth, A, b, kmax, None, {"seed": 5, "stoprule": {"type": "NCP", "res_dims": dims,
+                                                                  "ncp_smooth": smooth}})
+    assert info["stoprule"] == 1 and info["finaliter"] == want
+
+
 @pytest.mark.parametrize("p,f0", [(91, 5), (64, 1), (64, 32), (101, 50)])
 def test_ncp_statistic_single_frequency_closed_form(oracle, p, f0):
     """A pure cosine at frequency f0 puts all of pk(1:q) at f0, so c = [f >= f0]
