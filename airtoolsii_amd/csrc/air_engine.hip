@@ -911,6 +911,7 @@ static int build_geometry(air_engine* e, int N, int64_t nrays, int edge, int mat
     upload(e->geo.rays, e->geo.host_rays.data(), nrays, s);
     AIR_HIP(hipStreamSynchronize(s));
     if (matrix_free) {
+        mf_build_tiles(e->geo, s);
         e->kind = 2;
         e->m = nrays;
         e->n = (int64_t)N * N;

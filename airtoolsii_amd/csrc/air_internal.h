@@ -143,6 +143,17 @@ struct Geometry {
     int64_t nrays = 0;
     DevBuf rays;          // RayParam [nrays]
     std::vector<RayParam> host_rays;
+    // matrix-free projector: rays binned by T x T image tile (tile-major item
+    // lists), and per ray the positions of its items for the forward sums
+    int T = 0, ntiles_side = 0;
+    int64_t nitems = 0;
+    DevBuf tile_ptr;      // int64 [tiles + 1]
+    DevBuf tile_rays;     // int32 [nitems]: ray of each item, ascending within a tile
+    DevBuf ray_ptr;       // int64 [nrays + 1]
+    DevBuf ray_items;     // int32 [nitems]: item positions of each ray
+    DevBuf partial;       // float [nitems]: forward partial sums
+    int splits = 1;       // work-groups per tile
+    DevBuf gpart;         // double [splits * N * N]: adjoint slices when splits > 1
 };
 
 // ---------------------------------------------------------------- tracer (air_tracer.hip)
@@ -172,6 +183,7 @@ void circles_to_csr(int N, const double* angles, int nA, int nC, int dtype, DevS
 // Trace all rays into CSR (values narrowed to dtype).  Returns nnz.
 void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s);
 // Matrix-free products: y = A u (m) or g = A' v (n, FP64 accumulators).
+void mf_build_tiles(Geometry& g, hipStream_t s);
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s);
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s);
 // Matrix-free row stats (sum a^2, sum |a|, sum a^2 * w[col]) and column stats.

@@ -211,18 +211,26 @@ struct Pt {
 };
 
 // Walk one ray's kept points in traversal order (fwd: increasing t) and call
-// seg(x_u, y_u, x_v, y_v) for each pair of consecutive kept points.
+// seg(x_u, y_u, x_v, y_v) for each pair of consecutive kept points.  The walk
+// visits the x-lines kxa..kxb and the y-lines kya..kyb (line k at -N/2 + k)
+// and keeps a crossing when its other coordinate lies in the window spanned by
+// those lines; the whole grid (0..N, 0..N) is the reference's in-box test
+// (paralleltomo.m:236-238).  A smaller window yields the points of the full
+// sequence inside that rectangle, which is contiguous along the ray.
 template <class Seg>
-__device__ __forceinline__ void walk_ray(const RayParam& r, int N, bool fwd, Seg&& seg) {
+__device__ __forceinline__ void walk_window(const RayParam& r, int N, int kxa, int kxb, int kya, int kyb, bool fwd,
+                                            Seg&& seg) {
     const double half = (double)N / 2.0;
     const double lo = -half;
+    const double xlo = lo + (double)kxa, xhi = lo + (double)kxb;
+    const double ylo = lo + (double)kya, yhi = lo + (double)kyb;
     // x-line crossings: t ascending along k iff a > 0; none when a is +-0.
     const bool hasXlist = (r.a > 0.0) || (r.a < 0.0);
     const bool hasYlist = (r.b > 0.0) || (r.b < 0.0);
-    int kx = 0, dkx = 1, nx = hasXlist ? N + 1 : 0;
-    if (hasXlist && ((r.a > 0.0) != fwd)) { kx = N; dkx = -1; }
-    int ky = 0, dky = 1, ny = hasYlist ? N + 1 : 0;
-    if (hasYlist && ((r.b > 0.0) != fwd)) { ky = N; dky = -1; }
+    int kx = kxa, dkx = 1, nx = hasXlist ? kxb - kxa + 1 : 0;
+    if (hasXlist && ((r.a > 0.0) != fwd)) { kx = kxb; dkx = -1; }
+    int ky = kya, dky = 1, ny = hasYlist ? kyb - kya + 1 : 0;
+    if (hasYlist && ((r.b > 0.0) != fwd)) { ky = kyb; dky = -1; }
 
     Pt X, Y;
     bool hx = false, hy = false;
@@ -234,7 +242,7 @@ __device__ __forceinline__ void walk_ray(const RayParam& r, int N, bool fwd, Seg
             nx--;
             double tx = (g - r.x0) / r.a;
             double yx = r.b * tx + r.y0;
-            if (yx >= lo && yx <= half) { X.t = tx; X.x = g; X.y = yx; hx = true; return; }
+            if (yx >= ylo && yx <= yhi) { X.t = tx; X.x = g; X.y = yx; hx = true; return; }
         }
     };
     auto nextY = [&]() {
@@ -245,7 +253,7 @@ __device__ __forceinline__ void walk_ray(const RayParam& r, int N, bool fwd, Seg
             ny--;
             double ty = (g - r.y0) / r.b;
             double xy = r.a * ty + r.x0;
-            if (xy >= lo && xy <= half) { Y.t = ty; Y.x = xy; Y.y = g; hy = true; return; }
+            if (xy >= xlo && xy <= xhi) { Y.t = ty; Y.x = xy; Y.y = g; hy = true; return; }
         }
     };
     nextX();
@@ -273,6 +281,109 @@ __device__ __forceinline__ void walk_ray(const RayParam& r, int N, bool fwd, Seg
         px = x; py = y;
     }
     if (fwd && havePrev) kept(px, py);
+}
+
+// The forward walk of walk_window (fwd = true) with one division per step and
+// no inner scan loops, for the matrix-free projector.  Each list's accepted
+// lines form one contiguous run (the other coordinate is monotone in k); an
+// analytic estimate of that run, widened by 1 line on each side, bounds the
+// candidates, and candidates outside the window are consumed in t order
+// without being kept.  Merging the two sorted candidate lists and then
+// dropping the rejected ones gives the same sequence as merging the accepted
+// ones, so the kept points and segments equal walk_window's bit for bit (the
+// crossings use the same operations).
+__device__ __forceinline__ void cand_range(double p0, double q0, double dp, double dq, double qlo, double qhi,
+                                           double lo, int ka, int kb, int* k0, int* k1) {
+    // lines p = lo + k whose crossing q = dq * ((p - p0) / dp) + q0 lies in [qlo, qhi]
+    if (!(dq > 0.0 || dq < 0.0)) {                      // q constant
+        if (q0 >= qlo && q0 <= qhi) { *k0 = ka; *k1 = kb; }
+        else { *k0 = 1; *k1 = 0; }
+        return;
+    }
+    const double r1 = p0 + dp * ((qlo - q0) / dq) - lo, r2 = p0 + dp * ((qhi - q0) / dq) - lo;
+    double a = fmin(r1, r2), b = fmax(r1, r2);
+    if (!(a == a) || !(b == b)) { *k0 = ka; *k1 = kb; return; }
+    a = fmax(floor(a) - 1.0, (double)ka);
+    b = fmin(ceil(b) + 1.0, (double)kb);
+    *k0 = (int)a;
+    *k1 = (int)b;
+}
+
+template <class Seg>
+__device__ __forceinline__ void walk_window_fwd(const RayParam& r, int N, int kxa, int kxb, int kya, int kyb,
+                                                Seg&& seg) {
+    const double half = (double)N / 2.0;
+    const double lo = -half;
+    const double xlo = lo + (double)kxa, xhi = lo + (double)kxb;
+    const double ylo = lo + (double)kya, yhi = lo + (double)kyb;
+    const bool hasX = (r.a > 0.0) || (r.a < 0.0);
+    const bool hasY = (r.b > 0.0) || (r.b < 0.0);
+    int x0k = 1, x1k = 0, y0k = 1, y1k = 0;
+    if (hasX) cand_range(r.x0, r.y0, r.a, r.b, ylo, yhi, lo, kxa, kxb, &x0k, &x1k);
+    if (hasY) cand_range(r.y0, r.x0, r.b, r.a, xlo, xhi, lo, kya, kyb, &y0k, &y1k);
+    // t ascends with k when the direction component is positive
+    int kx = r.a > 0.0 ? x0k : x1k, dkx = r.a > 0.0 ? 1 : -1, nx = x1k - x0k + 1;
+    int ky = r.b > 0.0 ? y0k : y1k, dky = r.b > 0.0 ? 1 : -1, ny = y1k - y0k + 1;
+    if (nx < 0) nx = 0;
+    if (ny < 0) ny = 0;
+    // current candidate of each list: t, coordinates, inside the window
+    double Xt = 0, Xx = 0, Xy = 0, Yt = 0, Yx = 0, Yy = 0;
+    bool Xok = false, Yok = false;
+    bool hx = nx > 0, hy = ny > 0;
+    if (hx) {
+        const double g = lo + (double)kx;
+        Xt = (g - r.x0) / r.a;
+        Xy = r.b * Xt + r.y0;
+        Xx = g;
+        Xok = Xy >= ylo && Xy <= yhi;
+        kx += dkx;
+        nx--;
+    }
+    if (hy) {
+        const double g = lo + (double)ky;
+        Yt = (g - r.y0) / r.b;
+        Yx = r.a * Yt + r.x0;
+        Yy = g;
+        Yok = Yx >= xlo && Yx <= xhi;
+        ky += dky;
+        ny--;
+    }
+    bool havePrev = false, haveQ = false;
+    double px = 0, py = 0, qx = 0, qy = 0;
+    while (hx || hy) {
+        const bool sx = hx && (!hy || Xt <= Yt);
+        const double x = sx ? Xx : Yx, y = sx ? Xy : Yy;
+        const bool ok = sx ? Xok : Yok;
+        // refill the list just consumed: one division on the selected operands
+        const bool more = sx ? nx > 0 : ny > 0;
+        if (more) {
+            const double g = lo + (double)(sx ? kx : ky);
+            const double t = (g - (sx ? r.x0 : r.y0)) / (sx ? r.a : r.b);
+            const double o = (sx ? r.b : r.a) * t + (sx ? r.y0 : r.x0);
+            const bool in = sx ? (o >= ylo && o <= yhi) : (o >= xlo && o <= xhi);
+            if (sx) { Xt = t; Xx = g; Xy = o; Xok = in; kx += dkx; nx--; }
+            else { Yt = t; Yx = o; Yy = g; Yok = in; ky += dky; ny--; }
+        } else {
+            if (sx) hx = false;
+            else hy = false;
+        }
+        if (!ok) continue;
+        if (havePrev) {
+            const bool coinc = fabs(x - px) <= 1e-10 && fabs(y - py) <= 1e-10;
+            if (!coinc) {
+                if (haveQ) seg(qx, qy, px, py);
+                qx = px; qy = py; haveQ = true;
+            }
+        }
+        havePrev = true;
+        px = x; py = y;
+    }
+    if (havePrev && haveQ) seg(qx, qy, px, py);
+}
+
+template <class Seg>
+__device__ __forceinline__ void walk_ray(const RayParam& r, int N, bool fwd, Seg&& seg) {
+    walk_window(r, N, 0, N, 0, N, fwd, static_cast<Seg&&>(seg));
 }
 
 struct SegOut {
@@ -439,37 +550,148 @@ void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- matrix-free projector
-// Forward: y_i = sum_s aval_s * u[col_s] (paralleltomo.m:284).  One lane per ray.
-__global__ __launch_bounds__(256) void k_mf_forward(const RayParam* __restrict__ rays, int64_t nrays, int N,
-                                                    int edge_check, const float* __restrict__ u,
-                                                    float* __restrict__ y) {
-    int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray >= nrays) return;
-    RayParam r = rays[ray];
-    float acc = 0.f;
-    if (!ray_excluded(r, N, edge_check)) {
-        walk_ray(r, N, true, [&](double xu, double yu, double xv, double yv) {
-            SegOut o = seg_value(N, xu, yu, xv, yv);
-            if (o.col >= 0) acc += (float)o.val * u[o.col];
-        });
-    }
-    y[ray] = acc;
+// paralleltomo.m:282-287 (forward aval'*u(col), adjoint A(col) += u_row*aval)
+// with the products computed from the rays on every call.  The work is
+// tile-major: the image is cut into T x T tiles and every ray is listed under
+// the tiles it passes (a host binning with a half-pixel margin).  A
+// work-group owns one tile; each lane walks one listed ray through the tile's
+// window (the tile's grid lines plus one line on each side), which yields the
+// points of the full walk near the tile and so the full walk's segments
+// there, bit for bit, and keeps the segments whose midpoint pixel lies in the
+// tile.  Each segment of a ray belongs to exactly one tile.  The forward
+// product gathers x from the tile in LDS and writes one partial sum per
+// (ray, tile) item, summed per ray afterwards in a fixed order; the adjoint
+// accumulates into the tile in LDS (FP64) and stores it, so no global atomics
+// are needed.
+
+// window of tile (tx, ty): pixel columns fx in [tx T, tx T + T), image rows
+// iy = N-1-fy in [ty T, ty T + T); x-lines tx T - 1 .. tx T + T + 1, y-lines
+// N - ty T - T - 1 .. N - ty T + 1, clipped to the grid
+struct TileWin {
+    int fx0, iy0, kxa, kxb, kya, kyb;
+};
+
+__device__ __forceinline__ TileWin tile_window(int tile, int nt, int T, int N) {
+    TileWin w;
+    const int tx = tile % nt, ty = tile / nt;
+    w.fx0 = tx * T;
+    w.iy0 = ty * T;
+    w.kxa = max(w.fx0 - 1, 0);
+    w.kxb = min(w.fx0 + T + 1, N);
+    w.kya = max(N - w.iy0 - T - 1, 0);
+    w.kyb = min(N - w.iy0 + 1, N);
+    return w;
 }
 
-// Adjoint: g[col_s] += v_i * aval_s (paralleltomo.m:286), FP64 atomics.
-__global__ __launch_bounds__(256) void k_mf_adjoint(const RayParam* __restrict__ rays, int64_t nrays, int N,
-                                                    int edge_check, const float* __restrict__ v,
-                                                    double* __restrict__ g) {
-    int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray >= nrays) return;
-    float vi = v[ray];
-    if (vi == 0.f) return;
-    RayParam r = rays[ray];
-    if (ray_excluded(r, N, edge_check)) return;
-    walk_ray(r, N, true, [&](double xu, double yu, double xv, double yv) {
-        SegOut o = seg_value(N, xu, yu, xv, yv);
-        if (o.col >= 0) unsafeAtomicAdd(&g[o.col], (double)vi * (double)(float)o.val);
-    });
+// the segment's slot in the tile (column-major, T rows per column), or -1
+template <int T>
+__device__ __forceinline__ int tile_slot(const SegOut& o, const TileWin& w, int N) {
+    if (o.col < 0) return -1;
+    const int c = o.fx - w.fx0, rr = (o.col - o.fx * N) - w.iy0;
+    return ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T) ? c * T + rr : -1;
+}
+
+// A tile's items are split over `splits` work-groups (contiguous chunks, so
+// neighbouring rays share a wave); block b works on chunk b % splits of tile
+// b / splits.
+__device__ __forceinline__ void tile_chunk(const int64_t* __restrict__ tile_ptr, int splits, int* tile,
+                                           int64_t* q0, int64_t* q1) {
+    const int t = (int)(blockIdx.x / (unsigned)splits), part = (int)(blockIdx.x % (unsigned)splits);
+    const int64_t p0 = tile_ptr[t], len = tile_ptr[t + 1] - p0;
+    const int64_t chunk = (len + splits - 1) / splits;
+    *tile = t;
+    *q0 = p0 + std::min<int64_t>(len, chunk * part);
+    *q1 = p0 + std::min<int64_t>(len, chunk * (part + 1));
+}
+
+constexpr int MFT = 64;          // tile side in pixels
+constexpr int MFT_BLOCK = 512;   // threads per work-group (6 waves per SIMD at <= 85 VGPRs)
+
+__global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_forward(const RayParam* __restrict__ rays, int N, int nt,
+                                                             int splits, const int64_t* __restrict__ tile_ptr,
+                                                             const int* __restrict__ tile_rays,
+                                                             const float* __restrict__ u,
+                                                             float* __restrict__ partial) {
+    __shared__ float us[MFT * MFT];
+    int tile;
+    int64_t q0, q1;
+    tile_chunk(tile_ptr, splits, &tile, &q0, &q1);
+    const TileWin w = tile_window(tile, nt, MFT, N);
+    for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) {
+        const int fx = w.fx0 + i / MFT, iy = w.iy0 + i % MFT;
+        us[i] = (fx < N && iy < N) ? u[(int64_t)fx * N + iy] : 0.f;
+    }
+    __syncthreads();
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const RayParam r = rays[tile_rays[q]];
+        float acc = 0.f;
+        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb, [&](double xu, double yu, double xv, double yv) {
+            const SegOut o = seg_value(N, xu, yu, xv, yv);
+            const int k = tile_slot<MFT>(o, w, N);
+            if (k >= 0) acc += (float)o.val * us[k];
+        });
+        partial[q] = acc;
+    }
+}
+
+// y[ray] = sum of the ray's partials in its item order (8 lanes per ray)
+__global__ __launch_bounds__(256) void k_mft_raysum(const int64_t* __restrict__ ray_ptr,
+                                                    const int* __restrict__ ray_items,
+                                                    const float* __restrict__ partial, int64_t nrays,
+                                                    float* __restrict__ y) {
+    const int gl = threadIdx.x & 7;
+    for (int64_t ray = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3; ray < nrays;
+         ray += ((int64_t)gridDim.x * blockDim.x) >> 3) {
+        float a = 0.f;
+        for (int64_t q = ray_ptr[ray] + gl; q < ray_ptr[ray + 1]; q += 8) a += partial[ray_items[q]];
+        a += __shfl_xor(a, 4, 8);
+        a += __shfl_xor(a, 2, 8);
+        a += __shfl_xor(a, 1, 8);
+        if (gl == 0) y[ray] = a;
+    }
+}
+
+// Adjoint of one chunk into the tile in LDS (FP64); the chunk's tile goes to
+// slice (b % splits) of `out` (n values per slice), summed over the slices in
+// order afterwards, or straight to g when the tile is not split.
+__global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_adjoint(const RayParam* __restrict__ rays, int N, int nt,
+                                                             int splits, const int64_t* __restrict__ tile_ptr,
+                                                             const int* __restrict__ tile_rays,
+                                                             const float* __restrict__ v,
+                                                             double* __restrict__ out) {
+    __shared__ double gs[MFT * MFT];
+    int tile;
+    int64_t q0, q1;
+    tile_chunk(tile_ptr, splits, &tile, &q0, &q1);
+    const TileWin w = tile_window(tile, nt, MFT, N);
+    for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) gs[i] = 0.0;
+    __syncthreads();
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const int ray = tile_rays[q];
+        const float vi = v[ray];
+        if (vi == 0.f) continue;
+        const RayParam r = rays[ray];
+        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb, [&](double xu, double yu, double xv, double yv) {
+            const SegOut o = seg_value(N, xu, yu, xv, yv);
+            const int k = tile_slot<MFT>(o, w, N);
+            if (k >= 0) atomicAdd(&gs[k], (double)vi * (double)(float)o.val);
+        });
+    }
+    __syncthreads();
+    double* dst = out + (int64_t)(blockIdx.x % (unsigned)splits) * ((int64_t)N * N);
+    for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) {
+        const int fx = w.fx0 + i / MFT, iy = w.iy0 + i % MFT;
+        if (fx < N && iy < N) dst[(int64_t)fx * N + iy] = gs[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mft_sum_slices(const double* __restrict__ part, int splits, int64_t n,
+                                                        double* __restrict__ g) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a = part[i];
+        for (int sl = 1; sl < splits; ++sl) a += part[(int64_t)sl * n + i];
+        g[i] = a;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mf_row_stats(const RayParam* __restrict__ rays, int64_t nrays, int N,
@@ -511,18 +733,115 @@ __global__ __launch_bounds__(256) void k_mf_col_stats(const RayParam* __restrict
     });
 }
 
+// Host binning of the rays into image tiles: a ray is listed under every tile
+// that its line passes within half a pixel of (a superset of the tiles that
+// hold its segments).  Rays are visited in order, so each tile lists its rays
+// ascending; the forward sums then follow each ray's own tile order.
+void mf_build_tiles(Geometry& g, hipStream_t s) {
+    const int N = g.N;
+    const int T = MFT;
+    const int nt = (N + T - 1) / T;
+    const int64_t ntiles = (int64_t)nt * nt;
+    const double half = (double)N / 2.0, lo = -half, mg = 0.5;
+    auto fy_of = [&](double y) { return (int)std::min<double>(N - 1, std::max<double>(0, std::floor(y + half))); };
+    auto fx_of = fy_of;
+    auto for_tiles = [&](const RayParam& r, auto&& f) {
+        const bool excl = g.edge_check && ((r.b == 0.0 && std::fabs(r.y0 - half) < 1e-15) ||
+                                           (r.a == 0.0 && std::fabs(r.x0 - half) < 1e-15));
+        if (excl || !(std::fabs(r.a) > 0 || std::fabs(r.b) > 0)) return;
+        if (std::fabs(r.a) >= std::fabs(r.b)) {          // mostly along x: visit tile columns
+            for (int tx = 0; tx < nt; ++tx) {
+                const double xa = lo + (double)(tx * T) - mg, xb = lo + (double)std::min((tx + 1) * T, N) + mg;
+                const double ya = r.y0 + r.b * ((xa - r.x0) / r.a), yb = r.y0 + r.b * ((xb - r.x0) / r.a);
+                const double y0 = std::min(ya, yb) - mg, y1 = std::max(ya, yb) + mg;
+                if (y1 < lo || y0 > half) continue;
+                const int ty0 = (N - 1 - fy_of(y1)) / T, ty1 = (N - 1 - fy_of(y0)) / T;
+                for (int ty = ty0; ty <= ty1; ++ty) f(ty * nt + tx);
+            }
+        } else {                                          // mostly along y: visit tile rows
+            for (int ty = 0; ty < nt; ++ty) {
+                const double ya = lo + (double)(N - std::min((ty + 1) * T, N)) - mg, yb = lo + (double)(N - ty * T) + mg;
+                const double xa = r.x0 + r.a * ((ya - r.y0) / r.b), xb = r.x0 + r.a * ((yb - r.y0) / r.b);
+                const double x0 = std::min(xa, xb) - mg, x1 = std::max(xa, xb) + mg;
+                if (x1 < lo || x0 > half) continue;
+                const int tx0 = fx_of(x0) / T, tx1 = fx_of(x1) / T;
+                for (int tx = tx0; tx <= tx1; ++tx) f(ty * nt + tx);
+            }
+        }
+    };
+    std::vector<int64_t> tptr((size_t)ntiles + 1, 0), rptr((size_t)g.nrays + 1, 0);
+    for (int64_t i = 0; i < g.nrays; ++i) {
+        int64_t c = 0;
+        for_tiles(g.host_rays[(size_t)i], [&](int64_t t) { tptr[(size_t)t + 1]++; c++; });
+        rptr[(size_t)i + 1] = rptr[(size_t)i] + c;
+    }
+    for (int64_t t = 0; t < ntiles; ++t) tptr[(size_t)t + 1] += tptr[(size_t)t];
+    const int64_t nitems = tptr[(size_t)ntiles];
+    AIR_CHECK(nitems < INT32_MAX, AIR_E_UNSUPPORTED, "matrix-free projector: too many ray-tile items");
+    std::vector<int> trays((size_t)nitems), ritems((size_t)nitems);
+    std::vector<int64_t> fill(tptr.begin(), tptr.end() - 1);
+    for (int64_t i = 0; i < g.nrays; ++i) {
+        int64_t c = rptr[(size_t)i];
+        for_tiles(g.host_rays[(size_t)i], [&](int64_t t) {
+            const int64_t q = fill[(size_t)t]++;
+            trays[(size_t)q] = (int)i;
+            ritems[(size_t)c++] = (int)q;
+        });
+    }
+    g.T = T;
+    g.ntiles_side = nt;
+    g.nitems = nitems;
+    auto put = [&](DevBuf& d, const void* h, size_t bytes) {
+        d.alloc(bytes);
+        if (bytes) AIR_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+    };
+    put(g.tile_ptr, tptr.data(), 8 * tptr.size());
+    put(g.ray_ptr, rptr.data(), 8 * rptr.size());
+    put(g.tile_rays, trays.data(), 4 * trays.size());
+    put(g.ray_items, ritems.data(), 4 * ritems.size());
+    g.partial.alloc(4 * (size_t)std::max<int64_t>(nitems, 1));
+    // enough work-groups for 3 per CU, each with at least 1024 items
+    int dev = 0, cus = 0;
+    AIR_HIP(hipGetDevice(&dev));
+    AIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t by_cu = (3 * (int64_t)std::max(cus, 1) + ntiles - 1) / ntiles;
+    const int64_t by_items = std::max<int64_t>(1, nitems / (ntiles * 1024));
+    g.splits = (int)std::max<int64_t>(1, std::min<int64_t>({by_cu, by_items, 64}));
+    if (g.splits > 1) g.gpart.alloc(8 * (size_t)g.splits * (size_t)N * (size_t)N);
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
     if (g.nrays == 0) return;
-    k_mf_forward<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
-                                                                   g.edge_check, u, y);
+    AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
+    const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
+    k_mft_forward<<<nblk, MFT_BLOCK, 0, s>>>(g.rays.as<RayParam>(), g.N, g.ntiles_side, g.splits,
+                                             g.tile_ptr.as<int64_t>(), g.tile_rays.as<int>(), u,
+                                             g.partial.as<float>());
+    AIR_HIP(hipGetLastError());
+    const unsigned grid = (unsigned)std::min<int64_t>((g.nrays * 8 + 255) / 256, 65536);
+    k_mft_raysum<<<grid, 256, 0, s>>>(g.ray_ptr.as<int64_t>(), g.ray_items.as<int>(), g.partial.as<float>(),
+                                      g.nrays, y);
     AIR_HIP(hipGetLastError());
 }
+
+// g = A' v (every pixel written: the tiles cover the image)
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) {
     if (g.nrays == 0) return;
-    k_mf_adjoint<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
-                                                                   g.edge_check, v, gacc);
+    AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
+    const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
+    double* out = g.splits > 1 ? g.gpart.as<double>() : gacc;
+    k_mft_adjoint<<<nblk, MFT_BLOCK, 0, s>>>(g.rays.as<RayParam>(), g.N, g.ntiles_side, g.splits,
+                                             g.tile_ptr.as<int64_t>(), g.tile_rays.as<int>(), v, out);
     AIR_HIP(hipGetLastError());
+    if (g.splits > 1) {
+        const int64_t n = (int64_t)g.N * g.N;
+        k_mft_sum_slices<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
+            g.gpart.as<double>(), g.splits, n, gacc);
+        AIR_HIP(hipGetLastError());
+    }
 }
+
 void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t s) {
     if (g.nrays == 0) return;
     k_mf_row_stats<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,

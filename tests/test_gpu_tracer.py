@@ -23,6 +23,7 @@ PAR_CASES = {
     "rays_missing_grid": dict(N=16, theta=np.random.default_rng(4).uniform(0, 360, 40), p=23, d=60.0),
     "tiny_N1": dict(N=1, theta=np.array([0.0, 30.0, 90.0]), p=3, d=1.0),
     "c2_angle_subset": dict(N=512, theta=np.arange(0.0, 180.0, 0.5)[::24], p=725, d=None),
+    "N100_partial_tiles": dict(N=100, theta=np.arange(0.0, 180.0, 7.0), p=141, d=None),
 }
 FAN_CASES = {
     "fan_N64_even_p": dict(N=64, theta=np.arange(0.0, 360.0, 2.0), p=90),
@@ -44,19 +45,19 @@ SEISMIC_CASES = {
 }
 
 
-def build_pair(oracle, kind, cfg, dtype):
+def build_pair(oracle, kind, cfg, dtype, is_matrix=1):
     if kind == "par":
-        A = air.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"], dtype=dtype)
+        A = air.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"], 0, is_matrix, dtype=dtype)
         Ao = oracle.paralleltomo(cfg["N"], cfg["theta"], cfg["p"], cfg["d"])
     elif kind == "fan":
-        A = air.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"], dtype=dtype)
+        A = air.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"], None, None, None, 0, is_matrix, dtype=dtype)
         Ao = oracle.fanlineartomo(cfg["N"], cfg["theta"], cfg["p"])
     elif kind == "seismic":
-        A = air.seismictomo(cfg["N"], cfg["s"], cfg["p"], dtype=dtype)
+        A = air.seismictomo(cfg["N"], cfg["s"], cfg["p"], 0, is_matrix, dtype=dtype)
         Ao = oracle.seismictomo(cfg["N"], cfg["s"], cfg["p"])
     else:
         R = 2.0 if cfg["R"] is None else cfg["R"]
-        A = air.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], cfg["R"], cfg["d"], dtype=dtype)
+        A = air.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], cfg["R"], cfg["d"], 0, is_matrix, dtype=dtype)
         Ao = oracle.fancurvedtomo(cfg["N"], cfg["theta"], cfg["p"], R, cfg["d"])
     return A, Ao
 
@@ -145,6 +146,31 @@ def test_matrix_free_products_match_matrix(oracle, kind):
     assert np.linalg.norm(Af.T @ y - fy) <= 1e-6 * np.linalg.norm(fy)
     # adjoint identity <A x, y> = <x, A' y> (demos/demo_matrixfree.m:97-138)
     assert abs((Af @ x) @ y - x @ (Af.T @ y)) <= 1e-5 * abs(fx @ y)
+
+
+@pytest.mark.parametrize("kind,name", CASES)
+def test_matrix_free_tiles_reproduce_every_entry(oracle, kind, name):
+    """The tile-major projector walks each ray once per image tile it passes
+    and keeps the segments whose pixel lies in the tile: every entry of A must
+    come out exactly once with its exact value.  The adjoint sums in FP64, so
+    each pixel agrees with A32' y (A's values rounded to FP32) to the FP32
+    rounding of the result; a lost or doubled entry would be off by the size of
+    an entry.  The forward sums in FP32 in another order than a row of A."""
+    print(f"matrix-free tiles {kind} {name}", flush=True)
+    Af, Ao = build_pair(oracle, kind, TABLES[kind][name], "f32", is_matrix=0)
+    assert Af.matrix_free and Af.shape == Ao.shape
+    A32 = Ao.copy()
+    A32.data = A32.data.astype(np.float32).astype(np.float64)
+    Aabs = abs(Ao)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 1, Ao.shape[1])
+    y = rng.uniform(-1, 1, Ao.shape[0])
+    x32 = x.astype(np.float32).astype(np.float64)
+    y32 = y.astype(np.float32).astype(np.float64)
+    err_f = np.abs(Af @ x - A32 @ x32)
+    assert np.all(err_f <= 1e-5 * (Aabs @ np.abs(x32)) + 1e-30), err_f.max()
+    err_t = np.abs(Af.T @ y - A32.T @ y32)
+    assert np.all(err_t <= 2e-7 * (Aabs.T @ np.abs(y32)) + 1e-30), err_t.max()
 
 
 def test_matlab_csc_upload_matches_tracer(oracle):
