@@ -292,7 +292,7 @@ __device__ __forceinline__ void walk_window(const RayParam& r, int N, int kxa, i
 // dropping the rejected ones gives the same sequence as merging the accepted
 // ones, so the kept points and segments equal walk_window's bit for bit (the
 // crossings use the same operations).
-__device__ __forceinline__ void cand_range(double p0, double q0, double dp, double dq, double qlo, double qhi,
+__host__ __device__ inline void cand_range(double p0, double q0, double dp, double dq, double qlo, double qhi,
                                            double lo, int ka, int kb, int* k0, int* k1) {
     // lines p = lo + k whose crossing q = dq * ((p - p0) / dp) + q0 lies in [qlo, qhi]
     if (!(dq > 0.0 || dq < 0.0)) {                      // q constant
@@ -309,6 +309,13 @@ __device__ __forceinline__ void cand_range(double p0, double q0, double dp, doub
     *k1 = (int)b;
 }
 
+// seg(emit, x_u, y_u, x_v, y_v) is called on every step with the step's
+// candidate segment; only steps with emit set carry one of the walk's
+// segments.  Each step is straight-line code (selects, one division), so the
+// lanes of a wave stay converged while they walk different rays.  A list's
+// state is its current line k and that crossing's t; the crossing's other
+// coordinate is formed when the candidate is taken (the same operations as
+// walk_window).
 template <class Seg>
 __device__ __forceinline__ void walk_window_fwd(const RayParam& r, int N, int kxa, int kxb, int kya, int kyb,
                                                 Seg&& seg) {
@@ -322,63 +329,47 @@ __device__ __forceinline__ void walk_window_fwd(const RayParam& r, int N, int kx
     if (hasX) cand_range(r.x0, r.y0, r.a, r.b, ylo, yhi, lo, kxa, kxb, &x0k, &x1k);
     if (hasY) cand_range(r.y0, r.x0, r.b, r.a, xlo, xhi, lo, kya, kyb, &y0k, &y1k);
     // t ascends with k when the direction component is positive
-    int kx = r.a > 0.0 ? x0k : x1k, dkx = r.a > 0.0 ? 1 : -1, nx = x1k - x0k + 1;
-    int ky = r.b > 0.0 ? y0k : y1k, dky = r.b > 0.0 ? 1 : -1, ny = y1k - y0k + 1;
-    if (nx < 0) nx = 0;
-    if (ny < 0) ny = 0;
-    // current candidate of each list: t, coordinates, inside the window
-    double Xt = 0, Xx = 0, Xy = 0, Yt = 0, Yx = 0, Yy = 0;
-    bool Xok = false, Yok = false;
-    bool hx = nx > 0, hy = ny > 0;
-    if (hx) {
-        const double g = lo + (double)kx;
-        Xt = (g - r.x0) / r.a;
-        Xy = r.b * Xt + r.y0;
-        Xx = g;
-        Xok = Xy >= ylo && Xy <= yhi;
-        kx += dkx;
-        nx--;
-    }
-    if (hy) {
-        const double g = lo + (double)ky;
-        Yt = (g - r.y0) / r.b;
-        Yx = r.a * Yt + r.x0;
-        Yy = g;
-        Yok = Yx >= xlo && Yx <= xhi;
-        ky += dky;
-        ny--;
-    }
+    int kx = r.a > 0.0 ? x0k : x1k, dkx = r.a > 0.0 ? 1 : -1, nx = x1k - x0k;
+    int ky = r.b > 0.0 ? y0k : y1k, dky = r.b > 0.0 ? 1 : -1, ny = y1k - y0k;
+    bool hx = nx >= 0, hy = ny >= 0;       // nx, ny: candidates left after the current one
+    double Xt = hx ? (lo + (double)kx - r.x0) / r.a : 0.0;
+    double Yt = hy ? (lo + (double)ky - r.y0) / r.b : 0.0;
     bool havePrev = false, haveQ = false;
     double px = 0, py = 0, qx = 0, qy = 0;
     while (hx || hy) {
         const bool sx = hx && (!hy || Xt <= Yt);
-        const double x = sx ? Xx : Yx, y = sx ? Xy : Yy;
-        const bool ok = sx ? Xok : Yok;
-        // refill the list just consumed: one division on the selected operands
+        // the taken crossing: x-line (g, b t + y0) or y-line (a t + x0, g)
+        const int kc = sx ? kx : ky;
+        const double tc = sx ? Xt : Yt;
+        const double g = lo + (double)kc;
+        const double o = (sx ? r.b : r.a) * tc + (sx ? r.y0 : r.x0);
+        const bool ok = sx ? (o >= ylo && o <= yhi) : (o >= xlo && o <= xhi);
+        const double x = sx ? g : o, y = sx ? o : g;
+        // next candidate of that list: one division on the selected operands
         const bool more = sx ? nx > 0 : ny > 0;
-        if (more) {
-            const double g = lo + (double)(sx ? kx : ky);
-            const double t = (g - (sx ? r.x0 : r.y0)) / (sx ? r.a : r.b);
-            const double o = (sx ? r.b : r.a) * t + (sx ? r.y0 : r.x0);
-            const bool in = sx ? (o >= ylo && o <= yhi) : (o >= xlo && o <= xhi);
-            if (sx) { Xt = t; Xx = g; Xy = o; Xok = in; kx += dkx; nx--; }
-            else { Yt = t; Yx = o; Yy = g; Yok = in; ky += dky; ny--; }
-        } else {
-            if (sx) hx = false;
-            else hy = false;
-        }
-        if (!ok) continue;
-        if (havePrev) {
-            const bool coinc = fabs(x - px) <= 1e-10 && fabs(y - py) <= 1e-10;
-            if (!coinc) {
-                if (haveQ) seg(qx, qy, px, py);
-                qx = px; qy = py; haveQ = true;
-            }
-        }
-        havePrev = true;
-        px = x; py = y;
+        const int kn = kc + (sx ? dkx : dky);
+        const double tn = (lo + (double)kn - (sx ? r.x0 : r.y0)) / (sx ? r.a : r.b);
+        const bool upx = sx && more, upy = !sx && more;
+        Xt = upx ? tn : Xt;
+        Yt = upy ? tn : Yt;
+        kx = upx ? kn : kx;
+        ky = upy ? kn : ky;
+        nx -= upx ? 1 : 0;
+        ny -= upy ? 1 : 0;
+        hx = sx ? more : hx;
+        hy = sx ? hy : more;
+        // point (x, y) follows (px, py): keep (px, py) unless the two coincide
+        const bool coinc = fabs(x - px) <= 1e-10 && fabs(y - py) <= 1e-10;
+        const bool keep = ok && havePrev && !coinc;
+        seg(keep && haveQ, qx, qy, px, py);
+        qx = keep ? px : qx;
+        qy = keep ? py : qy;
+        haveQ = haveQ || keep;
+        havePrev = havePrev || ok;
+        px = ok ? x : px;
+        py = ok ? y : py;
     }
-    if (havePrev && haveQ) seg(qx, qy, px, py);
+    seg(havePrev && haveQ, qx, qy, px, py);
 }
 
 template <class Seg>
@@ -571,37 +562,55 @@ struct TileWin {
     int fx0, iy0, kxa, kxb, kya, kyb;
 };
 
-__device__ __forceinline__ TileWin tile_window(int tile, int nt, int T, int N) {
+__host__ __device__ inline TileWin tile_window(int tile, int nt, int T, int N) {
     TileWin w;
     const int tx = tile % nt, ty = tile / nt;
     w.fx0 = tx * T;
     w.iy0 = ty * T;
-    w.kxa = max(w.fx0 - 1, 0);
-    w.kxb = min(w.fx0 + T + 1, N);
-    w.kya = max(N - w.iy0 - T - 1, 0);
-    w.kyb = min(N - w.iy0 + 1, N);
+    w.kxa = w.fx0 - 1 > 0 ? w.fx0 - 1 : 0;
+    w.kxb = w.fx0 + T + 1 < N ? w.fx0 + T + 1 : N;
+    w.kya = N - w.iy0 - T - 1 > 0 ? N - w.iy0 - T - 1 : 0;
+    w.kyb = N - w.iy0 + 1 < N ? N - w.iy0 + 1 : N;
     return w;
 }
 
-// the segment's slot in the tile (column-major, T rows per column), or -1
-template <int T>
-__device__ __forceinline__ int tile_slot(const SegOut& o, const TileWin& w, int N) {
-    if (o.col < 0) return -1;
-    const int c = o.fx - w.fx0, rr = (o.col - o.fx * N) - w.iy0;
-    return ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T) ? c * T + rr : -1;
+// steps of walk_window_fwd for one ray in one window: its candidate lines
+__host__ __device__ inline int window_steps(const RayParam& r, int N, const TileWin& w) {
+    const double lo = -(double)N / 2.0;
+    const double xlo = lo + (double)w.kxa, xhi = lo + (double)w.kxb;
+    const double ylo = lo + (double)w.kya, yhi = lo + (double)w.kyb;
+    int x0k = 1, x1k = 0, y0k = 1, y1k = 0;
+    if (r.a > 0.0 || r.a < 0.0) cand_range(r.x0, r.y0, r.a, r.b, ylo, yhi, lo, w.kxa, w.kxb, &x0k, &x1k);
+    if (r.b > 0.0 || r.b < 0.0) cand_range(r.y0, r.x0, r.b, r.a, xlo, xhi, lo, w.kya, w.kyb, &y0k, &y1k);
+    return (x1k >= x0k ? x1k - x0k + 1 : 0) + (y1k >= y0k ? y1k - y0k + 1 : 0);
 }
 
-// A tile's items are split over `splits` work-groups (contiguous chunks, so
-// neighbouring rays share a wave); block b works on chunk b % splits of tile
-// b / splits.
+// seg_value restricted to a tile: the segment's length and its slot in the
+// tile (column-major, T rows per column), or -1 when its pixel (column
+// floor(xm) N + N-1-floor(ym)) lies outside the tile or outside the image
+template <int T>
+__device__ __forceinline__ double tile_seg(int N, const TileWin& w, double xu, double yu, double xv, double yv,
+                                           int* slot) {
+    const double half = (double)N / 2.0;
+    const double dx = xv - xu, dy = yv - yu;
+    const double xm = 0.5 * (xu + xv) + half;
+    const double ym = 0.5 * (yu + yv) + half;
+    const int fx = (int)floor(xm), iy = N - 1 - (int)floor(ym);
+    const int c = fx - w.fx0, rr = iy - w.iy0;
+    *slot = ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T && fx < N && iy < N) ? c * T + rr : -1;
+    return sqrt(dx * dx + dy * dy);
+}
+
+// A tile's items are sorted by their step count (longest first) and dealt
+// round-robin to `splits` work-groups: block b takes items part, part +
+// splits, ... of tile b / splits (part = b % splits), so the lanes of a wave
+// walk rays of about the same length and the blocks get equal shares.
 __device__ __forceinline__ void tile_chunk(const int64_t* __restrict__ tile_ptr, int splits, int* tile,
                                            int64_t* q0, int64_t* q1) {
     const int t = (int)(blockIdx.x / (unsigned)splits), part = (int)(blockIdx.x % (unsigned)splits);
-    const int64_t p0 = tile_ptr[t], len = tile_ptr[t + 1] - p0;
-    const int64_t chunk = (len + splits - 1) / splits;
     *tile = t;
-    *q0 = p0 + std::min<int64_t>(len, chunk * part);
-    *q1 = p0 + std::min<int64_t>(len, chunk * (part + 1));
+    *q0 = tile_ptr[t] + part;
+    *q1 = tile_ptr[t + 1];
 }
 
 constexpr int MFT = 64;          // tile side in pixels
@@ -622,13 +631,17 @@ __global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_forward(const RayParam* __
         us[i] = (fx < N && iy < N) ? u[(int64_t)fx * N + iy] : 0.f;
     }
     __syncthreads();
-    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int64_t stride = (int64_t)splits * blockDim.x;
+    for (int64_t q = q0 + (int64_t)splits * threadIdx.x; q < q1; q += stride) {
         const RayParam r = rays[tile_rays[q]];
         float acc = 0.f;
-        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb, [&](double xu, double yu, double xv, double yv) {
-            const SegOut o = seg_value(N, xu, yu, xv, yv);
-            const int k = tile_slot<MFT>(o, w, N);
-            if (k >= 0) acc += (float)o.val * us[k];
+        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb,
+                        [&](bool emit, double xu, double yu, double xv, double yv) {
+            int k;
+            const double val = tile_seg<MFT>(N, w, xu, yu, xv, yv, &k);
+            const bool use = emit && k >= 0;
+            const float xv32 = us[use ? k : 0];
+            acc += use ? (float)val * xv32 : 0.f;
         });
         partial[q] = acc;
     }
@@ -666,15 +679,17 @@ __global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_adjoint(const RayParam* __
     const TileWin w = tile_window(tile, nt, MFT, N);
     for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) gs[i] = 0.0;
     __syncthreads();
-    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int64_t stride = (int64_t)splits * blockDim.x;
+    for (int64_t q = q0 + (int64_t)splits * threadIdx.x; q < q1; q += stride) {
         const int ray = tile_rays[q];
         const float vi = v[ray];
         if (vi == 0.f) continue;
         const RayParam r = rays[ray];
-        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb, [&](double xu, double yu, double xv, double yv) {
-            const SegOut o = seg_value(N, xu, yu, xv, yv);
-            const int k = tile_slot<MFT>(o, w, N);
-            if (k >= 0) atomicAdd(&gs[k], (double)vi * (double)(float)o.val);
+        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb,
+                        [&](bool emit, double xu, double yu, double xv, double yv) {
+            int k;
+            const double val = tile_seg<MFT>(N, w, xu, yu, xv, yv, &k);
+            if (emit && k >= 0) atomicAdd(&gs[k], (double)vi * (double)(float)val);
         });
     }
     __syncthreads();
@@ -787,6 +802,29 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
             trays[(size_t)q] = (int)i;
             ritems[(size_t)c++] = (int)q;
         });
+    }
+    // sort each tile's items by step count, longest first (ties by ray)
+    {
+        std::vector<int> newpos((size_t)nitems), key;
+        std::vector<int64_t> ord;
+        std::vector<int> tmp;
+        for (int64_t t = 0; t < ntiles; ++t) {
+            const int64_t p0 = tptr[(size_t)t], len = tptr[(size_t)t + 1] - p0;
+            const TileWin w = tile_window((int)t, nt, T, N);
+            key.resize((size_t)len);
+            ord.resize((size_t)len);
+            for (int64_t j = 0; j < len; ++j) {
+                key[(size_t)j] = window_steps(g.host_rays[(size_t)trays[(size_t)(p0 + j)]], N, w);
+                ord[(size_t)j] = j;
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](int64_t u, int64_t v) { return key[(size_t)u] > key[(size_t)v]; });
+            tmp.assign(trays.begin() + p0, trays.begin() + p0 + len);
+            for (int64_t j = 0; j < len; ++j) {
+                trays[(size_t)(p0 + j)] = tmp[(size_t)ord[(size_t)j]];
+                newpos[(size_t)(p0 + ord[(size_t)j])] = (int)(p0 + j);
+            }
+        }
+        for (auto& q : ritems) q = newpos[(size_t)q];
     }
     g.T = T;
     g.ntiles_side = nt;
