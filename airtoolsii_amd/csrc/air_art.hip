@@ -265,6 +265,223 @@ void art_sweep_launch(const DevSparse& A, const ArtStep* st, int64_t L, double* 
     AIR_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- chunked sweep
+// The same sweep with the rows staged through LDS: the steps are cut into
+// chunks (at most ART_CMAX steps and E entries each); while wave 0 runs the
+// steps of chunk c from LDS, waves 1..15 copy the records and entries of
+// chunk c+1 into the other half of a double buffer, and the whole work-group
+// meets at one barrier per chunk.  x stays in LDS throughout.  The inner
+// product is reduced with DPP row shifts and row broadcasts in FP64 (lane 63
+// ends with the total), and the next step's record and first entries are read
+// while the current one finishes.
+constexpr int ART_CMAX = 256;
+constexpr int ART_BLOCK = 1024;
+
+struct ArtRec {
+    double b, nrm, lam;
+    int off, len;
+};
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, ROWMASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROWMASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the 64 lanes, returned in every lane
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_d<0x111, 0xf>(v);     // row_shr:1
+    v += dpp_d<0x112, 0xf>(v);     // row_shr:2
+    v += dpp_d<0x114, 0xf>(v);     // row_shr:4
+    v += dpp_d<0x118, 0xf>(v);     // row_shr:8: lane 15 of each row holds the row
+    v += dpp_d<0x142, 0xa>(v);     // row_bcast:15 -> rows 1, 3
+    v += dpp_d<0x143, 0xc>(v);     // row_bcast:31 -> rows 2, 3: lane 63 holds the total
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+struct ArtBuf {
+    ArtRec* rec;
+    double* val;
+    int* idx;
+};
+
+__device__ __forceinline__ ArtBuf art_buf(char* base, int E, int which) {
+    char* p = base + (size_t)which * ((size_t)ART_CMAX * sizeof(ArtRec) + (size_t)E * 12);
+    ArtBuf b;
+    b.rec = (ArtRec*)p;
+    b.val = (double*)(p + (size_t)ART_CMAX * sizeof(ArtRec));
+    b.idx = (int*)(p + (size_t)ART_CMAX * sizeof(ArtRec) + (size_t)E * 8);
+    return b;
+}
+
+// copy chunk c's records and entries into buffer B.  Producer wave w takes
+// steps w, w + nwaves, ...: lane l first loads the record of the wave's l-th
+// step (all records in one round trip), then the wave copies the entries of
+// two steps at a time.
+template <typename V>
+__device__ __forceinline__ void art_stage(const ArtStep* __restrict__ st, const int* __restrict__ cstart,
+                                          const int* __restrict__ soff, const int* __restrict__ idx,
+                                          const V* __restrict__ val, int c, ArtBuf B, int wave, int nwaves) {
+    const int lane = threadIdx.x & 63;
+    const int t0 = cstart[c], cnt = cstart[c + 1] - t0;
+    const int mine = cnt > wave ? (cnt - wave + nwaves - 1) / nwaves : 0;   // steps of this wave (<= 64)
+    int64_t s0 = 0;
+    int len = 0, off = 0;
+    if (lane < mine) {
+        const int q = wave + nwaves * lane;
+        const ArtStep a = st[t0 + q];
+        ArtRec r;
+        r.b = a.b;
+        r.nrm = a.nrm;
+        r.lam = a.lam;
+        r.off = soff[t0 + q];
+        r.len = (int)(a.e - a.s);
+        B.rec[q] = r;
+        s0 = a.s;
+        len = r.len;
+        off = r.off;
+    }
+    for (int u = 0; u < mine; u += 2) {
+        const int64_t sa = __shfl(s0, u), sb = __shfl(s0, u + 1 < mine ? u + 1 : u);
+        const int la = __shfl(len, u), lb = u + 1 < mine ? __shfl(len, u + 1) : 0;
+        const int oa = __shfl(off, u), ob = __shfl(off, u + 1 < mine ? u + 1 : u);
+        const int lmax = la > lb ? la : lb;
+        for (int i = lane; i < lmax; i += 64) {
+            int ia = 0, ib = 0;
+            double va = 0, vb = 0;
+            if (i < la) { ia = idx[sa + i]; va = to_f64(val[sa + i]); }
+            if (i < lb) { ib = idx[sb + i]; vb = to_f64(val[sb + i]); }
+            if (i < la) { B.idx[oa + i] = ia; B.val[oa + i] = va; }
+            if (i < lb) { B.idx[ob + i] = ib; B.val[ob + i] = vb; }
+        }
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(ART_BLOCK) void k_art_chunked(const ArtStep* __restrict__ st, const int* __restrict__ cstart,
+                                                           int nchunks, const int* __restrict__ soff,
+                                                           const int* __restrict__ idx, const V* __restrict__ val,
+                                                           double* xg, Bounds bd, int64_t n, int project_all_first,
+                                                           int E) {
+    extern __shared__ double smem[];
+    double* x = smem;
+    char* base = (char*)(smem + n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = ART_BLOCK / 64;
+    const bool bounded = bd.lb_kind != AIR_BOUND_NONE || bd.ub_kind != AIR_BOUND_NONE;
+    for (int64_t j = threadIdx.x; j < n; j += ART_BLOCK) x[j] = xg[j];
+    if (nchunks > 0) art_stage<V>(st, cstart, soff, idx, val, 0, art_buf(base, E, 0), wave, nw);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const ArtBuf B = art_buf(base, E, c & 1);
+        if (wave == 0) {
+            const int cnt = cstart[c + 1] - cstart[c];
+            ArtRec r = B.rec[0];
+            double a0 = 0, a1 = 0;
+            int j0 = 0, j1 = 0;
+            if (lane < r.len) { a0 = B.val[r.off + lane]; j0 = B.idx[r.off + lane]; }
+            if (lane + 64 < r.len) { a1 = B.val[r.off + lane + 64]; j1 = B.idx[r.off + lane + 64]; }
+            for (int q = 0; q < cnt; ++q) {
+                // the next step's record and first 128 entries (they do not depend on x)
+                ArtRec rn = r;
+                double b0 = 0, b1 = 0;
+                int k0 = 0, k1 = 0;
+                if (q + 1 < cnt) {
+                    rn = B.rec[q + 1];
+                    if (lane < rn.len) { b0 = B.val[rn.off + lane]; k0 = B.idx[rn.off + lane]; }
+                    if (lane + 64 < rn.len) { b1 = B.val[rn.off + lane + 64]; k1 = B.idx[rn.off + lane + 64]; }
+                }
+                // both gathers issued together (lanes past the row read x[0]
+                // and contribute nothing); the values are reused by the update,
+                // which is exact because a row's columns are distinct
+                const bool on0 = lane < r.len, on1 = lane + 64 < r.len;
+                const double x0v = x[j0], x1v = x[j1];
+                double dot = on0 ? a0 * x0v : 0.0;
+                dot += on1 ? a1 * x1v : 0.0;
+                for (int i = lane + 128; i < r.len; i += 64) dot += B.val[r.off + i] * x[B.idx[r.off + i]];
+                dot = wave_sum_dpp(dot);
+                const double coef = (r.lam * (r.b - dot)) / r.nrm;
+                if (on0) {
+                    double v = x0v + coef * a0;
+                    if (bounded) v = clamp_bounds(v, j0, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+                    x[j0] = v;
+                }
+                if (on1) {
+                    double v = x1v + coef * a1;
+                    if (bounded) v = clamp_bounds(v, j1, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+                    x[j1] = v;
+                }
+                for (int i = lane + 128; i < r.len; i += 64) {
+                    const int jj = B.idx[r.off + i];
+                    double v = x[jj] + coef * B.val[r.off + i];
+                    if (bounded) v = clamp_bounds(v, jj, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+                    x[jj] = v;
+                }
+                if (c == 0 && q == 0 && project_all_first && bounded) {
+                    // the whole vector is projected after every row; after the
+                    // first row only the row's own entries can change
+                    for (int64_t j = lane; j < n; j += 64)
+                        x[j] = clamp_bounds(x[j], j, bd.lb_kind, bd.lb, bd.lbv, bd.ub_kind, bd.ub, bd.ubv);
+                }
+                r = rn;
+                a0 = b0; a1 = b1; j0 = k0; j1 = k1;
+            }
+        } else if (c + 1 < nchunks) {
+            art_stage<V>(st, cstart, soff, idx, val, c + 1, art_buf(base, E, (c + 1) & 1), wave - 1, nw - 1);
+        }
+        __syncthreads();
+    }
+    for (int64_t j = threadIdx.x; j < n; j += ART_BLOCK) xg[j] = x[j];
+}
+
+// Chunk plan of one sweep: chunk boundaries (step indices) and each step's
+// entry offset inside its chunk's buffer.
+void art_plan_chunks(const std::vector<int>& steps, const std::vector<int64_t>& hp, int E, std::vector<int>& cstart,
+                     std::vector<int>& soff) {
+    cstart.assign(1, 0);
+    soff.resize(steps.size());
+    int used = 0, cnt = 0;
+    for (size_t t = 0; t < steps.size(); ++t) {
+        const int len = (int)(hp[(size_t)steps[t] + 1] - hp[(size_t)steps[t]]);
+        if (cnt == ART_CMAX || used + len > E) {
+            cstart.push_back((int)t);
+            used = 0;
+            cnt = 0;
+        }
+        soff[t] = used;
+        used += len;
+        cnt++;
+    }
+    if (!steps.empty()) cstart.push_back((int)steps.size());
+}
+
+// LDS entries per buffer for the chunked sweep, or 0 when x and two buffers
+// holding the longest row do not fit
+int art_chunk_entries(int64_t n, int64_t maxlen, int lim) {
+    const int64_t room = (int64_t)lim - 8 * n - 2 * (int64_t)ART_CMAX * (int64_t)sizeof(ArtRec);
+    int64_t E = std::min<int64_t>(4096, room / 24) & ~(int64_t)63;
+    return (E >= 512 && E >= maxlen) ? (int)E : 0;
+}
+
+void launch_art_chunked(const DevSparse& A, const ArtStep* st, const int* cstart, int nchunks, const int* soff,
+                        double* x, Bounds bd, int64_t n, int paf, int E, hipStream_t s) {
+    const size_t lds = 8 * (size_t)n + 2 * ((size_t)ART_CMAX * sizeof(ArtRec) + (size_t)E * 12);
+    dispatch_dtype(A.dtype, [&](auto v) {
+        using V = decltype(v);
+        if (lds > 64 * 1024)
+            AIR_HIP(hipFuncSetAttribute((const void*)k_art_chunked<V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+        k_art_chunked<V><<<1, ART_BLOCK, lds, s>>>(st, cstart, nchunks, soff, A.idx.as<int>(), A.val.as<V>(), x, bd,
+                                                   n, paf, E);
+        return 0;
+    });
+    AIR_HIP(hipGetLastError());
+}
+
 // pc: entries prefetched per step in units of 64 (2, 4 or 8)
 void launch_art_sweep(const DevSparse& A, const ArtStep* st, int64_t L, double* x, Bounds bd, int64_t n, int paf,
                       bool lds, int pc, hipStream_t s) {
@@ -501,9 +718,18 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
     if (lam_vec) dlam.alloc(8 * std::max<size_t>(L, 1));
     // ART: per-step records, x in LDS when it fits one work-group's LDS
     // (AIR_ART_LDS=0 keeps it in global memory), entries prefetched per step
-    DevBuf dinfo;
+    DevBuf dinfo, dcst, dsoff;
     bool use_lds = false;
-    int pc = 2;
+    int pc = 2, chunkE = 0;
+    std::vector<int64_t> hrowptr;
+    std::vector<int> cst, soff;
+    auto upload_plan = [&](const std::vector<int>& order_of_steps) {
+        art_plan_chunks(order_of_steps, hrowptr, chunkE, cst, soff);
+        dcst.alloc(4 * cst.size());
+        dsoff.alloc(4 * std::max<size_t>(soff.size(), 1));
+        AIR_HIP(hipMemcpyAsync(dcst.p, cst.data(), 4 * cst.size(), hipMemcpyHostToDevice, s));
+        if (!soff.empty()) AIR_HIP(hipMemcpyAsync(dsoff.p, soff.data(), 4 * soff.size(), hipMemcpyHostToDevice, s));
+    };
     if (!cart) {
         dinfo.alloc(sizeof(ArtStep) * std::max<size_t>(L, 1));
         int dev = 0, lim = 0;
@@ -517,6 +743,11 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
         int64_t maxlen = 0;
         for (int64_t q = 0; q < m; ++q) maxlen = std::max(maxlen, hp[q + 1] - hp[q]);
         pc = maxlen <= 128 ? 2 : (maxlen <= 256 ? 4 : 8);
+        // rows staged through LDS by the rest of a 1024-thread work-group
+        // (AIR_ART_CHUNK=0 keeps the one-wave form)
+        const char* envc = std::getenv("AIR_ART_CHUNK");
+        chunkE = use_lds && !(envc && envc[0] == '0') ? art_chunk_entries(n, maxlen, lim) : 0;
+        hrowptr.swap(hp);
     }
     if (cart) {
         dflag.alloc((size_t)std::max<int64_t>(n, 1));
@@ -607,6 +838,7 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
     double lam_last = lam;
     std::vector<int> steps(L);
     std::vector<double> lams(lam_vec ? L : 0);
+    if (chunkE > 0 && !rnd && L) upload_plan(I);
     const int nunflag = o->Nunflag >= 1 ? o->Nunflag : std::max(1, (int)std::lround(kmax / 4.0));
     auto ti = std::chrono::steady_clock::now();
     while (code < 0) {
@@ -650,8 +882,14 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
                     dsteps.as<int>(), (int64_t)L, A.ptr.as<int64_t>(), db.as<double>(), dnrm.as<double>(),
                     lam_vec ? dlam.as<double>() : nullptr, lam, dinfo.as<ArtStep>());
                 AIR_HIP(hipGetLastError());
-                launch_art_sweep(A, dinfo.as<ArtStep>(), (int64_t)L, dx.as<double>(), bd, n, k == 1 ? 1 : 0,
-                                 use_lds, pc, s);
+                if (chunkE > 0) {
+                    if (rnd) upload_plan(steps);
+                    launch_art_chunked(A, dinfo.as<ArtStep>(), dcst.as<int>(), (int)cst.size() - 1, dsoff.as<int>(),
+                                       dx.as<double>(), bd, n, k == 1 ? 1 : 0, chunkE, s);
+                } else {
+                    launch_art_sweep(A, dinfo.as<ArtStep>(), (int64_t)L, dx.as<double>(), bd, n, k == 1 ? 1 : 0,
+                                     use_lds, pc, s);
+                }
             }
             residual();                                             // art.m:288
         }

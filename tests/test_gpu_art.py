@@ -68,12 +68,15 @@ def test_art_fp32_storage_matches_oracle_on_rounded_matrix(oracle, prob):
     assert rel(X[:, -1], Xf[:, -1]) <= 1e-5
 
 
-@pytest.mark.parametrize("lds", ["0", "1"])
-@pytest.mark.parametrize("name", ["kaczmarz_bounds_damp", "randkaczmarz", "relax_function"])
-def test_art_global_and_lds_forms(oracle, prob, monkeypatch, lds, name):
-    """x in LDS (copied in and out of one work-group) or in global memory with
-    a fence per step: the same steps, so the same iterates."""
-    monkeypatch.setenv("AIR_ART_LDS", lds)
+@pytest.mark.parametrize("form", ["global", "lds_one_wave", "lds_chunked"])
+@pytest.mark.parametrize("name", ["kaczmarz_bounds_damp", "randkaczmarz", "relax_function", "symkaczmarz",
+                                  "custom_order"])
+def test_art_global_and_lds_forms(oracle, prob, monkeypatch, form, name):
+    """x in global memory with a fence per step, x in LDS with one wave, or x
+    in LDS with the rows staged chunk by chunk by the rest of the work-group:
+    the same steps, so the same iterates."""
+    monkeypatch.setenv("AIR_ART_LDS", "0" if form == "global" else "1")
+    monkeypatch.setenv("AIR_ART_CHUNK", "1" if form == "lds_chunked" else "0")
     meth, K, opts = ART_CASES[name]
     X, info = air.art(meth, prob["A64"], prob["b"], K, None, opts)
     Xo, io = oracle.art(meth, prob["Ao"], prob["b"], K, None, opts)
