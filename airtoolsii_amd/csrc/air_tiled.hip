@@ -241,33 +241,44 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
     int kopen = -1;                          // segment open at the end of the last round
     A carry = 0;
     A v[NV][VE];
-    unsigned c[EPL];
-    auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[EPL]) {
+    // codes stay packed two per word: entry 2h in the low half of c[h], entry
+    // 2h+1 in the high half; the LDS byte address and the start flag are cut
+    // straight out of the word
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    unsigned c[EPL / 2];
+    const unsigned mb = mask << 2;
+    auto addr = [&](const unsigned (&cc)[EPL / 2], int i) -> unsigned {
+        return (i & 1) ? (cc[i >> 1] >> 14) & mb : (cc[i >> 1] << 2) & mb;
+    };
+    auto flags = [&](const unsigned (&cc)[EPL / 2]) -> unsigned {
+        unsigned f = 0;
+#pragma unroll
+        for (int h = 0; h < EPL / 2; ++h) f |= (cc[h] & 0x80008000u) >> (15 - 2 * h);
+        return (f | (f >> 15)) & ((1u << EPL) - 1u);
+    };
+    auto lds_at = [&](unsigned byte) -> float { return *(const float*)((const char*)lds + byte); };
+    auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[EPL / 2]) {
         q += EPL * lane;
 #pragma unroll
         for (int h = 0; h < NV; ++h) load_vals<V, A>(val + q + VE * h, vv[h]);
 #pragma unroll
         for (int h = 0; h < NC; ++h) {
-            unsigned t8[8];
-            load_code<8>(code + q + 8 * h, t8);
+            const u4v w = __builtin_nontemporal_load((const u4v*)(code + q + 8 * h));
 #pragma unroll
-            for (int i = 0; i < 8; ++i) cc[8 * h + i] = t8[i];
+            for (int i = 0; i < 4; ++i) cc[4 * h + i] = w[i];
         }
     };
     load_round(rb, v, c);
     while (rb < E1) {
         A vn[NV][VE];
-        unsigned cn[EPL];
+        unsigned cn[EPL / 2];
         if (PF) load_round(rb + RND < E1 ? rb + RND : rb, vn, cn);
         A pr[EPL];
-        unsigned fl = 0;
+        unsigned fl = flags(c);
         if (rb >= E0 && rb + RND <= E1) {
             // interior round: every entry is valid
 #pragma unroll
-            for (int i = 0; i < EPL; ++i) {
-                pr[i] = v[i / VE][i % VE] * (A)lds[c[i] & mask];
-                fl |= (c[i] >> 15) << i;
-            }
+            for (int i = 0; i < EPL; ++i) pr[i] = v[i / VE][i % VE] * (A)lds_at(addr(c, i));
         } else {
             // edge round: entries outside [E0, E1) belong to other segments
             // (or tiles): they read slot 0 and contribute 0
@@ -278,12 +289,9 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
             const unsigned ok = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
             float xv[EPL];
 #pragma unroll
-            for (int i = 0; i < EPL; ++i) xv[i] = lds[((ok >> i) & 1u) ? (c[i] & mask) : 0u];
+            for (int i = 0; i < EPL; ++i) xv[i] = lds_at(((ok >> i) & 1u) ? addr(c, i) : 0u);
 #pragma unroll
-            for (int i = 0; i < EPL; ++i) {
-                pr[i] = mask_if(v[i / VE][i % VE] * (A)xv[i], (ok >> i) & 1u);
-                fl |= (c[i] >> 15) << i;
-            }
+            for (int i = 0; i < EPL; ++i) pr[i] = mask_if(v[i / VE][i % VE] * (A)xv[i], (ok >> i) & 1u);
             fl &= ok;
         }
         // head: products before the lane's first start; tail: from its last start
@@ -391,7 +399,7 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
 #pragma unroll
                 for (int i = 0; i < VE; ++i) v[h][i] = vn[h][i];
 #pragma unroll
-            for (int i = 0; i < EPL; ++i) c[i] = cn[i];
+            for (int i = 0; i < EPL / 2; ++i) c[i] = cn[i];
         } else if (rb < E1) {
             load_round(rb, v, c);
         }
