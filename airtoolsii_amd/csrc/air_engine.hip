@@ -149,8 +149,7 @@ void reset_operator(air_engine* e) {
     e->A = DevSparse();
     e->AT = DevSparse();
     e->T = TiledOp();
-    e->geo.rays.release();
-    e->geo.host_rays.clear();
+    e->geo = Geometry();
     e->kind = 0;
     e->m = e->n = 0;
     e->st.prepared = false;
@@ -997,11 +996,16 @@ int air_build_seismicwavetomo(air_engine* e, int32_t N, int32_t s, int32_t p, do
                               int dtype) {
     return guard([&] {
         AIR_CHECK(e && N > 0 && s > 0 && p > 0, AIR_E_ARG, "invalid seismicwavetomo arguments");
-        AIR_CHECK(!matrix_free, AIR_E_UNSUPPORTED,
-                  "seismicwavetomo: the matrix-free form is provided for the line-model tracers only");
         use_device(e);
         reset_operator(e);
         (void)dtype_size(dtype);
+        if (matrix_free) {                             // seismicwavetomo.m:94-99: kernels re-evaluated per product
+            fresnel_mf_prepare(e->geo, N, s, p, omega, e->stream);
+            e->kind = 2;
+            e->m = (int64_t)s * p;
+            e->n = (int64_t)N * N;
+            return;
+        }
         fresnel_to_csr(N, s, p, omega, dtype, e->A, e->stream);
         finish_matrix(e, N);
     });

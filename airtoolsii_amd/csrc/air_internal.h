@@ -154,6 +154,11 @@ struct Geometry {
     DevBuf partial;       // float [nitems]: forward partial sums
     int splits = 1;       // work-groups per tile
     DevBuf gpart;         // double [splits * N * N]: adjoint slices when splits > 1
+    // model 1: Fresnel-zone rows of seismicwavetomo (nrays = rows)
+    int model = 0;        // 0: line rays, 1: Fresnel zones
+    DevBuf frows;         // per-row source/receiver geometry
+    DevBuf frowsum;       // double [rows]: sum of the kernel over the image
+    double fw = 0;        // omega / N
 };
 
 // ---------------------------------------------------------------- tracer (air_tracer.hip)
@@ -184,6 +189,11 @@ void circles_to_csr(int N, const double* angles, int nA, int nC, int dtype, DevS
 void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s);
 // Matrix-free products: y = A u (m) or g = A' v (n, FP64 accumulators).
 void mf_build_tiles(Geometry& g, hipStream_t s);
+void fresnel_mf_prepare(Geometry& g, int N, int s, int p, double omega, hipStream_t st);
+void fresnel_mf_forward(const Geometry& g, const float* u, float* y, hipStream_t st);
+void fresnel_mf_adjoint(const Geometry& g, const float* v, double* gout, hipStream_t st);
+void fresnel_mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t st);
+void fresnel_mf_col_stats(const Geometry& g, double* colabs, int* colcnt, hipStream_t st);
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s);
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s);
 // Matrix-free row stats (sum a^2, sum |a|, sum a^2 * w[col]) and column stats.

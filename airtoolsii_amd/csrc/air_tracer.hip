@@ -850,6 +850,7 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
 }
 
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
+    if (g.model == 1) { fresnel_mf_forward(g, u, y, s); return; }
     if (g.nrays == 0) return;
     AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
     const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
@@ -865,6 +866,7 @@ void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
 
 // g = A' v (every pixel written: the tiles cover the image)
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) {
+    if (g.model == 1) { fresnel_mf_adjoint(g, v, gacc, s); return; }
     if (g.nrays == 0) return;
     AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
     const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
@@ -881,12 +883,14 @@ void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) 
 }
 
 void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t s) {
+    if (g.model == 1) { fresnel_mf_row_stats(g, colw, sq, ab, sqw, s); return; }
     if (g.nrays == 0) return;
     k_mf_row_stats<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
                                                                      g.edge_check, colw, sq, ab, sqw);
     AIR_HIP(hipGetLastError());
 }
 void mf_col_stats(const Geometry& g, double* colabs, int* colcnt, hipStream_t s) {
+    if (g.model == 1) { fresnel_mf_col_stats(g, colabs, colcnt, s); return; }
     if (g.nrays == 0) return;
     k_mf_col_stats<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
                                                                      g.edge_check, colabs, colcnt);

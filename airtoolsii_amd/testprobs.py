@@ -117,12 +117,14 @@ def sphericaltomo(N, angles=None, numCircles=None, isDisp=0, isMatrix=1, *, dtyp
 def seismicwavetomo(N, s=None, p=None, omega=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0,
                     return_all=False):
     """[A,b,x,s,p,omega] = seismicwavetomo(N,s,p,omega,isDisp,isMatrix),
-    seismicwavetomo.m:70-110 (x is the tectonic phantom).  Matrix form only."""
+    seismicwavetomo.m:70-110 (x is the tectonic phantom).  isMatrix=0 gives the
+    matrix-free operator, which re-evaluates the Fresnel kernels on every
+    product (seismicwavetomo.m:94-99)."""
     N = int(N)
     s = N if s is None else int(s)
     p = 2 * N if p is None else int(p)
     omega = 10.0 if omega is None else float(omega)
     e = Engine(device)
     check(load().air_build_seismicwavetomo(e.h, N, s, p, omega, 0 if isMatrix else 1, DTYPES[dtype]))
-    A = AirOperator(e, dtype, False, {"kind": "seismicwave", "N": N, "s": s, "p": p, "omega": omega})
+    A = AirOperator(e, dtype, not isMatrix, {"kind": "seismicwave", "N": N, "s": s, "p": p, "omega": omega})
     return _finish(A, return_all, (s, p, omega), tectonic)

@@ -277,3 +277,37 @@ def test_seismicwave_f32_and_return_all(oracle):
     np.testing.assert_allclose(v, Ao.data.astype(np.float32), rtol=1e-6)
     np.testing.assert_array_equal(x, oracle.tectonic(24).ravel(order="F"))
     np.testing.assert_allclose(b, Ao @ x, rtol=1e-5)
+
+
+@pytest.mark.parametrize("N,s,p,omega", [(24, None, None, None), (33, 5, 9, 20.0), (40, 8, 13, 4.0)])
+def test_seismicwave_matrix_free_products(oracle, N, s, p, omega):
+    """seismicwavetomo(..., isMatrix=0) re-evaluates the Fresnel kernels on
+    every product: its entries are the stored matrix's, so A x agrees with
+    A32 x to FP32 summation and A' y with A32' y to the result's FP32
+    rounding in every pixel (FP64 sums over the rows in order)."""
+    print(f"case seismicwave matrix-free N={N} s={s} p={p}", flush=True)
+    Af = air.seismicwavetomo(N, s, p, omega, 0, 0)
+    Ao = oracle.seismicwavetomo(N, s, p, 10.0 if omega is None else omega)
+    assert Af.matrix_free and Af.shape == Ao.shape
+    A32 = Ao.copy()
+    A32.data = A32.data.astype(np.float32).astype(np.float64)
+    Aabs = abs(Ao)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1, Ao.shape[1]).astype(np.float32).astype(np.float64)
+    y = rng.uniform(-1, 1, Ao.shape[0]).astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(Af @ x - A32 @ x) <= 1e-5 * (Aabs @ np.abs(x)) + 1e-30)
+    assert np.all(np.abs(Af.T @ y - A32.T @ y) <= 2e-7 * (Aabs.T @ np.abs(y)) + 1e-30)
+
+
+@pytest.mark.parametrize("method", ["cimmino", "cav", "drop", "sart"])
+def test_seismicwave_matrix_free_sirt_matches_matrix(method):
+    """Weights from the matrix-free row and column statistics, then 20 SIRT
+    iterations: the iterates of the stored and the matrix-free operator agree."""
+    A = air.seismicwavetomo(24)
+    Af = air.seismicwavetomo(24, None, None, None, 0, 0)
+    x = air.phantomgallery("tectonic", 24).ravel(order="F")
+    b = A @ x
+    opts = {"lbound": 0}
+    X, info, _ = air.sirt(method, A, b, 20, None, opts)
+    Xf, _, _ = air.sirt(method, Af, b, 20, None, dict(opts, rho=info["rho"]))
+    assert np.linalg.norm(Xf[:, -1] - X[:, -1]) <= 1e-5 * np.linalg.norm(X[:, -1])
