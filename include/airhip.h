@@ -146,7 +146,8 @@ int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int3
                             int32_t numCircles, int matrix_free, int dtype);
 /* testprobs/seismicwavetomo.m:1-288 (defaults resolved by the caller: s = N,
  * p = 2N, omega = 10): Fresnel-zone sensitivity of each source-receiver
- * pair, row = source*p + receiver.  matrix_free must be 0. */
+ * pair, row = source*p + receiver.  matrix_free = 1: no stored matrix, the
+ * kernels are re-evaluated on every product (seismicwavetomo.m:94-99). */
 int air_build_seismicwavetomo(air_engine* e, int32_t N, int32_t s, int32_t p, double omega,
                               int matrix_free, int dtype);
 /* ---- filtered back projection (auxil/fbp.m:31-80) ---------------------
