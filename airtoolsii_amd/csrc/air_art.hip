@@ -467,6 +467,193 @@ int art_chunk_entries(int64_t n, int64_t maxlen, int lim) {
     return (E >= 512 && E >= maxlen) ? (int)E : 0;
 }
 
+// ---------------------------------------------------------------- chunked CART sweep
+// The column-action sweep with the residual r in LDS: wave 0 runs the
+// column steps of one chunk while waves 1..15 stage the next chunk's column
+// records (column, norm, the current x_j, bounds, flags) and entries.  Used
+// when no column repeats in the order, so a staged x_j or flag is never
+// changed by a step of the chunk being run; x, flag and nflag are written
+// back to global memory by lane 0 as in the one-wave form.
+constexpr int CART_CMAX = 64;
+
+struct CartRec {
+    double nrm, xj, lb, ub;
+    int col, off, len, flag, nflag, pad;
+};
+
+struct CartBuf {
+    CartRec* rec;
+    double* val;
+    int* idx;
+};
+
+__device__ __forceinline__ CartBuf cart_buf(char* base, int E, int which) {
+    char* p = base + (size_t)which * ((size_t)CART_CMAX * sizeof(CartRec) + (size_t)E * 12);
+    CartBuf b;
+    b.rec = (CartRec*)p;
+    b.val = (double*)(p + (size_t)CART_CMAX * sizeof(CartRec));
+    b.idx = (int*)(p + (size_t)CART_CMAX * sizeof(CartRec) + (size_t)E * 8);
+    return b;
+}
+
+template <typename V>
+__device__ __forceinline__ void cart_stage(const int* __restrict__ cols, const int* __restrict__ cstart,
+                                           const int* __restrict__ soff, const int64_t* __restrict__ cptr,
+                                           const int* __restrict__ ridx, const V* __restrict__ val,
+                                           const double* __restrict__ nrm, const double* xg,
+                                           const unsigned char* flag, const int* nflag, const Bounds& bd, int c,
+                                           CartBuf B, int wave, int nwaves) {
+    const int lane = threadIdx.x & 63;
+    const int t0 = cstart[c], cnt = cstart[c + 1] - t0;
+    const int mine = cnt > wave ? (cnt - wave + nwaves - 1) / nwaves : 0;   // <= 64
+    int64_t s0 = 0;
+    int len = 0, off = 0;
+    if (lane < mine) {
+        const int q = wave + nwaves * lane;
+        const int j = cols[t0 + q];
+        CartRec r;
+        r.nrm = nrm[j];
+        r.xj = xg[j];
+        r.lb = bd.lb_kind == AIR_BOUND_SCALAR ? bd.lb : (bd.lb_kind == AIR_BOUND_VECTOR ? bd.lbv[j] : 0.0);
+        r.ub = bd.ub_kind == AIR_BOUND_SCALAR ? bd.ub : (bd.ub_kind == AIR_BOUND_VECTOR ? bd.ubv[j] : 0.0);
+        r.col = j;
+        r.off = soff[t0 + q];
+        r.flag = flag[j];
+        r.nflag = nflag[j];
+        r.len = r.flag ? (int)(cptr[j + 1] - cptr[j]) : 0;     // entries only for columns that step
+        r.pad = 0;
+        B.rec[q] = r;
+        s0 = cptr[j];
+        len = r.len;
+        off = r.off;
+    }
+    for (int u = 0; u < mine; ++u) {
+        const int64_t sa = __shfl(s0, u);
+        const int la = __shfl(len, u), oa = __shfl(off, u);
+        for (int i = lane; i < la; i += 64) {
+            B.idx[oa + i] = ridx[sa + i];
+            B.val[oa + i] = to_f64(val[sa + i]);
+        }
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(ART_BLOCK) void k_cart_chunked(const int* __restrict__ cols, const int* __restrict__ cstart,
+                                                            int nchunks, const int* __restrict__ soff,
+                                                            const int64_t* __restrict__ cptr,
+                                                            const int* __restrict__ ridx, const V* __restrict__ val,
+                                                            const double* __restrict__ nrm, double* xg, double* rg,
+                                                            unsigned char* flag, int* nflag, Bounds bd, int64_t n,
+                                                            int64_t m, CartParams cp, int E) {
+    extern __shared__ double smem[];
+    double* r = smem;
+    char* base = (char*)(smem + m);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = ART_BLOCK / 64;
+    const bool track = cp.k >= cp.kbegin;
+    for (int64_t i = threadIdx.x; i < m; i += ART_BLOCK) r[i] = rg[i];
+    int64_t arg = -1;
+    double mm = NAN;
+    if (wave == 0 && track) mm = wave_absmax(xg, n, &arg);
+    if (nchunks > 0)
+        cart_stage<V>(cols, cstart, soff, cptr, ridx, val, nrm, xg, flag, nflag, bd, 0, cart_buf(base, E, 0), wave,
+                      nw);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const CartBuf B = cart_buf(base, E, c & 1);
+        if (wave == 0) {
+            const int cnt = cstart[c + 1] - cstart[c];
+            for (int q = 0; q < cnt; ++q) {
+                const CartRec rc = B.rec[q];
+                const int j = rc.col;
+                if (rc.flag) {
+                    double av[4], rv[4];
+                    int iv[4];
+                    bool on[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int e = lane + 64 * u;
+                        on[u] = e < rc.len;
+                        av[u] = on[u] ? B.val[rc.off + e] : 0.0;
+                        iv[u] = on[u] ? B.idx[rc.off + e] : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) rv[u] = r[iv[u]];
+                    double dot = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) dot += on[u] ? av[u] * rv[u] : 0.0;
+                    for (int e = lane + 256; e < rc.len; e += 64) dot += B.val[rc.off + e] * r[B.idx[rc.off + e]];
+                    dot = wave_sum_dpp(dot);
+                    const double delta = dot / rc.nrm;
+                    double od = cp.lam * delta;
+                    const double xkj = rc.xj;
+                    if (bd.lb_kind != AIR_BOUND_NONE && od < rc.lb - xkj) od = rc.lb - xkj;   // cart.m:251-259
+                    if (bd.ub_kind != AIR_BOUND_NONE && od > rc.ub - xkj) od = rc.ub - xkj;   // :261-267
+                    const double xn = xkj + od;
+                    if (lane == 0) {
+                        xg[j] = xn;
+                        if (track && fabs(od) < cp.thr * mm) {                                // :273-276
+                            flag[j] = 0;
+                            nflag[j] = 1;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (on[u]) r[iv[u]] = rv[u] - od * av[u];                           // :279
+                    for (int e = lane + 256; e < rc.len; e += 64) {
+                        const int i = B.idx[rc.off + e];
+                        r[i] = r[i] - od * B.val[rc.off + e];
+                    }
+                    if (track) {                                                          // mm = max(abs(xk))
+                        const double an = fabs(xn);
+                        if (an >= mm || isnan(mm)) { if (!isnan(an)) { mm = an; arg = j; } }
+                        else if (j == arg) {
+                            order_stores();                                               // x[j] written above
+                            mm = wave_absmax(xg, n, &arg);
+                        }
+                    }
+                } else if (lane == 0) {                                                   // :281-285
+                    const int draw =
+                        1 + (int)floor(rng_u01(cp.seed, RNG_UNFLAG, (uint64_t)cp.k, (uint64_t)j) * cp.nunflag);
+                    if (rc.nflag < draw) nflag[j] = rc.nflag + 1;
+                    else flag[j] = 1;
+                }
+            }
+        } else if (c + 1 < nchunks) {
+            cart_stage<V>(cols, cstart, soff, cptr, ridx, val, nrm, xg, flag, nflag, bd, c + 1,
+                          cart_buf(base, E, (c + 1) & 1), wave - 1, nw - 1);
+        }
+        __syncthreads();
+    }
+    for (int64_t i = threadIdx.x; i < m; i += ART_BLOCK) rg[i] = r[i];
+}
+
+// LDS entries per buffer for the chunked CART sweep, or 0 when r and two
+// buffers holding the longest column do not fit
+int cart_chunk_entries(int64_t m, int64_t maxlen, int lim) {
+    const int64_t room = (int64_t)lim - 8 * m - 2 * (int64_t)CART_CMAX * (int64_t)sizeof(CartRec);
+    int64_t E = std::min<int64_t>(4096, room / 24) & ~(int64_t)63;
+    return (E >= 256 && E >= maxlen) ? (int)E : 0;
+}
+
+void cart_plan_chunks(const std::vector<int>& cols, const std::vector<int64_t>& cp, int E, std::vector<int>& cstart,
+                      std::vector<int>& soff) {
+    cstart.assign(1, 0);
+    soff.resize(cols.size());
+    int used = 0, cnt = 0;
+    for (size_t t = 0; t < cols.size(); ++t) {
+        const int len = (int)(cp[(size_t)cols[t] + 1] - cp[(size_t)cols[t]]);
+        if (cnt == CART_CMAX || used + len > E) {
+            cstart.push_back((int)t);
+            used = 0;
+            cnt = 0;
+        }
+        soff[t] = used;
+        used += len;
+        cnt++;
+    }
+    if (!cols.empty()) cstart.push_back((int)cols.size());
+}
+
 void launch_art_chunked(const DevSparse& A, const ArtStep* st, const int* cstart, int nchunks, const int* soff,
                         double* x, Bounds bd, int64_t n, int paf, int E, hipStream_t s) {
     const size_t lds = 8 * (size_t)n + 2 * ((size_t)ART_CMAX * sizeof(ArtRec) + (size_t)E * 12);
@@ -749,11 +936,41 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
         chunkE = use_lds && !(envc && envc[0] == '0') ? art_chunk_entries(n, maxlen, lim) : 0;
         hrowptr.swap(hp);
     }
+    int cartE = 0;
     if (cart) {
         dflag.alloc((size_t)std::max<int64_t>(n, 1));
         dnflag.alloc(4 * (size_t)std::max<int64_t>(n, 1));
         AIR_HIP(hipMemsetAsync(dflag.p, 1, (size_t)n, s));
         AIR_HIP(hipMemsetAsync(dnflag.p, 0, 4 * (size_t)n, s));
+        // r in LDS with the columns staged by the rest of a 1024-thread work-group,
+        // when r and two column buffers fit and no column repeats in the order
+        // (AIR_ART_CHUNK=0 keeps the one-wave form)
+        const char* envc = std::getenv("AIR_ART_CHUNK");
+        std::vector<char> seen((size_t)std::max<int64_t>(n, 1), 0);
+        bool repeats = false;
+        for (int j : I) {
+            repeats = repeats || seen[(size_t)j];
+            seen[(size_t)j] = 1;
+        }
+        if (!(envc && envc[0] == '0') && !repeats && L) {
+            int dev = 0, lim = 0;
+            AIR_HIP(hipGetDevice(&dev));
+            AIR_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            hrowptr.resize((size_t)n + 1);
+            AIR_HIP(hipMemcpyAsync(hrowptr.data(), AT.ptr.p, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+            int64_t maxlen = 0;
+            for (int64_t q = 0; q < n; ++q) maxlen = std::max(maxlen, hrowptr[q + 1] - hrowptr[q]);
+            cartE = cart_chunk_entries(m, maxlen, lim);
+            if (cartE > 0) {
+                cart_plan_chunks(I, hrowptr, cartE, cst, soff);
+                dcst.alloc(4 * cst.size());
+                dsoff.alloc(4 * std::max<size_t>(soff.size(), 1));
+                AIR_HIP(hipMemcpyAsync(dcst.p, cst.data(), 4 * cst.size(), hipMemcpyHostToDevice, s));
+                AIR_HIP(hipMemcpyAsync(dsoff.p, soff.data(), 4 * soff.size(), hipMemcpyHostToDevice, s));
+                AIR_HIP(hipStreamSynchronize(s));
+            }
+        }
     }
     auto residual = [&]() {
         dispatch_dtype(A.dtype, [&](auto v) {
@@ -870,10 +1087,22 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
             CartParams cp{lam, o->THR, k, o->Kbegin, nunflag, o->seed};
             dispatch_dtype(AT.dtype, [&](auto v) {
                 using V = decltype(v);
-                if (L) k_cart_sweep<V><<<1, 64, 0, s>>>(AT.ptr.as<int64_t>(), AT.idx.as<int>(), AT.val.as<V>(),
-                                                        dsteps.as<int>(), (int64_t)L, dnrm.as<double>(),
-                                                        dx.as<double>(), dr.as<double>(), dflag.as<unsigned char>(),
-                                                        dnflag.as<int>(), bd, n, cp);
+                if (L && cartE > 0) {
+                    const size_t lds = 8 * (size_t)m + 2 * ((size_t)CART_CMAX * sizeof(CartRec) + (size_t)cartE * 12);
+                    if (lds > 64 * 1024)
+                        AIR_HIP(hipFuncSetAttribute((const void*)k_cart_chunked<V>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    k_cart_chunked<V><<<1, ART_BLOCK, lds, s>>>(dsteps.as<int>(), dcst.as<int>(), (int)cst.size() - 1,
+                                                                dsoff.as<int>(), AT.ptr.as<int64_t>(), AT.idx.as<int>(),
+                                                                AT.val.as<V>(), dnrm.as<double>(), dx.as<double>(),
+                                                                dr.as<double>(), dflag.as<unsigned char>(),
+                                                                dnflag.as<int>(), bd, n, m, cp, cartE);
+                } else if (L) {
+                    k_cart_sweep<V><<<1, 64, 0, s>>>(AT.ptr.as<int64_t>(), AT.idx.as<int>(), AT.val.as<V>(),
+                                                     dsteps.as<int>(), (int64_t)L, dnrm.as<double>(), dx.as<double>(),
+                                                     dr.as<double>(), dflag.as<unsigned char>(), dnflag.as<int>(), bd,
+                                                     n, cp);
+                }
                 return 0;
             });
         } else {

@@ -119,6 +119,24 @@ def test_cart_matches_oracle_fp64(oracle, prob, name):
         assert rel(X[:, q], Xo[:, q]) <= 1e-10, (q, rel(X[:, q], Xo[:, q]))
 
 
+@pytest.mark.parametrize("form", ["one_wave", "chunked"])
+@pytest.mark.parametrize("name", ["columnaction_default", "flagging", "bounds_custom_order", "repeated_columns"])
+def test_cart_one_wave_and_chunked_forms(oracle, prob, monkeypatch, form, name):
+    """CART with the residual in global memory (one wave) or in LDS with the
+    columns staged chunk by chunk; an order that repeats columns always runs
+    the one-wave form.  Same steps, so the same iterates."""
+    monkeypatch.setenv("AIR_ART_CHUNK", "1" if form == "chunked" else "0")
+    if name == "repeated_columns":
+        meth, K, opts = list(range(576, 0, -1)) + [5, 300, 5, 17], [4], {"relaxpar": 0.9, "lbound": 0.0}
+    else:
+        meth, K, opts = CART_CASES[name]
+    X, info = air.cart(meth, prob["A64"], prob["b"], K, None, opts)
+    Xo, io = oracle.cart(meth, prob["Ao"], prob["b"], K, None, opts)
+    assert info["finaliter"] == io["finaliter"] and info["stoprule"] == io["stoprule"]
+    for q in range(X.shape[1]):
+        assert rel(X[:, q], Xo[:, q]) <= 1e-10, (q, rel(X[:, q], Xo[:, q]))
+
+
 def test_art_info_and_errors(prob):
     X, info = air.symkaczmarz(prob["A32"], prob["b"], [2, 6])
     assert info["finaliter"] == 6 and list(info["itersaved"]) == [2, 6] and X.shape[1] == 2
