@@ -14,21 +14,44 @@ from __future__ import annotations
 import numpy as np
 
 from .operators import as_operator
+from .art import art, cart
 from .sirt import sirt
 
 _SIRT = ("landweber", "cimmino", "cav", "drop", "sart")
 
 
+ROW_ACTION = ("kaczmarz", "symkaczmarz", "randkaczmarz", "columnaction")
+
+
 def _method_name(method):
     name = method if isinstance(method, str) else getattr(method, "__name__", "")
-    if name not in _SIRT:
+    if name not in _SIRT and name not in ROW_ACTION:
         raise ValueError(f"Unknown method {name}")
     return name
 
 
+def _run(name, A, b, K, x0, opts):
+    """[X, info] of the named SIRT, ART or CART method."""
+    if name == "columnaction":
+        return cart(name, A, b, K, x0, opts)
+    if name in ROW_ACTION:
+        return art(name, A, b, K, x0, opts)
+    X, info, _ = sirt(name, A, b, K, x0, opts)
+    return X, info
+
+
 def train_relaxpar(A, b, x_ex, method, kmax, options=None):
-    """relaxpar = train_relaxpar(A,b,x_ex,method,kmax,options)."""
+    """relaxpar = train_relaxpar(A,b,x_ex,method,kmax,options), train_relaxpar.m:43-271.
+
+    SIRT methods start from the default relaxpar with relaxparmax = 2/rho; the
+    ART methods and columnaction (:90-96) start from relaxpar 0.25 with
+    relaxparmax = 2, and their golden section search tests the upper probe
+    first (:213-268)."""
     name = _method_name(method)
+    is_sirt = name not in ROW_ACTION
+
+    def run(K_, x0_, opts_):
+        return _run(name, A, b, K_, x0_, opts_)
     A = as_operator(A)
     x_ex = np.ravel(np.asarray(x_ex, np.float64))
     if options is not None:
@@ -50,10 +73,15 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None):
             k = kmax
         else:
             k = k + stepk
-        X, info, _ = sirt(name, A, b, K, x0, opts)
+        if relaxparmax is None and not is_sirt:
+            opts["relaxpar"] = 0.25
+        X, info = run(K, x0, opts)
         if relaxparmax is None:
-            opts["rho"] = info["rho"]
-            relaxparmax = 2.0 / info["rho"]
+            if is_sirt:
+                opts["rho"] = info["rho"]
+                relaxparmax = 2.0 / info["rho"]
+            else:
+                relaxparmax = 2.0
         Er.extend(np.sqrt(np.sum((X - x_ex[:, None]) ** 2, axis=0)).tolist())
         kopt = int(np.argmin(Er)) + 1
         minE = Er[kopt - 1]
@@ -66,7 +94,7 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None):
     def probe(lam):
         o2 = dict(opts)
         o2["relaxpar"] = lam
-        X, _, _ = sirt(name, A, b, np.arange(1, kopt + 1), None, o2)
+        X, _ = run(np.arange(1, kopt + 1), None, o2)
         ee = np.sqrt(np.sum((X - x_ex[:, None]) ** 2, axis=0))
         kk = int(np.sum((ee[:-1] > meub) & (np.diff(ee) < 0))) + 1
         return kk, float(ee[kk - 1])
@@ -77,8 +105,12 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None):
     betap = alpha + (1 - r) * (beta - alpha)
     k_a, e_a = probe(alphap)
     k_b, e_b = probe(betap)
-    while abs(alphap - betap) > relaxparmax * 0.01:          # :160-229 (SIRT order)
-        if e_a > meub or (not e_b > meub and k_a >= k_b):
+    while abs(alphap - betap) > relaxparmax * 0.01:
+        if is_sirt:
+            right = e_a > meub or (not e_b > meub and k_a >= k_b)     # :160-211
+        else:
+            right = not e_b > meub and (e_a > meub or k_a >= k_b)     # :213-268
+        if right:
             z = alphap + (1 - r) * abs(beta - alphap)
             alpha, alphap, betap = alphap, betap, z
             k_a, e_a = k_b, e_b
@@ -125,7 +157,7 @@ def train_dpme(A, b_exact, x_exact, method, type, delta, s, kmax, options=None, 
         K = np.arange(1, k + 1)
         Ek = []
         while True:
-            Xny, _, _ = sirt(name, A, bi, K, x0, opts)
+            Xny, _ = _run(name, A, bi, K, x0, opts)
             Ek.extend(np.sqrt(np.sum((Xny - x_exact[:, None]) ** 2, axis=0)).tolist())
             cols.extend(Xny.T)
             I = int(np.argmin(Ek)) + 1

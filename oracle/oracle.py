@@ -440,8 +440,7 @@ def train_dpme(A, b_exact, x_exact, method, type, delta, s, kmax, options=None, 
     opts)`` defaults to this oracle's ``sirt``; ``resid(x, b)`` is b - A x."""
     if run is None:
         def run(A_, b_, K_, x0_, opts_):
-            X, info, _ = sirt(method, A_, b_, K_, x0_, opts_)
-            return X, info
+            return _run_method(method, A_, b_, K_, x0_, opts_)
     if not str(type)[:2].upper() in ("DP", "ME"):
         raise ValueError("The chosen type is invalid")
     opts = {} if options is None else dict(options)
@@ -699,13 +698,30 @@ def fbp(A, b, theta, filter="ram-lak"):
     return A.T @ P.ravel(order="F") * np.pi / (2 * na)
 
 
+ROW_ACTION = ("kaczmarz", "symkaczmarz", "randkaczmarz", "columnaction")
+
+
+def _run_method(method, A, b, K, x0, opts):
+    """[X, info] of the named SIRT, ART or CART method of this oracle."""
+    if method == "columnaction":
+        X, info = cart(method, A, b, K, x0, opts)
+        return X, info
+    if method in ROW_ACTION:
+        return art(method, A, b, K, x0, opts)
+    X, info, _ = sirt(method, A, b, K, x0, opts)
+    return X, info
+
+
 def train_relaxpar(A, b, x_ex, method, kmax, options=None, run=None):
-    """train/train_relaxpar.m:43-271, SIRT branch.  ``run(A,b,K,x0,opts)``
-    defaults to this oracle's ``sirt`` with the named method."""
+    """train/train_relaxpar.m:43-271.  SIRT methods start from the default
+    relaxpar with relaxparmax = 2/rho; the ART methods and columnaction
+    (:90-96) start from relaxpar 0.25 with relaxparmax = 2, and their golden
+    section search tests the upper probe first (:213-268).  ``run(A,b,K,x0,
+    opts)`` defaults to this oracle's sirt/art/cart with the named method."""
+    is_sirt = method not in ROW_ACTION
     if run is None:
         def run(A_, b_, K_, x0_, opts_):
-            X, info, _ = sirt(method, A_, b_, K_, x0_, opts_)
-            return X, info
+            return _run_method(method, A_, b_, K_, x0_, opts_)
     opts = {} if options is None else dict(options)
     if options is not None:
         opts["stoprule"] = {"type": "none"}
@@ -722,10 +738,15 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None, run=None):
             k = kmax
         else:
             k = k + stepk
+        if k <= stepk and not is_sirt:
+            opts["relaxpar"] = 0.25
         Xnew, info = run(A, b, K, x0, opts)
         if k <= stepk:
-            opts["rho"] = info["rho"]
-            relaxparmax = 2.0 / info["rho"]
+            if is_sirt:
+                opts["rho"] = info["rho"]
+                relaxparmax = 2.0 / info["rho"]
+            else:
+                relaxparmax = 2.0
         Er.extend(np.sqrt(np.sum((Xnew - x_ex[:, None]) ** 2, axis=0)).tolist())
         kopt = int(np.argmin(Er)) + 1
         minE = Er[kopt - 1]
@@ -749,7 +770,11 @@ def train_relaxpar(A, b, x_ex, method, kmax, options=None, run=None):
     k_a, e_a = probe(alphap)
     k_b, e_b = probe(betap)
     while abs(alphap - betap) > relaxparmax * 0.01:
-        if e_a > meub or (not e_b > meub and k_a >= k_b):
+        if is_sirt:
+            right = e_a > meub or (not e_b > meub and k_a >= k_b)     # :160-211
+        else:
+            right = not e_b > meub and (e_a > meub or k_a >= k_b)     # :213-268
+        if right:
             z = alphap + (1 - r) * abs(beta - alphap)
             alpha, alphap, betap = alphap, betap, z
             k_a, e_a = k_b, e_b

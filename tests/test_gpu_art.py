@@ -170,3 +170,23 @@ def test_art_ncp_stop_is_where_the_statistic_first_rises(oracle, prob):
     Af = air.paralleltomo(N, THETA, P, None, 0, 0)
     with pytest.raises(Exception, match="stored matrix"):
         air.kaczmarz(Af, prob["b"], 1)
+
+
+@pytest.mark.parametrize("meth", ["kaczmarz", "columnaction"])
+def test_train_relaxpar_row_action_matches_oracle(oracle, prob, meth):
+    """train_relaxpar.m's ART/CART branch (relaxpar 0.25 first, relaxparmax 2,
+    upper probe tested first) on the engine's sweeps and on the oracle's."""
+    print(f"train_relaxpar {meth}", flush=True)
+    lam = air.train_relaxpar(prob["A64"], prob["b"], prob["x"], meth, 30)
+    lam_o = oracle.train_relaxpar(prob["Ao"], prob["b"], prob["x"], meth, 30)
+    assert 0 < lam < 2 and abs(lam - lam_o) <= 1e-6, (lam, lam_o)
+
+
+def test_train_dpme_with_kaczmarz_matches_oracle(oracle, prob):
+    b0 = prob["Ao"] @ prob["x"]
+    E = np.random.default_rng(9).standard_normal((b0.size, 2))
+    delta = 0.02 * np.linalg.norm(b0)
+    for kind in ("DP", "ME"):
+        tau = air.train_dpme(prob["A64"], b0, prob["x"], "kaczmarz", kind, delta, 2, 20, noise=E)
+        tau_o = oracle.train_dpme(prob["Ao"], b0, prob["x"], "kaczmarz", kind, delta, 2, 20, noise=E)
+        assert abs(tau - tau_o) <= 1e-8 * abs(tau_o), (kind, tau, tau_o)

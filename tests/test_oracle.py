@@ -668,6 +668,30 @@ def test_train_relaxpar_on_synthetic_error_model(oracle):
     assert 0.9 * 2 / rho <= lam < 2 / rho
 
 
+def test_train_relaxpar_row_action_branch(oracle):
+    # ART/CART branch (train_relaxpar.m:90-96,213-268): the first runs use
+    # relaxpar 0.25, relaxparmax = 2, and the search tests the upper probe
+    # first.  Error floor 1 + 10 (lam - 0.5)^2, reached at k* = ceil(10/lam):
+    # meub = 1.02 * floor(0.25) admits lam in about [0.24, 0.76]; both first
+    # probes (0.76, 1.24) lie above it, so the search moves left and settles at
+    # the largest admissible lam (fewest iterations), near 0.76.
+    state = {"k_end": 0}
+
+    def run(A, b, K, x0, opts):
+        lam = opts["relaxpar"]
+        k0 = 0 if x0 is None else state["k_end"]
+        ks = k0 + np.asarray(K)
+        state["k_end"] = int(ks[-1])
+        kstar = math.ceil(10 / lam)
+        floor = 1 + 10 * (lam - 0.5) ** 2
+        err = floor + 0.5 * np.maximum(kstar - ks, 0) ** 2 + 0.001 * np.maximum(ks - kstar, 0)
+        return err[None, :], {}
+
+    lam = oracle.train_relaxpar(None, None, np.zeros(1), "kaczmarz", 100, None, run=run)
+    hi = 0.5 + math.sqrt((1.02 * (1 + 10 * 0.25 ** 2) - 1) / 10)
+    assert hi - 0.03 <= lam <= hi + 0.02, (lam, hi)
+
+
 def test_train_dpme_on_prescribed_iterates(oracle):
     # A = I and iterates x_k = x* + (k - 7)^2/100 * u (u a unit vector): the error
     # is smallest at k = 7, found in the first chunk of 10.  X keeps the start
