@@ -152,6 +152,8 @@ struct Geometry {
     DevBuf ray_ptr;       // int64 [nrays + 1]
     DevBuf ray_items;     // int32 [nitems]: item positions of each ray
     DevBuf partial;       // float [nitems]: forward partial sums
+    DevBuf raux;          // per-ray 1/a, 1/b, |d|, max(|a|,|b|) for the DDA walk
+    bool mf_exact = false; // crossing walk (entries equal to the tracer's bit for bit) instead of the DDA
     int splits = 1;       // work-groups per tile
     DevBuf gpart;         // double [splits * N * N]: adjoint slices when splits > 1
     // model 1: Fresnel-zone rows of seismicwavetomo (nrays = rows)

@@ -21,6 +21,7 @@
 // fx and reversing each run of equal fx when fy increases yields ascending
 // columns without a sort.  A separate check pass verifies the order.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -586,7 +587,8 @@ __host__ __device__ inline int window_steps(const RayParam& r, int N, const Tile
 }
 
 // seg_value restricted to a tile: the segment's length and its slot in the
-// tile (column-major, T rows per column), or -1 when its pixel (column
+// tile (column-major, columns padded to T+1 slots so that one row of the tile
+// spreads over the LDS banks), or -1 when its pixel (column
 // floor(xm) N + N-1-floor(ym)) lies outside the tile or outside the image
 template <int T>
 __device__ __forceinline__ double tile_seg(int N, const TileWin& w, double xu, double yu, double xv, double yv,
@@ -597,8 +599,79 @@ __device__ __forceinline__ double tile_seg(int N, const TileWin& w, double xu, d
     const double ym = 0.5 * (yu + yv) + half;
     const int fx = (int)floor(xm), iy = N - 1 - (int)floor(ym);
     const int c = fx - w.fx0, rr = iy - w.iy0;
-    *slot = ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T && fx < N && iy < N) ? c * T + rr : -1;
+    *slot = ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T && fx < N && iy < N) ? c * (T + 1) + rr : -1;
     return sqrt(dx * dx + dy * dy);
+}
+
+// The DDA form of the tile walk: t steps from grid line to grid line with
+// t = (line - x0) * (1/a) (no division), each segment's length is
+// (t_b - t_a) |d| (no square root) and its cell comes from the midpoint
+// (floor, the tracer's rule), so a segment falls in the same cell as the
+// tracer's.  Segments with |dx| and |dy| <= 1e-10 are dropped as the tracer
+// merges such points; the lengths then agree with the tracer's to rounding
+// (about 1e-15 relative), not bit for bit.  seg(len, slot) gets every kept
+// segment inside the tile.
+struct RayAux {
+    double ia, ib;      // 1/a, 1/b (0 when the component is 0)
+    double dn, mab;     // |d|, max(|a|, |b|)
+};
+
+template <int T, class Seg>
+__device__ __forceinline__ void walk_tile_dda(const RayParam& r, const RayAux& q, int N, const TileWin& w, Seg&& seg) {
+    const double half = (double)N / 2.0, lo = -half;
+    const int fxe = min(w.fx0 + T, N), iye = min(w.iy0 + T, N);
+    const double X0 = lo + (double)w.fx0, X1 = lo + (double)fxe;
+    const double Y0 = lo + (double)(N - iye), Y1 = lo + (double)(N - w.iy0);
+    const bool hx = (r.a > 0.0) || (r.a < 0.0), hy = (r.b > 0.0) || (r.b < 0.0);
+    double tin = -INFINITY, tout = INFINITY;
+    if (hx) {
+        const double t1 = (X0 - r.x0) * q.ia, t2 = (X1 - r.x0) * q.ia;
+        tin = fmax(tin, fmin(t1, t2));
+        tout = fmin(tout, fmax(t1, t2));
+    } else if (!(r.x0 >= X0 && r.x0 <= X1)) {
+        return;
+    }
+    if (hy) {
+        const double t1 = (Y0 - r.y0) * q.ib, t2 = (Y1 - r.y0) * q.ib;
+        tin = fmax(tin, fmin(t1, t2));
+        tout = fmin(tout, fmax(t1, t2));
+    } else if (!(r.y0 >= Y0 && r.y0 <= Y1)) {
+        return;
+    }
+    if (!(tin < tout)) return;
+    // the first line of each family after the entry point
+    int kx = 0, dkx = 0, ky = 0, dky = 0;
+    double txn = INFINITY, tyn = INFINITY;
+    if (hx) {
+        dkx = r.a > 0.0 ? 1 : -1;
+        const double xi = (r.x0 + r.a * tin) - lo;
+        kx = r.a > 0.0 ? (int)floor(xi) + 1 : (int)ceil(xi) - 1;
+        txn = ((lo + (double)kx) - r.x0) * q.ia;
+        if (!(txn > tin)) { kx += dkx; txn = ((lo + (double)kx) - r.x0) * q.ia; }
+    }
+    if (hy) {
+        dky = r.b > 0.0 ? 1 : -1;
+        const double yi = (r.y0 + r.b * tin) - lo;
+        ky = r.b > 0.0 ? (int)floor(yi) + 1 : (int)ceil(yi) - 1;
+        tyn = ((lo + (double)ky) - r.y0) * q.ib;
+        if (!(tyn > tin)) { ky += dky; tyn = ((lo + (double)ky) - r.y0) * q.ib; }
+    }
+    double ta = tin;
+    for (int guard = 0; guard < 2 * T + 8 && ta < tout; ++guard) {     // at most 2T+2 lines cross a tile
+        const double tb = fmin(fmin(txn, tyn), tout);
+        const double dt = tb - ta;
+        if (dt * q.mab > 1e-10) {
+            const double tm = 0.5 * (ta + tb);
+            const int fx = (int)floor((r.x0 + r.a * tm) + half);
+            const int iy = N - 1 - (int)floor((r.y0 + r.b * tm) + half);
+            const int c = fx - w.fx0, rr = iy - w.iy0;
+            if ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T && fx < N && iy < N)
+                seg(dt * q.dn, c * (T + 1) + rr);
+        }
+        if (txn <= tb) { kx += dkx; txn = ((lo + (double)kx) - r.x0) * q.ia; }
+        if (tyn <= tb) { ky += dky; tyn = ((lo + (double)ky) - r.y0) * q.ib; }
+        ta = tb;
+    }
 }
 
 // A tile's items are sorted by their step count (longest first) and dealt
@@ -616,33 +689,40 @@ __device__ __forceinline__ void tile_chunk(const int64_t* __restrict__ tile_ptr,
 constexpr int MFT = 64;          // tile side in pixels
 constexpr int MFT_BLOCK = 512;   // threads per work-group (6 waves per SIMD at <= 85 VGPRs)
 
-__global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_forward(const RayParam* __restrict__ rays, int N, int nt,
+template <bool EXACT>
+__global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_forward(const RayParam* __restrict__ rays,
+                                                             const RayAux* __restrict__ aux, int N, int nt,
                                                              int splits, const int64_t* __restrict__ tile_ptr,
                                                              const int* __restrict__ tile_rays,
                                                              const float* __restrict__ u,
                                                              float* __restrict__ partial) {
-    __shared__ float us[MFT * MFT];
+    __shared__ float us[MFT * (MFT + 1)];
     int tile;
     int64_t q0, q1;
     tile_chunk(tile_ptr, splits, &tile, &q0, &q1);
     const TileWin w = tile_window(tile, nt, MFT, N);
     for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) {
         const int fx = w.fx0 + i / MFT, iy = w.iy0 + i % MFT;
-        us[i] = (fx < N && iy < N) ? u[(int64_t)fx * N + iy] : 0.f;
+        us[(i / MFT) * (MFT + 1) + i % MFT] = (fx < N && iy < N) ? u[(int64_t)fx * N + iy] : 0.f;
     }
     __syncthreads();
     const int64_t stride = (int64_t)splits * blockDim.x;
     for (int64_t q = q0 + (int64_t)splits * threadIdx.x; q < q1; q += stride) {
-        const RayParam r = rays[tile_rays[q]];
+        const int ray = tile_rays[q];
+        const RayParam r = rays[ray];
         float acc = 0.f;
-        walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb,
-                        [&](bool emit, double xu, double yu, double xv, double yv) {
-            int k;
-            const double val = tile_seg<MFT>(N, w, xu, yu, xv, yv, &k);
-            const bool use = emit && k >= 0;
-            const float xv32 = us[use ? k : 0];
-            acc += use ? (float)val * xv32 : 0.f;
-        });
+        if (EXACT) {
+            walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb,
+                            [&](bool emit, double xu, double yu, double xv, double yv) {
+                int k;
+                const double val = tile_seg<MFT>(N, w, xu, yu, xv, yv, &k);
+                const bool use = emit && k >= 0;
+                const float xv32 = us[use ? k : 0];
+                acc += use ? (float)val * xv32 : 0.f;
+            });
+        } else {
+            walk_tile_dda<MFT>(r, aux[ray], N, w, [&](double len, int k) { acc += (float)len * us[k]; });
+        }
         partial[q] = acc;
     }
 }
@@ -666,18 +746,20 @@ __global__ __launch_bounds__(256) void k_mft_raysum(const int64_t* __restrict__ 
 
 // Adjoint of one chunk into the tile in LDS (FP64); the chunk's tile goes to
 // slice (b % splits) of `out` (n values per slice), summed over the slices in
-// order afterwards, or straight to g when the tile is not split.
+// order afterwards, or straight to g when the tile is not split.  It keeps the
+// crossing walk: its cost is bound by the LDS atomic adds (one per segment),
+// which a shorter walk step does not reduce.
 __global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_adjoint(const RayParam* __restrict__ rays, int N, int nt,
                                                              int splits, const int64_t* __restrict__ tile_ptr,
                                                              const int* __restrict__ tile_rays,
                                                              const float* __restrict__ v,
                                                              double* __restrict__ out) {
-    __shared__ double gs[MFT * MFT];
+    __shared__ double gs[MFT * (MFT + 1)];
     int tile;
     int64_t q0, q1;
     tile_chunk(tile_ptr, splits, &tile, &q0, &q1);
     const TileWin w = tile_window(tile, nt, MFT, N);
-    for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) gs[i] = 0.0;
+    for (int i = threadIdx.x; i < MFT * (MFT + 1); i += blockDim.x) gs[i] = 0.0;
     __syncthreads();
     const int64_t stride = (int64_t)splits * blockDim.x;
     for (int64_t q = q0 + (int64_t)splits * threadIdx.x; q < q1; q += stride) {
@@ -696,7 +778,7 @@ __global__ __launch_bounds__(MFT_BLOCK, 6) void k_mft_adjoint(const RayParam* __
     double* dst = out + (int64_t)(blockIdx.x % (unsigned)splits) * ((int64_t)N * N);
     for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) {
         const int fx = w.fx0 + i / MFT, iy = w.iy0 + i % MFT;
-        if (fx < N && iy < N) dst[(int64_t)fx * N + iy] = gs[i];
+        if (fx < N && iy < N) dst[(int64_t)fx * N + iy] = gs[(i / MFT) * (MFT + 1) + i % MFT];
     }
 }
 
@@ -833,6 +915,22 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
         d.alloc(bytes);
         if (bytes) AIR_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
     };
+    // per-ray reciprocals, |d| and max(|a|, |b|) for the DDA walk; AIR_MF_EXACT=1
+    // selects the crossing walk whose entries equal the tracer's bit for bit
+    std::vector<RayAux> aux((size_t)std::max<int64_t>(g.nrays, 1));
+    for (int64_t i = 0; i < g.nrays; ++i) {
+        const RayParam& r = g.host_rays[(size_t)i];
+        RayAux& q = aux[(size_t)i];
+        q.ia = (r.a > 0.0 || r.a < 0.0) ? 1.0 / r.a : 0.0;
+        q.ib = (r.b > 0.0 || r.b < 0.0) ? 1.0 / r.b : 0.0;
+        q.dn = std::sqrt(r.a * r.a + r.b * r.b);
+        q.mab = std::max(std::fabs(r.a), std::fabs(r.b));
+    }
+    put(g.raux, aux.data(), sizeof(RayAux) * aux.size());
+    {
+        const char* ex = std::getenv("AIR_MF_EXACT");
+        g.mf_exact = ex && ex[0] == '1';
+    }
     put(g.tile_ptr, tptr.data(), 8 * tptr.size());
     put(g.ray_ptr, rptr.data(), 8 * rptr.size());
     put(g.tile_rays, trays.data(), 4 * trays.size());
@@ -854,9 +952,9 @@ void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
     if (g.nrays == 0) return;
     AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
     const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
-    k_mft_forward<<<nblk, MFT_BLOCK, 0, s>>>(g.rays.as<RayParam>(), g.N, g.ntiles_side, g.splits,
-                                             g.tile_ptr.as<int64_t>(), g.tile_rays.as<int>(), u,
-                                             g.partial.as<float>());
+    auto kf = g.mf_exact ? k_mft_forward<true> : k_mft_forward<false>;
+    kf<<<nblk, MFT_BLOCK, 0, s>>>(g.rays.as<RayParam>(), g.raux.as<RayAux>(), g.N, g.ntiles_side, g.splits,
+                                  g.tile_ptr.as<int64_t>(), g.tile_rays.as<int>(), u, g.partial.as<float>());
     AIR_HIP(hipGetLastError());
     const unsigned grid = (unsigned)std::min<int64_t>((g.nrays * 8 + 255) / 256, 65536);
     k_mft_raysum<<<grid, 256, 0, s>>>(g.ray_ptr.as<int64_t>(), g.ray_items.as<int>(), g.partial.as<float>(),
