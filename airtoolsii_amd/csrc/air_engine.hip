@@ -931,6 +931,7 @@ int air_build_paralleltomo(air_engine* e, int32_t N, const double* theta, int32_
         (void)dtype_size(dtype);
         host_rays_paralleltomo(N, theta, nA, p, d, e->geo.host_rays);
         build_geometry(e, N, (int64_t)nA * p, 1, matrix_free, dtype);
+        if (matrix_free) mf_prepare_parallel(e->geo, nA, p, d, e->stream);
     });
 }
 

@@ -156,6 +156,11 @@ struct Geometry {
     bool mf_exact = false; // crossing walk (entries equal to the tracer's bit for bit) instead of the DDA
     int splits = 1;       // work-groups per tile
     DevBuf gpart;         // double [splits * N * N]: adjoint slices when splits > 1
+    // parallel beams: pixel-driven adjoint (pnA > 0 when prepared)
+    int pnA = 0, pp = 0, papc = 0, pchunks = 0;
+    DevBuf pang;          // per angle: direction, reciprocals, |d|, max(|a|,|b|)
+    DevBuf pdet;          // double [p]: detector positions
+    DevBuf ppart;         // double [pchunks * N * N]: per-chunk sums
     // model 1: Fresnel-zone rows of seismicwavetomo (nrays = rows)
     int model = 0;        // 0: line rays, 1: Fresnel zones
     DevBuf frows;         // per-row source/receiver geometry
@@ -164,7 +169,9 @@ struct Geometry {
 };
 
 // ---------------------------------------------------------------- tracer (air_tracer.hip)
+void parallel_detector(int p, double d, std::vector<double>& x0);
 void host_rays_paralleltomo(int N, const double* theta, int nA, int p, double d, std::vector<RayParam>& out);
+void mf_prepare_parallel(Geometry& g, int nA, int p, double d, hipStream_t s);
 void host_rays_fanlineartomo(int N, const double* theta, int nA, int p, double R, double dw, double sd,
                              std::vector<RayParam>& out);
 void host_rays_fancurvedtomo(int N, const double* theta, int nA, int p, double R, double d,

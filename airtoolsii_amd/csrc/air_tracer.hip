@@ -56,9 +56,8 @@ static void sincosd(double deg, double* s, double* c) {
 
 // paralleltomo.m:118-122,195-200: x0 = linspace(-d/2,d/2,p), y0 = 0, rotated by
 // theta; direction (a,b) = (-sind, cosd).
-void host_rays_paralleltomo(int N, const double* theta, int nA, int p, double d, std::vector<RayParam>& out) {
-    (void)N;
-    std::vector<double> x0(p);
+void parallel_detector(int p, double d, std::vector<double>& x0) {
+    x0.assign((size_t)p, 0.0);
     double d1 = -d / 2.0, d2 = d / 2.0;
     if (p == 1) x0[0] = d2;
     else {
@@ -67,6 +66,12 @@ void host_rays_paralleltomo(int N, const double* theta, int nA, int p, double d,
         x0[0] = d1;
         x0[p - 1] = d2;
     }
+}
+
+void host_rays_paralleltomo(int N, const double* theta, int nA, int p, double d, std::vector<RayParam>& out) {
+    (void)N;
+    std::vector<double> x0;
+    parallel_detector(p, d, x0);
     out.resize((size_t)nA * p);
     for (int i = 0; i < nA; ++i) {
         double s, c;
@@ -616,6 +621,15 @@ struct RayAux {
     double dn, mab;     // |d|, max(|a|, |b|)
 };
 
+static RayAux ray_aux(const RayParam& r) {
+    RayAux q;
+    q.ia = (r.a > 0.0 || r.a < 0.0) ? 1.0 / r.a : 0.0;
+    q.ib = (r.b > 0.0 || r.b < 0.0) ? 1.0 / r.b : 0.0;
+    q.dn = std::sqrt(r.a * r.a + r.b * r.b);
+    q.mab = std::max(std::fabs(r.a), std::fabs(r.b));
+    return q;
+}
+
 template <int T, class Seg>
 __device__ __forceinline__ void walk_tile_dda(const RayParam& r, const RayAux& q, int N, const TileWin& w, Seg&& seg) {
     const double half = (double)N / 2.0, lo = -half;
@@ -791,6 +805,104 @@ __global__ __launch_bounds__(256) void k_mft_sum_slices(const double* __restrict
     }
 }
 
+// Pixel-driven transpose of the DDA forward for parallel beams.  Ray j of
+// angle i is (c x_j, s x_j) + t (-s, c) (x_j the detector position), so it can
+// meet a pixel only when |x_j - u| <= (|c| + |s|) / 2, u = c x + s y at the
+// pixel's centre.  A thread owns one pixel and a chunk of angles; for each
+// candidate ray it clips the pixel's box with the DDA's own crossing
+// expressions ((line - x0) (1/a), (line - y0) (1/b)), so the entry and exit
+// parameters are the DDA's segment ends bit for bit, and keeps the segment
+// when it passes the same 1e-10 test and its midpoint cell (the same floor
+// rule, as the range test fx <= z < fx + 1) is this pixel.  Each pixel sums
+// its entries (rounded to FP32, as the stored matrix is) times v in FP64 in
+// angle order: no atomics, and the result does not depend on scheduling.
+// The candidate range is widened by 1e-6 detector units, far above the
+// rounding of u and of the positions.
+struct AngleAux {
+    double a, b, ia, ib, dn, mab;   // direction (-s, c), its reciprocals, |d|, max(|a|, |b|)
+    double ci, si, k0, k1;          // candidate j in [ceil(ci x + si y + k0), floor(ci x + si y + k1)]
+};
+
+__global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict__ ang,
+                                                     const double* __restrict__ det, int N, int nA, int p,
+                                                     int apc, int edge_check, const float* __restrict__ v,
+                                                     double* __restrict__ out) {
+    extern __shared__ double xs[];
+    for (int j = threadIdx.x; j < p; j += blockDim.x) xs[j] = det[j];
+    __syncthreads();
+    const int64_t n = (int64_t)N * N;
+    const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n) return;
+    const int fx = (int)(pix / N), iy = (int)(pix - (int64_t)fx * N);
+    const double half = (double)N / 2.0, lo = -half;
+    const double Xa = lo + (double)fx, Xb = lo + (double)(fx + 1);
+    const double Ya = lo + (double)(N - 1 - iy), Yb = lo + (double)(N - iy);
+    const double cx = Xa + 0.5, cy = Ya + 0.5;
+    // the midpoint cell test floor(z) == k as k <= z < k + 1
+    const double mx0 = (double)fx, mx1 = (double)(fx + 1);
+    const double my0 = (double)(N - 1 - iy), my1 = (double)(N - iy);
+    const int i0 = (int)blockIdx.y * apc, i1 = min(nA, i0 + apc);
+    double acc = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const AngleAux q = ang[i];
+        const double c = q.b, s = -q.a;
+        const bool hx = (q.a > 0.0) || (q.a < 0.0), hy = (q.b > 0.0) || (q.b < 0.0);
+        const double u = cx * q.ci + cy * q.si;
+        const int j0 = max((int)ceil(u + q.k0), 0);
+        const int j1 = min((int)floor(u + q.k1), p - 1);
+        const float* vi = v + (int64_t)i * p;
+        if (hx && hy) {
+            // (X - x0) (1/a) is monotone in X, so min/max of the two box
+            // lines are the lines taken in the order of the sign of 1/a
+            const double XI = q.ia > 0.0 ? Xa : Xb, XO = q.ia > 0.0 ? Xb : Xa;
+            const double YI = q.ib > 0.0 ? Ya : Yb, YO = q.ib > 0.0 ? Yb : Ya;
+            for (int j = j0; j <= j1; ++j) {
+                const double xj = xs[j];
+                // the ray's start as the ray table holds it: c x_j - s 0 and s x_j + c 0
+                const double rx0 = c * xj, ry0 = s * xj;
+                const double tin = fmax((XI - rx0) * q.ia, (YI - ry0) * q.ib);
+                const double tout = fmin((XO - rx0) * q.ia, (YO - ry0) * q.ib);
+                if (!(tin < tout)) continue;
+                const double dt = tout - tin;
+                if (!(dt * q.mab > 1e-10)) continue;
+                const double tm = 0.5 * (tin + tout);
+                const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
+                if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
+                acc += (double)(float)(dt * q.dn) * (double)vi[j];
+            }
+            continue;
+        }
+        for (int j = j0; j <= j1; ++j) {
+            const double xj = xs[j];
+            const double rx0 = c * xj, ry0 = s * xj;
+            if (edge_check && ((!hy && fabs(ry0 - half) < 1e-15) || (!hx && fabs(rx0 - half) < 1e-15))) continue;
+            double tin = -INFINITY, tout = INFINITY;
+            if (hx) {
+                const double t1 = (Xa - rx0) * q.ia, t2 = (Xb - rx0) * q.ia;
+                tin = fmax(tin, fmin(t1, t2));
+                tout = fmin(tout, fmax(t1, t2));
+            } else if (!(rx0 >= Xa && rx0 <= Xb)) {
+                continue;
+            }
+            if (hy) {
+                const double t1 = (Ya - ry0) * q.ib, t2 = (Yb - ry0) * q.ib;
+                tin = fmax(tin, fmin(t1, t2));
+                tout = fmin(tout, fmax(t1, t2));
+            } else if (!(ry0 >= Ya && ry0 <= Yb)) {
+                continue;
+            }
+            if (!(tin < tout)) continue;
+            const double dt = tout - tin;
+            if (!(dt * q.mab > 1e-10)) continue;
+            const double tm = 0.5 * (tin + tout);
+            const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
+            if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
+            acc += (double)(float)(dt * q.dn) * (double)vi[j];
+        }
+    }
+    out[(int64_t)blockIdx.y * n + pix] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_mf_row_stats(const RayParam* __restrict__ rays, int64_t nrays, int N,
                                                       int edge_check, const double* __restrict__ w,
                                                       double* __restrict__ sq, double* __restrict__ ab,
@@ -918,14 +1030,7 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
     // per-ray reciprocals, |d| and max(|a|, |b|) for the DDA walk; AIR_MF_EXACT=1
     // selects the crossing walk whose entries equal the tracer's bit for bit
     std::vector<RayAux> aux((size_t)std::max<int64_t>(g.nrays, 1));
-    for (int64_t i = 0; i < g.nrays; ++i) {
-        const RayParam& r = g.host_rays[(size_t)i];
-        RayAux& q = aux[(size_t)i];
-        q.ia = (r.a > 0.0 || r.a < 0.0) ? 1.0 / r.a : 0.0;
-        q.ib = (r.b > 0.0 || r.b < 0.0) ? 1.0 / r.b : 0.0;
-        q.dn = std::sqrt(r.a * r.a + r.b * r.b);
-        q.mab = std::max(std::fabs(r.a), std::fabs(r.b));
-    }
+    for (int64_t i = 0; i < g.nrays; ++i) aux[(size_t)i] = ray_aux(g.host_rays[(size_t)i]);
     put(g.raux, aux.data(), sizeof(RayAux) * aux.size());
     {
         const char* ex = std::getenv("AIR_MF_EXACT");
@@ -947,6 +1052,45 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
     AIR_HIP(hipStreamSynchronize(s));
 }
 
+// Tables for the pixel-driven adjoint of a parallel-beam geometry: per angle
+// the direction and the DDA's reciprocals (the ray table's own values), the
+// detector positions, and the angle chunks (enough threads for about 16 waves
+// per SIMD).  AIR_MF_PIXEL=0 keeps the tile adjoint.
+void mf_prepare_parallel(Geometry& g, int nA, int p, double d, hipStream_t s) {
+    g.pnA = 0;
+    const char* env = std::getenv("AIR_MF_PIXEL");
+    if (env && env[0] == '0') return;
+    if (g.mf_exact || nA < 1 || p < 2 || p > 8192 || !(d > 0.0) || g.nrays != (int64_t)nA * p) return;
+    std::vector<double> x0;
+    parallel_detector(p, d, x0);
+    std::vector<AngleAux> ang((size_t)nA);
+    for (int i = 0; i < nA; ++i) {
+        const RayParam& r = g.host_rays[(size_t)i * p];
+        const RayAux q = ray_aux(r);
+        const double c = r.b, sn = -r.a, idx = (double)(p - 1) / (x0[(size_t)p - 1] - x0[0]);
+        const double h = 0.5 * (std::fabs(c) + std::fabs(sn)) + 1e-6;
+        ang[(size_t)i] = AngleAux{r.a, r.b, q.ia, q.ib, q.dn, q.mab, c * idx, sn * idx, (-h - x0[0]) * idx,
+                                  (h - x0[0]) * idx};
+    }
+    g.pang.alloc(sizeof(AngleAux) * ang.size());
+    g.pdet.alloc(8 * x0.size());
+    AIR_HIP(hipMemcpyAsync(g.pang.p, ang.data(), sizeof(AngleAux) * ang.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(g.pdet.p, x0.data(), 8 * x0.size(), hipMemcpyHostToDevice, s));
+    const int64_t n = (int64_t)g.N * g.N;
+    const int64_t waves = (n + 63) / 64;
+    int dev = 0, cus = 0;
+    AIR_HIP(hipGetDevice(&dev));
+    AIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t want = 16 * 4 * (int64_t)std::max(cus, 1);
+    const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>({(want + waves - 1) / waves, 16, nA}));
+    g.papc = (nA + chunks - 1) / chunks;
+    g.pchunks = (nA + g.papc - 1) / g.papc;
+    if (g.pchunks > 1) g.ppart.alloc(8 * (size_t)g.pchunks * (size_t)n);
+    AIR_HIP(hipStreamSynchronize(s));
+    g.pnA = nA;
+    g.pp = p;
+}
+
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_forward(g, u, y, s); return; }
     if (g.nrays == 0) return;
@@ -966,6 +1110,20 @@ void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_adjoint(g, v, gacc, s); return; }
     if (g.nrays == 0) return;
+    if (g.pnA > 0) {
+        const int64_t n = (int64_t)g.N * g.N;
+        double* out = g.pchunks > 1 ? g.ppart.as<double>() : gacc;
+        dim3 grid((unsigned)((n + 255) / 256), (unsigned)g.pchunks);
+        k_mfp_adjoint<<<grid, 256, 8 * (size_t)g.pp, s>>>(g.pang.as<AngleAux>(), g.pdet.as<double>(), g.N, g.pnA,
+                                                          g.pp, g.papc, g.edge_check, v, out);
+        AIR_HIP(hipGetLastError());
+        if (g.pchunks > 1) {
+            k_mft_sum_slices<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
+                g.ppart.as<double>(), g.pchunks, n, gacc);
+            AIR_HIP(hipGetLastError());
+        }
+        return;
+    }
     AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
     const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
     double* out = g.splits > 1 ? g.gpart.as<double>() : gacc;

@@ -148,7 +148,7 @@ def test_matrix_free_products_match_matrix(oracle, kind):
     assert abs((Af @ x) @ y - x @ (Af.T @ y)) <= 1e-5 * abs(fx @ y)
 
 
-@pytest.mark.parametrize("walk", ["dda", "exact"])
+@pytest.mark.parametrize("walk", ["dda", "dda_tile_adjoint", "exact"])
 @pytest.mark.parametrize("kind,name", CASES)
 def test_matrix_free_tiles_reproduce_every_entry(oracle, monkeypatch, kind, name, walk):
     """The tile-major projector walks each ray once per image tile it passes
@@ -156,14 +156,17 @@ def test_matrix_free_tiles_reproduce_every_entry(oracle, monkeypatch, kind, name
     come out exactly once.  The crossing walk (AIR_MF_EXACT=1) forms the
     tracer's values bit for bit; the DDA walk forms them to rounding and drops
     segments below the tracer's 1e-10 merge tolerance, so its bar adds
-    1e-9 max|y| (a segment of that size is the only place the two rules can
-    differ; the adjoint always uses the crossing walk).  The adjoint sums in
-    FP64, so each pixel agrees with A32' y (A's values rounded to FP32) to the
-    FP32 rounding of the result; a lost or doubled entry would be off by the
-    size of an entry.  The forward sums in
+    1e-9 max|x| (a segment of that size is the only place the two rules can
+    differ).  For parallel beams the DDA's adjoint is pixel-driven (each pixel
+    clips its candidate rays with the DDA's expressions), so its bar adds
+    1e-9 max|y| likewise; AIR_MF_PIXEL=0 keeps the tile adjoint with the
+    crossing walk.  The adjoint sums in FP64, so each pixel agrees with A32' y
+    (A's values rounded to FP32) to the FP32 rounding of the result; a lost or
+    doubled entry would be off by the size of an entry.  The forward sums in
     FP32 in another order than a row of A."""
     print(f"matrix-free tiles {kind} {name} {walk}", flush=True)
     monkeypatch.setenv("AIR_MF_EXACT", "1" if walk == "exact" else "0")
+    monkeypatch.setenv("AIR_MF_PIXEL", "0" if walk == "dda_tile_adjoint" else "1")
     Af, Ao = build_pair(oracle, kind, TABLES[kind][name], "f32", is_matrix=0)
     assert Af.matrix_free and Af.shape == Ao.shape
     A32 = Ao.copy()
@@ -175,7 +178,7 @@ def test_matrix_free_tiles_reproduce_every_entry(oracle, monkeypatch, kind, name
     x32 = x.astype(np.float32).astype(np.float64)
     y32 = y.astype(np.float32).astype(np.float64)
     floor_f = 1e-30 if walk == "exact" else 1e-9 * np.abs(x32).max()
-    floor_t = 1e-30
+    floor_t = 1e-9 * np.abs(y32).max() if walk == "dda" else 1e-30
     rel_t = 2e-7
     err_f = np.abs(Af @ x - A32 @ x32)
     assert np.all(err_f <= 1e-5 * (Aabs @ np.abs(x32)) + floor_f), err_f.max()
