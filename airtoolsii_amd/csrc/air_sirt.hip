@@ -768,7 +768,8 @@ void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k,
 
 void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s) {
     (void)k;
-    k_zero_f64<<<vec_grid(n), BLOCK, 0, s>>>(B.g, n);
+    // every projector form writes each pixel of g; only an empty one leaves it
+    if (g.nrays == 0) k_zero_f64<<<vec_grid(n), BLOCK, 0, s>>>(B.g, n);
     mf_adjoint(g, B.Mr, B.g, s);
     if (line) {
         unsigned grid = std::min<unsigned>(vec_grid(n), 2048);

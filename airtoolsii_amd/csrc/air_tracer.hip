@@ -857,6 +857,7 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
             const double XI = q.ia > 0.0 ? Xa : Xb, XO = q.ia > 0.0 ? Xb : Xa;
             const double YI = q.ib > 0.0 ? Ya : Yb, YO = q.ib > 0.0 ? Yb : Ya;
             for (int j = j0; j <= j1; ++j) {
+                const float vj = vi[j];     // issued first: its latency overlaps the clip
                 const double xj = xs[j];
                 // the ray's start as the ray table holds it: c x_j - s 0 and s x_j + c 0
                 const double rx0 = c * xj, ry0 = s * xj;
@@ -868,7 +869,7 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
                 const double tm = 0.5 * (tin + tout);
                 const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
                 if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
-                acc += (double)(float)(dt * q.dn) * (double)vi[j];
+                acc += (double)(float)(dt * q.dn) * (double)vj;
             }
             continue;
         }
