@@ -670,11 +670,26 @@ __device__ __forceinline__ void walk_tile_dda(const RayParam& r, const RayAux& q
         tyn = ((lo + (double)ky) - r.y0) * q.ib;
         if (!(tyn > tin)) { ky += dky; tyn = ((lo + (double)ky) - r.y0) * q.ib; }
     }
+    // A segment between consecutive crossings lies in the cell bounded by the
+    // lines crossed so far: fx = kx - 1 (kx when moving left), fy likewise.
+    // Its midpoint is at least half its extent along each axis, dt |a| / 2 and
+    // dt |b| / 2, from every edge of that cell; with dt min(|a|,|b|) > 1e-9
+    // that is 5e-10, far above the rounding of the crossing parameters (about
+    // 1e-13), so the floor of the midpoint names that cell and the integer
+    // cell is used.  Shorter segments (including the first one when the entry
+    // lies within rounding of a line) take the midpoint rule.
+    const double mib = fmin(fabs(r.a), fabs(r.b));
     double ta = tin;
     for (int guard = 0; guard < 2 * T + 8 && ta < tout; ++guard) {     // at most 2T+2 lines cross a tile
         const double tb = fmin(fmin(txn, tyn), tout);
         const double dt = tb - ta;
-        if (dt * q.mab > 1e-10) {
+        if (dt * mib > 1e-9) {
+            const int fx = dkx > 0 ? kx - 1 : kx;
+            const int iy = N - 1 - (dky > 0 ? ky - 1 : ky);
+            const int c = fx - w.fx0, rr = iy - w.iy0;
+            if ((unsigned)c < (unsigned)T && (unsigned)rr < (unsigned)T && fx < N && iy < N)
+                seg(dt * q.dn, c * (T + 1) + rr);
+        } else if (dt * q.mab > 1e-10) {
             const double tm = 0.5 * (ta + tb);
             const int fx = (int)floor((r.x0 + r.a * tm) + half);
             const int iy = N - 1 - (int)floor((r.y0 + r.b * tm) + half);
@@ -820,6 +835,7 @@ __global__ __launch_bounds__(256) void k_mft_sum_slices(const double* __restrict
 // rounding of u and of the positions.
 struct AngleAux {
     double a, b, ia, ib, dn, mab;   // direction (-s, c), its reciprocals, |d|, max(|a|, |b|)
+    double mib;                     // min(|a|, |b|)
     double ci, si, k0, k1;          // candidate j in [ceil(ci x + si y + k0), floor(ci x + si y + k1)]
 };
 
@@ -863,12 +879,19 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
                 const double rx0 = c * xj, ry0 = s * xj;
                 const double tin = fmax((XI - rx0) * q.ia, (YI - ry0) * q.ib);
                 const double tout = fmin((XO - rx0) * q.ia, (YO - ry0) * q.ib);
-                if (!(tin < tout)) continue;
                 const double dt = tout - tin;
-                if (!(dt * q.mab > 1e-10)) continue;
-                const double tm = 0.5 * (tin + tout);
-                const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
-                if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
+                // The chord's ends lie on the box, so its midpoint is at least half
+                // its extent along each axis, dt |a| / 2 and dt |b| / 2, from every
+                // edge.  With dt min(|a|,|b|) > 1e-9 that is 5e-10, far above the
+                // rounding of the DDA's midpoint (about 1e-13 here), so its cell is
+                // this pixel and the 1e-10 test passes; shorter chords take both
+                // tests as the DDA does (dt <= 0 fails the first).
+                if (!(dt * q.mib > 1e-9)) {
+                    if (!(dt * q.mab > 1e-10)) continue;
+                    const double tm = 0.5 * (tin + tout);
+                    const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
+                    if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
+                }
                 acc += (double)(float)(dt * q.dn) * (double)vj;
             }
             continue;
@@ -1061,7 +1084,7 @@ void mf_prepare_parallel(Geometry& g, int nA, int p, double d, hipStream_t s) {
     g.pnA = 0;
     const char* env = std::getenv("AIR_MF_PIXEL");
     if (env && env[0] == '0') return;
-    if (g.mf_exact || nA < 1 || p < 2 || p > 8192 || !(d > 0.0) || g.nrays != (int64_t)nA * p) return;
+    if (g.mf_exact || nA < 1 || p < 2 || p > 6144 || !(d > 0.0) || g.nrays != (int64_t)nA * p) return;
     std::vector<double> x0;
     parallel_detector(p, d, x0);
     std::vector<AngleAux> ang((size_t)nA);
@@ -1070,8 +1093,9 @@ void mf_prepare_parallel(Geometry& g, int nA, int p, double d, hipStream_t s) {
         const RayAux q = ray_aux(r);
         const double c = r.b, sn = -r.a, idx = (double)(p - 1) / (x0[(size_t)p - 1] - x0[0]);
         const double h = 0.5 * (std::fabs(c) + std::fabs(sn)) + 1e-6;
-        ang[(size_t)i] = AngleAux{r.a, r.b, q.ia, q.ib, q.dn, q.mab, c * idx, sn * idx, (-h - x0[0]) * idx,
-                                  (h - x0[0]) * idx};
+        ang[(size_t)i] = AngleAux{r.a,      r.b,       q.ia,                  q.ib,
+                                  q.dn,     q.mab,     std::min(std::fabs(r.a), std::fabs(r.b)),
+                                  c * idx,  sn * idx,  (-h - x0[0]) * idx,    (h - x0[0]) * idx};
     }
     g.pang.alloc(sizeof(AngleAux) * ang.size());
     g.pdet.alloc(8 * x0.size());
