@@ -54,3 +54,28 @@ def test_c2_matrix_free_iterates_match_oracle(oracle, c2):
     err = rel(X[:, -1], Xo[:, -1])
     print(f"c2 parity: matrix-free cimmino rel err {err:.3e}", flush=True)
     assert err <= TOL
+
+
+def test_c2_matrix_free_products_match_matrix(c2):
+    """The matrix-free pair at full size: the DDA forward against A32 x (A's
+    values rounded to FP32) row by row, the pixel-driven adjoint against A32' y
+    pixel by pixel, and the adjoint identity <A x, y> = <x, A' y>, which holds
+    to rounding because both products use the same entries.  The floors of
+    1e-9 max|x| and 1e-9 max|y| cover segments below the 1e-10 merge
+    tolerance, where the DDA and the tracer may differ."""
+    Ao = c2["Ao"]
+    Af = air.paralleltomo(N, THETA, P, None, 0, 0)
+    A32 = Ao.copy()
+    A32.data = A32.data.astype(np.float32).astype(np.float64)
+    Aabs = abs(Ao)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0, 1, Ao.shape[1]).astype(np.float32).astype(np.float64)
+    y = rng.uniform(-1, 1, Ao.shape[0]).astype(np.float32).astype(np.float64)
+    print("c2 parity: matrix-free products", flush=True)
+    fx, ty = Af @ x, Af.T @ y
+    assert np.all(np.abs(fx - A32 @ x) <= 1e-5 * (Aabs @ np.abs(x)) + 1e-9 * np.abs(x).max())
+    assert np.all(np.abs(ty - A32.T @ y) <= 2e-7 * (Aabs.T @ np.abs(y)) + 1e-9 * np.abs(y).max())
+    scale = float((Aabs @ np.abs(x)) @ np.abs(y))
+    gap = abs(float(fx @ y) - float(x @ ty))
+    print(f"c2 parity: adjoint identity gap {gap / scale:.3e}", flush=True)
+    assert gap <= 1e-6 * scale
