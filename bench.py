@@ -387,7 +387,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine runs one problem per GPU and no multi-GPU path is "
-              "built (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "built (independent problems or right-hand sides would go one per GPU); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
