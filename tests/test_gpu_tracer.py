@@ -24,6 +24,9 @@ PAR_CASES = {
     "tiny_N1": dict(N=1, theta=np.array([0.0, 30.0, 90.0]), p=3, d=1.0),
     "c2_angle_subset": dict(N=512, theta=np.arange(0.0, 180.0, 0.5)[::24], p=725, d=None),
     "N100_partial_tiles": dict(N=100, theta=np.arange(0.0, 180.0, 7.0), p=141, d=None),
+    "near_axis_angles": dict(N=32, theta=np.array([1e-3, 0.5, 44.9999999, 45.0, 89.999, 90.0005, 134.99999,
+                                                   179.9999, 270.00001, 359.999]), p=47, d=None),
+    "two_rays": dict(N=12, theta=np.arange(0.0, 180.0, 10.0), p=2, d=3.0),
 }
 FAN_CASES = {
     "fan_N64_even_p": dict(N=64, theta=np.arange(0.0, 360.0, 2.0), p=90),
