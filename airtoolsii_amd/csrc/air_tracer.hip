@@ -773,6 +773,75 @@ __global__ __launch_bounds__(256) void k_mft_raysum(const int64_t* __restrict__ 
     }
 }
 
+// One-time row statistics with the forward's tile walk: per (ray, tile) item
+// the FP64 sums of a^2, |a| and a^2 w_j over the item's entries (a rounded
+// to FP32, as the stored matrix holds it), then per ray the sum of its items
+// in item order.  Entries are the forward's own (the DDA, or the crossing
+// walk with AIR_MF_EXACT=1), so the weights belong to the operator applied.
+template <bool EXACT>
+__global__ __launch_bounds__(MFT_BLOCK, 4) void k_mft_rowstats(const RayParam* __restrict__ rays,
+                                                              const RayAux* __restrict__ aux, int N, int nt,
+                                                              int splits, const int64_t* __restrict__ tile_ptr,
+                                                              const int* __restrict__ tile_rays,
+                                                              const double* __restrict__ colw,
+                                                              double* __restrict__ part, int64_t nitems) {
+    __shared__ double ws[MFT * (MFT + 1)];
+    int tile;
+    int64_t q0, q1;
+    tile_chunk(tile_ptr, splits, &tile, &q0, &q1);
+    const TileWin w = tile_window(tile, nt, MFT, N);
+    for (int i = threadIdx.x; i < MFT * MFT; i += blockDim.x) {
+        const int fx = w.fx0 + i / MFT, iy = w.iy0 + i % MFT;
+        ws[(i / MFT) * (MFT + 1) + i % MFT] = (colw && fx < N && iy < N) ? colw[(int64_t)fx * N + iy] : 0.0;
+    }
+    __syncthreads();
+    const int64_t stride = (int64_t)splits * blockDim.x;
+    for (int64_t q = q0 + (int64_t)splits * threadIdx.x; q < q1; q += stride) {
+        const int ray = tile_rays[q];
+        const RayParam r = rays[ray];
+        double s2 = 0.0, s1 = 0.0, s3 = 0.0;
+        auto add = [&](double len, int k) {
+            const double a = (double)(float)len;
+            s2 += a * a;
+            s1 += fabs(a);
+            s3 += (a * a) * ws[k];
+        };
+        if (EXACT) {
+            walk_window_fwd(r, N, w.kxa, w.kxb, w.kya, w.kyb,
+                            [&](bool emit, double xu, double yu, double xv, double yv) {
+                int k;
+                const double val = tile_seg<MFT>(N, w, xu, yu, xv, yv, &k);
+                if (emit && k >= 0) add(val, k);
+            });
+        } else {
+            walk_tile_dda<MFT>(r, aux[ray], N, w, add);
+        }
+        part[q] = s2;
+        part[nitems + q] = s1;
+        part[2 * nitems + q] = s3;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mft_raysum3(const int64_t* __restrict__ ray_ptr,
+                                                     const int* __restrict__ ray_items,
+                                                     const double* __restrict__ part, int64_t nitems,
+                                                     int64_t nrays, double* __restrict__ sq,
+                                                     double* __restrict__ ab, double* __restrict__ sqw) {
+    for (int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ray < nrays;
+         ray += (int64_t)gridDim.x * blockDim.x) {
+        double s2 = 0.0, s1 = 0.0, s3 = 0.0;
+        for (int64_t q = ray_ptr[ray]; q < ray_ptr[ray + 1]; ++q) {
+            const int it = ray_items[q];
+            s2 += part[it];
+            s1 += part[nitems + it];
+            s3 += part[2 * nitems + it];
+        }
+        if (sq) sq[ray] = s2;
+        if (ab) ab[ray] = s1;
+        if (sqw) sqw[ray] = s3;
+    }
+}
+
 // Adjoint of one chunk into the tile in LDS (FP64); the chunk's tile goes to
 // slice (b % splits) of `out` (n values per slice), summed over the slices in
 // order afterwards, or straight to g when the tile is not split.  It keeps the
@@ -1170,6 +1239,21 @@ void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) 
 void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_row_stats(g, colw, sq, ab, sqw, s); return; }
     if (g.nrays == 0) return;
+    if (g.T == MFT && g.splits >= 1 && g.nitems > 0) {
+        DevBuf part;
+        part.alloc(3 * 8 * (size_t)g.nitems);
+        const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
+        auto kr = g.mf_exact ? k_mft_rowstats<true> : k_mft_rowstats<false>;
+        kr<<<nblk, MFT_BLOCK, 0, s>>>(g.rays.as<RayParam>(), g.raux.as<RayAux>(), g.N, g.ntiles_side, g.splits,
+                                      g.tile_ptr.as<int64_t>(), g.tile_rays.as<int>(), colw, part.as<double>(),
+                                      g.nitems);
+        AIR_HIP(hipGetLastError());
+        k_mft_raysum3<<<(unsigned)std::min<int64_t>((g.nrays + 255) / 256, 65536), 256, 0, s>>>(
+            g.ray_ptr.as<int64_t>(), g.ray_items.as<int>(), part.as<double>(), g.nitems, g.nrays, sq, ab, sqw);
+        AIR_HIP(hipGetLastError());
+        AIR_HIP(hipStreamSynchronize(s));   // part is freed on return
+        return;
+    }
     k_mf_row_stats<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
                                                                      g.edge_check, colw, sq, ab, sqw);
     AIR_HIP(hipGetLastError());

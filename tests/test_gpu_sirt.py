@@ -141,6 +141,20 @@ def test_weights_match_oracle(c1, oracle, method):
     np.testing.assert_allclose(D, Do, rtol=2e-6)
 
 
+@pytest.mark.parametrize("walk", ["dda", "exact"])
+@pytest.mark.parametrize("method", ["cimmino", "cav", "drop", "sart"])
+def test_matrix_free_weights_match_oracle(c1, oracle, monkeypatch, method, walk):
+    """Weights of the matrix-free operator: row sums from the tile walk (FP64
+    sums of the FP32-rounded entries), column sums and counts from the ray
+    walk, against the oracle's FP64 weights of the stored matrix."""
+    monkeypatch.setenv("AIR_MF_EXACT", "1" if walk == "exact" else "0")
+    Mo, Do = oracle.weights(method, c1["Ao"])
+    Af = air.paralleltomo(c1["N"], c1["theta"], c1["p"], None, 0, 0)
+    M, D = air.weights(method, Af)
+    np.testing.assert_allclose(M, Mo, rtol=2e-6)
+    np.testing.assert_allclose(D, Do, rtol=2e-6)
+
+
 @pytest.mark.parametrize("method", ["landweber", "cimmino", "drop"])
 def test_spectral_radius(c1, method):
     rho = air.spectral_radius(method, c1["A"])
