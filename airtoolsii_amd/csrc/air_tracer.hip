@@ -908,10 +908,13 @@ struct AngleAux {
     double ci, si, k0, k1;          // candidate j in [ceil(ci x + si y + k0), floor(ci x + si y + k1)]
 };
 
+// STATS: the one-time column statistics instead (v taken as 1): out gets the
+// column's sum of |a| and cnt its number of entries.
+template <bool STATS>
 __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict__ ang,
                                                      const double* __restrict__ det, int N, int nA, int p,
                                                      int apc, int edge_check, const float* __restrict__ v,
-                                                     double* __restrict__ out) {
+                                                     double* __restrict__ out, int* __restrict__ cnt) {
     extern __shared__ double xs[];
     for (int j = threadIdx.x; j < p; j += blockDim.x) xs[j] = det[j];
     __syncthreads();
@@ -928,6 +931,7 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
     const double my0 = (double)(N - 1 - iy), my1 = (double)(N - iy);
     const int i0 = (int)blockIdx.y * apc, i1 = min(nA, i0 + apc);
     double acc = 0.0;
+    int nc = 0;
     for (int i = i0; i < i1; ++i) {
         const AngleAux q = ang[i];
         const double c = q.b, s = -q.a;
@@ -935,14 +939,14 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
         const double u = cx * q.ci + cy * q.si;
         const int j0 = max((int)ceil(u + q.k0), 0);
         const int j1 = min((int)floor(u + q.k1), p - 1);
-        const float* vi = v + (int64_t)i * p;
+        const float* vi = STATS ? nullptr : v + (int64_t)i * p;
         if (hx && hy) {
             // (X - x0) (1/a) is monotone in X, so min/max of the two box
             // lines are the lines taken in the order of the sign of 1/a
             const double XI = q.ia > 0.0 ? Xa : Xb, XO = q.ia > 0.0 ? Xb : Xa;
             const double YI = q.ib > 0.0 ? Ya : Yb, YO = q.ib > 0.0 ? Yb : Ya;
             for (int j = j0; j <= j1; ++j) {
-                const float vj = vi[j];     // issued first: its latency overlaps the clip
+                const float vj = STATS ? 1.f : vi[j];     // issued first: its latency overlaps the clip
                 const double xj = xs[j];
                 // the ray's start as the ray table holds it: c x_j - s 0 and s x_j + c 0
                 const double rx0 = c * xj, ry0 = s * xj;
@@ -962,6 +966,7 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
                     if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
                 }
                 acc += (double)(float)(dt * q.dn) * (double)vj;
+                if (STATS) ++nc;
             }
             continue;
         }
@@ -990,10 +995,12 @@ __global__ __launch_bounds__(256) void k_mfp_adjoint(const AngleAux* __restrict_
             const double tm = 0.5 * (tin + tout);
             const double zx = (rx0 + q.a * tm) + half, zy = (ry0 + q.b * tm) + half;
             if (!(zx >= mx0 && zx < mx1 && zy >= my0 && zy < my1)) continue;
-            acc += (double)(float)(dt * q.dn) * (double)vi[j];
+            acc += (double)(float)(dt * q.dn) * (STATS ? 1.0 : (double)vi[j]);
+            if (STATS) ++nc;
         }
     }
     out[(int64_t)blockIdx.y * n + pix] = acc;
+    if (STATS) cnt[(int64_t)blockIdx.y * n + pix] = nc;
 }
 
 __global__ __launch_bounds__(256) void k_mf_row_stats(const RayParam* __restrict__ rays, int64_t nrays, int N,
@@ -1212,8 +1219,8 @@ void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) 
         const int64_t n = (int64_t)g.N * g.N;
         double* out = g.pchunks > 1 ? g.ppart.as<double>() : gacc;
         dim3 grid((unsigned)((n + 255) / 256), (unsigned)g.pchunks);
-        k_mfp_adjoint<<<grid, 256, 8 * (size_t)g.pp, s>>>(g.pang.as<AngleAux>(), g.pdet.as<double>(), g.N, g.pnA,
-                                                          g.pp, g.papc, g.edge_check, v, out);
+        k_mfp_adjoint<false><<<grid, 256, 8 * (size_t)g.pp, s>>>(g.pang.as<AngleAux>(), g.pdet.as<double>(), g.N,
+                                                                 g.pnA, g.pp, g.papc, g.edge_check, v, out, nullptr);
         AIR_HIP(hipGetLastError());
         if (g.pchunks > 1) {
             k_mft_sum_slices<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
@@ -1261,6 +1268,14 @@ void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab,
 void mf_col_stats(const Geometry& g, double* colabs, int* colcnt, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_col_stats(g, colabs, colcnt, s); return; }
     if (g.nrays == 0) return;
+    if (g.pnA > 0 && colabs && colcnt) {   // parallel beams: a thread per pixel, all angles, no atomics
+        const int64_t n = (int64_t)g.N * g.N;
+        k_mfp_adjoint<true><<<dim3((unsigned)((n + 255) / 256), 1), 256, 8 * (size_t)g.pp, s>>>(
+            g.pang.as<AngleAux>(), g.pdet.as<double>(), g.N, g.pnA, g.pp, g.pnA, g.edge_check, nullptr, colabs,
+            colcnt);
+        AIR_HIP(hipGetLastError());
+        return;
+    }
     k_mf_col_stats<<<(unsigned)((g.nrays + 255) / 256), 256, 0, s>>>(g.rays.as<RayParam>(), g.nrays, g.N,
                                                                      g.edge_check, colabs, colcnt);
     AIR_HIP(hipGetLastError());
