@@ -1141,11 +1141,13 @@ void mf_build_tiles(Geometry& g, hipStream_t s) {
     put(g.tile_rays, trays.data(), 4 * trays.size());
     put(g.ray_items, ritems.data(), 4 * ritems.size());
     g.partial.alloc(4 * (size_t)std::max<int64_t>(nitems, 1));
-    // enough work-groups for 3 per CU, each with at least 1024 items
+    // enough work-groups for 4 per CU (AIR_MF_SPLITS_PER_CU=k for k), each with at least 1024 items
     int dev = 0, cus = 0;
     AIR_HIP(hipGetDevice(&dev));
     AIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int64_t by_cu = (3 * (int64_t)std::max(cus, 1) + ntiles - 1) / ntiles;
+    int64_t per_cu = 4;   // paralleltomo N=512, 360 angles, p=725: 3 -> 2249, 4 -> 2303, 6 -> 2279, 8 -> 2218 iter/s
+    if (const char* sp = std::getenv("AIR_MF_SPLITS_PER_CU")) per_cu = std::min(16, std::max(1, std::atoi(sp)));
+    const int64_t by_cu = (per_cu * (int64_t)std::max(cus, 1) + ntiles - 1) / ntiles;
     const int64_t by_items = std::max<int64_t>(1, nitems / (ntiles * 1024));
     g.splits = (int)std::max<int64_t>(1, std::min<int64_t>({by_cu, by_items, 64}));
     if (g.splits > 1) g.gpart.alloc(8 * (size_t)g.splits * (size_t)N * (size_t)N);
