@@ -122,7 +122,9 @@ int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colpt
                        const int64_t* rowind, const double* val, int dtype);
 /* testprobs/paralleltomo.m:1-110 (defaults resolved by the caller: p, d).
  * matrix_free=0: trace on the GPU into CSR (+CSC); 1: keep the geometry
- * only and project ray by ray on every product (paralleltomo.m:94-96). */
+ * only and project on every product (paralleltomo.m:94-96): A x walks each
+ * ray through the image tiles it crosses, A' y clips each pixel's candidate
+ * rays; both use the same entries, equal to the tracer's to FP64 rounding. */
 int air_build_paralleltomo(air_engine* e, int32_t N, const double* theta, int32_t nA,
                            int32_t p, double d, int matrix_free, int dtype);
 /* testprobs/fanlineartomo.m:1-136; R, dw, sd are in units of N as in the
