@@ -150,11 +150,19 @@ def check(rc):
     return rc
 
 
+class _ArrayPtr(C.c_void_p):
+    """A data pointer that keeps its array alive for as long as the pointer
+    object exists, so that ``ptr(np.ascontiguousarray(...))`` may be passed
+    straight into a call even when the conversion made a temporary copy."""
+
+
 def ptr(a):
     """Data pointer of a C-contiguous numpy array (None -> NULL)."""
     if a is None:
         return None
-    return C.c_void_p(a.ctypes.data)
+    p = _ArrayPtr(a.ctypes.data)
+    p._array = a
+    return p
 
 
 def f64(a):

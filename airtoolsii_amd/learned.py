@@ -194,14 +194,26 @@ def evaluate(A, N, seeds, params, lam_plain, kmax, method="sart", tau=1.02, boun
 
 
 def summarize(rows):
+    """Medians over all problems.  A learned run that never reaches the plain
+    method's error counts as an infinite ratio (a failure), so the medians are
+    over every problem, not only the ones that succeeded; the `_reached`
+    figures are the same medians over the problems that did reach it."""
     pe = np.array([r["plain_err"] for r in rows]); le = np.array([r["learned_err"] for r in rows])
     ps = np.array([r["plain_s"] for r in rows]); ls = np.array([r["learned_s"] for r in rows])
     hit = [r for r in rows if r["time_to_plain_error"] is not None]
-    ratio = np.array([r["time_to_plain_error"] / r["plain_s"] for r in hit]) if hit else np.array([np.nan])
-    ih = np.array([r["iters_to_plain_err"] / max(r["plain_iters"], 1) for r in hit]) if hit else np.array([np.nan])
+    ratio = np.array([np.inf if r["time_to_plain_error"] is None else r["time_to_plain_error"] / r["plain_s"]
+                      for r in rows])
+    ih = np.array([np.inf if r["iters_to_plain_err"] is None else r["iters_to_plain_err"] / max(r["plain_iters"], 1)
+                   for r in rows])
+    ok = np.isfinite(ratio)
+    # JSON has no infinity: a median that falls on a failure is reported as null
+    fin = lambda v: float(v) if np.isfinite(v) else None
     return {"problems": len(rows), "plain_err_median": float(np.median(pe)), "learned_err_median": float(np.median(le)),
             "learned_err_le_plain_frac": float(np.mean(le <= pe * 1.0)),
-            "iters_to_plain_err_over_plain_median": float(np.median(ih)),
+            "iters_to_plain_err_over_plain_median": fin(np.median(ih)),
+            "iters_to_plain_err_over_plain_median_reached": fin(np.median(ih[ok])) if ok.any() else None,
+            "time_to_plain_error_over_plain_median_reached": fin(np.median(ratio[ok])) if ok.any() else None,
+            "time_to_plain_error_over_plain_p75": fin(np.percentile(ratio, 75, method="higher")),
             "plain_s_total": float(ps.sum()), "learned_s_total": float(ls.sum()),
             "plain_iters_median": float(np.median([r["plain_iters"] for r in rows])),
             "learned_iters_median": float(np.median([r["learned_iters"] for r in rows])),
@@ -211,7 +223,7 @@ def summarize(rows):
                                                     for r in rows])),
             "learned_iter_s_median": float(np.median([(r["learned_s"] - r["learned_setup_s"]) / max(r["learned_iters"], 1)
                                                       for r in rows])),
-            "time_to_plain_error_over_plain_median": float(np.median(ratio)),
+            "time_to_plain_error_over_plain_median": fin(np.median(ratio)),
             "greedy_iters_to_plain_err_over_plain_median": float(np.median(
                 [r["greedy_iters_to_plain_err"] / max(r["plain_iters"], 1) for r in rows
                  if r.get("greedy_iters_to_plain_err") is not None] or [np.nan]))}

@@ -54,19 +54,25 @@ def smooth(N, p=4):
 def threephases(N, p=100, seed=None):
     """phantomgallery.m:257-292 (rand(p,2) fills column-major)."""
     rs = _rng(seed)
+    # meshgrid(1:N): I(r, c) = c + 1 varies along columns, J(r, c) = r + 1 along rows
+    ax = np.arange(1, N + 1, dtype=np.float64)
+    I, J = ax[None, :], ax[:, None]
+    cube = lambda t: t * t * t
     s1 = 0.025 * N
     c1 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
-    # each blob is separable in the grid coordinates: exp(-f(I) - g(J)) = exp(-f(I)) exp(-g(J))
-    ax = np.arange(1, N + 1, dtype=np.float64)
-    u = np.exp(-np.abs(ax[None, :] - c1[:, :1]) ** 3 / (2.5 * s1) ** 3)   # p x N, along I (columns)
-    v = np.exp(-np.abs(ax[None, :] - c1[:, 1:2]) ** 3 / s1 ** 3)          # p x N, along J (rows)
-    im1 = v.T @ u
+    d1, d2 = cube(2.5 * s1), cube(s1)
+    # one blob at a time, exp(-f - g) on the whole grid, accumulated in blob order
+    im1 = np.zeros((N, N))
+    for i in range(p):
+        im1 = im1 + np.exp(-cube(np.abs(I - c1[i, 0])) / d1 - cube(np.abs(J - c1[i, 1])) / d2)
     im1 = np.where(im1 < 0.35, 0.0, 2.0)
     s2 = 0.025 * N
     c2 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
-    u = np.exp(-((ax[None, :] - c2[:, :1]) ** 2) / (2 * s2) ** 2)
-    v = np.exp(-((ax[None, :] - c2[:, 1:2]) ** 2) / s2 ** 2)
-    im2 = v.T @ u
+    e1, e2 = (2 * s2) * (2 * s2), s2 * s2
+    im2 = np.zeros((N, N))
+    for i in range(p):
+        u, v = I - c2[i, 0], J - c2[i, 1]
+        im2 = im2 + np.exp(-(u * u) / e1 - (v * v) / e2)
     im2 = np.where(im2 < 0.55, 0.0, 1.0)
     im = im1 + im2
     im[im == 3] = 1

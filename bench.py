@@ -169,6 +169,9 @@ def learned_study(args):
     rows = Lm.evaluate(A, N, range(100000, 100000 + args.learn_eval), params, lam, 400)
     t_eval = time.perf_counter() - t0
     summ = Lm.summarize(rows)
+    if args.learn_save:
+        os.makedirs(os.path.dirname(os.path.abspath(args.learn_save)), exist_ok=True)
+        np.save(args.learn_save, np.asarray(params, np.float32), allow_pickle=False)
     out = {"metric": "time to the plain method's error, learned vs plain lambda (median ratio)",
            "value": round(summ["time_to_plain_error_over_plain_median"], 4), "unit": "ratio",
            "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
@@ -181,6 +184,42 @@ def learned_study(args):
            "seconds": {"build": round(t_build, 2), "trajectories_and_training": round(t_train, 1),
                        "evaluate": round(t_eval, 1)}}
     print(json.dumps(out), flush=True)
+
+
+LEARNED_PARAMS = os.path.join(HERE, "airtoolsii_amd", "data", "relaxation_net_sart_n256.npy")
+
+
+def learned_block(args):
+    """The metric's second half in the default run: the committed network
+    (trained on the GPU by `--study learned --learn-save`) against plain SIRT
+    with train_relaxpar's lambda and the discrepancy principle, on held-out
+    seeded problems of the training geometry.  Problems whose learned run never
+    reaches the plain method's error count as failures (infinite ratio)."""
+    import airtoolsii_amd as air
+    from airtoolsii_amd import learned as Lm
+    if not os.path.exists(LEARNED_PARAMS):
+        return {"skipped": "no committed network parameters"}
+    params = np.load(LEARNED_PARAMS, allow_pickle=False).astype(np.float32)
+    N, theta, p = 256, np.arange(0.0, 180.0, 1.0), 362
+    t0 = time.perf_counter()
+    A = air.paralleltomo(N, theta, p)
+    lam = Lm.plain_relaxpar(A, N, 999, 60)
+    seeds = range(100000, 100000 + args.learn_bench)
+    rows = Lm.evaluate(A, N, seeds, params, lam, 400)
+    summ = Lm.summarize(rows)
+    return {"workload": f"paralleltomo N={N}, theta=0:1:179 ({len(theta)} angles), p={p}, SART, box [0,1]",
+            "held_out_problems": len(rows), "seeds": f"{seeds.start}..{seeds.stop - 1}",
+            "phantoms": list(Lm.KINDS), "noise": "1-5% Gaussian, seeded",
+            "plain": f"lambda={lam:.4f} from train_relaxpar, DP with tau=1.02",
+            "learned": "committed network, DP with tau=1.02 as the fallback stop",
+            "time_to_plain_error_over_plain_median": summ["time_to_plain_error_over_plain_median"],
+            "iters_to_plain_error_over_plain_median": summ["iters_to_plain_err_over_plain_median"],
+            "reached_plain_error_frac": summ["reached_plain_err_frac"],
+            "time_ratio_median_reached_only": summ["time_to_plain_error_over_plain_median_reached"],
+            "time_ratio_p75": summ["time_to_plain_error_over_plain_p75"],
+            "plain_err_median": summ["plain_err_median"], "learned_err_median": summ["learned_err_median"],
+            "plain_iters_median": summ["plain_iters_median"], "learned_iters_median": summ["learned_iters_median"],
+            "seconds": round(time.perf_counter() - t0, 1)}
 
 
 def batch_study(args):
@@ -384,6 +423,9 @@ def main():
     ap.add_argument("--learn-K", type=int, default=150)
     ap.add_argument("--learn-epochs", type=int, default=40)
     ap.add_argument("--learn-rounds", type=int, default=3)
+    ap.add_argument("--learn-save", default=None)
+    ap.add_argument("--learn-bench", type=int, default=512)
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine runs one problem per GPU and no multi-GPU path is "
@@ -494,6 +536,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(N, theta, p, b, args.method, info["rho"], args.cpu_seconds, threads)
         out["vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
         out["vs_cpu_one_core"] = round(value / out["cpu_baseline"]["one_core_value"], 1)
+    if not args.no_learned and N == 512:
+        del run
+        out["learned"] = learned_block(args)
     print(json.dumps(out), flush=True)
 
 

@@ -63,3 +63,16 @@ def test_sirt_argument_errors_before_any_device_work(kwargs, msg):
     args.update(kwargs)
     with pytest.raises(ValueError, match=msg):
         sirt(args["method"], args["A"], args["b"], args["K"], args["x0"], args["options"])
+
+
+def test_ptr_keeps_converted_arrays_alive():
+    """ptr(np.ascontiguousarray(a, dtype)) of an array that needs converting
+    points at a temporary copy: the pointer object must keep it alive."""
+    import ctypes as C
+    import gc
+    from airtoolsii_amd import _lib
+    p = _lib.ptr(np.ascontiguousarray(np.arange(1000, dtype=np.int32), np.int64))
+    junk = [np.full(1000, -7, np.int64) for _ in range(50)]
+    gc.collect()
+    got = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int64)), (1000,))
+    assert np.array_equal(got, np.arange(1000)) and len(junk) == 50

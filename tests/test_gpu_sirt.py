@@ -276,3 +276,71 @@ def test_ncp_stopping_rule(c1, oracle, method, smooth, dims):
 def test_ncp_rejects_bad_res_dims(c1):
     with pytest.raises(Exception):
         air.cimmino(c1["A"], c1["b"], 5, None, {"stoprule": {"type": "NCP", "res_dims": [90, 180]}})
+
+
+def test_train_relaxpar_search_exact_on_engine_trajectories(oracle):
+    """train_relaxpar.m:43-271's search on the engine: the oracle's search driven
+    by the engine's own SIRT runs (deterministic: the same call returns the same
+    iterates) must return the product's lambda exactly, for the SART branch
+    (rho = 1) and for Cimmino (lambda range 2/rho from the first run)."""
+    N, theta, p = 32, np.arange(0.0, 180.0, 3.0), 45
+    A = air.paralleltomo(N, theta, p)
+    Ao = oracle.paralleltomo(N, theta, p)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    e = np.random.default_rng(2).standard_normal(b.size)
+    b = b + 0.03 * np.linalg.norm(b) * e / np.linalg.norm(e)
+    for method, opts in (("sart", None), ("cimmino", {"lbound": 0})):
+        lam = air.train_relaxpar(A, b, x, method, 100, opts)
+
+        def run(A_, b_, K_, x0_, o_, method=method):
+            X, info, _ = air.sirt(method, A, b_, K_, x0_, o_)
+            return X, info
+        lam_o = oracle.train_relaxpar(Ao, b, x, method, 100, opts, run=run)
+        print(method, lam, lam_o)
+        assert abs(lam - lam_o) <= 1e-12 * abs(lam_o), (method, lam, lam_o)
+
+
+@pytest.mark.parametrize("method,strategy", [("landweber", None), ("cimmino", None), ("cav", None), ("drop", None),
+                                             ("sart", None), ("sart", "line"), ("cimmino", "psi2")])
+def test_compressed_row_fallback_c1(c1, oracle, monkeypatch, method, strategy):
+    """With the tiled layout switched off (AIR_DISABLE_TILED=1) the SIRT loop runs
+    on the CSR/CSC kernels (k_resid, k_update, the line-search pass)."""
+    monkeypatch.setenv("AIR_DISABLE_TILED", "1")
+    A = air.paralleltomo(c1["N"], c1["theta"], c1["p"])
+    assert A.layout()["tiled"] == 0
+    opts = {"rho": c1["rho"][method], "lbound": 0}
+    if strategy is not None:
+        opts["relaxpar"] = strategy
+    X, info, _ = air.sirt(method, A, c1["b"], [7, 40], None, opts)
+    Xo, info_o, _ = oracle.sirt(method, c1["Ao"], c1["b"], [7, 40], None, opts)
+    for j in range(2):
+        assert rel(X[:, j], Xo[:, j]) <= TOL, (method, strategy, j, rel(X[:, j], Xo[:, j]))
+    # the ME stop rule's partial sums on the same kernels
+    if method == "cimmino" and strategy is None:
+        o2 = dict(opts, stoprule={"type": "ME", "taudelta": 0.0})
+        X2, info2, _ = air.sirt(method, A, c1["b"], 25, None, o2)
+        X2o, info2o, _ = oracle.sirt(method, c1["Ao"], c1["b"], 25, None, o2)
+        assert info2["finaliter"] == info2o["finaliter"] and rel(X2[:, -1], X2o[:, -1]) <= TOL
+
+
+def test_fallback_when_a_tile_exceeds_lds(oracle):
+    """A matrix whose 256-column tiles hold more distinct rows than the
+    backward kernel can stage in LDS: the build declines the tiled layout by
+    itself and SIRT runs on the CSR/CSC kernels."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    m, n = 24000, 512
+    Ao = sp.random(m, n, density=0.02, random_state=rng, format="csr")
+    Ao.data = rng.uniform(0.1, 1.0, Ao.nnz)
+    Ao.sort_indices()
+    A = air.as_operator(Ao)
+    assert A.layout()["tiled"] == 0
+    x = rng.uniform(0, 1, n)
+    b = Ao @ x
+    for method in ("sart", "cimmino"):
+        M, D = oracle.weights(method, Ao)
+        opts = {"rho": 1.0 if method == "sart" else oracle.spectral_radius(Ao, M, D), "lbound": 0}
+        X, _, _ = air.sirt(method, A, b, 30, None, opts)
+        Xo, _, _ = oracle.sirt(method, Ao, b, 30, None, opts)
+        assert rel(X[:, -1], Xo[:, -1]) <= TOL, method

@@ -883,14 +883,151 @@ def test_phantoms_basic_properties(oracle):
 
 def test_product_phantoms_match_oracle(oracle):
     from airtoolsii_amd import phantomgallery
-    for name, args in (("shepplogan", ()), ("smooth", (None,)), ("grains", (None, 1)), ("grains", (30, 7)),
-                       ("threephases", (None, 3))):
-        a = phantomgallery(name, 48, *args)
-        b = oracle.phantomgallery(name, 48, *args)
-        if name == "threephases":
-            assert np.mean(a != b) < 1e-3
-        else:
-            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+    for N in (48, 65):
+        for name, args in (("shepplogan", ()), ("smooth", (None,)), ("grains", (None, 1)), ("grains", (30, 7)),
+                           ("threephases", (None, 3)), ("threephases", (40, 9))):
+            a = phantomgallery(name, N, *args)
+            b = oracle.phantomgallery(name, N, *args)
+            if name == "smooth":
+                np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+            else:
+                assert np.array_equal(a, b), (name, N, args)
+
+
+# Shepp-Logan pixels summed by hand (phantomgallery.m:178-214).  The pixel
+# nearest to a point (x, y) of [-1,1]^2 is column round((x+1)(N-1)/2) and row
+# N-1-round((y+1)(N-1)/2): rot90 puts y = +1 on the top row.  Each point lies
+# well inside (or outside) every ellipse, so one pixel either way does not
+# change its sum; the table gives the ellipses that contain it:
+#   e1  A=1    .69 x .92  at (0, 0)
+#   e2  A=-.8  .6624 x .874  at (0, -.0184)
+#   e3  A=-.2  .11 x .31  at (.22, 0), phi = -18 deg
+#   e4  A=-.2  .16 x .41  at (-.22, 0), phi = 18 deg
+#   e5  A=.1   .21 x .25  at (0, .35)
+#   e6/e7 A=.1 .046 circles at (0, +-.1)
+#   e8  A=.1   .046 x .023 at (-.08, -.605); e9 .023 circle at (0, -.606); e10 .023 x .046 at (.06, -.605)
+# For the tilted ellipses the point sits 0.85 of the semi-major axis out along
+# the tilted axis (direction (-sin phi, cos phi)), which lies outside the
+# ellipse tilted the other way.
+def _sl_points():
+    d3 = np.array([-math.sin(math.radians(-18)), math.cos(math.radians(-18))])
+    d4 = np.array([-math.sin(math.radians(18)), math.cos(math.radians(18))])
+    return [
+        ((0.0, 0.89), 1.0),                     # e1 only (above e2's top at .8556)
+        ((0.0, -0.3), 1.0 - 0.8),               # e1 + e2
+        ((-0.08, -0.605), 1.0 - 0.8 + 0.1),     # e8 (bottom left small ellipse)
+        ((0.0, -0.606), 1.0 - 0.8 + 0.1),       # e9
+        ((0.06, -0.605), 1.0 - 0.8 + 0.1),      # e10
+        ((0.0, 0.45), 1.0 - 0.8 + 0.1),         # e5 (above e6)
+        ((0.0, 0.13), 1.0 - 0.8 + 0.1 + 0.1),   # e5 (y from .1 to .6) and e6
+        ((0.0, -0.1), 1.0 - 0.8 + 0.1),         # e7 (below e5)
+        (tuple(np.array([0.22, 0.0]) + 0.85 * 0.31 * d3), 0.0),    # e3 tip: 1 - .8 - .2 -> clamped to 0
+        (tuple(np.array([-0.22, 0.0]) + 0.85 * 0.41 * d4), 0.0),   # e4 tip
+        ((0.0, -0.95), 0.0),                    # outside the head
+        ((0.7, 0.7), 0.0),
+    ]
+
+
+def test_shepplogan_hand_summed_pixels(oracle):
+    from airtoolsii_amd import phantomgallery
+    N = 511
+    im = oracle.phantomgallery("shepplogan", N)
+    for (x, y), val in _sl_points():
+        c = int(round((x + 1) * (N - 1) / 2))
+        r = N - 1 - int(round((y + 1) * (N - 1) / 2))
+        assert abs(im[r, c] - val) < 1e-12, ((x, y), r, c, im[r, c], val)
+    # the head is not symmetric: an upside-down or mirrored image differs
+    assert not np.array_equal(im, np.flipud(im)) and not np.array_equal(im, np.fliplr(im))
+    assert np.array_equal(phantomgallery("shepplogan", N), im)
+
+
+def test_shepplogan_against_ellipse_membership(oracle):
+    """Every pixel against a direct evaluation of the ten quadratic forms on the
+    pixel's (x, y) (x = -1 + 2c/(N-1), y = 1 - 2r/(N-1)), skipping pixels whose
+    form is within 1e-9 of an ellipse boundary."""
+    N = 96
+    im = oracle.phantomgallery("shepplogan", N)
+    E = [(1, .69, .92, 0, 0, 0), (-.8, .6624, .8740, 0, -.0184, 0), (-.2, .11, .31, .22, 0, -18),
+         (-.2, .16, .41, -.22, 0, 18), (.1, .21, .25, 0, .35, 0), (.1, .046, .046, 0, .1, 0),
+         (.1, .046, .046, 0, -.1, 0), (.1, .046, .023, -.08, -.605, 0), (.1, .023, .023, 0, -.606, 0),
+         (.1, .023, .046, .06, -.605, 0)]
+    c, r = np.meshgrid(np.arange(N), np.arange(N))
+    X = -1 + 2.0 * c / (N - 1)
+    Y = 1 - 2.0 * r / (N - 1)
+    ref = np.zeros((N, N))
+    near = np.zeros((N, N), bool)
+    for A, a, b, x0, y0, deg in E:
+        t = math.radians(deg)
+        R = np.array([[math.cos(t), math.sin(t)], [-math.sin(t), math.cos(t)]])
+        u = R[0, 0] * (X - x0) + R[0, 1] * (Y - y0)
+        v = R[1, 0] * (X - x0) + R[1, 1] * (Y - y0)
+        q = (u / a) ** 2 + (v / b) ** 2
+        ref += A * (q <= 1)
+        near |= np.abs(q - 1) < 1e-9
+    ref = np.maximum(ref, 0)
+    assert near.mean() < 0.01
+    np.testing.assert_allclose(im[~near], ref[~near], atol=1e-12)
+
+
+def test_grains_against_full_grid_voronoi(oracle):
+    """grains, phantomgallery.m:361-389 taken literally: labels of the whole
+    Nbig x Nbig grid (meshgrid(1:Nbig), column-major, first centre on ties),
+    reshaped, cropped to rows and columns dN+(1:N), then scaled."""
+    for N, ng_arg, seed in ((40, None, 1), (64, 20, 5), (37, 11, 2)):
+        ng = int(math.floor(3 * math.sqrt(N) + 0.5)) if ng_arg is None else ng_arg
+        rs = np.random.RandomState(seed)
+        dN = int(math.floor(N / 10 + 0.5))
+        Nbig = N + 2 * dN
+        xG = np.ceil(Nbig * rs.random_sample(ng))
+        yG = np.ceil(Nbig * rs.random_sample(ng))
+        Xg, Yg = np.meshgrid(np.arange(1, Nbig + 1.0), np.arange(1, Nbig + 1.0))
+        Xc, Yc = Xg.ravel(order="F"), Yg.ravel(order="F")
+        dist = (Xc[:, None] - xG[None, :]) ** 2 + (Yc[:, None] - yG[None, :]) ** 2
+        lab = np.argmin(dist, axis=1).reshape((Nbig, Nbig), order="F").astype(np.float64)
+        lab = lab[dN:dN + N, dN:dN + N]
+        ref = lab / lab.max()
+        got = oracle.phantomgallery("grains", N, ng_arg, seed)
+        assert np.array_equal(got, ref), (N, ng, seed)
+        # ties exist at these sizes (integer distances), so first-index matters
+        assert (np.sort(dist, axis=1)[:, 0] == np.sort(dist, axis=1)[:, 1]).any()
+
+
+def _threephases_literal(N, p, seed):
+    """phantomgallery.m:257-292 with MATLAB's array semantics: [I,J] = meshgrid(1:N),
+    c = rand(p,2)*N filled column-major, blobs added one at a time."""
+    rs = np.random.RandomState(seed)
+    I, J = np.meshgrid(np.arange(1, N + 1.0), np.arange(1, N + 1.0))
+    s1 = 0.025 * N
+    c1 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
+    im1 = np.zeros((N, N))
+    for i in range(p):
+        im1 = im1 + np.exp(-np.abs(I - c1[i, 0]) ** 3 / (2.5 * s1) ** 3 - np.abs(J - c1[i, 1]) ** 3 / s1 ** 3)
+    im1 = np.where(im1 < 0.35, 0.0, 2.0)
+    s2 = 0.025 * N
+    c2 = rs.random_sample(2 * p).reshape((p, 2), order="F") * N
+    im2 = np.zeros((N, N))
+    for i in range(p):
+        im2 = im2 + np.exp(-(I - c2[i, 0]) ** 2 / (2 * s2) ** 2 - (J - c2[i, 1]) ** 2 / s2 ** 2)
+    im2 = np.where(im2 < 0.55, 0.0, 1.0)
+    im = im1 + im2
+    im[im == 3] = 1
+    return im / im.max()
+
+
+def test_threephases_and_smooth_against_literal_loops(oracle):
+    for N, p, seed in ((48, 100, 3), (64, 30, 1), (33, 100, 8)):
+        ref = _threephases_literal(N, p, seed)
+        assert np.array_equal(oracle.phantomgallery("threephases", N, p, seed), ref), (N, p, seed)
+        assert len(np.unique(ref)) == 3
+    for N, p in ((40, 4), (31, 2)):
+        I, J = np.meshgrid(np.arange(1, N + 1.0), np.arange(1, N + 1.0))
+        sigma = 0.25 * N
+        c = np.array([[0.6 * N, 0.6 * N], [0.5 * N, 0.3 * N], [0.2 * N, 0.7 * N], [0.8 * N, 0.2 * N]])
+        a = [1, 0.5, 0.7, 0.9]
+        im = np.zeros((N, N))
+        for i in range(p):
+            im = im + a[i] * np.exp(-(I - c[i, 0]) ** 2 / (1.2 * sigma) ** 2 - (J - c[i, 1]) ** 2 / sigma ** 2)
+        np.testing.assert_allclose(oracle.phantomgallery("smooth", N, p), im / im.max(), rtol=1e-14, atol=0)
 
 
 # ------------------------------------------------------------------ NCP statistic
