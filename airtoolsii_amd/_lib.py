@@ -31,7 +31,7 @@ EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "ai
            "air_build_sphericaltomo", "air_build_seismicwavetomo", "air_fbp", "air_art", "air_cart",
            "air_set_graph", "air_graph_launches",
            "air_size", "air_get_csr",
-           "air_apply", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
+           "air_apply", "air_apply_device", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
            "air_layout_info", "air_mlp_forward", "air_mlp_grad", "air_mlp_train", "air_sirt_records",
            "air_build_paralleltomo_batch", "air_batch_info", "air_sirt_multi"]
@@ -116,6 +116,7 @@ def load():
     L.air_size.argtypes = [_P, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]
     L.air_get_csr.argtypes = [_P, _P, _P, _P]
     L.air_apply.argtypes = [_P, C.c_int, _P, _P]
+    L.air_apply_device.argtypes = [_P, C.c_int, _P, _P]
     L.air_weights.argtypes = [_P, C.c_int, _P, _P]
     L.air_spectral_radius.argtypes = [_P, C.c_int, _P, _P, _dp]
     L.air_calczeta.argtypes = [C.c_int32, _P]

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -83,6 +84,30 @@ struct air_engine {
 using namespace air;
 
 namespace {
+
+// Scratch memory of air_apply_device, the entry point that allocates nothing:
+// one fixed array per GPU, held until the process ends.  In host code the
+// name `scratch` is only a stand-in for the symbol; its device address on each
+// GPU comes from hipGetSymbolAddress (taken once per device in air_create, with
+// that device current) and only that address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[AIR_APPLY_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;   // per device; null until the device's first air_create
+
+void init_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    AIR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AIR_CHECK(p != nullptr, AIR_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = p;
+}
+
+void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 template <typename F>
 int guard(F&& f) {
@@ -814,6 +839,7 @@ int air_create(int device, air_engine** out) {
         AIR_HIP(hipGetDeviceProperties(&prop, device));
         e->num_cu = prop.multiProcessorCount;
         AIR_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        init_scratch(device);
         *out = e.release();
     });
 }
@@ -1059,6 +1085,50 @@ int air_apply(air_engine* e, int transp, const double* in, double* out) {
         dout.alloc(4 * std::max<int64_t>(nout, 1));
         apply_dev(e, transp, din.as<float>(), dout.as<float>());
         download_f64(dout.as<float>(), out, nout, s);
+    });
+}
+
+int air_apply_device(air_engine* e, int transp, const float* in, float* out) {
+    return guard([&] {
+        need_operator(e);
+        AIR_CHECK(in && out, AIR_E_ARG, "null vector");
+        AIR_CHECK(transp == 0 || transp == 1, AIR_E_ARG, "transp_flag must be 'size', 'notransp' or 'transp'");
+        use_device(e);
+        hipStream_t s = e->stream;
+        void* sc = scratch_ptr(e->device);
+        AIR_CHECK(sc != nullptr, AIR_E_STATE, "scratch memory of this device not initialised");
+        const size_t cap = AIR_APPLY_SCRATCH_BYTES;
+        if (e->kind == 1 && e->A.nnz == 0 && e->T.ready) {
+            // a batch (tiled layout only): the internal-order vector lives in scratch
+            const size_t need = 4 * (size_t)std::max<int64_t>(e->T.n_int, 1);
+            AIR_CHECK(need <= cap, AIR_E_ARG,
+                      "air_apply_device: the operator needs " + std::to_string(need) +
+                          " bytes of scratch, more than the fixed " + std::to_string(cap));
+            float* xi = (float*)sc;
+            if (transp) {
+                tiled_backward_plain(e->T, in, xi, s);
+                permute_to_user(e->T, xi, out, s);
+            } else {
+                permute_to_internal(e->T, in, xi, s);
+                tiled_forward_plain(e->T, xi, out, s);
+            }
+        } else if (e->kind == 1) {
+            if (transp) spmv(e->AT, e->shAT, in, out, s);
+            else spmv(e->A, e->shA, in, out, s);
+        } else if (transp) {
+            // matrix-free adjoint: FP64 accumulators in scratch, cleared by a fill kernel
+            const size_t need = 8 * (size_t)std::max<int64_t>(e->n, 1);
+            AIR_CHECK(need <= cap, AIR_E_ARG,
+                      "air_apply_device: the operator needs " + std::to_string(need) +
+                          " bytes of scratch, more than the fixed " + std::to_string(cap));
+            double* acc = (double*)sc;
+            zero_f64(acc, e->n, s);
+            mf_adjoint(e->geo, in, acc, s);
+            vec_f64_to_f32(acc, out, e->n, s);
+        } else {
+            mf_forward(e->geo, in, out, s);
+        }
+        AIR_HIP(hipGetLastError());
     });
 }
 

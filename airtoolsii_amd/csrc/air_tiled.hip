@@ -1356,8 +1356,15 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
     (void)hipEventDestroy(e1);
 }
 
+__global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = 0.f;
+}
+
+// The padding columns of the internal order are cleared by a fill kernel (not
+// a memset node), so the permutation may also be enqueued inside a captured graph.
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s) {
-    AIR_HIP(hipMemsetAsync(x_int, 0, 4 * (size_t)T.n_int, s));
+    k_zero_f32<<<grid_for(T.n_int), BLOCK, 0, s>>>(x_int, T.n_int);
     k_scatter_perm<<<grid_for(T.n), BLOCK, 0, s>>>(x_user, T.perm.as<int>(), T.n, x_int);
     AIR_HIP(hipGetLastError());
 }

@@ -169,6 +169,18 @@ int air_get_csr(air_engine* e, int64_t* rowptr, int32_t* colind, double* val);
 /* afun convention, auxil/afun_matrix.m:47-50 / paralleltomo.m:282-287:
  * transp=0 -> out = A*in (m), transp=1 -> out = A'*in (n).  Host FP64. */
 int air_apply(air_engine* e, int transp, const double* in, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the product of
+ * air_apply on device memory, calling no hipMalloc/hipFree.  in, out: device
+ * FP32 vectors in the user's column order (n and m values for transp=0, m and n
+ * for transp=1).  The call enqueues the product on the engine's stream and
+ * returns without waiting (air_synchronize waits).  Matrix-free adjoints and
+ * batch operators work in a fixed scratch array of AIR_APPLY_SCRATCH_BYTES per
+ * GPU (8 bytes per column, resp. 4 per padded column); an operator that needs
+ * more is refused with AIR_E_ARG.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (two engines on two streams) race. */
+#define AIR_APPLY_SCRATCH_BYTES (32u << 20)
+int air_apply_device(air_engine* e, int transp, const float* in, float* out);
 
 /* ---- SIRT building blocks ---------------------------------------------
  * auxil/get_mfun_dfun.m:47-250: diagonals of M (m) and D (n), FP64. */
