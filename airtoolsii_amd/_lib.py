@@ -60,7 +60,8 @@ class Layout(C.Structure):
                 ("tiles_fwd", C.c_int64), ("tiles_bwd", C.c_int64), ("staged_rows", C.c_int64),
                 ("lds_bytes", C.c_int64), ("fwd_stream", C.c_int), ("fwd_lanes", C.c_int), ("fwd_loads", C.c_int),
                 ("bwd_lanes", C.c_int), ("bwd_loads", C.c_int), ("fwd_pieces_per_block", C.c_int),
-                ("tune_fwd_us", C.c_float * 32), ("tune_bwd_us", C.c_float * 8)]
+                ("fwd_entries", C.c_int64), ("bwd_entries", C.c_int64),
+                ("tune_fwd_us", C.c_float * 40), ("tune_bwd_us", C.c_float * 12)]
 
 
 class RowOptions(C.Structure):

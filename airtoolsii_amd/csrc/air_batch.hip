@@ -22,7 +22,7 @@ __global__ void k_shift_i32(const int* __restrict__ src, int64_t n, int off, int
 }
 __global__ void k_clear_start_flags(unsigned short* __restrict__ code, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        code[i] &= (unsigned short)0x7fffu;
+        code[i] &= (unsigned short)0xfffeu;
 }
 unsigned grid_of(int64_t n) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
@@ -52,7 +52,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     B = TiledOp();
     AIR_CHECK(!parts.empty(), AIR_E_ARG, "empty batch");
     const int dt = parts[0].dtype;
-    int64_t m = 0, n = 0, nint = 0, nnz = 0, P = 0, R = 0, tf = 0, tb = 0, staged = 0, rpi = 0;
+    int64_t m = 0, n = 0, nint = 0, nnz = 0, nfe = 0, nbe = 0, P = 0, R = 0, tf = 0, tb = 0, staged = 0, rpi = 0;
     int maxruns = 0, maxrows = 0;
     bool flags = true;
     for (auto& T : parts) {
@@ -61,6 +61,8 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         n += T.n;
         nint += T.n_int;
         nnz += T.nnz;
+        nfe += T.nfe;
+        nbe += T.nbe;
         P += T.npieces;
         R += T.nruns;
         tf += T.ntiles_f;
@@ -80,6 +82,8 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     B.n = n;
     B.n_int = nint;
     B.nnz = nnz;
+    B.nfe = nfe;
+    B.nbe = nbe;
     B.npieces = P;
     B.nruns = R;
     B.ntiles_f = tf;
@@ -91,14 +95,14 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     B.bwd_flags = flags;
     B.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
     B.perm.alloc(4 * (size_t)std::max<int64_t>(n, 1));
-    B.fval.alloc(vb * (size_t)(nnz + AIR_SPARSE_PAD));
-    B.fcode.alloc(2 * (size_t)(nnz + AIR_SPARSE_PAD));
-    B.bval.alloc(vb * (size_t)(nnz + AIR_SPARSE_PAD));
-    B.blr.alloc(2 * (size_t)(nnz + AIR_SPARSE_PAD));
-    AIR_HIP(hipMemsetAsync((char*)B.fval.p + vb * nnz, 0, vb * AIR_SPARSE_PAD, s));
-    AIR_HIP(hipMemsetAsync((char*)B.fcode.p + 2 * nnz, 0, 2 * AIR_SPARSE_PAD, s));
-    AIR_HIP(hipMemsetAsync((char*)B.bval.p + vb * nnz, 0, vb * AIR_SPARSE_PAD, s));
-    AIR_HIP(hipMemsetAsync((char*)B.blr.p + 2 * nnz, 0, 2 * AIR_SPARSE_PAD, s));
+    B.fval.alloc(vb * (size_t)(nfe + AIR_SPARSE_PAD));
+    B.fcode.alloc(2 * (size_t)(nfe + AIR_SPARSE_PAD));
+    B.bval.alloc(vb * (size_t)(nbe + AIR_SPARSE_PAD));
+    B.blr.alloc(2 * (size_t)(nbe + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((char*)B.fval.p + vb * nfe, 0, vb * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.fcode.p + 2 * nfe, 0, 2 * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.bval.p + vb * nbe, 0, vb * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)B.blr.p + 2 * nbe, 0, 2 * AIR_SPARSE_PAD, s));
     B.piece_ptr.alloc(8 * (size_t)(P + 1));
     B.rp_ptr.alloc(8 * (size_t)(m + 1));
     B.rp_idx.alloc(4 * (size_t)std::max<int64_t>(P, 1));
@@ -110,27 +114,27 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     B.run_len.alloc(4 * (size_t)std::max<int64_t>(R, 1));
     B.piece_tile_h.reserve((size_t)P);
     B.piece_ptr_h.reserve((size_t)P + 1);
-    int64_t om = 0, on = 0, oint = 0, oe = 0, oP = 0, oR = 0, otf = 0, otb = 0;
+    int64_t om = 0, on = 0, oint = 0, oef = 0, oeb = 0, oP = 0, oR = 0, otf = 0, otb = 0;
     for (auto& T : parts) {
         // forward
         shift32(T.perm, T.n, oint, B.perm, on, s);
-        copy_bytes(T.fval, vb * (size_t)T.nnz, B.fval, vb * (size_t)oe, s);
-        copy_bytes(T.fcode, 2 * (size_t)T.nnz, B.fcode, 2 * (size_t)oe, s);
-        shift64(T.piece_ptr, T.npieces, oe, B.piece_ptr, oP, s);
+        copy_bytes(T.fval, vb * (size_t)T.nfe, B.fval, vb * (size_t)oef, s);
+        copy_bytes(T.fcode, 2 * (size_t)T.nfe, B.fcode, 2 * (size_t)oef, s);
+        shift64(T.piece_ptr, T.npieces, oef, B.piece_ptr, oP, s);
         shift64(T.rp_ptr, T.m, oP, B.rp_ptr, om, s);      // rp_idx positions of a part start at its piece offset
         shift32(T.rp_idx, T.npieces, oP, B.rp_idx, oP, s);
         for (int64_t k = 0; k < T.npieces; ++k) {
             B.piece_tile_h.push_back(T.piece_tile_h[k] + (int)otf);
-            B.piece_ptr_h.push_back(T.piece_ptr_h[k] + oe);
+            B.piece_ptr_h.push_back(T.piece_ptr_h[k] + oef);
         }
         // backward
-        copy_bytes(T.bval, vb * (size_t)T.nnz, B.bval, vb * (size_t)oe, s);
-        copy_bytes(T.blr, 2 * (size_t)T.nnz, B.blr, 2 * (size_t)oe, s);
-        if (!flags && T.bwd_flags && T.nnz > 0) {        // one code format for the whole batch
-            k_clear_start_flags<<<grid_of(T.nnz), 256, 0, s>>>(B.blr.as<unsigned short>() + oe, T.nnz);
+        copy_bytes(T.bval, vb * (size_t)T.nbe, B.bval, vb * (size_t)oeb, s);
+        copy_bytes(T.blr, 2 * (size_t)T.nbe, B.blr, 2 * (size_t)oeb, s);
+        if (!flags && T.bwd_flags && T.nbe > 0) {        // one code format for the whole batch
+            k_clear_start_flags<<<grid_of(T.nbe), 256, 0, s>>>(B.blr.as<unsigned short>() + oeb, T.nbe);
             AIR_HIP(hipGetLastError());
         }
-        shift64(T.cell_ptr, T.n_int, oe, B.cell_ptr, oint, s);
+        shift64(T.cell_ptr, T.n_int, oeb, B.cell_ptr, oint, s);
         shift32(T.run_ptr, T.ntiles_b, oR, B.run_ptr, otb, s);
         shift32(T.run_row, T.nruns, om, B.run_row, oR, s);
         copy_bytes(T.run_off, 4 * (size_t)T.nruns, B.run_off, 4 * (size_t)oR, s);
@@ -138,7 +142,8 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         om += T.m;
         on += T.n;
         oint += T.n_int;
-        oe += T.nnz;
+        oef += T.nfe;
+        oeb += T.nbe;
         oP += T.npieces;
         oR += T.nruns;
         otf += T.ntiles_f;
@@ -147,12 +152,12 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         T = TiledOp();                                    // release the part
     }
     // closing entries of the pointer arrays
-    AIR_HIP(hipMemcpyAsync(B.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(B.piece_ptr.as<int64_t>() + P, &nfe, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(B.rp_ptr.as<int64_t>() + m, &P, 8, hipMemcpyHostToDevice, s));
-    AIR_HIP(hipMemcpyAsync(B.cell_ptr.as<int64_t>() + nint, &nnz, 8, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(B.cell_ptr.as<int64_t>() + nint, &nbe, 8, hipMemcpyHostToDevice, s));
     const int Ri = (int)R;
     AIR_HIP(hipMemcpyAsync(B.run_ptr.as<int>() + tb, &Ri, 4, hipMemcpyHostToDevice, s));
-    B.piece_ptr_h.push_back(nnz);
+    B.piece_ptr_h.push_back(nfe);
     AIR_HIP(hipStreamSynchronize(s));
     set_fwd_blocks(B, 256, s);
     set_fwd_waves(B, B.wave_target, s);

@@ -41,6 +41,8 @@ struct SirtState {
     LearnBufs learn;                 // learned / greedy relaxation
     DevBuf l_mlp, l_xstar, l_state, l_gpart, l_feat, l_target, l_out, l_err;
     DevBuf lamdev;                   // graph mode: the lambda schedule on the device
+    DevBuf ring;                     // deferred norms: block partials of the last iterations
+    int defer_first = 1;             // first iteration whose deferred norms are not reduced yet
     SirtBufs bufs{};
     double setup_s = 0, iter_s = 0;
 };
@@ -509,6 +511,15 @@ void drop_graph(air_engine* e) {
     e->gexec = nullptr;
 }
 
+// deferred norms: reduce the pending iterations' block partials into hist and
+// run the stop check of the last one
+void flush_deferred(air_engine* e) {
+    SirtState& st = e->st;
+    if (!st.bufs.ring || st.defer_first > st.k) return;
+    launch_finalize_deferred(st.bufs, (int)tiled_reduce_grid(e->T.m), st.sp, st.defer_first, st.k, e->stream);
+    st.defer_first = st.k + 1;
+}
+
 void enqueue_iteration(air_engine* e, int k, bool devk = false) {
     SirtState& st = e->st;
     hipStream_t s = e->stream;
@@ -528,6 +539,10 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
         launch_tiled_update(e->T, st.bufs, lam, line ? 1 : 0, k, s);
         record(e, 1);
         launch_tiled_resid(e->T, st.bufs, st.sp, k, s, mid_event(e));
+        record(e, 3);
+        if (e->timing) e->ev_used++;
+        if (st.bufs.ring && k - st.defer_first + 1 >= st.bufs.ring_slots) flush_deferred(e);
+        return;
     } else if (e->kind == 1) {
         if (line) launch_line_update(e->AT, e->shAT, st.bufs, k, s);
         else launch_update(e->AT, e->shAT, st.bufs, lam, k, s);
@@ -552,6 +567,7 @@ void enqueue_iterations(air_engine* e, int count) {
     hipStream_t s = e->stream;
     if (!graph_mode(e)) {
         for (int i = 0; i < count; ++i) enqueue_iteration(e, ++st.k);
+        flush_deferred(e);
         return;
     }
     const int G = e->graph_iters;
@@ -711,8 +727,17 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     st.Mr.alloc(4 * std::max<int64_t>(m, 1));
     st.g.alloc(8 * std::max<int64_t>(nx, 1));
     st.Ax.alloc(4 * std::max<int64_t>(m, 1));
+    // Deferred norms (the tiled loop with the stop rule 'none' and a host
+    // lambda schedule, outside graph mode): no iteration reads its own norms,
+    // so the row pass leaves them for a later reduction instead of a finalize
+    // kernel per iteration.
+    const bool defer = use_tiled(e) && o->stoprule == AIR_STOP_NONE && !device_lambda(o->relax_kind) &&
+                       o->relax_kind != AIR_RELAX_LEARNED && o->relax_kind != AIR_RELAX_GREEDY &&
+                       e->graph_iters == 0;
+    constexpr int RING_SLOTS = 32;
     // row-pass reductions: 4 doubles per block; column-pass ones: 1 per block
-    const int64_t row_slots = 4 * std::max<int64_t>(2048, use_tiled(e) ? tiled_reduce_grid(m) : 2048);
+    const int64_t row_blocks = use_tiled(e) ? tiled_reduce_grid(m) : 2048;
+    const int64_t row_slots = 4 * std::max<int64_t>(2048, row_blocks);
     const int64_t col_slots = std::max<int64_t>(2048, e->T.ntiles_b);
     st.partial.alloc(8 * (size_t)(row_slots + col_slots));
     st.hist.alloc(8 * (size_t)(kmax + 1));
@@ -746,6 +771,12 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     B.partial_cols = st.partial.as<double>() + row_slots;
     B.hist = st.hist.as<double>();
     B.ctrl = st.ctrl.as<Ctrl>();
+    if (defer) {
+        st.ring.alloc(8 * 4 * (size_t)row_blocks * RING_SLOTS);
+        B.ring = st.ring.as<double>();
+        B.ring_slots = RING_SLOTS;
+    }
+    st.defer_first = 1;
     if (o->relax_kind == AIR_RELAX_LEARNED || o->relax_kind == AIR_RELAX_GREEDY) {
         LearnBufs& L = st.learn;
         L = LearnBufs{};
@@ -1439,8 +1470,10 @@ int air_layout_info(air_engine* e, air_layout* out) {
             tiled_shapes(T, &L.fwd_lanes, &L.fwd_loads, &L.bwd_lanes, &L.bwd_loads);
             L.fwd_stream = tiled_fwd_stream(T) ? 1 : 0;
             L.fwd_pieces_per_block = L.fwd_stream ? T.wave_target : T.ppb;
-            for (int q = 0; q < 32; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
-            for (int q = 0; q < 8; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
+            L.fwd_entries = T.nfe;
+            L.bwd_entries = T.nbe;
+            for (int q = 0; q < 40; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
+            for (int q = 0; q < 12; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
         }
         *out = L;
     });

@@ -230,6 +230,7 @@ struct TiledOp {
     int dtype = AIR_F32;
     int N = 0;                  // image side for 2-D geometries (0: generic column blocks)
     int64_t m = 0, n = 0, n_int = 0, nnz = 0;
+    int64_t nfe = 0, nbe = 0;   // entries of the forward / backward streams (segments padded to 8)
     int64_t ntiles_f = 0, ntiles_b = 0, npieces = 0, nruns = 0;
     int bt_size = 256;          // columns per backward tile
     int nblk_f = 0;             // forward blocks
@@ -249,7 +250,7 @@ struct TiledOp {
     // backward codes carry a column-start flag in bit 15 when every tile has
     // fewer than 32768 distinct rows (required by the stream form)
     bool bwd_flags = false;
-    float tune_fwd_ms[32] = {}, tune_bwd_ms[8] = {};
+    float tune_fwd_ms[40] = {}, tune_bwd_ms[12] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
@@ -374,6 +375,12 @@ struct SirtBufs {
     NcpState* ncp = nullptr;  // NCP stop rule state (null otherwise)
     const double* lamv = nullptr;  // graph mode: lambda_1..lambda_kmax on the device (null otherwise)
     const LearnBufs* learn = nullptr;  // learned / greedy relaxation (null otherwise)
+    // deferred norms (tiled loop, stop rule none, lambda from the host): the row
+    // pass of iteration k >= 1 leaves its block partials in slot k % ring_slots
+    // and launch_finalize_deferred turns a run of slots into hist[k] and the
+    // stop check afterwards, so no finalize kernel sits between the two passes
+    double* ring = nullptr;
+    int ring_slots = 0;
 };
 
 // r = b - A x; Mr = M r; FP64 reductions and the stop check of iteration k.
@@ -400,7 +407,10 @@ void launch_finalize_line(const SirtBufs& B, int nblocks, int k, hipStream_t s);
 void launch_learned_select(const SirtBufs& B, int k, hipStream_t s);
 // true when the finalize kernel of the row pass computes the learned lambda (no separate kernel)
 bool fused_select(const SirtBufs& B);
+// blocks of the tiled row pass
 unsigned tiled_reduce_grid(int64_t m);
+// hist[k], k = k0..k1, from the deferred block partials; the stop check of k1
+void launch_finalize_deferred(const SirtBufs& B, int nblocks, StopParams sp, int k0, int k1, hipStream_t s);
 // matrix-free iteration pieces
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s);
 void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s);

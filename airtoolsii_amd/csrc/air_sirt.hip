@@ -465,6 +465,48 @@ bool fused_select(const SirtBufs& B) {
     return B.learn && B.learn->mode == AIR_RELAX_LEARNED && B.learn->mlp && !B.learn->xstar;
 }
 
+// Deferred norms: block j of the launch reduces the row-pass partials of
+// iteration k0 + j (ring slot (k0 + j) % slots, in the block order of
+// k_finalize_rows, so hist matches the direct form bit for bit); the last
+// block also runs the stop check of k1 (the rule is 'none': k1 >= kmax stops).
+__global__ __launch_bounds__(BLOCK) void k_finalize_deferred(const double* __restrict__ ring, int nblocks, int slots,
+                                                             int k0, int k1, double* hist, Ctrl* ctrl, StopParams sp) {
+    __shared__ double sh[4][BLOCK / 64];
+    if (load_stop(ctrl)) return;
+    const int k = k0 + (int)blockIdx.x;
+    const double* partial = ring + (size_t)(k % slots) * (size_t)nblocks * 4;
+    double v[4] = {0, 0, 0, 0};
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const d4* p4 = (const d4*)partial;
+    int b = threadIdx.x;
+    for (; b + 3 * BLOCK < nblocks; b += 4 * BLOCK) {
+        d4 a0 = p4[b], a1 = p4[b + BLOCK], a2 = p4[b + 2 * BLOCK], a3 = p4[b + 3 * BLOCK];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] += ((a0[q] + a1[q]) + (a2[q] + a3[q]));
+    }
+    for (; b < nblocks; b += BLOCK) {
+        d4 a0 = p4[b];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] += a0[q];
+    }
+    block_reduce<4>(v, sh);
+    if (threadIdx.x == 0) {
+        if (k < k1) {
+            hist[k] = sqrt(v[0]);
+        } else {
+            evaluate_stop(ctrl, sp, k, v[0], v[1], v[2], v[3], 0.0, hist);
+            ctrl->kcur = k;
+        }
+    }
+}
+
+void launch_finalize_deferred(const SirtBufs& B, int nblocks, StopParams sp, int k0, int k1, hipStream_t s) {
+    if (k1 < k0) return;
+    k_finalize_deferred<<<(unsigned)(k1 - k0 + 1), BLOCK, 0, s>>>(B.ring, nblocks, B.ring_slots, k0, k1, B.hist,
+                                                                   B.ctrl, sp);
+    AIR_HIP(hipGetLastError());
+}
+
 void launch_finalize_rows(const SirtBufs& B, int nblocks, StopParams sp, int k, hipStream_t s) {
     if (sp.type == AIR_STOP_NCP && B.ncp) ncp_enqueue(*B.ncp, B.r, B.ncp->m, B.ctrl, s);
     const bool sel = fused_select(B);

@@ -77,8 +77,7 @@ struct SegRound {
     }
     // Branch-free: masked entries read LDS slot 0 and contribute 0, so the
     // U*VE LDS reads of a round issue back to back.
-    __device__ __forceinline__ A dot(const float* lds, int64_t base, int64_t s, int64_t e, int gl,
-                                     unsigned mask) const {
+    __device__ __forceinline__ A dot(const float* lds, int64_t base, int64_t s, int64_t e, int gl) const {
         A xv[U][VE];
         bool ok[U][VE];
 #pragma unroll
@@ -87,7 +86,7 @@ struct SegRound {
             for (int i = 0; i < VE; ++i) {
                 const int64_t q = base + (int64_t)u * STEP + (int64_t)gl * VE + i;
                 ok[u][i] = q >= s && q < e;
-                const unsigned idx = ok[u][i] ? (c[u][i] & mask) : 0u;
+                const unsigned idx = ok[u][i] ? (c[u][i] >> 2) : 0u;
                 xv[u][i] = (A)lds[idx];
             }
         A acc = 0;
@@ -108,7 +107,7 @@ struct SegRound {
 template <typename V, int G, int U, int DEPTH, typename Bounds, typename Done>
 __device__ __forceinline__ void pipelined_segments(int first, int end, int stride, const V* __restrict__ val,
                                                    const unsigned short* __restrict__ code, const float* lds,
-                                                   unsigned mask, int gl, Bounds&& bounds, Done&& done) {
+                                                   int gl, Bounds&& bounds, Done&& done) {
     using R = SegRound<V, G, U>;
     constexpr int VE = R::VE;
     static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
@@ -131,11 +130,11 @@ __device__ __forceinline__ void pipelined_segments(int first, int end, int strid
         buf[DEPTH].load(val, code, (kk[DEPTH] < end ? ss[DEPTH] : ss[0]) & ~(int64_t)(VE - 1), gl);
         const int64_t s = ss[0], e = ee[0];
         const int64_t b0 = s & ~(int64_t)(VE - 1);
-        auto acc = buf[0].dot(lds, b0, s, e, gl, mask);
+        auto acc = buf[0].dot(lds, b0, s, e, gl);
         for (int64_t base = b0 + R::SPAN; base < e; base += R::SPAN) {
             R more;
             more.load(val, code, base, gl);
-            acc += more.dot(lds, base, s, e, gl, mask);
+            acc += more.dot(lds, base, s, e, gl);
         }
         done(kk[0], acc);
 #pragma unroll
@@ -169,7 +168,7 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd(const int* __restrict__ blk_tile
     constexpr int GPB = BLOCK / G;
     const int gl = threadIdx.x % G;
     pipelined_segments<V, G, U, DEPTH>(
-        p0 + threadIdx.x / G, p1, GPB, fval, fcode, xs, 0x0fffu, gl,
+        p0 + threadIdx.x / G, p1, GPB, fval, fcode, xs, gl,
         [&](int k, int64_t& s, int64_t& e) { s = piece_ptr[k]; e = piece_ptr[k + 1]; },
         [&](int k, auto acc) {
             acc = group_sum<G>(acc);
@@ -219,9 +218,9 @@ __device__ __forceinline__ float seg_step(float v, uint64_t sm, int lane) {
 // (val, code), walked as a flat stream in rounds of 64*EPL entries, EPL
 // consecutive entries per lane, so that every vector load is useful and no
 // load depends on data fetched in the same round.  Every segment is
-// non-empty and its first code carries the start flag (bit 15); the other
-// code bits (& mask) index the LDS vector.  The flag splits each lane's
-// products into a head (the segment open on the left), the segments closed
+// non-empty and its first code carries the start flag (bit 0); the code with
+// bits 0-1 cleared is the byte address of the entry's operand in the LDS
+// vector.  The flag splits each lane's products into a head (the segment open on the left), the segments closed
 // inside the lane and a tail; a reverse segmented scan over the lanes adds
 // the heads of the following lanes up to the next start, and the segment left
 // open at the end of a round is carried (wave-uniform) into the next round.
@@ -231,7 +230,7 @@ __device__ __forceinline__ float seg_step(float v, uint64_t sm, int lane) {
 template <typename V, int EPL, int PF, typename Close>
 __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V* __restrict__ val,
                                                 const unsigned short* __restrict__ code, const float* lds,
-                                                unsigned mask, int k0, Close&& close) {
+                                                int k0, Close&& close) {
     using A = typename AccOf<V>::T;
     constexpr int VE = VecOf<V>::VE, NV = EPL / VE, NC = EPL / 8, RND = 64 * EPL;
     static_assert(EPL == 8 || EPL == 16, "8 or 16 entries per lane");
@@ -242,19 +241,18 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
     A carry = 0;
     A v[NV][VE];
     // codes stay packed two per word: entry 2h in the low half of c[h], entry
-    // 2h+1 in the high half; the LDS byte address and the start flag are cut
-    // straight out of the word
+    // 2h+1 in the high half; each code is the LDS byte address of its operand
+    // with the start flag in bit 0
     typedef unsigned u4v __attribute__((ext_vector_type(4)));
     unsigned c[EPL / 2];
-    const unsigned mb = mask << 2;
     auto addr = [&](const unsigned (&cc)[EPL / 2], int i) -> unsigned {
-        return (i & 1) ? (cc[i >> 1] >> 14) & mb : (cc[i >> 1] << 2) & mb;
+        return (i & 1) ? (cc[i >> 1] >> 16) & 0xfffcu : cc[i >> 1] & 0xfffcu;
     };
     auto flags = [&](const unsigned (&cc)[EPL / 2]) -> unsigned {
         unsigned f = 0;
 #pragma unroll
-        for (int h = 0; h < EPL / 2; ++h) f |= (cc[h] & 0x80008000u) >> (15 - 2 * h);
-        return (f | (f >> 15)) & ((1u << EPL) - 1u);
+        for (int h = 0; h < EPL / 2; ++h) f |= ((cc[h] & 1u) | ((cc[h] >> 15) & 2u)) << (2 * h);
+        return f;
     };
     auto lds_at = [&](unsigned byte) -> float { return *(const float*)((const char*)lds + byte); };
     auto load_round = [&](int64_t q, A (&vv)[NV][VE], unsigned (&cc)[EPL / 2]) {
@@ -407,9 +405,119 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
     if (lane == 0 && kopen >= 0) close(kopen, carry);
 }
 
+// Segmented sums over one wave's run of whole segments in the lane form of the
+// layout: every segment starts at a multiple of 8 entries and is padded to a
+// multiple of 8 (zero values, codes pointing into the segment), so the 8
+// consecutive entries a lane takes in a round of 512 belong to one segment.
+// A lane's products are summed in registers (no per-entry masks or splits);
+// its first code carries the start flag (bit 0).  Across the lanes a forward
+// segmented scan (DPP row shifts and row broadcasts, each step gated by the
+// lane's distance to the last start at or below it) gives every lane the sum
+// of its segment so far; the lane before each start closes its segment, the
+// segment open at lane 63 is carried (wave-uniform) into the next round, and
+// the last valid lane of the last round closes the wave's last segment.
+// close(k, sum) is called once per segment, k counted from k0.  PF = 1 issues
+// the next round's loads before the current one is summed.
+template <typename V, int PF, typename Close>
+__device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __restrict__ val,
+                                             const unsigned short* __restrict__ code, const float* lds, int k0,
+                                             Close&& close) {
+    static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
+    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t le_mask = (2ull << lane) - 1ull;       // lanes 0..lane (all 64 for lane 63)
+    auto lds_at = [&](unsigned byte) -> float { return *(const float*)((const char*)lds + byte); };
+    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned (&cc)[4]) {
+        const int64_t q = rb + 8 * lane;
+        if (q < E1) {
+#pragma unroll
+            for (int h = 0; h < NV; ++h) {
+                float t[VE];
+                load_vals<V, float>(val + q + VE * h, t);
+#pragma unroll
+                for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
+            }
+            const u4v w = __builtin_nontemporal_load((const u4v*)(code + q));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cc[i] = w[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) vv[i] = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cc[i] = 0u;
+        }
+    };
+    int64_t rb = E0;
+    int knext = k0;          // index of the next segment start
+    bool open = false;       // a segment is carried in from the previous round
+    float carry = 0.f;
+    float v[8];
+    unsigned c[4];
+    load_round(rb, v, c);
+    while (rb < E1) {
+        const bool last_round = rb + 512 >= E1;
+        float vn[8];
+        unsigned cn[4];
+        if (PF && !last_round) load_round(rb + 512, vn, cn);
+        // the lane's 8 products; entry 0 masks the flag out of its address
+        float s = v[0] * lds_at(c[0] & 0xfffcu);
+        s += v[1] * lds_at(c[0] >> 16);
+#pragma unroll
+        for (int h = 1; h < 4; ++h) {
+            s += v[2 * h] * lds_at(c[h] & 0xffffu);
+            s += v[2 * h + 1] * lds_at(c[h] >> 16);
+        }
+        const uint64_t sm = __ballot((c[0] & 1u) != 0u);
+        const uint64_t below = sm & le_mask;
+        const int dist = lane - (below ? 63 - __builtin_clzll(below) : -1);
+        if (open && (sm & 1ull)) {
+            if (lane == 0) close(knext - 1, carry);        // the carried segment ends before lane 0
+        } else if (lane == 0) {
+            s += carry;                                    // lane 0 continues the carried segment
+        }
+        float w = s;
+#pragma unroll
+        for (int D = 1; D <= 8; D <<= 1) {
+            float t;
+            if (D == 1) t = dpp_f<0x111, 0xf>(w);
+            else if (D == 2) t = dpp_f<0x112, 0xf>(w);
+            else if (D == 4) t = dpp_f<0x114, 0xf>(w);
+            else t = dpp_f<0x118, 0xf>(w);
+            if (dist >= D) w += t;                         // no start in (lane - D, lane]
+        }
+        {
+            const float t = dpp_f<0x142, 0xa>(w);          // lane 15 of rows 0, 2 -> rows 1, 3
+            if (((lane >> 4) & 1) && dist > (lane & 15)) w += t;
+        }
+        {
+            const float t = dpp_f<0x143, 0xc>(w);          // lane 31 -> rows 2, 3
+            if (lane >= 32 && dist > lane - 32) w += t;
+        }
+        const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
+        const bool next_starts = lane < 63 && ((sm >> (lane + 1)) & 1ull);
+        if (next_starts || (last_round && lane == nvalid - 1)) close(knext - 1 + __popcll(below), w);
+        knext += __popcll(sm);
+        carry = readlane_f(w, 63);
+        open = true;
+        rb += 512;
+        if (!last_round) {
+            if (PF) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = vn[i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) c[i] = cn[i];
+            } else {
+                load_round(rb, v, c);
+            }
+        }
+    }
+}
+
 // Forward product as a flat stream: every wave owns a run of whole pieces of
 // one tile (a block is 4 waves of one tile, which share the tile of x in
-// LDS) and writes each piece's sum to partial[piece].
+// LDS) and writes each piece's sum to partial[piece].  EPL = 8 or 16 walks the
+// entries with stream_segments, EPL = 0 with stream_lanes (the lane form).
 template <typename V, int EPL, int PF>
 __global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
                                                      int nwaves, int nblk, const int64_t* __restrict__ piece_ptr,
@@ -435,8 +543,9 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk
     if (w >= nwaves) return;
     const int pw0 = wave_p0[w], pw1 = wave_p0[w + 1];
     if (pw0 >= pw1) return;
-    stream_segments<V, EPL, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, 0x0fffu, pw0,
-                           [&](int k, auto sum) { partial[k] = (float)sum; });
+    auto close = [&](int k, auto sum) { partial[k] = (float)sum; };
+    if constexpr (EPL == 0) stream_lanes<V, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, pw0, close);
+    else stream_segments<V, EPL, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, pw0, close);
 }
 
 // y = sum of a row's partials (8 lanes per row)
@@ -467,20 +576,59 @@ __global__ __launch_bounds__(BLOCK) void k_treduce(const int64_t* __restrict__ r
     if (load_stop(ctrl)) return;
     const int gl = threadIdx.x & 7;
     double v[4] = {0, 0, 0, 0};
-    for (int64_t row = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 3; row < m; row += (int64_t)gridDim.x * (BLOCK / 8)) {
-        float ax = row_partials(rp_ptr, rp_idx, partial, row, gl);
-        if (gl == 0) {
-            float ri = b[row] - ax;
-            float mri = M ? M[row] * ri : ri;
-            if (ME) {
-                double ro = (double)r[row];
-                v[2] += ro * (ro + (double)ri);
-                v[3] += ro * ro;
+    // Each 8-lane group takes RB rows a grid stride apart and issues each level
+    // of their dependent loads (row pointers, piece lists, partials, the
+    // epilogue vectors) for all of them before it uses any, so a batch waits
+    // for three memory round trips instead of three per row.  Per row the sum
+    // is that of row_partials (lane partials in piece order, then the 8-lane tree).
+    constexpr int RB = 4;
+    const int64_t G = (int64_t)gridDim.x * (BLOCK / 8);
+    for (int64_t r0 = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 3; r0 < m; r0 += RB * G) {
+        int64_t s[RB], e[RB];
+        float bb[RB], mm[RB], ro[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int64_t row = r0 + u * G;
+            s[u] = e[u] = 0;
+            bb[u] = mm[u] = ro[u] = 0.f;
+            if (row < m) {
+                s[u] = rp_ptr[row];
+                e[u] = rp_ptr[row + 1];
+                if (gl == 0) {
+                    bb[u] = b[row];
+                    mm[u] = M ? M[row] : 1.f;
+                    if (ME) ro[u] = r[row];
+                }
             }
-            r[row] = ri;
-            Mr[row] = mri;
-            v[0] += (double)ri * (double)ri;
-            v[1] += (double)ri * (double)mri;
+        }
+        int idx[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) idx[u] = s[u] + gl < e[u] ? rp_idx[s[u] + gl] : -1;
+        float a[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) a[u] = idx[u] >= 0 ? 0.f + partial[idx[u]] : 0.f;
+#pragma unroll
+        for (int u = 0; u < RB; ++u)
+            for (int64_t q = s[u] + gl + 8; q < e[u]; q += 8) a[u] += partial[rp_idx[q]];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) a[u] = group_sum<8>(a[u]);
+        if (gl == 0) {
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int64_t row = r0 + u * G;
+                if (row >= m) continue;
+                const float ri = bb[u] - a[u];
+                const float mri = M ? mm[u] * ri : ri;
+                if (ME) {
+                    const double rold = (double)ro[u];
+                    v[2] += rold * (rold + (double)ri);
+                    v[3] += rold * rold;
+                }
+                r[row] = ri;
+                Mr[row] = mri;
+                v[0] += (double)ri * (double)ri;
+                v[1] += (double)ri * (double)mri;
+            }
         }
     }
     block_reduce<4>(v, sh);
@@ -539,7 +687,6 @@ struct BwdArgs {
     const unsigned short* blr;
     int maxruns;
     int bt;                   // columns per tile
-    unsigned mask;            // local-row bits of a code (bit 15 is the column-start flag when set)
 };
 
 // MODE 0: g = A'y (FP32 out); 1: SIRT update of x; 2: line search (g in FP64, sum D g^2, lam)
@@ -568,7 +715,7 @@ __global__ __launch_bounds__(BLOCK) void k_tbwd(BwdArgs a, const V* __restrict__
         __syncthreads();
     }
     pipelined_segments<V, G, U, DEPTH>(
-        w, a.bt, BLOCK / G, bval, a.blr, ys, a.mask, lane,
+        w, a.bt, BLOCK / G, bval, a.blr, ys, lane,
         [&](int c, int64_t& s, int64_t& e) { s = a.cell_ptr[c0 + c]; e = a.cell_ptr[c0 + c + 1]; },
         [&](int c, auto acc) {
             acc = group_sum<G>(acc);
@@ -660,9 +807,13 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __res
             return lo;
         };
         const int ka = first_at((int64_t)Et * wv / NW), kb = first_at((int64_t)Et * (wv + 1) / NW);
-        if (ka < kb)
-            stream_segments<V, EPL, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.blr, ys, a.mask, ka,
-                                   [&](int k, auto sum) { gt[necol[k]] = (float)sum; });
+        auto close = [&](int k, auto sum) { gt[necol[k]] = (float)sum; };
+        if (ka < kb) {
+            if constexpr (EPL == 0)
+                stream_lanes<V, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.blr, ys, ka, close);
+            else
+                stream_segments<V, EPL, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.blr, ys, ka, close);
+        }
     }
     __syncthreads();
     double dsum[1] = {0};
@@ -689,7 +840,7 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __res
 
 __global__ void k_flag_col_starts(const int64_t* __restrict__ cell_ptr, int64_t ncols, unsigned short* __restrict__ blr) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x)
-        if (cell_ptr[j] < cell_ptr[j + 1]) blr[cell_ptr[j]] |= (unsigned short)0x8000u;
+        if (cell_ptr[j] < cell_ptr[j + 1]) blr[cell_ptr[j]] |= (unsigned short)1u;
 }
 
 // ---------------------------------------------------------------- layout build kernels
@@ -733,7 +884,7 @@ __global__ void k_fwd_fill(const uint32_t* __restrict__ order, const uint32_t* _
         bool start = q == 0;
         if (!start) start = ftile_s[q] != ftile_s[q - 1] || rowof[order[q - 1]] != row;
         fval[q] = val[e];
-        fcode[q] = (unsigned short)((cp[e] % FT) | (start ? 0x8000u : 0u));
+        fcode[q] = (unsigned short)(((cp[e] % FT) << 2) | (start ? 1u : 0u));
         flag[q] = start ? 1u : 0u;
     }
 }
@@ -794,7 +945,7 @@ __global__ void k_local_rows(const uint64_t* __restrict__ key2s, const uint32_t*
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t tile = (uint32_t)(key2s[q] >> 32);
         const uint32_t u = uid[q] + flag[q] - 1;        // inclusive unique index
-        blr[order2[q]] = (unsigned short)(u - tile_first[tile]);
+        blr[order2[q]] = (unsigned short)((u - tile_first[tile]) << 2);
     }
 }
 
@@ -826,6 +977,36 @@ __global__ void k_scatter_perm(const float* __restrict__ xu, const int* __restri
 __global__ void k_gather_perm(const float* __restrict__ xi, const int* __restrict__ perm, int64_t n, float* __restrict__ xu) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
         xu[j] = xi[perm[j]];
+}
+
+// Copy segments into the lane form: segment k moves from [ptr[k], ptr[k+1]) to
+// [ptr_pad[k], ptr_pad[k+1]), whose length is rounded up to a multiple of 8;
+// the padding entries get value 0 and the code of the segment's last entry
+// (start flag cleared), so they read an operand the segment reads anyway.
+// One wave per segment.
+template <typename V>
+__global__ void k_pad_segments(const int64_t* __restrict__ ptr, const int64_t* __restrict__ ptr_pad, int64_t nseg,
+                               const V* __restrict__ val, const unsigned short* __restrict__ code,
+                               V* __restrict__ val_out, unsigned short* __restrict__ code_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nseg; k += waves) {
+        const int64_t s = ptr[k], len = ptr[k + 1] - s, o = ptr_pad[k], olen = ptr_pad[k + 1] - o;
+        for (int64_t j = lane; j < olen; j += 64) {
+            if (j < len) {
+                val_out[o + j] = val[s + j];
+                code_out[o + j] = code[s + j];
+            } else {
+                val_out[o + j] = V{0};
+                code_out[o + j] = (unsigned short)(code[s + len - 1] & 0xfffeu);
+            }
+        }
+    }
+}
+
+__global__ void k_pad_lengths(const int64_t* __restrict__ ptr, int64_t nseg, int64_t* __restrict__ len) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nseg; k += (int64_t)gridDim.x * blockDim.x)
+        len[k] = (ptr[k + 1] - ptr[k] + 7) & ~(int64_t)7;
 }
 
 // ---------------------------------------------------------------- host: build
@@ -931,6 +1112,43 @@ static void pad_alloc(DevBuf& b, int64_t n, hipStream_t s) {
     AIR_HIP(hipMemsetAsync((T*)b.p + n, 0, sizeof(T) * AIR_SPARSE_PAD, s));
 }
 
+// Replace (ptr, val, code) of nseg segments by their lane form; returns the
+// padded entry count.  The arrays keep AIR_SPARSE_PAD zero entries at the end.
+static int64_t pad_segments(DevBuf& ptr, int64_t nseg, DevBuf& val, DevBuf& code, int dtype, hipStream_t s) {
+    DevBuf len, ptr_pad;
+    len.alloc(8 * (size_t)std::max<int64_t>(nseg, 1));
+    ptr_pad.alloc(8 * (size_t)(nseg + 1));
+    if (nseg > 0) {
+        k_pad_lengths<<<grid_for(nseg), BLOCK, 0, s>>>(ptr.as<int64_t>(), nseg, len.as<int64_t>());
+        AIR_HIP(hipGetLastError());
+    }
+    scan_exclusive_i64(len.as<int64_t>(), ptr_pad.as<int64_t>(), nseg, s);
+    int64_t total = 0;
+    AIR_HIP(hipMemcpyAsync(&total, ptr_pad.as<int64_t>() + nseg, 8, hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+    const size_t vs = dtype_size(dtype);
+    DevBuf v2, c2;
+    v2.alloc(vs * (size_t)(total + AIR_SPARSE_PAD));
+    c2.alloc(2 * (size_t)(total + AIR_SPARSE_PAD));
+    AIR_HIP(hipMemsetAsync((char*)v2.p + vs * total, 0, vs * AIR_SPARSE_PAD, s));
+    AIR_HIP(hipMemsetAsync((char*)c2.p + 2 * total, 0, 2 * AIR_SPARSE_PAD, s));
+    if (nseg > 0) {
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nseg * 64 + BLOCK - 1) / BLOCK, 65536));
+        dispatch_dtype(dtype, [&](auto v) {
+            using V = decltype(v);
+            k_pad_segments<V><<<grid, BLOCK, 0, s>>>(ptr.as<int64_t>(), ptr_pad.as<int64_t>(), nseg, val.as<V>(),
+                                                     code.as<unsigned short>(), v2.as<V>(), c2.as<unsigned short>());
+            return 0;
+        });
+        AIR_HIP(hipGetLastError());
+    }
+    AIR_HIP(hipStreamSynchronize(s));
+    ptr = std::move(ptr_pad);
+    val = std::move(v2);
+    code = std::move(c2);
+    return total;
+}
+
 bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune) {
     T = TiledOp();
     const int64_t m = A.nrows, n = A.ncols, nnz = A.nnz;
@@ -989,6 +1207,9 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
                                               piece_row.as<uint32_t>(), piece_tile.as<int>(), iota_p.as<uint32_t>());
     AIR_HIP(hipGetLastError());
     AIR_HIP(hipMemcpyAsync(T.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipStreamSynchronize(s));
+    // pieces padded to whole lanes of 8 entries (the lane form)
+    T.nfe = pad_segments(T.piece_ptr, P, T.fval, T.fcode, T.dtype, s);
     // forward blocks are cut per tile from the piece -> tile map (kept on the host)
     T.piece_tile_h.resize(P);
     AIR_HIP(hipMemcpyAsync(T.piece_tile_h.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
@@ -1107,8 +1328,9 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         T.maxruns = maxruns;
         T.maxrows = maxrows;
         T.lds_b = 4 * (2 * (size_t)maxruns + 1 + (size_t)maxrows);
-        if (T.lds_b > MAX_LDS_B || maxrows > 65535) return false;
-        T.bwd_flags = maxrows <= 32768 && T.bt_size == MAX_BT && T.ntiles_b * MAX_BT == T.n_int;
+        // a backward code is the staged row's LDS byte address (4 * row) in 16 bits
+        if (T.lds_b > MAX_LDS_B || maxrows > 16383) return false;
+        T.bwd_flags = T.bt_size == MAX_BT && T.ntiles_b * MAX_BT == T.n_int;
         if (T.bwd_flags) {
             k_flag_col_starts<<<grid_for(T.n_int), BLOCK, 0, s>>>(T.cell_ptr.as<int64_t>(), T.n_int,
                                                                  T.blr.as<unsigned short>());
@@ -1118,6 +1340,8 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
         AIR_HIP(hipStreamSynchronize(s));
     }
+    // columns padded to whole lanes of 8 entries (the lane form)
+    T.nbe = pad_segments(T.cell_ptr, T.n_int, T.bval, T.blr, T.dtype, s);
     T.ready = true;
     if (tune) autotune_tiled(T, s);
     return true;
@@ -1134,7 +1358,6 @@ static BwdArgs bwd_args(const TiledOp& T) {
     a.blr = T.blr.as<unsigned short>();
     a.maxruns = T.maxruns;
     a.bt = T.bt_size;
-    a.mask = T.bwd_flags ? 0x7fffu : 0xffffu;
     return a;
 }
 
@@ -1143,8 +1366,12 @@ static BwdArgs bwd_args(const TiledOp& T) {
 static const int FWD_CFG[][3] = {{16, 2, 1}, {16, 2, 2}, {16, 1, 2}, {8, 2, 2}, {32, 1, 2}};
 static const int BWD_CFG[][3] = {{64, 2, 1}, {64, 2, 2}, {64, 1, 2}, {32, 2, 2}};
 static constexpr int N_FWD_CFG = 5, N_BWD_CFG = 4;
-static constexpr int N_FWD_STREAM = 4;     // stream forward: 8/16 entries per lane, without/with prefetch
-static constexpr int N_BWD_STREAM = 3;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch
+static constexpr int N_FWD_STREAM = 6;     // stream forward: 8/16 entries per lane, without/with prefetch,
+                                           // then the lane form without/with prefetch
+static constexpr int N_BWD_STREAM = 5;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch,
+                                           // then the lane form without/with prefetch
+// the lane form sums in FP32 registers: FP64 storage runs the 8-entry stream form instead
+static constexpr bool lane_form_ok(int dtype) { return dtype != AIR_F64; }
 template <int A, int B, int C> using Shape3 = std::integer_sequence<int, A, B, C>;
 
 template <typename F>
@@ -1180,11 +1407,24 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                                                 T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
                                                 T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
             };
-            switch (cfg - N_FWD_CFG) {
-            case 0: launch(k_tfwd_flat<V, 8, 0>); break;
-            case 1: launch(k_tfwd_flat<V, 8, 1>); break;
-            case 2: launch(k_tfwd_flat<V, 16, 0>); break;
-            default: launch(k_tfwd_flat<V, 16, 1>); break;
+            int sv = cfg - N_FWD_CFG;
+            if constexpr (std::is_same<V, double>::value) {
+                if (sv >= 4) sv -= 4;
+                switch (sv) {
+                case 0: launch(k_tfwd_flat<V, 8, 0>); break;
+                case 1: launch(k_tfwd_flat<V, 8, 1>); break;
+                case 2: launch(k_tfwd_flat<V, 16, 0>); break;
+                default: launch(k_tfwd_flat<V, 16, 1>); break;
+                }
+            } else {
+                switch (sv) {
+                case 0: launch(k_tfwd_flat<V, 8, 0>); break;
+                case 1: launch(k_tfwd_flat<V, 8, 1>); break;
+                case 2: launch(k_tfwd_flat<V, 16, 0>); break;
+                case 3: launch(k_tfwd_flat<V, 16, 1>); break;
+                case 4: launch(k_tfwd_flat<V, 0, 0>); break;
+                default: launch(k_tfwd_flat<V, 0, 1>); break;
+                }
             }
             return 0;
         }
@@ -1218,10 +1458,22 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
                                                                B->partial_cols, B->ctrl);
         };
         if (cfg >= N_BWD_CFG) {
-            switch (cfg - N_BWD_CFG) {
-            case 0: launch(k_tbwd_flat<V, MODE, 8, 8, 1>, 512); break;
-            case 1: launch(k_tbwd_flat<V, MODE, 8, 16, 0>, 512); break;
-            default: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
+            int sv = cfg - N_BWD_CFG;
+            if constexpr (std::is_same<V, double>::value) {
+                if (sv >= 3) sv = 0;
+                switch (sv) {
+                case 0: launch(k_tbwd_flat<V, MODE, 8, 8, 1>, 512); break;
+                case 1: launch(k_tbwd_flat<V, MODE, 8, 16, 0>, 512); break;
+                default: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
+                }
+            } else {
+                switch (sv) {
+                case 0: launch(k_tbwd_flat<V, MODE, 8, 8, 1>, 512); break;
+                case 1: launch(k_tbwd_flat<V, MODE, 8, 16, 0>, 512); break;
+                case 2: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
+                case 3: launch(k_tbwd_flat<V, MODE, 8, 0, 0>, 512); break;
+                default: launch(k_tbwd_flat<V, MODE, 8, 0, 1>, 512); break;
+                }
             }
             return 0;
         }
@@ -1253,16 +1505,17 @@ void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStr
 void launch_tiled_resid(const TiledOp& T, const SirtBufs& B, StopParams sp, int k, hipStream_t s, hipEvent_t mid) {
     launch_fwd_pieces(T, B.x, B.ctrl, s);
     if (mid) AIR_HIP(hipEventRecord(mid, s));
+    const unsigned grid = tiled_reduce_grid(T.m);
+    const bool defer = B.ring != nullptr && k >= 1;
+    double* red = defer ? B.ring + (size_t)(k % B.ring_slots) * grid * 4 : B.partial;
     if (sp.type == AIR_STOP_ME)
-        k_treduce<true><<<reduce_grid(T.m), BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(),
-                                                           T.partial.as<float>(), T.m, B.b, B.M, B.r, B.Mr, B.partial,
-                                                           B.hist, B.ctrl, sp, k);
+        k_treduce<true><<<grid, BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(), T.partial.as<float>(),
+                                               T.m, B.b, B.M, B.r, B.Mr, red, B.hist, B.ctrl, sp, k);
     else
-        k_treduce<false><<<reduce_grid(T.m), BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(),
-                                                            T.partial.as<float>(), T.m, B.b, B.M, B.r, B.Mr, B.partial,
-                                                            B.hist, B.ctrl, sp, k);
+        k_treduce<false><<<grid, BLOCK, 0, s>>>(T.rp_ptr.as<int64_t>(), T.rp_idx.as<int>(), T.partial.as<float>(),
+                                                T.m, B.b, B.M, B.r, B.Mr, red, B.hist, B.ctrl, sp, k);
     AIR_HIP(hipGetLastError());
-    launch_finalize_rows(B, (int)reduce_grid(T.m), sp, k, s);
+    if (!defer) launch_finalize_rows(B, (int)grid, sp, k, s);
 }
 
 void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int line, int k, hipStream_t s) {
@@ -1278,8 +1531,9 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
 #include <utility>
 void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
     // stream forms: entries per lane and the prefetch flag
-    static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}};
-    static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}};
+    // entries per lane (0: the lane form, 8 per lane in whole-lane segments) and the prefetch flag
+    static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}};
+    static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}};
     const int fs = T.fwd_cfg - N_FWD_CFG, bs = T.bwd_cfg - N_BWD_CFG;
     *fl = fs >= 0 ? FS[fs][0] : FWD_CFG[T.fwd_cfg][0];
     *fu = fs >= 0 ? FS[fs][1] : FWD_CFG[T.fwd_cfg][1];
@@ -1297,6 +1551,10 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         if (got >= 2) {
             T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + N_FWD_STREAM - 1);
             T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0) - 1);
+            if (!lane_form_ok(T.dtype)) {          // the lane forms' FP64 stand-ins
+                if (T.fwd_cfg >= N_FWD_CFG + 4) T.fwd_cfg -= 4;
+                if (T.bwd_cfg >= N_BWD_CFG + 3) T.bwd_cfg = N_BWD_CFG;
+            }
             if (got == 3 && T.fwd_cfg < N_FWD_CFG) set_fwd_blocks(T, std::min(std::max(par, 64), 4096), s);
             if (got == 3 && T.fwd_cfg >= N_FWD_CFG) set_fwd_waves(T, std::min(std::max(par, 512), 1 << 20), s);
             return;
@@ -1339,6 +1597,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
     for (int wi = 0; wi < 3; ++wi) {
         set_fwd_waves(T, WT[wi], s);
         for (int sv = 0; sv < N_FWD_STREAM; ++sv) {
+            if (sv >= 4 && !lane_form_ok(T.dtype)) continue;
             const int c = N_FWD_CFG + sv;
             float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
             T.tune_fwd_ms[3 * N_FWD_CFG + 3 * sv + wi] = t / 4;
@@ -1348,6 +1607,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
     set_fwd_waves(T, best_wt, s);
     best = 1e30f;
     for (int c = 0; c < N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0); ++c) {
+        if (c >= N_BWD_CFG + 3 && !lane_form_ok(T.dtype)) continue;
         float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });
         T.tune_bwd_ms[c] = t / 4;
         if (t < best) { best = t; T.bwd_cfg = c; }

@@ -520,7 +520,8 @@ def main():
                             "fwd_stream": L["fwd_stream"],
                             "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
                                        L["bwd_loads"]],
-                            "tune_fwd_us": L["tune_fwd_us"][:27], "tune_bwd_us": L["tune_bwd_us"]}
+                            "tune_fwd_us": L["tune_fwd_us"][:33], "tune_bwd_us": L["tune_bwd_us"][:9],
+                            "fwd_entries": L["fwd_entries"], "bwd_entries": L["bwd_entries"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     # the whole C-ABI call as a host caller sees it: b and x0 from host memory,
     # weights, r0, the K iterations and X back over PCIe (rho given)

@@ -250,14 +250,18 @@ typedef struct {
     int64_t lds_bytes;    /* dynamic LDS of the backward kernel */
     int     fwd_stream;             /* forward kernel: 0 per-piece lane groups, 1 flat stream of whole waves */
     int     fwd_lanes, fwd_loads;   /* autotuned shape: lanes per piece, vector loads per round
-                                       (stream form: entries per lane, 1 if the next round is prefetched) */
+                                       (stream form: entries per lane, 0 for the lane form, and 1 if the
+                                       next round is prefetched) */
     int     bwd_lanes, bwd_loads;   /* lanes per column, vector loads per round (stream form as forward) */
     int     fwd_pieces_per_block;   /* pieces per block (stream form: target entries per wave) */
-    float   tune_fwd_us[32];        /* autotuner timings: per-piece shape c at 256/512/1024 pieces per
-                                       block -> [3c+i], c < 5; stream form v (8 or 16 entries per lane,
-                                       without/with prefetch) at 2048/4096/8192 entries per wave -> [15+3v+i] */
-    float   tune_bwd_us[8];         /* backward shapes: per-column 0-3, stream form 4-6 (8 entries per lane
-                                       with prefetch, 16 without, 16 with) */
+    int64_t fwd_entries, bwd_entries;  /* stored entries of the two streams: every row piece and every
+                                          column padded to a multiple of 8 (the lane form) */
+    float   tune_fwd_us[40];        /* autotuner timings: per-piece shape c at 256/512/1024 pieces per
+                                       block -> [3c+i], c < 5; stream form v (v < 4: 8 or 16 entries per
+                                       lane, without/with prefetch; v = 4, 5: the lane form without/with
+                                       prefetch) at 2048/4096/8192 entries per wave -> [15+3v+i] */
+    float   tune_bwd_us[12];        /* backward shapes: per-column 0-3, stream form 4-8 (8 entries per
+                                       lane with prefetch, 16 without, 16 with, lane form without, with) */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 

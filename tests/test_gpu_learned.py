@@ -98,10 +98,12 @@ def test_learned_stop_and_dp_fallback(setup):
     assert info["stoprule"] == 2 and 0 < info["finaliter"] < 200
 
 
-def test_trained_network_gains_over_plain(setup):
-    """Trained on 600 problems (three rounds of dataset aggregation) and judged on
-    32 held-out ones: the learned run reaches the plain method's final error in
-    fewer iterations and usually ends at a lower error."""
+def test_training_on_engine_trajectories(setup):
+    """Trained on 600 problems (three rounds of dataset aggregation) with the
+    engine's kernels: the loss falls and, on 32 held-out problems, the learned
+    run mostly reaches the plain method's final error and usually ends lower.
+    A training set this small varies from run to run; the gain itself is
+    measured on the committed network below."""
     A, _ = setup
     params, hist = Lm.train_on_policy(A, N, range(200, 800), 120, rounds=3, epochs=40)
     assert np.all(np.isfinite(params)) and hist[-1] < hist[0]
@@ -109,8 +111,29 @@ def test_trained_network_gains_over_plain(setup):
     rows = Lm.evaluate(A, N, range(9000, 9032), params, lam, 300)
     s = Lm.summarize(rows)
     print(s)
-    # 32 problems and a small training set: the bounds leave room for the
-    # run-to-run spread of training; the 512-problem study is the measurement
     assert s["reached_plain_err_frac"] >= 0.65
-    assert s["iters_to_plain_err_over_plain_median"] <= 0.85
     assert s["learned_err_le_plain_frac"] >= 0.6
+
+
+def test_committed_network_gains_over_plain():
+    """The committed network (trained on the GPU on paralleltomo N=256, 180
+    angles, p=362) against plain SIRT (train_relaxpar's lambda, DP with tau
+    1.02 ||e||) on 48 held-out seeded problems of that geometry: the learned
+    run reaches the plain method's error in well under the plain method's
+    iterations and wall time.  Problems where it never does count as failures
+    (infinite ratio) in the medians."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "airtoolsii_amd", "data",
+                        "relaxation_net_sart_n256.npy")
+    params = np.load(path, allow_pickle=False).astype(np.float32)
+    assert params.shape == (5378,)
+    A = air.paralleltomo(256, np.arange(0.0, 180.0, 1.0), 362)
+    lam = Lm.plain_relaxpar(A, 256, 999, 60)
+    rows = Lm.evaluate(A, 256, range(100000, 100048), params, lam, 400)
+    s = Lm.summarize(rows)
+    print(s)
+    assert s["reached_plain_err_frac"] >= 0.8
+    assert s["iters_to_plain_err_over_plain_median"] is not None
+    assert s["iters_to_plain_err_over_plain_median"] <= 0.6
+    assert s["time_to_plain_error_over_plain_median"] is not None
+    assert s["time_to_plain_error_over_plain_median"] <= 0.65

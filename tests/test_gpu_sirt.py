@@ -216,12 +216,16 @@ def test_train_relaxpar_matches_oracle(oracle):
 # Every forward/backward kernel shape of the tiled layout (the autotuner picks
 # one per matrix): (forward shape, backward shape, pieces per block or entries
 # per wave).  Forward shapes 0-4 are per-piece lane groups, 5-8 the stream
-# form (8 or 16 entries per lane, without/with prefetch); backward shapes 0-3
-# are per-column lane groups, 4-6 the stream form (8 entries with prefetch,
-# 16 without, 16 with).  N=96 gives 4 forward tiles with padded columns.
+# form (8 or 16 entries per lane, without/with prefetch), 9-10 the lane form
+# (whole-lane segments, without/with prefetch); backward shapes 0-3 are
+# per-column lane groups, 4-6 the stream form (8 entries with prefetch, 16
+# without, 16 with), 7-8 the lane form.  N=96 gives 4 forward tiles with
+# padded columns.
 @pytest.mark.parametrize("cfg", ["0,0,256", "1,1,512", "2,2,1024", "3,3,256", "4,0,512",
                                  "5,1,512", "6,2,2048", "7,3,8192", "8,0,100000",
-                                 "5,4,2048", "7,5,4096", "0,6,256", "8,6,512"])
+                                 "5,4,2048", "7,5,4096", "0,6,256", "8,6,512",
+                                 "9,7,2048", "10,8,8192", "9,8,512", "10,7,100000", "0,7,256", "10,0,4096",
+                                 "5,8,1024"])
 def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
     monkeypatch.setenv("AIR_TILED_CFG", cfg)
@@ -229,9 +233,13 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     L = A.layout()
     f, bcfg, par = (int(v) for v in cfg.split(","))
     assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
-    assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16}.get(bcfg, L["bwd_lanes"])
+    assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16, 7: 0, 8: 0}.get(bcfg, L["bwd_lanes"])
     if f >= 5:
-        assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1)][f - 5]
+        assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1), (0, 0), (0, 1)][f - 5]
+    # the lane form's streams: every segment padded to whole lanes of 8 entries
+    assert L["fwd_entries"] % 8 == 0 and L["bwd_entries"] % 8 == 0
+    assert L["nnz"] <= L["fwd_entries"] < L["nnz"] + 8 * L["pieces"]
+    assert L["nnz"] <= L["bwd_entries"] < L["nnz"] + 8 * L["n_internal"]
     assert L["fwd_pieces_per_block"] == (max(par, 512) if f >= 5 else par)
     Ao = oracle.paralleltomo(N, theta, p)
     x = oracle.phantomgallery("threephases", N, None, 3).ravel(order="F")
@@ -244,7 +252,7 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
-@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512"])
+@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512", "10,8,4096", "9,7,512"])
 def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
     """Per-piece and stream forward kernels agree on every storage type."""
     opts = {"rho": c1["rho"]["sart"], "lbound": 0}
@@ -344,3 +352,30 @@ def test_fallback_when_a_tile_exceeds_lds(oracle):
         X, _, _ = air.sirt(method, A, b, 30, None, opts)
         Xo, _, _ = oracle.sirt(method, Ao, b, 30, None, opts)
         assert rel(X[:, -1], Xo[:, -1]) <= TOL, method
+
+
+def test_deferred_norms_equal_direct_finalize(c1, oracle):
+    """With the stop rule 'none' the tiled loop defers the residual norms (block
+    partials kept in a ring of 32 iterations and reduced afterwards).  The run
+    in uneven chunks that cross the ring's wrap must give the iterates of the
+    same run with a per-iteration finalize (DP with taudelta 0, which never
+    fires), and the oracle's residual history."""
+    from airtoolsii_amd.sirt import SirtRun
+    opts = {"rho": c1["rho"]["cimmino"], "lbound": 0}
+    K = 90
+    run = SirtRun("cimmino", c1["A"], c1["b"], K, None, opts)
+    for n in (7, 40, 1, 25, 30):                       # 103 > K: the last call stops at K
+        run.iterate(n)
+    x1, info1 = run.fetch()
+    run2 = SirtRun("cimmino", c1["A"], c1["b"], K, None, dict(opts, stoprule={"type": "DP", "taudelta": 0.0}))
+    run2.iterate(K)
+    x2, info2 = run2.fetch()
+    assert info1["finaliter"] == info2["finaliter"] == K and info1["stoprule"] == info2["stoprule"] == 0
+    assert np.array_equal(x1, x2)
+    np.testing.assert_allclose(info1["resnorm"], info2["resnorm"], rtol=1e-12)
+    _, _, _, rn = oracle.sirt("cimmino", c1["Ao"], c1["b"], K, None, opts, return_resnorm=True)
+    np.testing.assert_allclose(info1["resnorm"], rn[: K + 1], rtol=1e-5)
+    # the whole call (save list, chunked loop) on the deferred path
+    X, info, _ = air.cimmino(c1["A"], c1["b"], [5, 33, 64, K], None, opts)
+    assert info["finaliter"] == K and np.array_equal(X[:, -1], x1)
+    np.testing.assert_allclose(info["resnorm"], info1["resnorm"], rtol=1e-12)
