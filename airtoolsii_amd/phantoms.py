@@ -7,6 +7,8 @@ is what NumPy's legacy ``RandomState(seed).random_sample`` produces.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 
@@ -101,6 +103,114 @@ def grains(N, numGrains=None, seed=None):
     return im / im.max()
 
 
+_TI_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "binary_training_image.txt")
+
+
+def binary_training_image():
+    """The 81 x 84 training image of phantomgallery.m:677-2641 (getTI)."""
+    with open(_TI_PATH) as f:
+        return np.array([[int(c) for c in line.strip()] for line in f if line.strip()], np.uint8)
+
+
+def clique_histogram(Z):
+    """CompHistTrain(Z, ones(3), 1), phantomgallery.m:458-509: counts of the
+    3 x 3 patterns of Z, pattern id = 1 + sum_i n_i 2^(i-1) over the window
+    read column-major; H[id - 1] is returned (512 values)."""
+    Z = np.asarray(Z, np.int64)
+    r, c = Z.shape
+    ids = np.zeros((r - 2, c - 2), np.int64)
+    for dc in range(3):
+        for dr in range(3):
+            ids += Z[dr:dr + r - 2, dc:dc + c - 2] << (dr + 3 * dc)
+    return np.bincount(ids.ravel(), minlength=512).astype(np.float64)
+
+
+def binary(N, seed=None):
+    """binary, phantomgallery.m:236-253 with sequential_simulation_full_clique
+    (:511-611): a 3 x 3 clique histogram of the training image and its mirror
+    (+eps) drives a sequential simulation.  The first 3 x 3 pattern is drawn
+    from the whole histogram, then the first three rows are extended column by
+    column (three new pixels per draw), then each later row starts with three
+    new pixels at its left end and continues one pixel at a time.  Every draw
+    takes one rand: the patterns compatible with the known pixels (marg2id,
+    :630-659, ids ascending) get pdf = H/sum(H), cdf = cumsum(pdf)/sum(pdf), and
+    the first pattern with cdf > rand is taken.  Sums run left to right.
+
+    A one-pixel draw has two candidates, ids v and v + 256 for the 8 known
+    pixels v, so its outcome is 1 exactly when rand >= cdf(v), the first cdf
+    entry; that value is tabulated once per v with the same operations."""
+    N = int(N)
+    if N < 3:
+        raise ValueError("binary needs N >= 3")
+    rnd = iter(_rng(seed).random_sample(1 + (N - 3) * (N - 1)).tolist())
+    TI = binary_training_image()
+    Hl = (clique_histogram(TI) + clique_histogram(TI[:, ::-1]) + np.finfo(float).eps).tolist()
+
+    def seqsum(v):
+        s = 0.0
+        for t in v:
+            s += t
+        return s
+
+    def pick(ids, hs, total):
+        """First id whose cdf = cumsum(hs)/total exceeds the next rand."""
+        r = next(rnd)
+        c = 0.0
+        for q, t in zip(ids, hs):
+            c += t
+            if c / total > r:
+                return q
+        raise ValueError("Zero probabiltiy pattern event! Please add a homogeneous distribution")
+
+    def draw(ids):
+        h = [Hl[q] for q in ids]
+        sh = seqsum(h)
+        pdf = [t / sh for t in h]
+        return pick(ids, pdf, seqsum(pdf))
+
+    def first_cdf(v):
+        h0, h1 = Hl[v], Hl[v + 256]
+        sh = h0 + h1
+        p0, p1 = h0 / sh, h1 / sh
+        return p0 / (p0 + p1)
+
+    cut = [first_cdf(v) for v in range(256)]
+    sim = [[-1] * N for _ in range(N)]
+
+    def window(i0, j0):
+        """0-based ids compatible with the known pixels of the window at (i0, j0)."""
+        known = value = 0
+        for dc in range(3):
+            for dr in range(3):
+                v = sim[i0 + dr][j0 + dc]
+                if v >= 0:
+                    bit = 1 << (dr + 3 * dc)
+                    known |= bit
+                    if v:
+                        value |= bit
+        return [q for q in range(512) if (q & known) == value]
+
+    # the first pattern: cdf = cumsum(H)/sum(H) over all 512 ids
+    q = pick(range(512), Hl, seqsum(Hl))
+    for dc in range(3):
+        for dr in range(3):
+            sim[dr][dc] = (q >> (dr + 3 * dc)) & 1
+    for j in range(3, N):                          # rows 1..3, columns 4..N: column j of the window is new
+        q = draw(window(0, j - 2))
+        for dr in range(3):
+            sim[dr][j] = (q >> (dr + 6)) & 1
+    for i in range(3, N):
+        q = draw(window(i - 2, 0))                 # left end: row i of the window is new
+        for dc in range(3):
+            sim[i][dc] = (q >> (2 + 3 * dc)) & 1
+        row, up1, up2 = sim[i], sim[i - 1], sim[i - 2]
+        for j in range(3, N):                      # one new pixel, the window's bottom right
+            v = (up2[j - 2] | (up1[j - 2] << 1) | (row[j - 2] << 2) | (up2[j - 1] << 3) | (up1[j - 1] << 4)
+                 | (row[j - 1] << 5) | (up2[j] << 6) | (up1[j] << 7))
+            row[j] = 0 if cut[v] > next(rnd) else 1
+    return np.array(sim, np.float64)
+
+
 def tectonic(N):
     """phantomgallery.m:422-454: two plates, the right one (0.75) tilted up at its
     left end, the left one (1) bending down past column 5*round(N/13)."""
@@ -127,7 +237,7 @@ def tectonic(N):
 
 
 def phantomgallery(name, N, P1=None, P2=None, P3=None):
-    """phantomgallery.m:79-151 dispatch (shepplogan, smooth, threephases, grains, tectonic)."""
+    """phantomgallery.m:79-151 dispatch (shepplogan, smooth, binary, threephases, grains, tectonic)."""
     if name == "shepplogan":
         return shepplogan(N)
     if name == "smooth":
@@ -138,7 +248,9 @@ def phantomgallery(name, N, P1=None, P2=None, P3=None):
         return grains(N, P1, P2)
     if name == "tectonic":
         return tectonic(N)
-    if name in ("binary", "threephasessmooth", "fourphases", "ppower"):
+    if name == "binary":
+        return binary(N, P1)
+    if name in ("threephasessmooth", "fourphases", "ppower"):
         raise NotImplementedError(f"phantom '{name}' is not provided by this engine")
     raise ValueError("Illegal phantom name")
 

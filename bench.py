@@ -232,7 +232,7 @@ def batch_study(args):
     Ns = rng.integers(64, 385, P)
     nAs = rng.integers(30, 361, P)
     thetas = [np.arange(nA) * (180.0 / nA) for nA in nAs]
-    kinds = ("shepplogan", "smooth", "threephases", "grains")
+    kinds = ("shepplogan", "smooth", "binary", "threephases", "grains")
     K = args.batch_K
     t0 = time.perf_counter()
     A = air.paralleltomo_batch(Ns, thetas, method="cimmino")
@@ -244,6 +244,7 @@ def batch_study(args):
         kind = kinds[int(rng.integers(len(kinds)))]
         ps = int(rng.integers(1, 2 ** 31 - 1))
         img = {"shepplogan": lambda: phantoms.shepplogan(N), "smooth": lambda: phantoms.smooth(N, 4),
+               "binary": lambda: phantoms.binary(N, ps),
                "threephases": lambda: phantoms.threephases(N, 100, ps),
                "grains": lambda: phantoms.grains(N, None, ps)}[kind]()
         xs_true.append(np.asarray(img, np.float64).ravel(order="F"))

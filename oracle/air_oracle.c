@@ -219,6 +219,102 @@ void ora_phantom_grains(int N, int numGrains, int has_seed, uint32_t seed, doubl
     free(xG); free(yG);
 }
 
+/* CompHistTrain (phantomgallery.m:458-509) for the full 3x3 mask and two
+ * categories: every inner voxel's window, ids counted. */
+void ora_clique_hist(const uint8_t* Z, int nr, int nc, double* H)
+{
+    for (int q = 0; q < 512; ++q) H[q] = 0.0;
+    for (int j = 0; j + 2 < nc; ++j)
+        for (int i = 0; i + 2 < nr; ++i) {
+            int id = 0, bit = 0;
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r, ++bit)
+                    if (Z[(int64_t)(i + r) + (int64_t)(j + c) * nr]) id |= 1 << bit;
+            H[id] += 1.0;
+        }
+}
+
+/* One draw of sequential_simulation_full_clique: the ids (0-based) whose bits
+ * match the known window entries w[0..8] (-1 unknown, column-major), in
+ * ascending order as marg2id returns them; pdf = H/sum, cdf = cumsum/sum, the
+ * first id with cdf > rand.  Sums run left to right. */
+static int binary_draw(const double* H, const int* w, ora_mt* st, int64_t* ndraw)
+{
+    int ids[512], n = 0;
+    for (int q = 0; q < 512; ++q) {
+        int ok = 1;
+        for (int b = 0; b < 9 && ok; ++b)
+            if (w[b] >= 0 && ((q >> b) & 1) != w[b]) ok = 0;
+        if (ok) ids[n++] = q;
+    }
+    double sh = 0.0;
+    for (int k = 0; k < n; ++k) sh += H[ids[k]];
+    double pdf[512], sp = 0.0;
+    for (int k = 0; k < n; ++k) { pdf[k] = H[ids[k]] / sh; sp += pdf[k]; }
+    const double r = ora_mt_rand(st);
+    (*ndraw)++;
+    double c = 0.0;
+    for (int k = 0; k < n; ++k) {
+        c += pdf[k];
+        if (c / sp > r) return ids[k];
+    }
+    return -1;
+}
+
+int64_t ora_phantom_binary(int N, int has_seed, uint32_t seed, const uint8_t* ti, int tr, int tc, double* im)
+{
+    if (N < 3) return -1;
+    ora_mt st; ora_mt_seed(&st, has_seed ? seed : 0);
+    /* H = CompHistTrain(TI) + CompHistTrain(fliplr(TI)) + eps */
+    double H[512], Hf[512];
+    uint8_t* fl = (uint8_t*)malloc((size_t)tr * tc);
+    for (int j = 0; j < tc; ++j)
+        for (int i = 0; i < tr; ++i) fl[i + (int64_t)j * tr] = ti[i + (int64_t)(tc - 1 - j) * tr];
+    ora_clique_hist(ti, tr, tc, H);
+    ora_clique_hist(fl, tr, tc, Hf);
+    free(fl);
+    for (int q = 0; q < 512; ++q) H[q] = H[q] + Hf[q] + 2.220446049250313e-16;
+    int* sim = (int*)malloc(sizeof(int) * (size_t)N * N);
+#define SIM(i, j) sim[(int64_t)(i) + (int64_t)(j) * N]
+    for (int64_t q = 0; q < (int64_t)N * N; ++q) sim[q] = -1;
+    int64_t ndraw = 0;
+    int w[9];
+    /* the first pattern from the whole histogram: cdf = cumsum(H)/sum(H) */
+    {
+        double norm = 0.0;
+        for (int q = 0; q < 512; ++q) norm += H[q];
+        const double r = ora_mt_rand(&st);
+        ndraw++;
+        double c = 0.0;
+        int id = -1;
+        for (int q = 0; q < 512; ++q) {
+            c += H[q];
+            if (c / norm > r) { id = q; break; }
+        }
+        for (int b = 0; b < 9; ++b) SIM(b % 3, b / 3) = (id >> b) & 1;
+    }
+    /* rows 1..3, columns 4..N: the window's last column is new */
+    for (int j = 3; j < N; ++j) {
+        for (int b = 0; b < 9; ++b) w[b] = SIM(b % 3, j - 2 + b / 3);
+        const int id = binary_draw(H, w, &st, &ndraw);
+        for (int r = 0; r < 3; ++r) SIM(r, j) = (id >> (6 + r)) & 1;
+    }
+    for (int i = 3; i < N; ++i)
+        for (int j = 2; j < N; ++j) {
+            for (int b = 0; b < 9; ++b) w[b] = SIM(i - 2 + b % 3, j - 2 + b / 3);
+            const int id = binary_draw(H, w, &st, &ndraw);
+            if (j == 2) {                           /* left side: sim(i, 1:3) = n(end, :) */
+                for (int c = 0; c < 3; ++c) SIM(i, c) = (id >> (2 + 3 * c)) & 1;
+            } else {                                /* sim(i, j) = n(end, end) */
+                SIM(i, j) = (id >> 8) & 1;
+            }
+        }
+    for (int64_t q = 0; q < (int64_t)N * N; ++q) im[q] = (double)sim[q];
+#undef SIM
+    free(sim);
+    return ndraw;
+}
+
 /* ------------------------------------------------------------------ */
 /* Ray tracers                                                         */
 /* ------------------------------------------------------------------ */

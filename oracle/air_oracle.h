@@ -33,6 +33,12 @@ void ora_phantom_shepplogan(int N, double* im);                 /* :155-214 */
 void ora_phantom_smooth(int N, int p, double* im);              /* :216-232 */
 void ora_phantom_threephases(int N, int p, int has_seed, uint32_t seed, double* im); /* :257-292 */
 void ora_phantom_grains(int N, int numGrains, int has_seed, uint32_t seed, double* im); /* :352-389 */
+/* CompHistTrain(Z, ones(3), 1), :458-509: H[id-1] for the 3x3 windows of the
+ * nr x nc column-major 0/1 image Z (id = 1 + sum n_i 2^(i-1), window read column-major) */
+void ora_clique_hist(const uint8_t* Z, int nr, int nc, double* H);
+/* binary, :236-253 + sequential_simulation_full_clique :511-611 on the training
+ * image ti (tr x tc column-major, getTI :677-2641); returns the number of rand draws */
+int64_t ora_phantom_binary(int N, int has_seed, uint32_t seed, const uint8_t* ti, int tr, int tc, double* im);
 
 /* ---- ray tracers -> CSR (0-based, columns ascending, duplicates summed) */
 typedef struct ora_csr ora_csr;

@@ -77,6 +77,9 @@ def lib():
         L.ora_phantom_smooth.argtypes = [C.c_int, C.c_int, _f64p]
         L.ora_phantom_threephases.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, _f64p]
         L.ora_phantom_grains.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, _f64p]
+        L.ora_clique_hist.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
+        L.ora_phantom_binary.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_int, C.c_int, _f64p]
+        L.ora_phantom_binary.restype = C.c_int64
         L.ora_paralleltomo.restype = C.c_void_p
         L.ora_paralleltomo.argtypes = [C.c_int, _f64p, C.c_int, C.c_int, C.c_double]
         L.ora_fanlineartomo.restype = C.c_void_p
@@ -160,8 +163,33 @@ def mt_rand(seed, count):
 
 
 # ---------------------------------------------------------------- phantoms
+_TI_TXT = os.path.join(_HERE, "..", "airtoolsii_amd", "data", "binary_training_image.txt")
+
+
+def binary_training_image():
+    """getTI, phantomgallery.m:677-2641, read from the data file (one image row
+    per line of '0'/'1' characters) as an 81 x 84 uint8 array."""
+    rows = [line.strip() for line in open(_TI_TXT) if line.strip()]
+    return (np.frombuffer("".join(rows).encode(), np.uint8) - ord("0")).reshape(len(rows), len(rows[0]))
+
+
+def clique_histogram(Z):
+    """CompHistTrain(Z, ones(3), 1), phantomgallery.m:458-509, in C: H[id-1]."""
+    Z = np.asfortranarray(np.asarray(Z, np.uint8))
+    H = np.zeros(512)
+    lib().ora_clique_hist(Z.ctypes.data, Z.shape[0], Z.shape[1], H)
+    return H
+
+
+def binary_draws(N, seed):
+    """Number of rand draws binary(N, seed) takes (the C loop's count)."""
+    ti = np.asfortranarray(binary_training_image())
+    out = np.zeros(N * N)
+    return int(lib().ora_phantom_binary(N, 1, int(seed), ti.ctypes.data, ti.shape[0], ti.shape[1], out))
+
+
 def phantomgallery(name, N, P1=None, P2=None):
-    """phantomgallery.m:79-151 for shepplogan/smooth/threephases/grains.
+    """phantomgallery.m:79-151 for shepplogan/smooth/threephases/grains/binary.
     Returns the N x N image (column-major semantics: use ``im.ravel('F')``)."""
     L = lib()
     out = np.zeros(N * N, np.float64)
@@ -175,6 +203,12 @@ def phantomgallery(name, N, P1=None, P2=None):
     elif name == "grains":
         L.ora_phantom_grains(N, 0 if P1 is None else int(P1),
                              0 if P2 is None else 1, 0 if P2 is None else int(P2), out)
+    elif name == "binary":
+        ti = np.asfortranarray(binary_training_image())
+        nd = L.ora_phantom_binary(N, 0 if P1 is None else 1, 0 if P1 is None else int(P1),
+                                  ti.ctypes.data, ti.shape[0], ti.shape[1], out)
+        if nd < 0:
+            raise ValueError("binary needs N >= 3")
     else:
         raise ValueError("Illegal phantom name")
     return out.reshape((N, N), order="F")

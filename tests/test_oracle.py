@@ -1146,3 +1146,111 @@ def test_mlp_bf16_rounding():
     r = mlp.bf16(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -7      # ties to even
     assert abs(r[3] - v[3]) <= 2 ** -9 * abs(v[3]) and r[4] == 0.0
+
+
+# ------------------------------------------------------------------ binary phantom
+def _marg2id_literal(marginal):
+    """marg2id, phantomgallery.m:630-659, with its index arithmetic: 1-based
+    ids, index_tmp = reshape(1:2^sN, 2^(i-1), []) (column-major), the columns
+    marginal(i)+1 : 2 : end kept for each known entry."""
+    sN = len(marginal)
+    index = np.arange(1, 2 ** sN + 1)
+    known = [i for i in range(sN) if marginal[i] >= 0]
+    Index = np.zeros((2 ** sN, len(known)), np.int64)
+    for c, i in enumerate(known):
+        index_tmp = index.reshape((2 ** i, 2 ** sN // 2 ** i), order="F")
+        index_use = index_tmp[:, int(marginal[i])::2]
+        Index[index_use.ravel(order="F") - 1, c] = 1
+    return np.nonzero(Index.sum(axis=1) == len(known))[0] + 1
+
+
+def _id2n_literal(id_):
+    """id2n, :613-628: the base-2 digits of id-1, least significant first."""
+    n, q = np.zeros(9), id_ - 1
+    for i in range(9):
+        qi = q // 2
+        n[i] = q - 2 * qi
+        q = qi
+    return n
+
+
+def _binary_literal(N, seed, H):
+    """sequential_simulation_full_clique, :511-611, with NaN for unknown pixels,
+    column-major marginals and find(cdf > r, 1)."""
+    rs = np.random.RandomState(seed)
+    sim = np.full((N, N), np.nan)
+    cdf = np.cumsum(H) / H.sum()
+    id_ = int(np.nonzero(cdf > rs.random_sample())[0][0]) + 1
+    sim[0:3, 0:3] = _id2n_literal(id_).reshape((3, 3), order="F")
+
+    def draw(block):
+        marginal = block.ravel(order="F").copy()
+        marginal[np.isnan(marginal)] = -1
+        ids = _marg2id_literal(marginal)
+        pdf = H[ids - 1] / np.sum(H[ids - 1])
+        c = np.cumsum(pdf) / np.sum(pdf)
+        pos = int(np.nonzero(c > rs.random_sample())[0][0])
+        return _id2n_literal(ids[pos]).reshape((3, 3), order="F")
+
+    for j in range(3, N):
+        sim[0:3, j] = draw(sim[0:3, j - 2:j + 1])[:, -1]
+    for i in range(3, N):
+        for j in range(2, N):
+            n = draw(sim[i - 2:i + 1, j - 2:j + 1])
+            if j == 2:
+                sim[i, 0:3] = n[-1, :]
+            else:
+                sim[i, j] = n[-1, -1]
+    return sim
+
+
+def test_binary_training_image_and_histogram(oracle):
+    """getTI: 81 x 84, 1957 ones (one per IJ pair of phantomgallery.m:682-2638).
+    The clique histogram counts every 3 x 3 window once; its all-zero and
+    all-one bins equal the window sums 0 and 9 of a box filter; each bin equals
+    the count of windows matching its pattern (bits read column-major, as
+    id2n/reshape define them); the mirror's histogram is the histogram with the
+    window's first and last columns exchanged."""
+    from scipy.ndimage import correlate
+    from airtoolsii_amd import phantoms
+    TI = oracle.binary_training_image()
+    assert TI.shape == (81, 84) and int(TI.sum()) == 1957 and set(np.unique(TI)) == {0, 1}
+    assert np.array_equal(phantoms.binary_training_image(), TI)
+    H = oracle.clique_histogram(TI)
+    assert H.sum() == 79 * 82
+    box = correlate(TI.astype(np.int64), np.ones((3, 3), np.int64), mode="constant")[1:-1, 1:-1]
+    assert H[0] == np.sum(box == 0) and H[511] == np.sum(box == 9)
+    T = 2 * TI.astype(np.int64) - 1
+    for id1 in (2, 3, 5, 9, 17, 33, 65, 129, 257, 300, 448, 73, 146, 292):
+        P = _id2n_literal(id1).reshape((3, 3), order="F")
+        score = correlate(T, (2 * P - 1).astype(np.int64), mode="constant")[1:-1, 1:-1]
+        assert H[id1 - 1] == np.sum(score == 9), id1
+    Hf = oracle.clique_histogram(TI[:, ::-1])
+    swap = [((q & 0o007) << 6) | (q & 0o070) | ((q >> 6) & 0o007) for q in range(512)]
+    assert np.array_equal(Hf, H[swap])
+    assert not np.array_equal(Hf, H)
+    np.testing.assert_array_equal(phantoms.clique_histogram(TI), H)
+
+
+@pytest.mark.parametrize("N,seed", [(3, 1), (4, 7), (6, 2), (9, 11)])
+def test_binary_against_literal_simulation(oracle, N, seed):
+    from airtoolsii_amd import phantoms
+    TI = oracle.binary_training_image()
+    H = oracle.clique_histogram(TI) + oracle.clique_histogram(TI[:, ::-1]) + np.finfo(float).eps
+    ref = _binary_literal(N, seed, H)
+    assert np.array_equal(oracle.phantomgallery("binary", N, seed), ref)
+    assert np.array_equal(phantoms.binary(N, seed), ref)
+    assert oracle.binary_draws(N, seed) == 1 + (N - 3) * (N - 1)
+
+
+def test_binary_product_matches_oracle(oracle):
+    from airtoolsii_amd import phantomgallery
+    for N, seed in ((64, 1), (97, 5), (128, 3)):
+        a = phantomgallery("binary", N, seed)
+        b = oracle.phantomgallery("binary", N, seed)
+        assert np.array_equal(a, b), (N, seed)
+        # the training image is about a quarter ones; the simulation follows it loosely
+        assert 0.05 < a.mean() < 0.6
+    assert not np.array_equal(phantomgallery("binary", 64, 1), phantomgallery("binary", 64, 2))
+    with pytest.raises(ValueError):
+        phantomgallery("binary", 2, 1)
