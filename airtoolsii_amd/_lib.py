@@ -61,7 +61,8 @@ class Layout(C.Structure):
                 ("lds_bytes", C.c_int64), ("fwd_stream", C.c_int), ("fwd_lanes", C.c_int), ("fwd_loads", C.c_int),
                 ("bwd_lanes", C.c_int), ("bwd_loads", C.c_int), ("fwd_pieces_per_block", C.c_int),
                 ("fwd_entries", C.c_int64), ("bwd_entries", C.c_int64),
-                ("tune_fwd_us", C.c_float * 40), ("tune_bwd_us", C.c_float * 12)]
+                ("tune_fwd_us", C.c_float * 64), ("tune_bwd_us", C.c_float * 16),
+                ("fwd_step", C.c_int), ("fwd_step_ok", C.c_int)]
 
 
 class RowOptions(C.Structure):

@@ -54,7 +54,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     const int dt = parts[0].dtype;
     int64_t m = 0, n = 0, nint = 0, nnz = 0, nfe = 0, nbe = 0, P = 0, R = 0, tf = 0, tb = 0, staged = 0, rpi = 0;
     int maxruns = 0, maxrows = 0;
-    bool flags = true;
+    bool flags = true, step = true;
     for (auto& T : parts) {
         AIR_CHECK(T.ready && T.dtype == dt, AIR_E_STATE, "batch parts must be tiled with one storage type");
         m += T.m;
@@ -71,6 +71,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         maxruns = std::max(maxruns, T.maxruns);
         maxrows = std::max(maxrows, T.maxrows);
         flags = flags && T.bwd_flags;
+        step = step && T.fstep_ok;
         rpi += T.npieces;
     }
     AIR_CHECK(nint < INT32_MAX && m < INT32_MAX && P < INT32_MAX && R < INT32_MAX, AIR_E_UNSUPPORTED,
@@ -103,6 +104,12 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     AIR_HIP(hipMemsetAsync((char*)B.fcode.p + 2 * nfe, 0, 2 * AIR_SPARSE_PAD, s));
     AIR_HIP(hipMemsetAsync((char*)B.bval.p + vb * nbe, 0, vb * AIR_SPARSE_PAD, s));
     AIR_HIP(hipMemsetAsync((char*)B.blr.p + 2 * nbe, 0, 2 * AIR_SPARSE_PAD, s));
+    // step words are tile-local, so they concatenate as they are (one per 8 forward entries)
+    B.fstep_ok = step;
+    if (step) {
+        B.fstep.alloc(4 * (size_t)(nfe / 8 + AIR_SPARSE_PAD));
+        AIR_HIP(hipMemsetAsync((char*)B.fstep.p + 4 * (nfe / 8), 0, 4 * AIR_SPARSE_PAD, s));
+    }
     B.piece_ptr.alloc(8 * (size_t)(P + 1));
     B.rp_ptr.alloc(8 * (size_t)(m + 1));
     B.rp_idx.alloc(4 * (size_t)std::max<int64_t>(P, 1));
@@ -120,6 +127,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         shift32(T.perm, T.n, oint, B.perm, on, s);
         copy_bytes(T.fval, vb * (size_t)T.nfe, B.fval, vb * (size_t)oef, s);
         copy_bytes(T.fcode, 2 * (size_t)T.nfe, B.fcode, 2 * (size_t)oef, s);
+        if (step) copy_bytes(T.fstep, 4 * (size_t)(T.nfe / 8), B.fstep, 4 * (size_t)(oef / 8), s);
         shift64(T.piece_ptr, T.npieces, oef, B.piece_ptr, oP, s);
         shift64(T.rp_ptr, T.m, oP, B.rp_ptr, om, s);      // rp_idx positions of a part start at its piece offset
         shift32(T.rp_idx, T.npieces, oP, B.rp_idx, oP, s);

@@ -925,11 +925,23 @@ static void upload_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowpt
     AIR_HIP(hipStreamSynchronize(s));
 }
 
+// A caller's matrix with a square number of columns is tiled as an N x N
+// image (the column-major pixel order of the reference's test problems), so a
+// traced matrix handed over as plain CSR/CSC gets the image tiles and, where
+// its rows are straight rays, the step-coded forward; any other matrix is
+// just as valid in that column order.
+static int image_side(int64_t n) {
+    int64_t r = (int64_t)std::llround(std::sqrt((double)n));
+    while (r * r > n) --r;
+    while ((r + 1) * (r + 1) <= n) ++r;
+    return (r * r == n && r > 0 && r < (1 << 20)) ? (int)r : 0;
+}
+
 int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* colind,
                        const double* val, int dtype) {
     return guard([&] {
         upload_csr(e, m, n, rowptr, colind, val, dtype);
-        finish_matrix(e, 0);
+        finish_matrix(e, image_side(n));
     });
 }
 
@@ -955,7 +967,7 @@ int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colpt
         e->m = e->A.nrows;
         e->n = e->A.ncols;
         e->T = TiledOp();
-        if (!tiled_disabled()) build_tiled(e->A, 0, e->T, e->stream);
+        if (!tiled_disabled()) build_tiled(e->A, image_side(e->n), e->T, e->stream, !e->no_tune);
     });
 }
 
@@ -1472,8 +1484,10 @@ int air_layout_info(air_engine* e, air_layout* out) {
             L.fwd_pieces_per_block = L.fwd_stream ? T.wave_target : T.ppb;
             L.fwd_entries = T.nfe;
             L.bwd_entries = T.nbe;
-            for (int q = 0; q < 40; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
-            for (int q = 0; q < 12; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
+            for (int q = 0; q < 64; ++q) L.tune_fwd_us[q] = 1000.f * T.tune_fwd_ms[q];
+            for (int q = 0; q < 16; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
+            L.fwd_step = tiled_fwd_step(T) ? 1 : 0;
+            L.fwd_step_ok = T.fstep_ok ? 1 : 0;
         }
         *out = L;
     });

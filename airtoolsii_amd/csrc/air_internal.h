@@ -250,7 +250,11 @@ struct TiledOp {
     // backward codes carry a column-start flag in bit 15 when every tile has
     // fewer than 32768 distinct rows (required by the stream form)
     bool bwd_flags = false;
-    float tune_fwd_ms[40] = {}, tune_bwd_ms[12] = {};
+    // step-coded forward lanes (one word per 8 entries); fstep_ok when every
+    // piece of the matrix is a unit-step staircase
+    DevBuf fstep;
+    bool fstep_ok = false;
+    float tune_fwd_ms[64] = {}, tune_bwd_ms[16] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
@@ -262,6 +266,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s);
 void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);
 void set_fwd_waves(TiledOp& T, int target_entries, hipStream_t s);
 bool tiled_fwd_stream(const TiledOp& T);
+bool tiled_fwd_step(const TiledOp& T);
 void tiled_shapes(const TiledOp& T, int* fwd_lanes, int* fwd_loads, int* bwd_lanes, int* bwd_loads);
 void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s);
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);

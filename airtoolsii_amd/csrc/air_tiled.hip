@@ -405,19 +405,59 @@ __device__ __forceinline__ void stream_segments(int64_t E0, int64_t E1, const V*
     if (lane == 0 && kopen >= 0) close(kopen, carry);
 }
 
+// One round of the lane form's segmented scan.  Lane l holds s, the sum of
+// its 8 products, and start, whether a segment starts at its first entry.
+// Across the lanes a forward segmented scan (DPP row shifts and row
+// broadcasts, each step gated by the lane's distance to the last start at or
+// below it) gives every lane the sum of its segment so far; the lane before
+// each start closes its segment, the segment open at lane 63 is carried
+// (wave-uniform) into the next round, and on the last round lane nvalid-1
+// closes the wave's last segment.  knext counts segment starts seen so far.
+template <typename Close>
+__device__ __forceinline__ void lanes_round(float s, bool start, bool last_round, int nvalid, int lane,
+                                            int& knext, bool& open, float& carry, Close&& close) {
+    const uint64_t le_mask = (2ull << lane) - 1ull;       // lanes 0..lane (all 64 for lane 63)
+    const uint64_t sm = __ballot(start);
+    const uint64_t below = sm & le_mask;
+    const int dist = lane - (below ? 63 - __builtin_clzll(below) : -1);
+    if (open && (sm & 1ull)) {
+        if (lane == 0) close(knext - 1, carry);        // the carried segment ends before lane 0
+    } else if (lane == 0) {
+        s += carry;                                    // lane 0 continues the carried segment
+    }
+    float w = s;
+#pragma unroll
+    for (int D = 1; D <= 8; D <<= 1) {
+        float t;
+        if (D == 1) t = dpp_f<0x111, 0xf>(w);
+        else if (D == 2) t = dpp_f<0x112, 0xf>(w);
+        else if (D == 4) t = dpp_f<0x114, 0xf>(w);
+        else t = dpp_f<0x118, 0xf>(w);
+        if (dist >= D) w += t;                         // no start in (lane - D, lane]
+    }
+    {
+        const float t = dpp_f<0x142, 0xa>(w);          // lane 15 of rows 0, 2 -> rows 1, 3
+        if (((lane >> 4) & 1) && dist > (lane & 15)) w += t;
+    }
+    {
+        const float t = dpp_f<0x143, 0xc>(w);          // lane 31 -> rows 2, 3
+        if (lane >= 32 && dist > lane - 32) w += t;
+    }
+    const bool next_starts = lane < 63 && ((sm >> (lane + 1)) & 1ull);
+    if (next_starts || (last_round && lane == nvalid - 1)) close(knext - 1 + __popcll(below), w);
+    knext += __popcll(sm);
+    carry = readlane_f(w, 63);
+    open = true;
+}
+
 // Segmented sums over one wave's run of whole segments in the lane form of the
 // layout: every segment starts at a multiple of 8 entries and is padded to a
 // multiple of 8 (zero values, codes pointing into the segment), so the 8
 // consecutive entries a lane takes in a round of 512 belong to one segment.
 // A lane's products are summed in registers (no per-entry masks or splits);
-// its first code carries the start flag (bit 0).  Across the lanes a forward
-// segmented scan (DPP row shifts and row broadcasts, each step gated by the
-// lane's distance to the last start at or below it) gives every lane the sum
-// of its segment so far; the lane before each start closes its segment, the
-// segment open at lane 63 is carried (wave-uniform) into the next round, and
-// the last valid lane of the last round closes the wave's last segment.
-// close(k, sum) is called once per segment, k counted from k0.  PF = 1 issues
-// the next round's loads before the current one is summed.
+// its first code carries the start flag (bit 0), and lanes_round does the
+// rest.  close(k, sum) is called once per segment, k counted from k0.  PF = 1
+// issues the next round's loads before the current one is summed.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __restrict__ val,
                                              const unsigned short* __restrict__ code, const float* lds, int k0,
@@ -426,7 +466,6 @@ __device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __
     constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
     typedef unsigned u4v __attribute__((ext_vector_type(4)));
     const int lane = (int)(threadIdx.x & 63);
-    const uint64_t le_mask = (2ull << lane) - 1ull;       // lanes 0..lane (all 64 for lane 63)
     auto lds_at = [&](unsigned byte) -> float { return *(const float*)((const char*)lds + byte); };
     auto load_round = [&](int64_t rb, float (&vv)[8], unsigned (&cc)[4]) {
         const int64_t q = rb + 8 * lane;
@@ -468,38 +507,8 @@ __device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __
             s += v[2 * h] * lds_at(c[h] & 0xffffu);
             s += v[2 * h + 1] * lds_at(c[h] >> 16);
         }
-        const uint64_t sm = __ballot((c[0] & 1u) != 0u);
-        const uint64_t below = sm & le_mask;
-        const int dist = lane - (below ? 63 - __builtin_clzll(below) : -1);
-        if (open && (sm & 1ull)) {
-            if (lane == 0) close(knext - 1, carry);        // the carried segment ends before lane 0
-        } else if (lane == 0) {
-            s += carry;                                    // lane 0 continues the carried segment
-        }
-        float w = s;
-#pragma unroll
-        for (int D = 1; D <= 8; D <<= 1) {
-            float t;
-            if (D == 1) t = dpp_f<0x111, 0xf>(w);
-            else if (D == 2) t = dpp_f<0x112, 0xf>(w);
-            else if (D == 4) t = dpp_f<0x114, 0xf>(w);
-            else t = dpp_f<0x118, 0xf>(w);
-            if (dist >= D) w += t;                         // no start in (lane - D, lane]
-        }
-        {
-            const float t = dpp_f<0x142, 0xa>(w);          // lane 15 of rows 0, 2 -> rows 1, 3
-            if (((lane >> 4) & 1) && dist > (lane & 15)) w += t;
-        }
-        {
-            const float t = dpp_f<0x143, 0xc>(w);          // lane 31 -> rows 2, 3
-            if (lane >= 32 && dist > lane - 32) w += t;
-        }
         const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
-        const bool next_starts = lane < 63 && ((sm >> (lane + 1)) & 1ull);
-        if (next_starts || (last_round && lane == nvalid - 1)) close(knext - 1 + __popcll(below), w);
-        knext += __popcll(sm);
-        carry = readlane_f(w, 63);
-        open = true;
+        lanes_round(s, (c[0] & 1u) != 0u, last_round, nvalid, lane, knext, open, carry, close);
         rb += 512;
         if (!last_round) {
             if (PF) {
@@ -509,6 +518,82 @@ __device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __
                 for (int i = 0; i < 4; ++i) c[i] = cn[i];
             } else {
                 load_round(rb, v, c);
+            }
+        }
+    }
+}
+
+// The step-coded lane form of the forward stream.  Inside a 64 x 64 tile a
+// straight ray's cells form a monotone staircase: ordered along the ray with
+// the tile's x increasing, each cell is the previous one moved by +1 in x,
+// by dy = +1 or -1 in y, or by both (a corner crossing).  A lane's 8 entries
+// are then one 32-bit word instead of 8 column codes:
+//   bits  0-11  tile position x * 64 + y of the lane's first entry
+//   bit   12    the lane's first entry starts a segment
+//   bit   13    dy = -1 (else +1)
+//   bits 14+2i, 15+2i (i = 1..7)  entry i's move: x step, y step (both 0 on
+//                padding entries, which carry value 0)
+// and the tile of the operand sits in LDS in the same x * 64 + y order.
+__device__ __forceinline__ float step_products(const float (&v)[8], unsigned w, const float* lds) {
+    unsigned p = w & 0xfffu;
+    const int yd = (w & 0x2000u) ? -1 : 1;
+    float s = v[0] * lds[p];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const unsigned sx = (w >> (14 + 2 * i)) & 1u, sy = (w >> (15 + 2 * i)) & 1u;
+        p = (unsigned)((int)(p + (sx << 6)) + __mul24((int)sy, yd));
+        s += v[i] * lds[p & 0xfffu];
+    }
+    return s;
+}
+
+template <typename V, int PF, typename Close>
+__device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __restrict__ val,
+                                             const unsigned* __restrict__ words, const float* lds, int k0,
+                                             Close&& close) {
+    static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
+    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
+    const int lane = (int)(threadIdx.x & 63);
+    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) {
+        const int64_t q = rb + 8 * lane;
+        if (q < E1) {
+#pragma unroll
+            for (int h = 0; h < NV; ++h) {
+                float t[VE];
+                load_vals<V, float>(val + q + VE * h, t);
+#pragma unroll
+                for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
+            }
+            ww = __builtin_nontemporal_load(words + (q >> 3));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) vv[i] = 0.f;
+            ww = 0u;
+        }
+    };
+    int64_t rb = E0;
+    int knext = k0;
+    bool open = false;
+    float carry = 0.f;
+    float v[8];
+    unsigned w;
+    load_round(rb, v, w);
+    while (rb < E1) {
+        const bool last_round = rb + 512 >= E1;
+        float vn[8];
+        unsigned wn = 0u;
+        if (PF && !last_round) load_round(rb + 512, vn, wn);
+        const float s = step_products(v, w, lds);
+        const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
+        lanes_round(s, ((w >> 12) & 1u) != 0u, last_round, nvalid, lane, knext, open, carry, close);
+        rb += 512;
+        if (!last_round) {
+            if (PF) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = vn[i];
+                w = wn;
+            } else {
+                load_round(rb, v, w);
             }
         }
     }
@@ -546,6 +631,47 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd_flat(const int* __restrict__ blk
     auto close = [&](int k, auto sum) { partial[k] = (float)sum; };
     if constexpr (EPL == 0) stream_lanes<V, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, pw0, close);
     else stream_segments<V, EPL, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fcode, xs, pw0, close);
+}
+
+// Tile position x * 64 + y of the nested internal index q (bits 0-3: y mod 16,
+// 4-7: x mod 16, 8-9: y / 16, 10-11: x / 16, as nested_index numbers a tile).
+__device__ __host__ __forceinline__ unsigned nested_to_plain(unsigned q) {
+    const unsigned x = ((q >> 10) & 3u) * 16u + ((q >> 4) & 15u);
+    const unsigned y = ((q >> 8) & 3u) * 16u + (q & 15u);
+    return x * 64u + y;
+}
+
+// Forward product in the step-coded lane form: as k_tfwd_flat, with the tile
+// of x staged in LDS in plain x * 64 + y order (each aligned group of 4
+// nested entries is 4 consecutive y of one x, so it moves as one float4).
+template <typename V, int PF>
+__global__ __launch_bounds__(BLOCK) void k_tfwd_step(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
+                                                     int nwaves, int nblk, const int64_t* __restrict__ piece_ptr,
+                                                     const V* __restrict__ fval, const unsigned* __restrict__ fstep,
+                                                     const float* __restrict__ x, float* __restrict__ partial,
+                                                     const Ctrl* ctrl) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int WPB = BLOCK / 64;
+    __shared__ __align__(16) float xs[FT];
+    if (ctrl && load_stop(ctrl)) return;
+    const int b = xcd_block(blockIdx.x, nblk);
+    const int t = blk_tile[b];
+    {
+        const f4* xt = (const f4*)(x + (int64_t)t * FT);
+        f4 tmp[FT / (4 * BLOCK)];
+#pragma unroll
+        for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
+#pragma unroll
+        for (int i = 0; i < FT / (4 * BLOCK); ++i)
+            ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + BLOCK * i)) >> 2] = tmp[i];
+    }
+    __syncthreads();
+    const int w = b * WPB + (int)(threadIdx.x >> 6);
+    if (w >= nwaves) return;
+    const int pw0 = wave_p0[w], pw1 = wave_p0[w + 1];
+    if (pw0 >= pw1) return;
+    auto close = [&](int k, float sum) { partial[k] = sum; };
+    stream_steps<V, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fstep, xs, pw0, close);
 }
 
 // y = sum of a row's partials (8 lanes per row)
@@ -1009,6 +1135,84 @@ __global__ void k_pad_lengths(const int64_t* __restrict__ ptr, int64_t nseg, int
         len[k] = (ptr[k + 1] - ptr[k] + 7) & ~(int64_t)7;
 }
 
+// Step-coded forward: put each piece's entries in ray order and check that
+// consecutive cells are unit moves.  A piece arrives in column order, which
+// for a tile's cells is (x, y) lexicographic; a staircase that descends in y
+// (the lowest or the highest y of its last x below that of its first x) is put
+// in ray order by reversing every run of equal x.  Moves must then be (0, dy), (1, 0) or
+// (1, dy); anything else
+// (a general matrix) sets *bad, and the layout keeps only the column codes.
+// One thread per piece, before the pieces are padded; the start flag stays on
+// the piece's first entry.
+template <typename V>
+__global__ void k_step_order(const int64_t* __restrict__ ptr, int64_t npieces, V* __restrict__ val,
+                             unsigned short* __restrict__ code, unsigned* __restrict__ bad) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npieces; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = ptr[k], e = ptr[k + 1];
+        if (e - s < 2) continue;
+        auto pos = [&](int64_t q) { return nested_to_plain((unsigned)code[q] >> 2); };
+        int64_t l = e - 1, f = s;     // first entry of the last x, last entry of the first x
+        while (l > s && (pos(l - 1) >> 6) == (pos(e - 1) >> 6)) --l;
+        while (f + 1 < e && (pos(f + 1) >> 6) == (pos(s) >> 6)) ++f;
+        const int dy = ((pos(l) & 63u) < (pos(s) & 63u) || (pos(e - 1) & 63u) < (pos(f) & 63u)) ? -1 : 1;
+        if (dy < 0) {
+            code[s] &= (unsigned short)0xfffeu;
+            int64_t a = s;
+            while (a < e) {
+                int64_t b = a + 1;
+                while (b < e && (pos(b) >> 6) == (pos(a) >> 6)) ++b;
+                for (int64_t i = a, j = b - 1; i < j; ++i, --j) {
+                    const V tv = val[i]; val[i] = val[j]; val[j] = tv;
+                    const unsigned short tc = code[i]; code[i] = code[j]; code[j] = tc;
+                }
+                a = b;
+            }
+            code[s] |= (unsigned short)1u;
+        }
+        unsigned prev = pos(s);
+        for (int64_t q = s + 1; q < e; ++q) {
+            const unsigned cur = pos(q);
+            const int dx = (int)(cur >> 6) - (int)(prev >> 6), dyy = (int)(cur & 63u) - (int)(prev & 63u);
+            const bool ok = (dx == 0 && dyy == dy) || (dx == 1 && (dyy == 0 || dyy == dy));
+            if (!ok) atomicOr(bad, 1u);
+            prev = cur;
+        }
+    }
+}
+
+// One step word per lane of 8 entries of the padded forward stream (layout in
+// step_products).  Padding entries repeat the last code, so they do not move.
+__global__ void k_step_words(const unsigned short* __restrict__ code, int64_t nlanes, unsigned* __restrict__ words,
+                             unsigned* __restrict__ bad) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nlanes; g += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned c0 = code[8 * g];
+        unsigned prev = nested_to_plain(c0 >> 2);
+        unsigned w = prev | ((c0 & 1u) << 12);
+        int neg = -1;
+        bool ok = true;
+        for (int i = 1; i < 8; ++i) {
+            const unsigned cur = nested_to_plain((unsigned)code[8 * g + i] >> 2);
+            const int dx = (int)(cur >> 6) - (int)(prev >> 6), dyy = (int)(cur & 63u) - (int)(prev & 63u);
+            unsigned st = 0u;
+            if (dx == 1) st |= 1u;
+            else if (dx != 0) ok = false;
+            if (dyy == 1 || dyy == -1) {
+                st |= 2u;
+                const int sg = dyy < 0 ? 1 : 0;
+                if (neg < 0) neg = sg;
+                else if (neg != sg) ok = false;
+            } else if (dyy != 0) {
+                ok = false;
+            }
+            w |= st << (14 + 2 * i);
+            prev = cur;
+        }
+        if (neg == 1) w |= 1u << 13;
+        words[g] = w;
+        if (!ok) atomicOr(bad, 1u);
+    }
+}
+
 // ---------------------------------------------------------------- host: build
 // Forward blocks: up to ppb consecutive pieces of one tile each.
 void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s) {
@@ -1208,8 +1412,34 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
     AIR_HIP(hipGetLastError());
     AIR_HIP(hipMemcpyAsync(T.piece_ptr.as<int64_t>() + P, &nnz, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipStreamSynchronize(s));
+    // step-coded lanes: traced image geometries with FP32/BF16/FP16 storage
+    const bool try_step = N > 0 && T.dtype != AIR_F64;
+    DevBuf bad;
+    if (try_step) {
+        bad.alloc(4);
+        AIR_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+        dispatch_dtype(T.dtype, [&](auto v) {
+            using V = decltype(v);
+            k_step_order<V><<<grid_for(P), BLOCK, 0, s>>>(T.piece_ptr.as<int64_t>(), P, T.fval.as<V>(),
+                                                          T.fcode.as<unsigned short>(), bad.as<unsigned>());
+            return 0;
+        });
+        AIR_HIP(hipGetLastError());
+    }
     // pieces padded to whole lanes of 8 entries (the lane form)
     T.nfe = pad_segments(T.piece_ptr, P, T.fval, T.fcode, T.dtype, s);
+    if (try_step) {
+        const int64_t nl = T.nfe / 8;
+        pad_alloc<unsigned>(T.fstep, nl, s);
+        k_step_words<<<grid_for(nl), BLOCK, 0, s>>>(T.fcode.as<unsigned short>(), nl, T.fstep.as<unsigned>(),
+                                                    bad.as<unsigned>());
+        AIR_HIP(hipGetLastError());
+        unsigned hb = 1u;
+        AIR_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        T.fstep_ok = hb == 0u;
+        if (!T.fstep_ok) T.fstep.release();
+    }
     // forward blocks are cut per tile from the piece -> tile map (kept on the host)
     T.piece_tile_h.resize(P);
     AIR_HIP(hipMemcpyAsync(T.piece_tile_h.data(), piece_tile.p, 4 * P, hipMemcpyDeviceToHost, s));
@@ -1366,8 +1596,11 @@ static BwdArgs bwd_args(const TiledOp& T) {
 static const int FWD_CFG[][3] = {{16, 2, 1}, {16, 2, 2}, {16, 1, 2}, {8, 2, 2}, {32, 1, 2}};
 static const int BWD_CFG[][3] = {{64, 2, 1}, {64, 2, 2}, {64, 1, 2}, {32, 2, 2}};
 static constexpr int N_FWD_CFG = 5, N_BWD_CFG = 4;
-static constexpr int N_FWD_STREAM = 6;     // stream forward: 8/16 entries per lane, without/with prefetch,
-                                           // then the lane form without/with prefetch
+static constexpr int N_FWD_STREAM = 8;     // stream forward: 8/16 entries per lane, without/with prefetch,
+                                           // then the lane form and the step-coded lane form, each
+                                           // without/with prefetch
+static constexpr int N_WAVE_TARGETS = 4;   // entries per wave the autotuner tries for the stream forms
+static const int WAVE_TARGETS[N_WAVE_TARGETS] = {2048, 4096, 8192, 16384};
 static constexpr int N_BWD_STREAM = 5;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch,
                                            // then the lane form without/with prefetch
 // the lane form sums in FP32 registers: FP64 storage runs the 8-entry stream form instead
@@ -1407,7 +1640,13 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                                                 T.piece_ptr.as<int64_t>(), T.fval.as<V>(),
                                                 T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
             };
+            auto launch_step = [&](auto kern) {
+                kern<<<T.nblk_w, BLOCK, 0, s>>>(T.wblk_tile.as<int>(), T.wave_p0.as<int>(), T.nwaves, T.nblk_w,
+                                                T.piece_ptr.as<int64_t>(), T.fval.as<V>(), T.fstep.as<unsigned>(),
+                                                x_int, T.partial.as<float>(), ctrl);
+            };
             int sv = cfg - N_FWD_CFG;
+            if (sv >= 6 && !T.fstep_ok) sv -= 2;            // no step words: the lane form
             if constexpr (std::is_same<V, double>::value) {
                 if (sv >= 4) sv -= 4;
                 switch (sv) {
@@ -1423,7 +1662,9 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                 case 2: launch(k_tfwd_flat<V, 16, 0>); break;
                 case 3: launch(k_tfwd_flat<V, 16, 1>); break;
                 case 4: launch(k_tfwd_flat<V, 0, 0>); break;
-                default: launch(k_tfwd_flat<V, 0, 1>); break;
+                case 5: launch(k_tfwd_flat<V, 0, 1>); break;
+                case 6: launch_step(k_tfwd_step<V, 0>); break;
+                default: launch_step(k_tfwd_step<V, 1>); break;
                 }
             }
             return 0;
@@ -1440,6 +1681,7 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
 }
 
 bool tiled_fwd_stream(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG; }
+bool tiled_fwd_step(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG + 6 && T.fstep_ok; }
 
 template <int MODE>
 static void launch_bwd(const TiledOp& T, const float* y, float* gout, const SirtBufs* B, double lam, hipStream_t s,
@@ -1532,7 +1774,7 @@ void launch_tiled_update(const TiledOp& T, const SirtBufs& B, double lam, int li
 void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
     // stream forms: entries per lane and the prefetch flag
     // entries per lane (0: the lane form, 8 per lane in whole-lane segments) and the prefetch flag
-    static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}};
+    static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}, {0, 0}, {0, 1}};
     static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}};
     const int fs = T.fwd_cfg - N_FWD_CFG, bs = T.bwd_cfg - N_BWD_CFG;
     *fl = fs >= 0 ? FS[fs][0] : FWD_CFG[T.fwd_cfg][0];
@@ -1551,6 +1793,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         if (got >= 2) {
             T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + N_FWD_STREAM - 1);
             T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0) - 1);
+            if (T.fwd_cfg >= N_FWD_CFG + 6 && !T.fstep_ok) T.fwd_cfg -= 2;   // the lane form instead
             if (!lane_form_ok(T.dtype)) {          // the lane forms' FP64 stand-ins
                 if (T.fwd_cfg >= N_FWD_CFG + 4) T.fwd_cfg -= 4;
                 if (T.bwd_cfg >= N_BWD_CFG + 3) T.bwd_cfg = N_BWD_CFG;
@@ -1591,17 +1834,17 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         }
     }
     set_fwd_blocks(T, best_ppb, s);
-    // stream forms at 2048/4096/8192 entries per wave
-    static const int WT[] = {2048, 4096, 8192};
+    // stream forms at WAVE_TARGETS entries per wave
     int best_wt = T.wave_target;
-    for (int wi = 0; wi < 3; ++wi) {
-        set_fwd_waves(T, WT[wi], s);
+    for (int wi = 0; wi < N_WAVE_TARGETS; ++wi) {
+        set_fwd_waves(T, WAVE_TARGETS[wi], s);
         for (int sv = 0; sv < N_FWD_STREAM; ++sv) {
             if (sv >= 4 && !lane_form_ok(T.dtype)) continue;
+            if (sv >= 6 && !T.fstep_ok) continue;
             const int c = N_FWD_CFG + sv;
             float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
-            T.tune_fwd_ms[3 * N_FWD_CFG + 3 * sv + wi] = t / 4;
-            if (t < best) { best = t; T.fwd_cfg = c; best_wt = WT[wi]; }
+            T.tune_fwd_ms[3 * N_FWD_CFG + N_WAVE_TARGETS * sv + wi] = t / 4;
+            if (t < best) { best = t; T.fwd_cfg = c; best_wt = WAVE_TARGETS[wi]; }
         }
     }
     set_fwd_waves(T, best_wt, s);

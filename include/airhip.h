@@ -256,12 +256,17 @@ typedef struct {
     int     fwd_pieces_per_block;   /* pieces per block (stream form: target entries per wave) */
     int64_t fwd_entries, bwd_entries;  /* stored entries of the two streams: every row piece and every
                                           column padded to a multiple of 8 (the lane form) */
-    float   tune_fwd_us[40];        /* autotuner timings: per-piece shape c at 256/512/1024 pieces per
+    float   tune_fwd_us[64];        /* autotuner timings: per-piece shape c at 256/512/1024 pieces per
                                        block -> [3c+i], c < 5; stream form v (v < 4: 8 or 16 entries per
                                        lane, without/with prefetch; v = 4, 5: the lane form without/with
-                                       prefetch) at 2048/4096/8192 entries per wave -> [15+3v+i] */
-    float   tune_bwd_us[12];        /* backward shapes: per-column 0-3, stream form 4-8 (8 entries per
+                                       prefetch; v = 6, 7: the step-coded lane form without/with prefetch)
+                                       at 2048/4096/8192/16384 entries per wave -> [15+4v+i]; 0: not run */
+    float   tune_bwd_us[16];        /* backward shapes: per-column 0-3, stream form 4-8 (8 entries per
                                        lane with prefetch, 16 without, 16 with, lane form without, with) */
+    int     fwd_step;               /* 1: the forward runs the step-coded lane form (one 32-bit word of
+                                       unit moves per 8 entries instead of 16-bit column codes) */
+    int     fwd_step_ok;            /* 1: every row piece is a unit-step staircase, so the step-coded
+                                       form is available to the autotuner */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 

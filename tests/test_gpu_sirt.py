@@ -217,7 +217,8 @@ def test_train_relaxpar_matches_oracle(oracle):
 # one per matrix): (forward shape, backward shape, pieces per block or entries
 # per wave).  Forward shapes 0-4 are per-piece lane groups, 5-8 the stream
 # form (8 or 16 entries per lane, without/with prefetch), 9-10 the lane form
-# (whole-lane segments, without/with prefetch); backward shapes 0-3 are
+# (whole-lane segments, without/with prefetch), 11-12 the step-coded lane form
+# (one word of unit moves per 8 entries); backward shapes 0-3 are
 # per-column lane groups, 4-6 the stream form (8 entries with prefetch, 16
 # without, 16 with), 7-8 the lane form.  N=96 gives 4 forward tiles with
 # padded columns.
@@ -225,7 +226,7 @@ def test_train_relaxpar_matches_oracle(oracle):
                                  "5,1,512", "6,2,2048", "7,3,8192", "8,0,100000",
                                  "5,4,2048", "7,5,4096", "0,6,256", "8,6,512",
                                  "9,7,2048", "10,8,8192", "9,8,512", "10,7,100000", "0,7,256", "10,0,4096",
-                                 "5,8,1024"])
+                                 "5,8,1024", "11,7,4096", "12,8,2048", "11,0,100000", "12,5,512"])
 def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
     monkeypatch.setenv("AIR_TILED_CFG", cfg)
@@ -235,7 +236,9 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
     assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16, 7: 0, 8: 0}.get(bcfg, L["bwd_lanes"])
     if f >= 5:
-        assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1), (0, 0), (0, 1)][f - 5]
+        assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1), (0, 0), (0, 1),
+                                                    (0, 0), (0, 1)][f - 5]
+    assert L["fwd_step_ok"] == 1 and L["fwd_step"] == (1 if f >= 11 else 0)
     # the lane form's streams: every segment padded to whole lanes of 8 entries
     assert L["fwd_entries"] % 8 == 0 and L["bwd_entries"] % 8 == 0
     assert L["nnz"] <= L["fwd_entries"] < L["nnz"] + 8 * L["pieces"]
@@ -252,7 +255,8 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
-@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512", "10,8,4096", "9,7,512"])
+@pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512", "10,8,4096", "9,7,512",
+                                 "11,8,4096", "12,7,512"])
 def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
     """Per-piece and stream forward kernels agree on every storage type."""
     opts = {"rho": c1["rho"]["sart"], "lbound": 0}
@@ -379,3 +383,32 @@ def test_deferred_norms_equal_direct_finalize(c1, oracle):
     X, info, _ = air.cimmino(c1["A"], c1["b"], [5, 33, 64, K], None, opts)
     assert info["finaliter"] == K and np.array_equal(X[:, -1], x1)
     np.testing.assert_allclose(info["resnorm"], info1["resnorm"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("geom", ["fanlinear", "fancurved", "seismic"])
+def test_step_coded_forward_other_geometries(oracle, monkeypatch, geom):
+    """The step-coded forward on other traced geometries: straight rays from a
+    point source cross every tile as unit-step staircases (both y directions,
+    corner crossings); a geometry the build cannot encode keeps the column
+    codes.  Either way the iterates match the oracle's."""
+    monkeypatch.setenv("AIR_TILED_CFG", "12,8,4096")
+    N = 80
+    if geom == "fanlinear":
+        theta = np.arange(0.0, 360.0, 4.0)
+        A, Ao = air.fanlineartomo(N, theta, 113), oracle.fanlineartomo(N, theta, 113)
+    elif geom == "fancurved":
+        theta = np.arange(0.0, 360.0, 4.0)
+        A, Ao = air.fancurvedtomo(N, theta, 113), oracle.fancurvedtomo(N, theta, 113)
+    else:
+        A, Ao = air.seismictomo(N, 60, 90), oracle.seismictomo(N, 60, 90)
+    L = A.layout()
+    assert L["tiled"] == 1
+    if geom != "seismic":
+        assert L["fwd_step_ok"] == 1 and L["fwd_step"] == 1
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    opts = {"rho": oracle.spectral_radius(Ao, *oracle.weights("sart", Ao)), "lbound": 0}
+    X, _, _ = air.sart(A, b, [4, 20], None, opts)
+    Xo, _, _ = oracle.sirt("sart", Ao, b, [4, 20], None, opts)
+    for j in range(2):
+        assert rel(X[:, j], Xo[:, j]) <= TOL, (geom, j, rel(X[:, j], Xo[:, j]))
