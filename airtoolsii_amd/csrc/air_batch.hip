@@ -54,7 +54,9 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     const int dt = parts[0].dtype;
     int64_t m = 0, n = 0, nint = 0, nnz = 0, nfe = 0, nbe = 0, P = 0, R = 0, tf = 0, tb = 0, staged = 0, rpi = 0;
     int maxruns = 0, maxrows = 0;
-    bool flags = true, step = true;
+    bool flags = true, step = true, bstep = true;
+    int64_t nbse = 0;
+    size_t lds_bs = 0;
     for (auto& T : parts) {
         AIR_CHECK(T.ready && T.dtype == dt, AIR_E_STATE, "batch parts must be tiled with one storage type");
         m += T.m;
@@ -72,6 +74,9 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         maxrows = std::max(maxrows, T.maxrows);
         flags = flags && T.bwd_flags;
         step = step && T.fstep_ok;
+        bstep = bstep && T.bstep_ok;
+        nbse += T.nbse;
+        if (T.bstep_ok) lds_bs = std::max(lds_bs, T.lds_bs - 8 * (size_t)T.maxruns);   // the slots
         rpi += T.npieces;
     }
     AIR_CHECK(nint < INT32_MAX && m < INT32_MAX && P < INT32_MAX && R < INT32_MAX, AIR_E_UNSUPPORTED,
@@ -110,6 +115,21 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         B.fstep.alloc(4 * (size_t)(nfe / 8 + AIR_SPARSE_PAD));
         AIR_HIP(hipMemsetAsync((char*)B.fstep.p + 4 * (nfe / 8), 0, 4 * AIR_SPARSE_PAD, s));
     }
+    // step-coded backward: words and strides are tile-local; column pointers are
+    // shifted; the staging's run tables take the batch's longest run list
+    lds_bs += 8 * (size_t)maxruns;
+    bstep = bstep && lds_bs <= AIR_TILE_LDS_MAX;
+    B.bstep_ok = bstep;
+    if (bstep) {
+        B.nbse = nbse;
+        B.lds_bs = lds_bs;
+        B.bsval.alloc(vb * (size_t)(nbse + AIR_SPARSE_PAD));
+        AIR_HIP(hipMemsetAsync((char*)B.bsval.p + vb * nbse, 0, vb * AIR_SPARSE_PAD, s));
+        B.bsword.alloc(4 * (size_t)(nbse / 8 + AIR_SPARSE_PAD));
+        AIR_HIP(hipMemsetAsync((char*)B.bsword.p + 4 * (nbse / 8), 0, 4 * AIR_SPARSE_PAD, s));
+        B.bstep_ptr.alloc(8 * (size_t)(nint + 1));
+        B.bstep_R.alloc(4 * (size_t)std::max<int64_t>(tb, 1));
+    }
     B.piece_ptr.alloc(8 * (size_t)(P + 1));
     B.rp_ptr.alloc(8 * (size_t)(m + 1));
     B.rp_idx.alloc(4 * (size_t)std::max<int64_t>(P, 1));
@@ -121,7 +141,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     B.run_len.alloc(4 * (size_t)std::max<int64_t>(R, 1));
     B.piece_tile_h.reserve((size_t)P);
     B.piece_ptr_h.reserve((size_t)P + 1);
-    int64_t om = 0, on = 0, oint = 0, oef = 0, oeb = 0, oP = 0, oR = 0, otf = 0, otb = 0;
+    int64_t om = 0, on = 0, oint = 0, oef = 0, oeb = 0, oP = 0, oR = 0, otf = 0, otb = 0, obs = 0;
     for (auto& T : parts) {
         // forward
         shift32(T.perm, T.n, oint, B.perm, on, s);
@@ -143,6 +163,13 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
             AIR_HIP(hipGetLastError());
         }
         shift64(T.cell_ptr, T.n_int, oeb, B.cell_ptr, oint, s);
+        if (bstep) {
+            copy_bytes(T.bsval, vb * (size_t)T.nbse, B.bsval, vb * (size_t)obs, s);
+            copy_bytes(T.bsword, 4 * (size_t)(T.nbse / 8), B.bsword, 4 * (size_t)(obs / 8), s);
+            shift64(T.bstep_ptr, T.n_int, obs, B.bstep_ptr, oint, s);
+            copy_bytes(T.bstep_R, 4 * (size_t)T.ntiles_b, B.bstep_R, 4 * (size_t)otb, s);
+            obs += T.nbse;
+        }
         shift32(T.run_ptr, T.ntiles_b, oR, B.run_ptr, otb, s);
         shift32(T.run_row, T.nruns, om, B.run_row, oR, s);
         copy_bytes(T.run_off, 4 * (size_t)T.nruns, B.run_off, 4 * (size_t)oR, s);
@@ -163,6 +190,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     AIR_HIP(hipMemcpyAsync(B.piece_ptr.as<int64_t>() + P, &nfe, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(B.rp_ptr.as<int64_t>() + m, &P, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(B.cell_ptr.as<int64_t>() + nint, &nbe, 8, hipMemcpyHostToDevice, s));
+    if (bstep) AIR_HIP(hipMemcpyAsync(B.bstep_ptr.as<int64_t>() + nint, &nbse, 8, hipMemcpyHostToDevice, s));
     const int Ri = (int)R;
     AIR_HIP(hipMemcpyAsync(B.run_ptr.as<int>() + tb, &Ri, 4, hipMemcpyHostToDevice, s));
     B.piece_ptr_h.push_back(nfe);

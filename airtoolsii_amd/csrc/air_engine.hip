@@ -1488,6 +1488,9 @@ int air_layout_info(air_engine* e, air_layout* out) {
             for (int q = 0; q < 16; ++q) L.tune_bwd_us[q] = 1000.f * T.tune_bwd_ms[q];
             L.fwd_step = tiled_fwd_step(T) ? 1 : 0;
             L.fwd_step_ok = T.fstep_ok ? 1 : 0;
+            L.bwd_step = tiled_bwd_step(T) ? 1 : 0;
+            L.bwd_step_ok = T.bstep_ok ? 1 : 0;
+            L.bwd_step_entries = T.nbse;
         }
         *out = L;
     });

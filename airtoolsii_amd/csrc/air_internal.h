@@ -225,6 +225,9 @@ void csr_to_csc(const DevSparse& A, DevSparse& AT, hipStream_t s);
 //  backward (A' y): per 16x16 sub-tile, the columns of A with (value, local
 //    row); the sub-tile's distinct rows of y are staged in LDS (as runs of
 //    consecutive rows) and each column is summed with LDS gathers.
+// dynamic LDS a backward tile may stage (two 512-thread blocks per CU)
+constexpr size_t AIR_TILE_LDS_MAX = 60 * 1024;
+
 struct TiledOp {
     bool ready = false;
     int dtype = AIR_F32;
@@ -254,6 +257,13 @@ struct TiledOp {
     // piece of the matrix is a unit-step staircase
     DevBuf fstep;
     bool fstep_ok = false;
+    // step-coded backward lanes: values with bridging zeros, their column
+    // pointers, one word per 8 entries, the slot stride per tile, and the LDS
+    // of the strided staging; bstep_ok when every column could be encoded
+    DevBuf bsval, bstep_ptr, bsword, bstep_R;
+    int64_t nbse = 0;
+    size_t lds_bs = 0;
+    bool bstep_ok = false;
     float tune_fwd_ms[64] = {}, tune_bwd_ms[16] = {};
 };
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
@@ -267,6 +277,7 @@ void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);
 void set_fwd_waves(TiledOp& T, int target_entries, hipStream_t s);
 bool tiled_fwd_stream(const TiledOp& T);
 bool tiled_fwd_step(const TiledOp& T);
+bool tiled_bwd_step(const TiledOp& T);
 void tiled_shapes(const TiledOp& T, int* fwd_lanes, int* fwd_loads, int* bwd_lanes, int* bwd_loads);
 void tiled_backward_plain(const TiledOp& T, const float* y, float* g_int, hipStream_t s);
 void permute_to_internal(const TiledOp& T, const float* x_user, float* x_int, hipStream_t s);

@@ -34,7 +34,7 @@
 namespace air {
 
 static constexpr int FT = 4096;            // columns per forward tile (64 x 64)
-static constexpr size_t MAX_LDS_B = 60 * 1024;
+static constexpr size_t MAX_LDS_B = AIR_TILE_LDS_MAX;
 static constexpr int MAX_BT = 256;         // columns per backward tile
 
 // 16-bit index loads matching VecOf<V>::VE
@@ -964,6 +964,230 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __res
     }
 }
 
+// ---------------------------------------------------------------- backward, step-coded
+// The step-coded lane form of the backward stream.  A tile's rows are staged
+// with a fixed stride: run r (consecutive rows; for a sweep of parallel or fan
+// rays one run per angle) occupies LDS slots r * R .. r * R + R - 1, R the
+// tile's longest run, unused slots 0.  A column's entries (rows ascending) go
+// through the runs in order; consecutive entries of one run form a group.  An
+// entry either continues the group (next slot) or opens a group in the next
+// run at base + R + d, base the slot of the previous group's first entry and
+// d in {-1, 0, 1}, which covers a ray sweep in fine angular steps; anything
+// else is bridged at build time by entries with value 0 (k_bstep_walk).  A
+// lane's 8 entries are one 32-bit word:
+//   bits  0-13  slot of the first entry's group start (base)
+//   bits 14-16  first entry's slot minus base
+//   bits 15+2i, 16+2i (i = 1..7)  entry i: 0 continue the group (slot + 1),
+//                s > 0 open a group at base + R + s - 2
+//   bit   31    the lane's first entry starts a column
+// Padding entries continue the group with value 0; the staging leaves 8 zero
+// slots after the last run so they stay inside it.
+__device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w, const float* ys, int R) {
+    unsigned base = w & 0x3fffu;
+    unsigned addr = base + ((w >> 14) & 7u);
+    const int adv = R - 2;
+    float s = v[0] * ys[addr];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const unsigned st = (w >> (15 + 2 * i)) & 3u;
+        const unsigned nb = (unsigned)((int)(base + st) + adv);
+        base = st ? nb : base;
+        addr = st ? nb : addr + 1u;
+        s += v[i] * ys[addr];
+    }
+    return s;
+}
+
+template <typename V, int PF, typename Close>
+__device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* __restrict__ val,
+                                              const unsigned* __restrict__ words, const float* ys, int R, int k0,
+                                              Close&& close) {
+    static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
+    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
+    const int lane = (int)(threadIdx.x & 63);
+    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) {
+        const int64_t q = rb + 8 * lane;
+        if (q < E1) {
+#pragma unroll
+            for (int h = 0; h < NV; ++h) {
+                float t[VE];
+                load_vals<V, float>(val + q + VE * h, t);
+#pragma unroll
+                for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
+            }
+            ww = __builtin_nontemporal_load(words + (q >> 3));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) vv[i] = 0.f;
+            ww = 0u;
+        }
+    };
+    int64_t rb = E0;
+    int knext = k0;
+    bool open = false;
+    float carry = 0.f;
+    float v[8];
+    unsigned w;
+    load_round(rb, v, w);
+    while (rb < E1) {
+        const bool last_round = rb + 512 >= E1;
+        float vn[8];
+        unsigned wn = 0u;
+        if (PF && !last_round) load_round(rb + 512, vn, wn);
+        const float s = bstep_products(v, w, ys, R);
+        const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
+        lanes_round(s, (w >> 31) != 0u, last_round, nvalid, lane, knext, open, carry, close);
+        rb += 512;
+        if (!last_round) {
+            if (PF) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = vn[i];
+                w = wn;
+            } else {
+                load_round(rb, v, w);
+            }
+        }
+    }
+}
+
+// Stage the tile's runs of y with stride R (slot r * R + o = y[run_row + o]
+// for o < run_len, 0 elsewhere, plus 8 zero slots); returns the slot array.
+template <int NT>
+__device__ __forceinline__ float* stage_runs_strided(float* smem, int maxruns, int t, int R,
+                                                     const int* __restrict__ run_ptr, const int* __restrict__ run_row,
+                                                     const int* __restrict__ run_len, const float* __restrict__ y) {
+    int* rrow = (int*)smem;                 // [maxruns]
+    int* rlen = rrow + maxruns;             // [maxruns]
+    float* ys = (float*)(rlen + maxruns);
+    const int r0 = run_ptr[t], nr = run_ptr[t + 1] - r0;
+    for (int i = threadIdx.x; i < nr; i += NT) {
+        rrow[i] = run_row[r0 + i];
+        rlen[i] = run_len[r0 + i];
+    }
+    __syncthreads();
+    const int nslot = nr * R + 8;
+    const float invR = 1.0f / (float)R;
+    constexpr int UNR = 8;
+    for (int k0 = threadIdx.x; k0 < nslot; k0 += NT * UNR) {
+        float vv[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int k = k0 + u * NT;
+            const int r = (int)(((float)k + 0.5f) * invR);
+            const int o = k - r * R;
+            vv[u] = (k < nslot && r < nr && o < rlen[r]) ? y[rrow[r] + o] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int k = k0 + u * NT;
+            if (k < nslot) ys[k] = vv[u];
+        }
+    }
+    __syncthreads();
+    return ys;
+}
+
+struct BStepArgs {
+    const int64_t* col_ptr;   // column pointers of the step-coded stream
+    const unsigned* words;
+    const int* R;             // slot stride per tile
+    const int* run_ptr;
+    const int* run_row;
+    const int* run_len;
+    int maxruns;
+};
+
+// Backward product in the step-coded lane form: as k_tbwd_flat with the
+// strided staging and stream_bsteps.
+template <typename V, int MODE, int NW, int PF>
+__global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __restrict__ bval,
+                                                       const float* __restrict__ y, float* __restrict__ gout,
+                                                       double* __restrict__ g64, const float* __restrict__ D,
+                                                       float* __restrict__ x, double lam, int lb_kind, float lb,
+                                                       const float* __restrict__ lbv, int ub_kind, float ub,
+                                                       const float* __restrict__ ubv, double* red, Ctrl* ctrl) {
+    constexpr int NT = NW * 64;
+    static_assert(NT >= MAX_BT, "one thread per column in the epilogue");
+    extern __shared__ float smem[];
+    __shared__ int cpl[MAX_BT + 1];
+    __shared__ short necol[MAX_BT];
+    __shared__ float gt[MAX_BT];
+    __shared__ int wtot[MAX_BT / 64];
+    __shared__ double sh[1][NW];
+    if (MODE != 0 && load_stop(ctrl)) return;
+    if (MODE == 1 && isnan(lam)) lam = ctrl->lam_cur;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t c0 = (int64_t)t * MAX_BT;
+    const int64_t Et0 = a.col_ptr[c0];
+    const int R = a.R[t];
+    const int c = (int)threadIdx.x;
+    float xj = 0.f, dj = 1.f;
+    if (c < MAX_BT) cpl[c] = (int)(a.col_ptr[c0 + c] - Et0);
+    if (c == 0) cpl[MAX_BT] = (int)(a.col_ptr[c0 + MAX_BT] - Et0);
+    if (c < MAX_BT) {
+        gt[c] = 0.f;
+        if (MODE == 1) {
+            xj = x[c0 + c];
+            if (D) dj = D[c0 + c];
+        }
+    }
+    const float* ys = stage_runs_strided<NT>(smem, a.maxruns, t, R, a.run_ptr, a.run_row, a.run_len, y);
+    bool ne = false;
+    uint64_t nm = 0;
+    if (c < MAX_BT) {
+        ne = cpl[c + 1] > cpl[c];
+        nm = __ballot(ne);
+        if ((c & 63) == 0) wtot[c >> 6] = __popcll(nm);
+    }
+    __syncthreads();
+    int nne = 0, before = 0;
+#pragma unroll
+    for (int u = 0; u < MAX_BT / 64; ++u) {
+        before += u < (c >> 6) ? wtot[u] : 0;
+        nne += wtot[u];
+    }
+    if (ne) necol[before + __builtin_amdgcn_mbcnt_hi((unsigned)(nm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)nm, 0u))] =
+        (short)c;
+    __syncthreads();
+    {
+        const int wv = c >> 6;
+        const int Et = cpl[MAX_BT];
+        auto start_of = [&](int k) { return k < nne ? cpl[necol[k]] : Et; };
+        auto first_at = [&](int64_t goal) {
+            int lo = 0, hi = nne;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (start_of(mid) >= goal) hi = mid; else lo = mid + 1;
+            }
+            return lo;
+        };
+        const int ka = first_at((int64_t)Et * wv / NW), kb = first_at((int64_t)Et * (wv + 1) / NW);
+        auto close = [&](int k, float sum) { gt[necol[k]] = sum; };
+        if (ka < kb) stream_bsteps<V, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.words, ys, R, ka, close);
+    }
+    __syncthreads();
+    double dsum[1] = {0};
+    if (c < MAX_BT) {
+        const float g = gt[c];
+        const int64_t j = c0 + c;
+        if (MODE == 0) {
+            gout[j] = g;
+        } else if (MODE == 1) {
+            const double dg = D ? (double)dj * (double)g : (double)g;
+            x[j] = project((double)xj + lam * dg, (int)j, lb_kind, lb, lbv, ub_kind, ub, ubv);
+        } else {
+            const double gj = (double)g;
+            g64[j] = gj;
+            dsum[0] = D ? (double)D[j] * (gj * gj) : gj * gj;
+        }
+    }
+    if (MODE == 1 && threadIdx.x == 0 && blockIdx.x == 0) ctrl->lam_cur = lam;
+    if (MODE == 2) {
+        block_reduce<1, NT>(dsum, sh);
+        store_block_partials<1>(dsum, red);
+    }
+}
+
 __global__ void k_flag_col_starts(const int64_t* __restrict__ cell_ptr, int64_t ncols, unsigned short* __restrict__ blr) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x)
         if (cell_ptr[j] < cell_ptr[j + 1]) blr[cell_ptr[j]] |= (unsigned short)1u;
@@ -1210,6 +1434,124 @@ __global__ void k_step_words(const unsigned short* __restrict__ code, int64_t nl
         if (neg == 1) w |= 1u << 13;
         words[g] = w;
         if (!ok) atomicOr(bad, 1u);
+    }
+}
+
+// Step-coded backward, build: one thread per column walks its entries (rows
+// ascending) as (run, offset) pairs of the strided staging and emits the
+// stream that stream_bsteps decodes.  Each group of entries in one run gets a
+// base offset b_g <= its first offset o_g, reached from the previous base by
+// moves of -1, 0 or +1 per run; the entries from b_g up to the group's real
+// entries are zeros, and so is one entry per skipped run.  The bases come from
+// two sweeps: backwards, U_g = min(o_g, U_{g+1} + k) (k runs to the next
+// group) bounds each base so that every later group stays reachable going
+// down; forwards, b_g = min(U_g, b_{g-1} + k) takes the highest base reachable
+// going up, which is never below b_{g-1} - k.  With fine angular steps the
+// offsets move by at most one per run and b_g = o_g almost everywhere.  The
+// column is padded to whole lanes.  Pass 1 (FILL false) stores U_g at every
+// group's first entry and the padded count per column; pass 2 writes values
+// and lane words at optr.  A column whose stream does not fit the word (a
+// slot outside the staging, a lane starting more than 7 entries into its
+// group) sets *bad and the matrix keeps the column codes.
+template <typename V, bool FILL>
+__global__ void k_bstep_walk(const int64_t* __restrict__ cell_ptr, int64_t ncols, const unsigned short* __restrict__ blr,
+                             const V* __restrict__ bval, const int* __restrict__ run_ptr,
+                             const int* __restrict__ run_off, const int* __restrict__ Rt, int* __restrict__ ubuf,
+                             int64_t* __restrict__ count, const int64_t* __restrict__ optr, V* __restrict__ oval,
+                             unsigned* __restrict__ oword, unsigned* __restrict__ bad) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncols; c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = cell_ptr[c], e = cell_ptr[c + 1];
+        if (s == e) {
+            if (!FILL) count[c] = 0;
+            continue;
+        }
+        const int t = (int)(c / MAX_BT);
+        const int r0 = run_ptr[t], nr = run_ptr[t + 1] - r0;
+        const int R = Rt[t];
+        const unsigned lim = (unsigned)(nr * R + 8);
+        auto ro = [&](int64_t q, int& r, int& o) {
+            const int u = (int)(blr[q] >> 2);
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (run_off[r0 + mid] <= u) lo = mid; else hi = mid - 1;
+            }
+            r = lo;
+            o = u - run_off[r0 + lo];
+        };
+        if (!FILL) {                                    // backward sweep: U at each group's first entry
+            int rq, oq;
+            ro(e - 1, rq, oq);
+            int un = 0, rn = -1;
+            for (int64_t q = e - 1; q >= s; --q) {
+                int rp = -1, op = 0;
+                if (q > s) ro(q - 1, rp, op);
+                if (rp != rq) {                         // q opens its group
+                    const int u = rn < 0 ? oq : min(oq, un + (rn - rq));
+                    ubuf[q] = u;
+                    un = u;
+                    rn = rq;
+                }
+                rq = rp;
+                oq = op;
+            }
+        }
+        const int64_t o0 = FILL ? optr[c] : 0;
+        int64_t n = 0;
+        unsigned word = 0u;
+        bool ok = true;
+        auto emit = [&](V v, unsigned sym, unsigned addr, unsigned base) {
+            if (addr >= lim || base >= lim) ok = false;
+            const int i = (int)(n & 7);
+            if (i == 0) {
+                if (addr - base > 7u) ok = false;
+                word = (base & 0x3fffu) | (((addr - base) & 7u) << 14) | (n == 0 ? (1u << 31) : 0u);
+            } else {
+                word |= sym << (15 + 2 * i);
+            }
+            if (FILL) oval[o0 + n] = v;
+            ++n;
+            if (FILL && (n & 7) == 0) oword[(o0 + n) / 8 - 1] = word;
+        };
+        int cr, o;
+        ro(s, cr, o);
+        int b = ubuf[s];                                // the group's base offset
+        int co = b;                                     // offset of the last emitted entry
+        unsigned base = (unsigned)(cr * R + b), addr = base;
+        emit(b == o ? bval[s] : V{0}, 0u, addr, base);
+        while (co + 1 < o) { ++co; ++addr; emit(V{0}, 0u, addr, base); }
+        if (co < o) { ++co; ++addr; emit(bval[s], 0u, addr, base); }
+        for (int64_t q = s + 1; q < e && ok; ++q) {
+            int r;
+            ro(q, r, o);
+            if (r == cr) {
+                if (o <= co) { ok = false; break; }
+            } else {
+                const int k = r - cr;
+                if (k < 0) { ok = false; break; }
+                const int bn = min(ubuf[q], b + k);
+                for (int j = 1; j <= k; ++j) {
+                    const int d = max(-1, min(1, bn - b + (k - j)));
+                    b += d;
+                    cr += 1;
+                    base = (unsigned)(cr * R + b);
+                    addr = base;
+                    emit(j == k && bn == o ? bval[q] : V{0}, (unsigned)(d + 2), addr, base);
+                }
+                co = b;
+                if (co == o) continue;
+            }
+            while (co + 1 < o) { ++co; ++addr; emit(V{0}, 0u, addr, base); }
+            ++co;
+            ++addr;
+            emit(bval[q], 0u, addr, base);
+        }
+        while (ok && (n & 7) != 0) {
+            ++addr;
+            emit(V{0}, 0u, addr, base);
+        }
+        if (!ok) atomicOr(bad, 1u);
+        if (!FILL) count[c] = ok ? n : 0;
     }
 }
 
@@ -1569,6 +1911,71 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         T.run_ptr.alloc(4 * rp.size());
         AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
         AIR_HIP(hipStreamSynchronize(s));
+        // strided staging of the step-coded backward: slot stride R per tile (its longest run)
+        if (N > 0 && T.dtype != AIR_F64 && T.bwd_flags) {
+            std::vector<int> Rt(T.ntiles_b, 1);
+            size_t need = 0;
+            bool fits = true;
+            for (int64_t t = 0; t < T.ntiles_b; ++t) {
+                for (int r = rp[t]; r < rp[t + 1]; ++r) Rt[t] = std::max(Rt[t], rl[r]);
+                const int64_t slots = (int64_t)(rp[t + 1] - rp[t]) * Rt[t] + 8;
+                fits = fits && slots < 16384;
+                need = std::max(need, 4 * (2 * (size_t)maxruns) + 4 * (size_t)slots);
+            }
+            if (fits && need <= MAX_LDS_B) {
+                T.lds_bs = need;
+                T.bstep_R.alloc(4 * Rt.size());
+                AIR_HIP(hipMemcpyAsync(T.bstep_R.p, Rt.data(), 4 * Rt.size(), hipMemcpyHostToDevice, s));
+                AIR_HIP(hipStreamSynchronize(s));
+            }
+        }
+    }
+    if (T.lds_bs > 0) {
+        // step-coded backward lanes, walked from the unpadded columns
+        DevBuf cnt, bad, ubuf;
+        cnt.alloc(8 * (size_t)(T.n_int + 1));
+        ubuf.alloc(4 * (size_t)nnz);
+        bad.alloc(4);
+        AIR_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+        dispatch_dtype(T.dtype, [&](auto v) {
+            using V = decltype(v);
+            k_bstep_walk<V, false><<<grid_for(T.n_int), BLOCK, 0, s>>>(
+                T.cell_ptr.as<int64_t>(), T.n_int, T.blr.as<unsigned short>(), T.bval.as<V>(), T.run_ptr.as<int>(),
+                T.run_off.as<int>(), T.bstep_R.as<int>(), ubuf.as<int>(), cnt.as<int64_t>(), nullptr, nullptr,
+                nullptr, bad.as<unsigned>());
+            return 0;
+        });
+        AIR_HIP(hipGetLastError());
+        T.bstep_ptr.alloc(8 * (size_t)(T.n_int + 1));
+        scan_exclusive_i64(cnt.as<int64_t>(), T.bstep_ptr.as<int64_t>(), T.n_int, s);
+        unsigned hb = 1u;
+        int64_t total = 0;
+        AIR_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipMemcpyAsync(&total, T.bstep_ptr.as<int64_t>() + T.n_int, 8, hipMemcpyDeviceToHost, s));
+        AIR_HIP(hipStreamSynchronize(s));
+        if (hb == 0u) {
+            const size_t vs = dtype_size(T.dtype);
+            T.bsval.alloc(vs * (size_t)(total + AIR_SPARSE_PAD));
+            AIR_HIP(hipMemsetAsync((char*)T.bsval.p + vs * total, 0, vs * AIR_SPARSE_PAD, s));
+            pad_alloc<unsigned>(T.bsword, total / 8, s);
+            dispatch_dtype(T.dtype, [&](auto v) {
+                using V = decltype(v);
+                k_bstep_walk<V, true><<<grid_for(T.n_int), BLOCK, 0, s>>>(
+                    T.cell_ptr.as<int64_t>(), T.n_int, T.blr.as<unsigned short>(), T.bval.as<V>(),
+                    T.run_ptr.as<int>(), T.run_off.as<int>(), T.bstep_R.as<int>(), ubuf.as<int>(), nullptr,
+                    T.bstep_ptr.as<int64_t>(), T.bsval.as<V>(), T.bsword.as<unsigned>(), bad.as<unsigned>());
+                return 0;
+            });
+            AIR_HIP(hipGetLastError());
+            AIR_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+        }
+        T.bstep_ok = hb == 0u;
+        T.nbse = T.bstep_ok ? total : 0;
+        if (!T.bstep_ok) {
+            T.bsval.release(); T.bsword.release(); T.bstep_ptr.release(); T.bstep_R.release();
+            T.lds_bs = 0;
+        }
     }
     // columns padded to whole lanes of 8 entries (the lane form)
     T.nbe = pad_segments(T.cell_ptr, T.n_int, T.bval, T.blr, T.dtype, s);
@@ -1601,8 +2008,9 @@ static constexpr int N_FWD_STREAM = 8;     // stream forward: 8/16 entries per l
                                            // without/with prefetch
 static constexpr int N_WAVE_TARGETS = 4;   // entries per wave the autotuner tries for the stream forms
 static const int WAVE_TARGETS[N_WAVE_TARGETS] = {2048, 4096, 8192, 16384};
-static constexpr int N_BWD_STREAM = 5;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch,
-                                           // then the lane form without/with prefetch
+static constexpr int N_BWD_STREAM = 7;     // stream backward (8 waves): 8 + prefetch, 16, 16 + prefetch,
+                                           // then the lane form and the step-coded lane form, each
+                                           // without/with prefetch
 // the lane form sums in FP32 registers: FP64 storage runs the 8-entry stream form instead
 static constexpr bool lane_form_ok(int dtype) { return dtype != AIR_F64; }
 template <int A, int B, int C> using Shape3 = std::integer_sequence<int, A, B, C>;
@@ -1682,6 +2090,7 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
 
 bool tiled_fwd_stream(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG; }
 bool tiled_fwd_step(const TiledOp& T) { return T.fwd_cfg >= N_FWD_CFG + 6 && T.fstep_ok; }
+bool tiled_bwd_step(const TiledOp& T) { return T.bwd_cfg >= N_BWD_CFG + 5 && T.bstep_ok; }
 
 template <int MODE>
 static void launch_bwd(const TiledOp& T, const float* y, float* gout, const SirtBufs* B, double lam, hipStream_t s,
@@ -1699,8 +2108,27 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
                                                                B->lb_kind, B->lb, B->lbv, B->ub_kind, B->ub, B->ubv,
                                                                B->partial_cols, B->ctrl);
         };
+        auto launch_step = [&](auto kern) {
+            BStepArgs b;
+            b.col_ptr = T.bstep_ptr.as<int64_t>();
+            b.words = T.bsword.as<unsigned>();
+            b.R = T.bstep_R.as<int>();
+            b.run_ptr = T.run_ptr.as<int>();
+            b.run_row = T.run_row.as<int>();
+            b.run_len = T.run_len.as<int>();
+            b.maxruns = T.maxruns;
+            if (MODE == 0)
+                kern<<<(unsigned)T.ntiles_b, 512, T.lds_bs, s>>>(b, T.bsval.as<V>(), y, gout, nullptr, nullptr,
+                                                                 nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f, nullptr,
+                                                                 nullptr, nullptr);
+            else
+                kern<<<(unsigned)T.ntiles_b, 512, T.lds_bs, s>>>(b, T.bsval.as<V>(), y, nullptr, B->g, B->D, B->x,
+                                                                 lam, B->lb_kind, B->lb, B->lbv, B->ub_kind, B->ub,
+                                                                 B->ubv, B->partial_cols, B->ctrl);
+        };
         if (cfg >= N_BWD_CFG) {
             int sv = cfg - N_BWD_CFG;
+            if (sv >= 5 && !T.bstep_ok) sv -= 2;            // no step words: the lane form
             if constexpr (std::is_same<V, double>::value) {
                 if (sv >= 3) sv = 0;
                 switch (sv) {
@@ -1714,7 +2142,9 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
                 case 1: launch(k_tbwd_flat<V, MODE, 8, 16, 0>, 512); break;
                 case 2: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
                 case 3: launch(k_tbwd_flat<V, MODE, 8, 0, 0>, 512); break;
-                default: launch(k_tbwd_flat<V, MODE, 8, 0, 1>, 512); break;
+                case 4: launch(k_tbwd_flat<V, MODE, 8, 0, 1>, 512); break;
+                case 5: launch_step(k_tbwd_step<V, MODE, 8, 0>); break;
+                default: launch_step(k_tbwd_step<V, MODE, 8, 1>); break;
                 }
             }
             return 0;
@@ -1775,7 +2205,7 @@ void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
     // stream forms: entries per lane and the prefetch flag
     // entries per lane (0: the lane form, 8 per lane in whole-lane segments) and the prefetch flag
     static const int FS[N_FWD_STREAM][2] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}, {0, 0}, {0, 1}};
-    static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}};
+    static const int BS[N_BWD_STREAM][2] = {{8, 1}, {16, 0}, {16, 1}, {0, 0}, {0, 1}, {0, 0}, {0, 1}};
     const int fs = T.fwd_cfg - N_FWD_CFG, bs = T.bwd_cfg - N_BWD_CFG;
     *fl = fs >= 0 ? FS[fs][0] : FWD_CFG[T.fwd_cfg][0];
     *fu = fs >= 0 ? FS[fs][1] : FWD_CFG[T.fwd_cfg][1];
@@ -1794,6 +2224,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
             T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + N_FWD_STREAM - 1);
             T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0) - 1);
             if (T.fwd_cfg >= N_FWD_CFG + 6 && !T.fstep_ok) T.fwd_cfg -= 2;   // the lane form instead
+            if (T.bwd_cfg >= N_BWD_CFG + 5 && !T.bstep_ok) T.bwd_cfg -= 2;
             if (!lane_form_ok(T.dtype)) {          // the lane forms' FP64 stand-ins
                 if (T.fwd_cfg >= N_FWD_CFG + 4) T.fwd_cfg -= 4;
                 if (T.bwd_cfg >= N_BWD_CFG + 3) T.bwd_cfg = N_BWD_CFG;
@@ -1851,6 +2282,7 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
     best = 1e30f;
     for (int c = 0; c < N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0); ++c) {
         if (c >= N_BWD_CFG + 3 && !lane_form_ok(T.dtype)) continue;
+        if (c >= N_BWD_CFG + 5 && !T.bstep_ok) continue;
         float t = time_it([&] { launch_bwd<0>(T, y.as<float>(), g.as<float>(), nullptr, 0.0, s, c); });
         T.tune_bwd_ms[c] = t / 4;
         if (t < best) { best = t; T.bwd_cfg = c; }

@@ -52,8 +52,8 @@ def kernel_bytes(L, method, s_v):
       cols   A'(M r) + update: entries, column pointers, staging runs, M r once,
              x read+write, D (DROP/SART);
       rows   forward pieces: entries, piece pointers, x once, one partial per piece;
-             the step-coded form stores values plus one 4-byte word per 8
-             stored entries instead of the 16-bit indices;
+             the step-coded forms (both passes) store values plus one 4-byte
+             word per 8 stored entries instead of the 16-bit indices;
       reduce residual epilogue: row->piece lists, partials, b, M, r and M r.
     CSR+CSC layout: entries (value + int32 index), pointers, gathered vector
     once, the epilogue vectors (the reduce figure is 0)."""
@@ -62,7 +62,8 @@ def kernel_bytes(L, method, s_v):
     hasD = method in ("drop", "sart")
     if L["tiled"]:
         ni, P, R = L["n_internal"], L["pieces"], L["runs"]
-        cols = nnz * (s_v + 2) + 8 * (ni + 1) + 12 * R + 4 * (L["tiles_bwd"] + 1) + 4 * m + 8 * ni \
+        idx_b = 4 * (L["bwd_step_entries"] // 8) if L.get("bwd_step") else 2 * nnz
+        cols = nnz * s_v + idx_b + 8 * (ni + 1) + 12 * R + 4 * (L["tiles_bwd"] + 1) + 4 * m + 8 * ni \
             + (4 * ni if hasD else 0)
         idx_f = 4 * (L["fwd_entries"] // 8) if L.get("fwd_step") else 2 * nnz
         rows = nnz * s_v + idx_f + 8 * (P + 1) + 4 * ni + 4 * P
@@ -521,10 +522,11 @@ def main():
                             "canonical_frac": round(canon * value / 1e9 / HBM_PEAK_GBS, 4),
                             "cols_us": round(t_cols * 1e6, 2), "rows_us": round(t_rows * 1e6, 2),
                             "reduce_us": round(t_red * 1e6, 2), "pieces": L["pieces"], "runs": L["runs"],
-                            "fwd_stream": L["fwd_stream"], "fwd_step": L["fwd_step"],
+                            "fwd_stream": L["fwd_stream"], "fwd_step": L["fwd_step"], "bwd_step": L["bwd_step"],
+                            "bwd_step_entries": L["bwd_step_entries"],
                             "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
                                        L["bwd_loads"]],
-                            "tune_fwd_us": L["tune_fwd_us"][:47], "tune_bwd_us": L["tune_bwd_us"][:9],
+                            "tune_fwd_us": L["tune_fwd_us"][:47], "tune_bwd_us": L["tune_bwd_us"][:11],
                             "fwd_entries": L["fwd_entries"], "bwd_entries": L["bwd_entries"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     # the whole C-ABI call as a host caller sees it: b and x0 from host memory,

@@ -261,12 +261,19 @@ typedef struct {
                                        lane, without/with prefetch; v = 4, 5: the lane form without/with
                                        prefetch; v = 6, 7: the step-coded lane form without/with prefetch)
                                        at 2048/4096/8192/16384 entries per wave -> [15+4v+i]; 0: not run */
-    float   tune_bwd_us[16];        /* backward shapes: per-column 0-3, stream form 4-8 (8 entries per
-                                       lane with prefetch, 16 without, 16 with, lane form without, with) */
+    float   tune_bwd_us[16];        /* backward shapes: per-column 0-3, stream form 4-10 (8 entries per
+                                       lane with prefetch, 16 without, 16 with, lane form without, with,
+                                       step-coded lane form without, with) */
     int     fwd_step;               /* 1: the forward runs the step-coded lane form (one 32-bit word of
                                        unit moves per 8 entries instead of 16-bit column codes) */
     int     fwd_step_ok;            /* 1: every row piece is a unit-step staircase, so the step-coded
                                        form is available to the autotuner */
+    int     bwd_step;               /* 1: the backward runs the step-coded lane form (rows staged with a
+                                       fixed stride per run, one 32-bit word of moves per 8 entries) */
+    int     bwd_step_ok;            /* 1: every column's moves could be encoded and the strided
+                                       staging fits LDS */
+    int64_t bwd_step_entries;       /* entries of the step-coded backward stream (bridging zeros and
+                                       lane padding included) */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 

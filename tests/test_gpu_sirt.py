@@ -226,7 +226,8 @@ def test_train_relaxpar_matches_oracle(oracle):
                                  "5,1,512", "6,2,2048", "7,3,8192", "8,0,100000",
                                  "5,4,2048", "7,5,4096", "0,6,256", "8,6,512",
                                  "9,7,2048", "10,8,8192", "9,8,512", "10,7,100000", "0,7,256", "10,0,4096",
-                                 "5,8,1024", "11,7,4096", "12,8,2048", "11,0,100000", "12,5,512"])
+                                 "5,8,1024", "11,7,4096", "12,8,2048", "11,0,100000", "12,5,512",
+                                 "11,9,4096", "12,10,2048", "0,9,256", "9,10,512", "5,9,8192"])
 def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
     monkeypatch.setenv("AIR_TILED_CFG", cfg)
@@ -234,7 +235,9 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     L = A.layout()
     f, bcfg, par = (int(v) for v in cfg.split(","))
     assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
-    assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16, 7: 0, 8: 0}.get(bcfg, L["bwd_lanes"])
+    assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16, 7: 0, 8: 0, 9: 0, 10: 0}.get(bcfg, L["bwd_lanes"])
+    assert L["bwd_step_ok"] == 1 and L["bwd_step"] == (1 if bcfg >= 9 else 0)
+    assert L["bwd_step_entries"] >= L["nnz"] and L["bwd_step_entries"] % 8 == 0
     if f >= 5:
         assert (L["fwd_lanes"], L["fwd_loads"]) == [(8, 0), (8, 1), (16, 0), (16, 1), (0, 0), (0, 1),
                                                     (0, 0), (0, 1)][f - 5]
@@ -256,7 +259,7 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
 @pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512", "10,8,4096", "9,7,512",
-                                 "11,8,4096", "12,7,512"])
+                                 "11,8,4096", "12,7,512", "12,10,4096", "11,9,512"])
 def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
     """Per-piece and stream forward kernels agree on every storage type."""
     opts = {"rho": c1["rho"]["sart"], "lbound": 0}
@@ -387,11 +390,12 @@ def test_deferred_norms_equal_direct_finalize(c1, oracle):
 
 @pytest.mark.parametrize("geom", ["fanlinear", "fancurved", "seismic"])
 def test_step_coded_forward_other_geometries(oracle, monkeypatch, geom):
-    """The step-coded forward on other traced geometries: straight rays from a
-    point source cross every tile as unit-step staircases (both y directions,
-    corner crossings); a geometry the build cannot encode keeps the column
-    codes.  Either way the iterates match the oracle's."""
-    monkeypatch.setenv("AIR_TILED_CFG", "12,8,4096")
+    """The step-coded forward and backward on other traced geometries: straight
+    rays from a point source cross every tile as unit-step staircases (both y
+    directions, corner crossings), and every pixel's rays sweep through the
+    runs of its backward tile; a geometry the build cannot encode keeps the
+    column codes.  Either way the iterates match the oracle's."""
+    monkeypatch.setenv("AIR_TILED_CFG", "12,10,4096")
     N = 80
     if geom == "fanlinear":
         theta = np.arange(0.0, 360.0, 4.0)
@@ -405,6 +409,9 @@ def test_step_coded_forward_other_geometries(oracle, monkeypatch, geom):
     assert L["tiled"] == 1
     if geom != "seismic":
         assert L["fwd_step_ok"] == 1 and L["fwd_step"] == 1
+    if geom == "fanlinear":
+        assert L["bwd_step_ok"] == 1 and L["bwd_step"] == 1
+    assert L["bwd_step"] == L["bwd_step_ok"]
     x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
     b = Ao @ x
     opts = {"rho": oracle.spectral_radius(Ao, *oracle.weights("sart", Ao)), "lbound": 0}
