@@ -1,7 +1,7 @@
 // air_mlp.hip -- the learned relaxation network.
 //
 // A 16-64-64-2 perceptron with ReLU hidden layers maps per-iteration residual
-// statistics of a SIRT run to (o0, o1): lambda_k * rho = 4 sigmoid(o0) and a
+// statistics of a SIRT run to (o0, o1): lambda_k * rho = MLP_LAMBDA_MAX sigmoid(o0) and a
 // stop logit o1.  Training runs here with hand-written kernels: forward and
 // backward on MFMA (BF16 operands, FP32 accumulation) over tiles of 16
 // samples per wave, per-block weight gradients reduced in a fixed order, and
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_mlp_forward(const float* __restrict__ W,
 
 // Loss and gradient of one minibatch (samples idx[0..nb)): per-block partial
 // gradients part[block][NPARAM] and loss partials.  Loss per sample:
-//   (4 sigmoid(o0) - t0)^2 + stop_w * BCE(o1, t1),   averaged over nb.
+//   (MLP_LAMBDA_MAX sigmoid(o0) - t0)^2 + stop_w * BCE(o1, t1),   averaged over nb.
 // A target t0 < 0 drops the lambda term of that sample.
 __global__ __launch_bounds__(NT) void k_mlp_grad(const float* __restrict__ W, const float* __restrict__ X,
                                                  const float* __restrict__ T, const int* __restrict__ idx,

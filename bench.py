@@ -104,12 +104,14 @@ def cpu_baseline(N, theta, p, b, method, rho, seconds, threads):
     samples = int(min(max(seconds / per_iter, 6), 20000))
     dt = timed(samples + 1) - d1
     per_iter = max(dt, 1e-9) / samples
-    # one core as well: a few iterations (the setup call measured at one thread too)
+    # one core as well: about a quarter of `seconds` of iterations (the setup call
+    # measured at one thread too)
     t1 = time.perf_counter()
     O.sirt(method, A, b, 1, None, opts, nthreads=1)
     d1s = time.perf_counter() - t1
+    k1 = int(min(max(0.25 * seconds / max(per_iter * threads, 1e-9), 3), 200))
     t1 = time.perf_counter()
-    O.sirt(method, A, b, 4, None, opts, nthreads=1)
+    O.sirt(method, A, b, k1 + 1, None, opts, nthreads=1)
     dt1 = time.perf_counter() - t1 - d1s
     cpu = "unknown"
     try:
@@ -121,7 +123,8 @@ def cpu_baseline(N, theta, p, b, method, rho, seconds, threads):
             "sample": f"{samples} {method} iterations of the FP64 oracle (CSR A.x, CSC A'.y, OpenMP, "
                       f"{threads} threads) on the same {A.shape[0]}x{A.shape[1]} problem, nnz={A.nnz}, "
                       f"{dt:.1f} s; oracle tracer {t_trace:.1f} s and per-call setup not included",
-            "one_core_value": round(3.0 / max(dt1, 1e-9), 3), "one_core_sample": "3 iterations at 1 thread",
+            "one_core_value": round(k1 / max(dt1, 1e-9), 3),
+            "one_core_sample": f"{k1} iterations at 1 thread, {dt1:.1f} s",
             "cpu_model": cpu, "host_cpus": os.cpu_count()}
 
 

@@ -1,12 +1,12 @@
 """FP64 reference of the learned relaxation network (test infrastructure only).
 
 The network maps 16 residual statistics of a SIRT iteration to (o0, o1):
-lambda_k * rho = 4 sigmoid(o0) and a stop logit o1.  Layers 16 -> 64 -> 64 -> 2
+lambda_k * rho = LAMBDA_MAX sigmoid(o0) (LAMBDA_MAX = 12) and a stop logit o1.  Layers 16 -> 64 -> 64 -> 2
 with ReLU; parameters are 5378 values laid out as W1[64][16], b1[64],
 W2[64][64], b2[64], W3[2][64], b3[2] (row = output unit).
 
 The loss of n samples with targets (t0, t1) is
-    mean[(4 sigmoid(o0) - t0)^2 * [t0 >= 0] + stop_w * BCE(o1, t1)].
+    mean[(LAMBDA_MAX sigmoid(o0) - t0)^2 * [t0 >= 0] + stop_w * BCE(o1, t1)].
 
 `emulate_bf16=True` rounds to BF16 at the points where the device kernels feed
 their BF16 matrix units (inputs, weights, activations and back-propagated
