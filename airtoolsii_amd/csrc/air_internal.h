@@ -248,7 +248,7 @@ struct TiledOp {
     std::vector<int> piece_tile_h;         // forward tile of every piece (host copy)
     std::vector<int64_t> piece_ptr_h;      // first entry of every piece (host copy)
     // stream form of the forward product: waves own runs of whole pieces
-    DevBuf wblk_tile, wave_p0;
+    DevBuf wblk_tile, wave_p0, wave_e;     // wave_e: first stream entry of every wave (wave_p0's pieces)
     int nblk_w = 0, nwaves = 0, wave_target = 4096;
     // backward codes carry a column-start flag in bit 15 when every tile has
     // fewer than 32768 distinct rows (required by the stream form)

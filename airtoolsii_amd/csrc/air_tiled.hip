@@ -547,37 +547,41 @@ __device__ __forceinline__ float step_products(const float (&v)[8], unsigned w, 
     return s;
 }
 
+// One round of a step-coded stream: lane l's 8 values and its word (zeros past E1).
+template <typename V>
+__device__ __forceinline__ void steps_load_round(const V* __restrict__ val, const unsigned* __restrict__ words,
+                                                 int64_t rb, int64_t E1, float (&vv)[8], unsigned& ww) {
+    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
+    const int64_t q = rb + 8 * (int64_t)(threadIdx.x & 63);
+    if (q < E1) {
+#pragma unroll
+        for (int h = 0; h < NV; ++h) {
+            float t[VE];
+            load_vals<V, float>(val + q + VE * h, t);
+#pragma unroll
+            for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
+        }
+        ww = __builtin_nontemporal_load(words + (q >> 3));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) vv[i] = 0.f;
+        ww = 0u;
+    }
+}
+
+// The wave's stream [E0, E1) with its first round (v, w) already loaded by
+// the caller, so that those loads can be in flight with the block's prologue.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                              const unsigned* __restrict__ words, const float* lds, int k0,
-                                             Close&& close) {
+                                             float (&v)[8], unsigned w, Close&& close) {
     static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
-    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
     const int lane = (int)(threadIdx.x & 63);
-    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) {
-        const int64_t q = rb + 8 * lane;
-        if (q < E1) {
-#pragma unroll
-            for (int h = 0; h < NV; ++h) {
-                float t[VE];
-                load_vals<V, float>(val + q + VE * h, t);
-#pragma unroll
-                for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
-            }
-            ww = __builtin_nontemporal_load(words + (q >> 3));
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) vv[i] = 0.f;
-            ww = 0u;
-        }
-    };
+    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) { steps_load_round<V>(val, words, rb, E1, vv, ww); };
     int64_t rb = E0;
     int knext = k0;
     bool open = false;
     float carry = 0.f;
-    float v[8];
-    unsigned w;
-    load_round(rb, v, w);
     while (rb < E1) {
         const bool last_round = rb + 512 >= E1;
         float vn[8];
@@ -646,7 +650,7 @@ __device__ __host__ __forceinline__ unsigned nested_to_plain(unsigned q) {
 // nested entries is 4 consecutive y of one x, so it moves as one float4).
 template <typename V, int PF>
 __global__ __launch_bounds__(BLOCK) void k_tfwd_step(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
-                                                     int nwaves, int nblk, const int64_t* __restrict__ piece_ptr,
+                                                     int nwaves, int nblk, const int64_t* __restrict__ wave_e,
                                                      const V* __restrict__ fval, const unsigned* __restrict__ fstep,
                                                      const float* __restrict__ x, float* __restrict__ partial,
                                                      const Ctrl* ctrl) {
@@ -656,22 +660,30 @@ __global__ __launch_bounds__(BLOCK) void k_tfwd_step(const int* __restrict__ blk
     if (ctrl && load_stop(ctrl)) return;
     const int b = xcd_block(blockIdx.x, nblk);
     const int t = blk_tile[b];
-    {
-        const f4* xt = (const f4*)(x + (int64_t)t * FT);
-        f4 tmp[FT / (4 * BLOCK)];
-#pragma unroll
-        for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
-#pragma unroll
-        for (int i = 0; i < FT / (4 * BLOCK); ++i)
-            ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + BLOCK * i)) >> 2] = tmp[i];
-    }
-    __syncthreads();
+    // the wave's range and first round are loaded with the tile of x, so the
+    // block waits for one round trip before the tile is staged
     const int w = b * WPB + (int)(threadIdx.x >> 6);
-    if (w >= nwaves) return;
-    const int pw0 = wave_p0[w], pw1 = wave_p0[w + 1];
-    if (pw0 >= pw1) return;
+    int pw0 = 0;
+    int64_t E0 = 0, E1 = 0;
+    if (w < nwaves) {
+        pw0 = wave_p0[w];
+        E0 = wave_e[w];
+        E1 = wave_e[w + 1];
+    }
+    const f4* xt = (const f4*)(x + (int64_t)t * FT);
+    f4 tmp[FT / (4 * BLOCK)];
+#pragma unroll
+    for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
+    float v[8];
+    unsigned wd;
+    steps_load_round<V>(fval, fstep, E0, E1, v, wd);
+#pragma unroll
+    for (int i = 0; i < FT / (4 * BLOCK); ++i)
+        ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + BLOCK * i)) >> 2] = tmp[i];
+    __syncthreads();
+    if (E0 >= E1) return;
     auto close = [&](int k, float sum) { partial[k] = sum; };
-    stream_steps<V, PF>(piece_ptr[pw0], piece_ptr[pw1], fval, fstep, xs, pw0, close);
+    stream_steps<V, PF>(E0, E1, fval, fstep, xs, pw0, v, wd, close);
 }
 
 // y = sum of a row's partials (8 lanes per row)
@@ -1613,6 +1625,10 @@ void set_fwd_waves(TiledOp& T, int target, hipStream_t s) {
     T.nblk_w = (int)bt.size();
     T.nwaves = (int)wp.size() - 1;
     T.wblk_tile.alloc(4 * std::max<size_t>(bt.size(), 1));
+    std::vector<int64_t> we(wp.size());
+    for (size_t i = 0; i < wp.size(); ++i) we[i] = pp[wp[i]];
+    T.wave_e.alloc(8 * we.size());
+    AIR_HIP(hipMemcpyAsync(T.wave_e.p, we.data(), 8 * we.size(), hipMemcpyHostToDevice, s));
     T.wave_p0.alloc(4 * wp.size());
     if (!bt.empty()) AIR_HIP(hipMemcpyAsync(T.wblk_tile.p, bt.data(), 4 * bt.size(), hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(T.wave_p0.p, wp.data(), 4 * wp.size(), hipMemcpyHostToDevice, s));
@@ -2050,7 +2066,7 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
             };
             auto launch_step = [&](auto kern) {
                 kern<<<T.nblk_w, BLOCK, 0, s>>>(T.wblk_tile.as<int>(), T.wave_p0.as<int>(), T.nwaves, T.nblk_w,
-                                                T.piece_ptr.as<int64_t>(), T.fval.as<V>(), T.fstep.as<unsigned>(),
+                                                T.wave_e.as<int64_t>(), T.fval.as<V>(), T.fstep.as<unsigned>(),
                                                 x_int, T.partial.as<float>(), ctrl);
             };
             int sv = cfg - N_FWD_CFG;
