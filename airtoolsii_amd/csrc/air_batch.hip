@@ -190,7 +190,10 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     AIR_HIP(hipMemcpyAsync(B.piece_ptr.as<int64_t>() + P, &nfe, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(B.rp_ptr.as<int64_t>() + m, &P, 8, hipMemcpyHostToDevice, s));
     AIR_HIP(hipMemcpyAsync(B.cell_ptr.as<int64_t>() + nint, &nbe, 8, hipMemcpyHostToDevice, s));
-    if (bstep) AIR_HIP(hipMemcpyAsync(B.bstep_ptr.as<int64_t>() + nint, &nbse, 8, hipMemcpyHostToDevice, s));
+    if (bstep) {
+        AIR_HIP(hipMemcpyAsync(B.bstep_ptr.as<int64_t>() + nint, &nbse, 8, hipMemcpyHostToDevice, s));
+        set_bstep_waves(B, s);
+    }
     const int Ri = (int)R;
     AIR_HIP(hipMemcpyAsync(B.run_ptr.as<int>() + tb, &Ri, 4, hipMemcpyHostToDevice, s));
     B.piece_ptr_h.push_back(nfe);

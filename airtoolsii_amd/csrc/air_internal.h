@@ -261,6 +261,7 @@ struct TiledOp {
     // pointers, one word per 8 entries, the slot stride per tile, and the LDS
     // of the strided staging; bstep_ok when every column could be encoded
     DevBuf bsval, bstep_ptr, bsword, bstep_R;
+    DevBuf bstep_we, bstep_wk;  // per (tile, wave of 8): entry range and first column ordinal
     int64_t nbse = 0;
     size_t lds_bs = 0;
     bool bstep_ok = false;
@@ -275,6 +276,7 @@ void tiled_forward_plain(const TiledOp& T, const float* x_int, float* y, hipStre
 void autotune_tiled(TiledOp& T, hipStream_t s);
 void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s);
 void set_fwd_waves(TiledOp& T, int target_entries, hipStream_t s);
+void set_bstep_waves(TiledOp& T, hipStream_t s);
 bool tiled_fwd_stream(const TiledOp& T);
 bool tiled_fwd_step(const TiledOp& T);
 bool tiled_bwd_step(const TiledOp& T);

@@ -1010,37 +1010,19 @@ __device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w,
     return s;
 }
 
+// As stream_steps, for the backward words; the first round (v, w) is loaded
+// by the caller before the block stages its rows.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                               const unsigned* __restrict__ words, const float* ys, int R, int k0,
-                                              Close&& close) {
+                                              float (&v)[8], unsigned w, Close&& close) {
     static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
-    constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
     const int lane = (int)(threadIdx.x & 63);
-    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) {
-        const int64_t q = rb + 8 * lane;
-        if (q < E1) {
-#pragma unroll
-            for (int h = 0; h < NV; ++h) {
-                float t[VE];
-                load_vals<V, float>(val + q + VE * h, t);
-#pragma unroll
-                for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
-            }
-            ww = __builtin_nontemporal_load(words + (q >> 3));
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) vv[i] = 0.f;
-            ww = 0u;
-        }
-    };
+    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) { steps_load_round<V>(val, words, rb, E1, vv, ww); };
     int64_t rb = E0;
     int knext = k0;
     bool open = false;
     float carry = 0.f;
-    float v[8];
-    unsigned w;
-    load_round(rb, v, w);
     while (rb < E1) {
         const bool last_round = rb + 512 >= E1;
         float vn[8];
@@ -1101,6 +1083,8 @@ __device__ __forceinline__ float* stage_runs_strided(float* smem, int maxruns, i
 
 struct BStepArgs {
     const int64_t* col_ptr;   // column pointers of the step-coded stream
+    const int64_t* wave_e;    // per (tile, wave of 8): entry range [wave_e[2i], wave_e[2i+1])
+    const int* wave_k;        // per (tile, wave): ordinal of its first non-empty column in the tile
     const unsigned* words;
     const int* R;             // slot stride per tile
     const int* run_ptr;
@@ -1130,12 +1114,19 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
     if (MODE == 1 && isnan(lam)) lam = ctrl->lam_cur;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int64_t c0 = (int64_t)t * MAX_BT;
+    // the wave's range and first round first, so that they are in flight
+    // while the block stages the tile's rows
+    const int wv = (int)(threadIdx.x >> 6);
+    const int64_t ea = a.wave_e[2 * (t * NW + wv)], eb = a.wave_e[2 * (t * NW + wv) + 1];
+    const int ka = a.wave_k[t * NW + wv];
+    float v[8];
+    unsigned wd;
+    steps_load_round<V>(bval, a.words, ea, eb, v, wd);
     const int64_t Et0 = a.col_ptr[c0];
     const int R = a.R[t];
     const int c = (int)threadIdx.x;
     float xj = 0.f, dj = 1.f;
-    if (c < MAX_BT) cpl[c] = (int)(a.col_ptr[c0 + c] - Et0);
-    if (c == 0) cpl[MAX_BT] = (int)(a.col_ptr[c0 + MAX_BT] - Et0);
+    if (c <= MAX_BT) cpl[c] = (int)(a.col_ptr[c0 + c] - Et0);
     if (c < MAX_BT) {
         gt[c] = 0.f;
         if (MODE == 1) {
@@ -1152,30 +1143,17 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
         if ((c & 63) == 0) wtot[c >> 6] = __popcll(nm);
     }
     __syncthreads();
-    int nne = 0, before = 0;
+    int before = 0;
 #pragma unroll
     for (int u = 0; u < MAX_BT / 64; ++u) {
         before += u < (c >> 6) ? wtot[u] : 0;
-        nne += wtot[u];
     }
     if (ne) necol[before + __builtin_amdgcn_mbcnt_hi((unsigned)(nm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)nm, 0u))] =
         (short)c;
     __syncthreads();
     {
-        const int wv = c >> 6;
-        const int Et = cpl[MAX_BT];
-        auto start_of = [&](int k) { return k < nne ? cpl[necol[k]] : Et; };
-        auto first_at = [&](int64_t goal) {
-            int lo = 0, hi = nne;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (start_of(mid) >= goal) hi = mid; else lo = mid + 1;
-            }
-            return lo;
-        };
-        const int ka = first_at((int64_t)Et * wv / NW), kb = first_at((int64_t)Et * (wv + 1) / NW);
         auto close = [&](int k, float sum) { gt[necol[k]] = sum; };
-        if (ka < kb) stream_bsteps<V, PF>(Et0 + start_of(ka), Et0 + start_of(kb), bval, a.words, ys, R, ka, close);
+        if (ea < eb) stream_bsteps<V, PF>(ea, eb, bval, a.words, ys, R, ka, v, wd, close);
     }
     __syncthreads();
     double dsum[1] = {0};
@@ -1711,6 +1689,49 @@ static int64_t pad_segments(DevBuf& ptr, int64_t nseg, DevBuf& val, DevBuf& code
     return total;
 }
 
+// Wave ranges of the step-coded backward (8 waves per tile): wave w of tile t
+// takes the non-empty columns whose entries start in [Et w / 8, Et (w+1) / 8)
+// of the tile's Et entries, as whole columns; its entry range and the ordinal
+// of its first non-empty column, so that the kernel can load its first round
+// before the tile's rows are staged.
+void set_bstep_waves(TiledOp& T, hipStream_t s) {
+    constexpr int NW = 8;
+    std::vector<int64_t> ptr(T.n_int + 1);
+    AIR_HIP(hipMemcpyAsync(ptr.data(), T.bstep_ptr.p, 8 * ptr.size(), hipMemcpyDeviceToHost, s));
+    AIR_HIP(hipStreamSynchronize(s));
+    const int64_t nt = T.ntiles_b;
+    std::vector<int64_t> we(2 * nt * NW);
+    std::vector<int> wk(nt * NW);
+    std::vector<int> ne;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t c0 = t * MAX_BT, Et0 = ptr[c0], Et = ptr[c0 + MAX_BT] - Et0;
+        ne.clear();
+        for (int c = 0; c < MAX_BT; ++c)
+            if (ptr[c0 + c + 1] > ptr[c0 + c]) ne.push_back(c);
+        const int nne = (int)ne.size();
+        auto start_of = [&](int k) { return k < nne ? ptr[c0 + ne[k]] - Et0 : Et; };
+        auto first_at = [&](int64_t goal) {
+            int lo = 0, hi = nne;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (start_of(mid) >= goal) hi = mid; else lo = mid + 1;
+            }
+            return lo;
+        };
+        for (int w = 0; w < NW; ++w) {
+            const int ka = first_at(Et * w / NW), kb = first_at(Et * (w + 1) / NW);
+            we[2 * (t * NW + w)] = Et0 + start_of(ka);
+            we[2 * (t * NW + w) + 1] = Et0 + start_of(kb);
+            wk[t * NW + w] = ka;
+        }
+    }
+    T.bstep_we.alloc(8 * we.size());
+    T.bstep_wk.alloc(4 * wk.size());
+    AIR_HIP(hipMemcpyAsync(T.bstep_we.p, we.data(), 8 * we.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipMemcpyAsync(T.bstep_wk.p, wk.data(), 4 * wk.size(), hipMemcpyHostToDevice, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune) {
     T = TiledOp();
     const int64_t m = A.nrows, n = A.ncols, nnz = A.nnz;
@@ -1988,6 +2009,7 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         }
         T.bstep_ok = hb == 0u;
         T.nbse = T.bstep_ok ? total : 0;
+        if (T.bstep_ok) set_bstep_waves(T, s);
         if (!T.bstep_ok) {
             T.bsval.release(); T.bsword.release(); T.bstep_ptr.release(); T.bstep_R.release();
             T.lds_bs = 0;
@@ -2127,6 +2149,8 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
         auto launch_step = [&](auto kern) {
             BStepArgs b;
             b.col_ptr = T.bstep_ptr.as<int64_t>();
+            b.wave_e = T.bstep_we.as<int64_t>();
+            b.wave_k = T.bstep_wk.as<int>();
             b.words = T.bsword.as<unsigned>();
             b.R = T.bstep_R.as<int>();
             b.run_ptr = T.run_ptr.as<int>();
