@@ -14,7 +14,7 @@ The compute runs in libairhip.so (HIP kernels for gfx950, C ABI in
 include/airhip.h); build it with ``python -m airtoolsii_amd.build``.
 """
 from .batch import BatchOperator, batch_sirt, paralleltomo_batch
-from .operators import AirOperator, as_operator, from_matlab_csc
+from .operators import AirOperator, FunctionOperator, as_operator, from_matlab_csc
 from .phantoms import add_noise, phantomgallery
 from .sirt import (SirtRun, calczeta, cav, cimmino, drop, landweber, sart, sirt, sirt_multi, spectral_radius,
                    weights)
@@ -24,7 +24,7 @@ from .art import art, cart, columnaction, kaczmarz, randkaczmarz, symkaczmarz
 from .fbp import fbp
 from .train import train_dpme, train_relaxpar
 
-__all__ = ["BatchOperator", "batch_sirt", "paralleltomo_batch", "AirOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
+__all__ = ["BatchOperator", "batch_sirt", "paralleltomo_batch", "AirOperator", "FunctionOperator", "as_operator", "from_matlab_csc", "add_noise", "phantomgallery", "SirtRun",
            "calczeta", "cav", "cimmino", "drop", "landweber", "sart", "sirt", "sirt_multi", "spectral_radius", "weights",
            "fancurvedtomo", "fanlineartomo", "paralleltomo", "seismictomo", "seismicwavetomo", "sphericaltomo", "train_relaxpar", "train_dpme", "fbp", "art", "cart", "columnaction",
            "kaczmarz", "randkaczmarz", "symkaczmarz"]

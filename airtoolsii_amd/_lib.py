@@ -34,7 +34,11 @@ EXPORTS = ["air_last_error", "air_create", "air_destroy", "air_synchronize", "ai
            "air_apply", "air_apply_device", "air_weights", "air_spectral_radius", "air_calczeta", "air_sirt", "air_sirt_prepare",
            "air_sirt_iterate", "air_sirt_fetch", "air_set_kernel_timing", "air_get_kernel_timing",
            "air_layout_info", "air_mlp_forward", "air_mlp_grad", "air_mlp_train", "air_sirt_records",
-           "air_build_paralleltomo_batch", "air_batch_info", "air_sirt_multi"]
+           "air_build_paralleltomo_batch", "air_batch_info", "air_sirt_multi", "air_set_afun"]
+
+
+# int fn(void* ctx, int transp, const double* in, double* out): air_set_afun's operator
+AFUN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 class AirError(RuntimeError):
@@ -103,6 +107,7 @@ def load():
     L.air_synchronize.argtypes = [_P]
     L.air_set_matrix_csr.argtypes = [_P, _i64, _i64, _P, _P, _P, C.c_int]
     L.air_set_matrix_csc.argtypes = [_P, _i64, _i64, _P, _P, _P, C.c_int]
+    L.air_set_afun.argtypes = [_P, _i64, _i64, AFUN, C.c_void_p]
     L.air_build_paralleltomo.argtypes = [_P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_double, C.c_int, C.c_int]
     L.air_build_fanlineartomo.argtypes = [_P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                           C.c_double, C.c_int, C.c_int]

@@ -53,7 +53,10 @@ struct air_engine {
     int device = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
-    int kind = 0;   // 0 none, 1 stored matrix, 2 matrix-free geometry
+    int kind = 0;   // 0 none, 1 stored matrix, 2 matrix-free geometry, 3 a caller's function
+    air_afun_fn afun = nullptr;   // kind 3: the caller's products on host FP64 vectors
+    void* afun_ctx = nullptr;
+    std::vector<double> ext_in, ext_out;
     int64_t m = 0, n = 0;
     air::DevSparse A, AT;
     air::Shape shA, shAT;
@@ -177,6 +180,8 @@ void reset_operator(air_engine* e) {
     e->AT = DevSparse();
     e->T = TiledOp();
     e->geo = Geometry();
+    e->afun = nullptr;
+    e->afun_ctx = nullptr;
     e->kind = 0;
     e->m = e->n = 0;
     e->st.prepared = false;
@@ -216,9 +221,43 @@ void finish_matrix(air_engine* e, int N) {
 
 bool use_tiled(const air_engine* e) { return e->kind == 1 && e->T.ready; }
 
+// A caller's operator (kind 3): the device vector goes to host FP64, the
+// caller's function runs, and its result comes back.
+void ext_call(air_engine* e, int trans, const double* in, double* out) {
+    const int rc = e->afun(e->afun_ctx, trans, in, out);
+    AIR_CHECK(rc == 0, AIR_E_CALLBACK, "the operator function failed (returned " + std::to_string(rc) + ")");
+}
+void ext_product(air_engine* e, int trans, const float* in) {
+    const int64_t nin = trans ? e->m : e->n, nout = trans ? e->n : e->m;
+    e->ext_in.resize(std::max<int64_t>(nin, 1));
+    e->ext_out.assign(std::max<int64_t>(nout, 1), 0.0);
+    download_f64(in, e->ext_in.data(), nin, e->stream);
+    ext_call(e, trans, e->ext_in.data(), e->ext_out.data());
+}
+void ext_apply(air_engine* e, int trans, const float* in, float* out) {
+    hipStream_t s = e->stream;
+    ext_product(e, trans, in);
+    const int64_t nout = trans ? e->n : e->m;
+    DevBuf tmp;
+    upload(tmp, e->ext_out.data(), nout, s);
+    vec_f64_to_f32(tmp.as<double>(), out, nout, s);
+    AIR_HIP(hipStreamSynchronize(s));
+}
+void ext_apply_f64(air_engine* e, int trans, const float* in, double* out) {
+    hipStream_t s = e->stream;
+    ext_product(e, trans, in);
+    const int64_t nout = trans ? e->n : e->m;
+    if (nout > 0) AIR_HIP(hipMemcpyAsync(out, e->ext_out.data(), 8 * nout, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipStreamSynchronize(s));
+}
+
 // y = A x (trans=0) or y = A' x (trans=1), device FP32 in / out.
 void apply_dev(air_engine* e, int trans, const float* in, float* out) {
     hipStream_t s = e->stream;
+    if (e->kind == 3) {
+        ext_apply(e, trans, in, out);
+        return;
+    }
     if (e->kind == 1 && e->A.nnz == 0 && e->T.ready) {     // a batch: only the tiled layout exists
         DevBuf xi;
         xi.alloc(4 * (size_t)std::max<int64_t>(e->T.n_int, 1));
@@ -254,6 +293,61 @@ struct Stats {
     std::vector<double> rsq, rab, rsqw, cab, cnz;
 };
 
+// get_mfun_dfun.m's function-handle branches, with the caller's products on
+// unit vectors (host FP64, as the handle is called):
+//   Cimmino, DROP (:68-73, :150-155): rsq_i = norm(A' e_i)^2
+//   CAV (:108-121): s_j = 1 / nnz(A e_j), rsqw_i = s' (A' e_i).^2
+//   DROP (:179-184): cnz_j = nnz(A e_j)
+//   SART (:208, :229): rab = |A 1|, cab = |A' 1|
+void ext_stats(air_engine* e, int method, Stats& S) {
+    const int64_t m = e->m, n = e->n;
+    std::vector<double> u, v;
+    if (method == AIR_CIMMINO || method == AIR_DROP || method == AIR_CAV) {
+        std::vector<double> sj(n, 0.0);
+        if (method != AIR_CIMMINO) {
+            u.assign(n, 0.0);
+            v.assign(std::max<int64_t>(m, 1), 0.0);
+            for (int64_t j = 0; j < n; ++j) {
+                u[j] = 1.0;
+                ext_call(e, 0, u.data(), v.data());
+                u[j] = 0.0;
+                int64_t c = 0;
+                for (int64_t i = 0; i < m; ++i) c += v[i] != 0.0;
+                S.cnz[j] = (double)c;
+                sj[j] = 1.0 / (double)c;
+            }
+        }
+        {
+            u.assign(m, 0.0);
+            v.assign(std::max<int64_t>(n, 1), 0.0);
+            for (int64_t i = 0; i < m; ++i) {
+                u[i] = 1.0;
+                ext_call(e, 1, u.data(), v.data());
+                u[i] = 0.0;
+                if (method == AIR_CAV) {
+                    double a = 0.0;
+                    for (int64_t j = 0; j < n; ++j) a += sj[j] * (v[j] * v[j]);
+                    S.rsqw[i] = a;
+                } else {
+                    double a = 0.0;
+                    for (int64_t j = 0; j < n; ++j) a += v[j] * v[j];
+                    const double nv = std::sqrt(a);
+                    S.rsq[i] = nv * nv;
+                }
+            }
+        }
+    } else if (method == AIR_SART) {
+        u.assign(n, 1.0);
+        v.assign(std::max<int64_t>(m, 1), 0.0);
+        ext_call(e, 0, u.data(), v.data());
+        for (int64_t i = 0; i < m; ++i) S.rab[i] = std::fabs(v[i]);
+        u.assign(m, 1.0);
+        v.assign(std::max<int64_t>(n, 1), 0.0);
+        ext_call(e, 1, u.data(), v.data());
+        for (int64_t j = 0; j < n; ++j) S.cab[j] = std::fabs(v[j]);
+    }
+}
+
 void operator_stats(air_engine* e, int method, Stats& S) {
     hipStream_t s = e->stream;
     const int64_t m = e->m, n = e->n;
@@ -265,6 +359,10 @@ void operator_stats(air_engine* e, int method, Stats& S) {
     cnz.alloc(8 * std::max<int64_t>(n, 1));
     S.rsq.assign(m, 0); S.rab.assign(m, 0); S.rsqw.assign(m, 0); S.cab.assign(n, 0); S.cnz.assign(n, 0);
     bool need_cols = method == AIR_CAV || method == AIR_DROP || method == AIR_SART;
+    if (e->kind == 3) {
+        ext_stats(e, method, S);
+        return;
+    }
     if (e->kind == 1) {
         if (need_cols) row_stats(e->AT, nullptr, nullptr, cab.as<double>(), nullptr, cnz.as<double>(), s);
         if (method == AIR_CAV) {
@@ -548,6 +646,13 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
         else launch_update(e->AT, e->shAT, st.bufs, lam, k, s);
         record(e, 1);
         launch_resid(e->A, e->shA, st.bufs, st.sp, k, s);
+        record(e, 2);
+    } else if (e->kind == 3) {
+        ext_apply_f64(e, 1, st.bufs.Mr, st.bufs.g);
+        launch_g_update(e->n, st.bufs, lam, line ? 1 : 0, k, s);
+        record(e, 1);
+        ext_apply(e, 0, st.bufs.x, st.Ax.as<float>());
+        launch_vec_resid(e->m, st.Ax.as<float>(), st.bufs, st.sp, k, s);
         record(e, 2);
     } else {
         launch_mf_update(e->geo, e->n, st.bufs, lam, line ? 1 : 0, k, s);
@@ -836,7 +941,10 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     // r0 = b - A x0 and the k = 0 check (sirt.m:153-162)
     if (use_tiled(e)) launch_tiled_resid(e->T, B, st.sp, 0, s, nullptr);
     else if (e->kind == 1) launch_resid(e->A, e->shA, B, st.sp, 0, s);
-    else launch_mf_resid(e->geo, B, st.sp, 0, st.Ax.as<float>(), s);
+    else if (e->kind == 3) {
+        ext_apply(e, 0, B.x, st.Ax.as<float>());
+        launch_vec_resid(e->m, st.Ax.as<float>(), B, st.sp, 0, s);
+    } else launch_mf_resid(e->geo, B, st.sp, 0, st.Ax.as<float>(), s);
     AIR_HIP(hipStreamSynchronize(s));
     st.prepared = true;
     st.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -935,6 +1043,20 @@ static int image_side(int64_t n) {
     while (r * r > n) --r;
     while ((r + 1) * (r + 1) <= n) ++r;
     return (r * r == n && r > 0 && r < (1 << 20)) ? (int)r : 0;
+}
+
+int air_set_afun(air_engine* e, int64_t m, int64_t n, air_afun_fn fn, void* ctx) {
+    return guard([&] {
+        AIR_CHECK(e != nullptr && fn != nullptr, AIR_E_ARG, "null engine or operator function");
+        AIR_CHECK(m >= 0 && n >= 0, AIR_E_ARG, "invalid operator size");
+        use_device(e);
+        reset_operator(e);
+        e->afun = fn;
+        e->afun_ctx = ctx;
+        e->m = m;
+        e->n = n;
+        e->kind = 3;
+    });
 }
 
 int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* colind,
@@ -1138,6 +1260,8 @@ int air_apply_device(air_engine* e, int transp, const float* in, float* out) {
         AIR_CHECK(transp == 0 || transp == 1, AIR_E_ARG, "transp_flag must be 'size', 'notransp' or 'transp'");
         use_device(e);
         hipStream_t s = e->stream;
+        AIR_CHECK(e->kind != 3, AIR_E_UNSUPPORTED,
+                  "air_apply_device: a caller's operator function works on host vectors");
         void* sc = scratch_ptr(e->device);
         AIR_CHECK(sc != nullptr, AIR_E_STATE, "scratch memory of this device not initialised");
         const size_t cap = AIR_APPLY_SCRATCH_BYTES;
@@ -1257,7 +1381,7 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         // with a rule that can stop early, check the stop flag every 32 iterations
         // instead of enqueueing iterations that would only return
         const bool early = st.sp.type != AIR_STOP_NONE || st.relax_kind == AIR_RELAX_LEARNED;
-        const int chunk = graph_mode(e) ? e->graph_iters : 32;
+        const int chunk = graph_mode(e) ? e->graph_iters : (e->kind == 3 ? 1 : 32);
         int left = std::min(niter, st.kmax - st.k);
         while (left > 0) {
             const int c = early ? std::min(left, chunk) : left;
@@ -1352,7 +1476,7 @@ int air_sirt(air_engine* e, const air_sirt_options* opts, const double* b, const
         DevBuf snaps;
         if (nsnap) snaps.alloc(4 * (size_t)nx * nsnap);
         int l = 0;
-        const int chunk = graph_mode(e) ? e->graph_iters : 32;
+        const int chunk = graph_mode(e) ? e->graph_iters : (e->kind == 3 ? 1 : 32);
         bool stopped = read_stop(e) != 0;
         auto ti = std::chrono::steady_clock::now();
         while (!stopped && st.k < kmax) {

@@ -432,6 +432,8 @@ void launch_finalize_deferred(const SirtBufs& B, int nblocks, StopParams sp, int
 // matrix-free iteration pieces
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s);
 void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s);
+void launch_vec_resid(int64_t m, const float* Ax, const SirtBufs& B, StopParams sp, int k, hipStream_t s);
+void launch_g_update(int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s);
 // small helpers
 void vec_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 void vec_f32_to_f64(const float* in, double* out, int64_t n, hipStream_t s);

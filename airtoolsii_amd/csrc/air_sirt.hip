@@ -798,7 +798,11 @@ void launch_apply_line(int64_t n, const SirtBufs& B, hipStream_t s) {
 
 void launch_mf_resid(const Geometry& g, const SirtBufs& B, StopParams sp, int k, float* Ax, hipStream_t s) {
     mf_forward(g, B.x, Ax, s);
-    const int64_t m = g.nrays;
+    launch_vec_resid(g.nrays, Ax, B, sp, k, s);
+}
+
+// r = b - Ax, M r, the FP64 norms and the stop rule from a computed A x (m values)
+void launch_vec_resid(int64_t m, const float* Ax, const SirtBufs& B, StopParams sp, int k, hipStream_t s) {
     unsigned grid = std::min<unsigned>(vec_grid(m), 2048);
     if (sp.type == AIR_STOP_ME)
         k_resid_vec<true><<<grid, BLOCK, 0, s>>>(m, Ax, B.b, B.M, B.r, B.Mr, B.partial, B.hist, B.ctrl, sp, k);
@@ -813,6 +817,12 @@ void launch_mf_update(const Geometry& g, int64_t n, const SirtBufs& B, double la
     // every projector form writes each pixel of g; only an empty one leaves it
     if (g.nrays == 0) k_zero_f64<<<vec_grid(n), BLOCK, 0, s>>>(B.g, n);
     mf_adjoint(g, B.Mr, B.g, s);
+    launch_g_update(n, B, lam, line, k, s);
+}
+
+// x <- P(x + lam D g) from a computed g = A' M r (FP64, n values); line: the
+// line-search lambda from g first
+void launch_g_update(int64_t n, const SirtBufs& B, double lam, int line, int k, hipStream_t s) {
     if (line) {
         unsigned grid = std::min<unsigned>(vec_grid(n), 2048);
         k_gsum<<<grid, BLOCK, 0, s>>>(n, B.g, B.D, B.partial_cols, B.ctrl);

@@ -97,6 +97,43 @@ class AirOperator:
         return sp.csr_matrix((v, ci, rp), shape=self.shape)
 
 
+class FunctionOperator(AirOperator):
+    """A caller's function handle as the operator (sirt/sirt.m:104-115,
+    auxil/check_inputs.m:67-75): ``fn(None, 'size')`` gives (m, n),
+    ``fn(v, 'notransp')`` A v and ``fn(w, 'transp')`` A' w, all FP64.  The
+    engine calls it for every product (air_set_afun) and runs the rest of the
+    SIRT loop on the GPU; the weights are those of get_mfun_dfun.m's
+    function-handle branches.  An exception inside ``fn`` fails the engine
+    call; the exception is kept in ``self.error``."""
+
+    def __init__(self, fn, device: int = 0):
+        mn = np.ravel(np.asarray(fn(None, "size")))
+        if mn.size != 2:
+            raise ValueError("A(v, 'size') must return [m n]")
+        m, n = int(mn[0]), int(mn[1])
+        self.fn = fn
+        self.error = None
+
+        def call(ctx, transp, inp, out):
+            kin, kout = (m, n) if transp else (n, m)
+            try:
+                v = np.ctypeslib.as_array(inp, (kin,)).copy() if kin > 0 else np.zeros(0)
+                y = np.ravel(np.asarray(fn(v, "transp" if transp else "notransp"), np.float64))
+                if y.shape[0] != kout:
+                    raise ValueError(f"the operator function returned {y.shape[0]} values, expected {kout}")
+                if kout > 0:
+                    np.ctypeslib.as_array(out, (kout,))[:] = y
+                return 0
+            except Exception as ex:          # reported through AIR_E_CALLBACK
+                self.error = ex
+                return -1
+
+        self._cb = _lib.AFUN(call)           # kept alive with the operator
+        e = Engine(device)
+        check(load().air_set_afun(e.h, m, n, self._cb, None))
+        AirOperator.__init__(self, e, "f32", True)
+
+
 class _Transposed:
     def __init__(self, op):
         self.op = op
@@ -108,13 +145,13 @@ class _Transposed:
 
 def as_operator(A, dtype: str = "f32", device: int = 0) -> AirOperator:
     """Wrap ``A`` as an engine operator.  Accepts an :class:`AirOperator`
-    (returned unchanged), a SciPy sparse matrix or a dense 2-D array (uploaded
-    as CSR; the engine builds the CSC on the GPU)."""
+    (returned unchanged), a function handle with the afun convention (a
+    :class:`FunctionOperator`), a SciPy sparse matrix or a dense 2-D array
+    (uploaded as CSR; the engine builds the CSC on the GPU)."""
     if isinstance(A, AirOperator):
         return A
-    if callable(A):
-        raise TypeError("Only engine operators (AirOperator) are accepted as function handles; "
-                        "build A with paralleltomo/fanlineartomo(..., isMatrix=0) or pass a matrix")
+    if callable(A) and not hasattr(A, "shape"):
+        return FunctionOperator(A, device)
     import scipy.sparse as sp
     M = sp.csr_matrix(A, dtype=np.float64)
     M.sum_duplicates()
@@ -136,4 +173,4 @@ def from_matlab_csc(m, n, jc, ir, pr, dtype: str = "f32", device: int = 0) -> Ai
     return AirOperator(e, dtype, False)
 
 
-__all__ = ["AirOperator", "as_operator", "from_matlab_csc", "_lib"]
+__all__ = ["AirOperator", "FunctionOperator", "as_operator", "from_matlab_csc", "_lib"]

@@ -195,7 +195,7 @@ def sirt(sirt_method, A, b, K, x0=None, options=None):
     if A is None or b is None or K is None:
         raise ValueError("Too few input arguments.")
     if not hasattr(A, "shape"):
-        A = as_operator(A)                            # raises for foreign callables
+        A = as_operator(A)                            # a function handle: FunctionOperator
     m, n = A.shape
     b = f64(b)
     if b.shape[0] != m:

@@ -26,6 +26,7 @@ extern "C" {
 #define AIR_E_UNSUPPORTED -4   /* option not implemented for this operator */
 #define AIR_E_STATE       -5   /* call out of order (e.g. no operator set) */
 #define AIR_E_GEOMETRY    -6   /* tracer produced a column outside 1..N^2 */
+#define AIR_E_CALLBACK    -7   /* a caller's operator function (air_set_afun) returned non-zero */
 
 /* storage type of the matrix values; accumulation is FP32 (FP64 for AIR_F64) */
 #define AIR_F32   0
@@ -120,6 +121,19 @@ int air_set_matrix_csr(air_engine* e, int64_t m, int64_t n, const int64_t* rowpt
  * 0-based row indices): the operand of auxil/afun_matrix.m:44-53. */
 int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colptr,
                        const int64_t* rowind, const double* val, int dtype);
+/* A caller's operator: the function-handle afun of sirt/sirt.m:104-115 and
+ * auxil/check_inputs.m:67-75.  fn(ctx, 0, x, y) must write y = A x (x: n
+ * values, y: m values), fn(ctx, 1, w, z) z = A' w; host FP64 arrays owned by
+ * the engine; a non-zero return fails the calling entry point with
+ * AIR_E_CALLBACK.  Every product of the SIRT loop, of the spectral radius and
+ * of air_apply calls fn on the calling thread; the rest of each iteration runs
+ * on the device.  The weights follow get_mfun_dfun.m's function-handle
+ * branches: Cimmino and DROP take the row norms from m products A'e_i
+ * (:68-73, :150-155), CAV and DROP the column counts from n products A e_j
+ * (:108-116, :179-184; CAV then sums (A'e_i)^2 / count, :117-121), SART the
+ * products A 1 and A' 1 (:208, :229). */
+typedef int (*air_afun_fn)(void* ctx, int transp, const double* in, double* out);
+int air_set_afun(air_engine* e, int64_t m, int64_t n, air_afun_fn fn, void* ctx);
 /* testprobs/paralleltomo.m:1-110 (defaults resolved by the caller: p, d).
  * matrix_free=0: trace on the GPU into CSR (+CSC); 1: keep the geometry
  * only and project on every product (paralleltomo.m:94-96): A x walks each

@@ -334,6 +334,44 @@ def weights(method, A):
     return M, D
 
 
+def weights_handle(method, afun, m, n):
+    """get_mfun_dfun.m's function-handle branches (the handle called on unit
+    vectors and on ones, as the reference does): Cimmino :68-73, CAV :108-121,
+    DROP :150-155 and :179-184, SART :208 and :229.  Returns (M, D)."""
+    M, D = np.ones(m), np.ones(n)
+    if method == "landweber":
+        return M, D
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if method in ("cimmino", "drop"):
+            normAi = np.zeros(m)
+            for i in range(m):
+                e = np.zeros(m); e[i] = 1
+                normAi[i] = np.linalg.norm(afun(e, "transp")) ** 2
+            M = (1.0 / m) * (1.0 / normAi) if method == "cimmino" else 1.0 / normAi
+            M[M == np.inf] = 0
+        if method in ("cav", "drop"):
+            s = np.zeros(n)
+            for j in range(n):
+                e = np.zeros(n); e[j] = 1
+                s[j] = 1.0 / np.sum(afun(e, "notransp") != 0)
+            if method == "drop":
+                s[s == np.inf] = 0
+                D = s
+            else:
+                normAs = np.zeros(m)
+                for i in range(m):
+                    e = np.zeros(m); e[i] = 1
+                    normAs[i] = s @ (afun(e, "transp") ** 2)
+                M = 1.0 / normAs
+                M[M == np.inf] = 0
+        if method == "sart":
+            M = 1.0 / np.abs(afun(np.ones(n), "notransp"))
+            M[M == np.inf] = 0
+            D = 1.0 / np.abs(afun(np.ones(m), "transp"))
+            D[D == np.inf] = 0
+    return M, D
+
+
 def spmv(A, x, nthreads=0):
     rp, ci, v = _csr_parts(A)
     y = np.empty(A.shape[0])

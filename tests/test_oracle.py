@@ -1254,3 +1254,43 @@ def test_binary_product_matches_oracle(oracle):
     assert not np.array_equal(phantomgallery("binary", 64, 1), phantomgallery("binary", 64, 2))
     with pytest.raises(ValueError):
         phantomgallery("binary", 2, 1)
+
+
+# ------------------------------------------------------------------ function-handle weights
+def test_weights_handle_branches_closed_forms(oracle):
+    """get_mfun_dfun.m's function-handle branches against closed forms on the
+    matrix behind the handle: row norms (Cimmino, DROP), column counts (DROP),
+    sum_j a_ij^2 / nnz_j (CAV's handle branch, which divides where the matrix
+    branch multiplies), |A 1| and |A' 1| (SART); an empty row gives weight 0."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    A = sp.random(30, 20, density=0.2, random_state=4, format="csr")
+    A = sp.csr_matrix(A + sp.csr_matrix((np.ones(20), (np.arange(20) % 30, np.arange(20))), shape=(30, 20)))
+    A = sp.lil_matrix(A); A[7, :] = 0; A = A.tocsr(); A.eliminate_zeros()
+    fn = lambda v, t: A @ v if t == "notransp" else A.T @ v
+    m, n = A.shape
+    rsq = np.asarray(A.multiply(A).sum(axis=1)).ravel()
+    cnz = np.diff(A.tocsc().indptr).astype(float)
+    assert rsq[7] == 0 and (cnz > 0).all()
+    with np.errstate(divide="ignore"):
+        M, D = oracle.weights_handle("cimmino", fn, m, n)
+        ref = np.where(rsq > 0, 1.0 / (m * rsq), 0.0)
+        np.testing.assert_allclose(M, ref, rtol=1e-13); assert np.all(D == 1)
+        M, D = oracle.weights_handle("drop", fn, m, n)
+        np.testing.assert_allclose(M, np.where(rsq > 0, 1.0 / rsq, 0.0), rtol=1e-13)
+        np.testing.assert_allclose(D, 1.0 / cnz, rtol=1e-15)
+        M, D = oracle.weights_handle("cav", fn, m, n)
+        w = np.asarray(A.multiply(A) @ (1.0 / cnz)).ravel()
+        np.testing.assert_allclose(M, np.where(w > 0, 1.0 / w, 0.0), rtol=1e-13)
+        # the matrix branch weights the squares by the counts instead
+        Mm, _ = oracle.weights("cav", A)
+        assert not np.allclose(M, Mm)
+        M, D = oracle.weights_handle("sart", fn, m, n)
+        ra, ca = np.abs(A @ np.ones(n)), np.abs(A.T @ np.ones(m))
+        np.testing.assert_allclose(M, np.where(ra > 0, 1.0 / ra, 0.0), rtol=1e-14)
+        np.testing.assert_allclose(D, 1.0 / ca, rtol=1e-14)
+    # an empty column makes CAV's handle branch 0 * Inf = NaN in every row, as in MATLAB
+    B = sp.lil_matrix(A); B[:, 3] = 0; B = B.tocsr(); B.eliminate_zeros()
+    fb = lambda v, t: B @ v if t == "notransp" else B.T @ v
+    M, _ = oracle.weights_handle("cav", fb, m, n)
+    assert np.isnan(M).all()
