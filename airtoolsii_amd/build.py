@@ -38,6 +38,7 @@ SOURCES = {
     "air_mlp.hip": [],
     "air_batch.hip": [],
     "air_multi.hip": [],
+    "air_spmm.hip": [],
     "air_engine.hip": [],
 }
 

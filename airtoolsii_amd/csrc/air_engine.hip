@@ -62,6 +62,7 @@ struct air_engine {
     air::Shape shA, shAT;
     air::Geometry geo;
     air::TiledOp T;            // LDS-tiled layout used by the SIRT loop when ready
+    air::SpmmOp S;             // layout of the many-right-hand-side products (built on first use)
     bool no_tune = false;      // skip the kernel-shape autotuner (parts of a batch)
     // batch of independent problems (block-diagonal operator without CSR)
     std::vector<int64_t> b_row_off, b_col_off;
@@ -179,6 +180,7 @@ void reset_operator(air_engine* e) {
     e->A = DevSparse();
     e->AT = DevSparse();
     e->T = TiledOp();
+    e->S = SpmmOp();
     e->geo = Geometry();
     e->afun = nullptr;
     e->afun_ctx = nullptr;
@@ -206,6 +208,13 @@ bool tiled_disabled() {
     return v && v[0] && v[0] != '0';
 }
 
+// AIR_SPMM=0 runs many right-hand sides on the compressed rows and columns
+// (wave per row or column) instead of the LDS-staged chunk kernels.
+bool spmm_disabled() {
+    const char* v = std::getenv("AIR_SPMM");
+    return v && v[0] == '0';
+}
+
 // CSC by transpose, then the LDS-tiled layout of the SIRT loop (image side N
 // for traced geometries, 0 for a general matrix).
 void finish_matrix(air_engine* e, int N) {
@@ -216,6 +225,7 @@ void finish_matrix(air_engine* e, int N) {
     e->m = e->A.nrows;
     e->n = e->A.ncols;
     e->T = TiledOp();
+    e->S = SpmmOp();
     if (!tiled_disabled()) build_tiled(e->A, N, e->T, e->stream, !e->no_tune);
 }
 
@@ -1089,6 +1099,7 @@ int air_set_matrix_csc(air_engine* e, int64_t m, int64_t n, const int64_t* colpt
         e->m = e->A.nrows;
         e->n = e->A.ncols;
         e->T = TiledOp();
+        e->S = SpmmOp();
         if (!tiled_disabled()) build_tiled(e->A, image_side(e->n), e->T, e->stream, !e->no_tune);
     });
 }
@@ -1828,6 +1839,65 @@ int air_sirt_multi(air_engine* e, const air_sirt_options* o, int32_t nrhs, const
         }
         // RHS-minor FP32 blocks: element (i, j) at i * 64 + j
         const size_t L = 64;
+        if (use_tiled(e) && !spmm_disabled() && !e->S.ready) build_spmm(e->A, e->T, e->S, s);
+        if (use_tiled(e) && !spmm_disabled() && e->S.ready) {
+            // LDS-staged products on the tiled layout's internal column order
+            const SpmmOp& S = e->S;
+            const int64_t ni = S.n_int;
+            const std::vector<int>& perm = S.perm_h;
+            std::vector<float> hB((size_t)m * L, 0.f), hX((size_t)ni * L, 0.f);
+            for (int32_t j = 0; j < nrhs; ++j)
+                for (int64_t i = 0; i < m; ++i) hB[(size_t)i * L + j] = (float)B[(size_t)j * m + i];
+            if (X0)
+                for (int32_t j = 0; j < nrhs; ++j)
+                    for (int64_t c = 0; c < n; ++c) hX[(size_t)perm[c] * L + j] = (float)X0[(size_t)j * n + c];
+            DevBuf dB, dX, dMR, dM, dD, part, nrm;
+            dB.alloc(4 * (size_t)m * L);
+            dX.alloc(4 * (size_t)ni * L);
+            dMR.alloc(4 * (size_t)m * L);
+            AIR_HIP(hipMemcpyAsync(dB.p, hB.data(), 4 * hB.size(), hipMemcpyHostToDevice, s));
+            AIR_HIP(hipMemcpyAsync(dX.p, hX.data(), 4 * hX.size(), hipMemcpyHostToDevice, s));
+            if (hasM) upload_f32(dM, M.data(), m, s);
+            if (hasD) {
+                std::vector<double> Di((size_t)ni, 0.0);
+                for (int64_t c = 0; c < n; ++c) Di[perm[c]] = D[c];
+                upload_f32(dD, Di.data(), ni, s);
+            }
+            const unsigned gr = spmm_rows_grid(m, e->num_cu);
+            part.alloc(8 * (size_t)gr * L);
+            nrm.alloc(8 * L);
+            const float* Mp = hasM ? dM.as<float>() : nullptr;
+            const float* Dp = hasD ? dD.as<float>() : nullptr;
+            spmm_forward(S, dX.as<float>(), s);
+            spmm_rows(S, e->T, dB.as<float>(), Mp, dMR.as<float>(), part.as<double>(), gr, s);
+            hipEvent_t a, b;
+            AIR_HIP(hipEventCreate(&a));
+            AIR_HIP(hipEventCreate(&b));
+            AIR_HIP(hipEventRecord(a, s));
+            for (int k = 1; k <= K; ++k) {
+                spmm_backward_update(S, dMR.as<float>(), dX.as<float>(), Dp, lam, o->lb_kind, (float)o->lb,
+                                     o->ub_kind, (float)o->ub, s);
+                spmm_forward(S, dX.as<float>(), s);
+                spmm_rows(S, e->T, dB.as<float>(), Mp, dMR.as<float>(), part.as<double>(), gr, s);
+            }
+            AIR_HIP(hipEventRecord(b, s));
+            multi_norms(part.as<double>(), (int)gr, nrm.as<double>(), s);
+            AIR_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            AIR_HIP(hipEventElapsedTime(&ms, a, b));
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (gpu_ms) *gpu_ms = ms;
+            AIR_HIP(hipMemcpyAsync(hX.data(), dX.p, 4 * hX.size(), hipMemcpyDeviceToHost, s));
+            std::vector<double> hn(L);
+            AIR_HIP(hipMemcpyAsync(hn.data(), nrm.p, 8 * L, hipMemcpyDeviceToHost, s));
+            AIR_HIP(hipStreamSynchronize(s));
+            for (int32_t j = 0; j < nrhs; ++j)
+                for (int64_t c = 0; c < n; ++c) X_out[(size_t)j * n + c] = (double)hX[(size_t)perm[c] * L + j];
+            if (resnorm)
+                for (int32_t j = 0; j < nrhs; ++j) resnorm[j] = std::sqrt(hn[j]);
+            return;
+        }
         std::vector<float> hB((size_t)m * L, 0.f), hX((size_t)n * L, 0.f);
         for (int32_t j = 0; j < nrhs; ++j)
             for (int64_t i = 0; i < m; ++i) hB[(size_t)i * L + j] = (float)B[(size_t)j * m + i];

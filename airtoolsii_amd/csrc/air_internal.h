@@ -267,6 +267,33 @@ struct TiledOp {
     bool bstep_ok = false;
     float tune_fwd_ms[64] = {}, tune_bwd_ms[16] = {};
 };
+// Many right-hand sides (air_spmm.hip): per product, the launch order of the
+// 1024-thread work-groups, their table, every wave's first stream block and
+// block count, the operand chunks and their per-(wave, round) step counts,
+// and the record stream (1 KB blocks of 12 records).
+struct SpmmPass {
+    DevBuf lperm, wgt, wblk, wnb, chd, ctab, stream;
+    int nwg_launch = 0;
+    int64_t nblocks = 0;
+    int64_t step_slots = 0;     // 4 x the steps of all rounds (entries + the zeros padding them)
+    double wave_balance = 1.0;  // sum over chunks of the mean over waves of records / of the max
+};
+struct SpmmOp {
+    bool ready = false;
+    int64_t m = 0, n = 0, n_int = 0, npieces = 0;
+    std::vector<int> perm_h;    // user column -> internal column (the tiled layout's)
+    SpmmPass fwd, bwd;
+    DevBuf rowlist;             // distinct rows of every 32 x 32 block, in order
+    DevBuf partial;             // float [pieces][64]
+};
+bool build_spmm(const DevSparse& A, const struct TiledOp& T, SpmmOp& S, hipStream_t s);
+void spmm_forward(const SpmmOp& S, const float* X, hipStream_t s);
+void spmm_backward_update(const SpmmOp& S, const float* MR, float* X, const float* D, double lam, int lb_kind,
+                          float lb, int ub_kind, float ub, hipStream_t s);
+unsigned spmm_rows_grid(int64_t m, int num_cu);
+void spmm_rows(const SpmmOp& S, const struct TiledOp& T, const float* B, const float* M, float* MR, double* part,
+               unsigned grid, hipStream_t s);
+
 // Build the tiled operator from the CSR of A (N > 0: N x N image columns).
 // Returns false (and leaves T not ready) when a tile needs more LDS than allowed.
 bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune = true);
