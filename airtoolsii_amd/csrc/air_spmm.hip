@@ -52,11 +52,13 @@ namespace air {
 
 namespace {
 
-constexpr int SP_THREADS = 1024;
-constexpr int SP_WAVES = 16;
+constexpr int SP_THREADS = 512;
+constexpr int SP_WAVES = 8;
 constexpr int SP_ROUNDS = 16;
-constexpr int SP_TARGETS = SP_WAVES * SP_ROUNDS * 4;     // 1024
-constexpr int SP_CELLS = SP_WAVES * SP_ROUNDS;           // 256 (wave, round) cells per chunk
+constexpr int SP_TARGETS = SP_WAVES * SP_ROUNDS * 4;     // 512
+constexpr int SP_CELLS = SP_WAVES * SP_ROUNDS;           // 128 (wave, round) cells per chunk
+constexpr int SP_BB = 9;                                 // log2 of the columns of a backward block
+static_assert((1 << SP_BB) == SP_TARGETS, "a backward block has one column per target");
 constexpr int SP_ROWS = 256;                             // operand rows per chunk
 constexpr int SP_RHS = 64;
 constexpr int SP_REC = 80;                               // bytes per record
@@ -67,7 +69,7 @@ constexpr int SP_FT = 4096;                              // columns per 64 x 64 
 constexpr int NXCD = 8;
 
 static_assert(SP_BLK_RECS * SP_REC <= SP_BLK, "a block holds its records");
-static_assert(SP_OP_BYTES / 16 == 4 * SP_THREADS, "the operand is four 16-byte DMAs per thread");
+static_assert(SP_OP_BYTES / 16 == 8 * SP_THREADS, "the operand is eight 16-byte DMAs per thread");
 
 struct SpEpi {
     float* out;          // forward: partial [pieces][64]; backward: X [n_int][64]
@@ -85,12 +87,12 @@ __device__ __forceinline__ void fma4(float4& a, float v, const float4& x) {
 }
 
 // LDS byte offset of lane q's 16 bytes in the operand row named by byte J of
-// the slot word: one v_perm_b32 builds [qoff.b0, slot J, qoff.b2, 0] (selector
+// the slot word: one v_perm_b32 builds [qoff.b0, slot J, 0, 0] (selector
 // bytes 0-3 pick bytes of the second source, 4-7 of the first, 0x0c gives
-// zero); qoff holds 16 q in byte 0 and the operand buffer in byte 2.
+// zero); qoff holds 16 q.
 template <int J>
 __device__ __forceinline__ uint32_t sp_addr(uint32_t sl, uint32_t qoff) {
-    return __builtin_amdgcn_perm(sl, qoff, 0x0c020000u | ((4u + J) << 8));
+    return __builtin_amdgcn_perm(sl, qoff, 0x0c0c0000u | ((4u + J) << 8));
 }
 
 // NS steps of one record: all operand rows are read before the FMAs.
@@ -110,16 +112,11 @@ __device__ __forceinline__ void sp_steps(float4& a, const float4& v, uint32_t sl
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// LDS of the chunk kernel (the whole 160 KB of a CU): the record rings of the
-// 16 waves (2 KB each) and two operand buffers, as separate arrays so that the
-// compiler sees that the DMA into one operand buffer does not write the other
-// (it then adds no wait before reading it).  The rings come first (highest
-// alignment), the first operand buffer at 32 KB and the second 64 KB after
-// it, addressed through byte 2 of the v_perm result (see sp_addr), so every
-// base folds into the DS offset field.
+// LDS of the chunk kernel, 80 KB so that two work-groups share a CU (one
+// computes while the other waits at a barrier or for its operand): the
+// record rings of the 8 waves (2 KB each), then the operand chunk.
 __shared__ __align__(64) unsigned char sp_ring[SP_WAVES * 2 * SP_BLK];
-__shared__ __align__(16) unsigned char sp_op0[SP_OP_BYTES];
-__shared__ __align__(16) unsigned char sp_op1[SP_OP_BYTES];
+__shared__ __align__(16) unsigned char sp_op[SP_OP_BYTES];
 
 // 16 bytes per lane from global memory straight into LDS: lane l of the wave
 // writes dst + 16 l (dst is the same for the whole wave).
@@ -127,18 +124,17 @@ __device__ __forceinline__ void dma16(const void* src, unsigned char* dst) {
     __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 16, 0, 0);
 }
 
-// operand chunk {first row, rows} into buffer OB: wave w, DMA k covers rows
-// 16 w + 4 k .. + 3 (row 16 w + 4 k + lane / 16, columns 4 (lane % 16) ..)
-template <int MODE, int OB>
+// operand chunk {first row, rows}: wave w, DMA k covers rows 32 w + 4 k .. + 3
+// (row 32 w + 4 k + lane / 16, columns 4 (lane % 16) ..)
+template <int MODE>
 __device__ __forceinline__ void sp_dma_operand(const int4& ch, const int* __restrict__ rowlist,
                                                const float* __restrict__ Y, int w, int lane) {
-    unsigned char* buf = OB ? sp_op1 : sp_op0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int r = w * 16 + k * 4 + (lane >> 4);
+    for (int k = 0; k < 8; ++k) {
+        const int r = w * 32 + k * 4 + (lane >> 4);
         if (r < ch.y) {
             const int64_t row = MODE == 0 ? (int64_t)ch.x + r : (int64_t)rowlist[(int64_t)ch.x + r];
-            dma16(Y + row * SP_RHS + (lane & 15) * 4, buf + (w * 16 + k * 4) * 256);
+            dma16(Y + row * SP_RHS + (lane & 15) * 4, sp_op + (w * 32 + k * 4) * 256);
         }
     }
 }
@@ -183,21 +179,18 @@ __device__ __forceinline__ void sp_take(SpStream& S, unsigned char* ring, int la
     S.sl = *(const uint32_t*)(ring + S.off + 64 + g * 4);
 }
 
-// One chunk with its operand in buffer OB: start the DMA of the next chunk's
-// operand into the other buffer, run this wave's 16 rounds, then wait for
-// that DMA and meet at a barrier.
-template <int MODE, int OB>
+// One chunk: this wave's 16 rounds on the staged operand, then, if another
+// chunk follows, a barrier (every wave is done with the operand), the DMA of
+// the next operand, a wait for it and a second barrier.
+template <int MODE>
 __device__ __forceinline__ void sp_chunk(float4 (&acc)[SP_ROUNDS], SpStream& S, unsigned char* ring,
                                          const uint32_t* __restrict__ cw, bool has_next, int4 next,
                                          const int* rowlist, const float* Y, int w, int lane) {
-    const int g = lane >> 4, q = lane & 15;
-    S.sw = false;
-    if (has_next) sp_dma_operand<MODE, 1 - OB>(next, rowlist, Y, w, lane);
+    const int q = lane & 15;
     uint32_t cnt[SP_ROUNDS / 2];
 #pragma unroll
     for (int i = 0; i < SP_ROUNDS / 2; ++i) cnt[i] = __builtin_amdgcn_readfirstlane(cw[i]);
-    const unsigned char* ob = OB ? sp_op1 - SP_OP_BYTES : sp_op0;   // byte 2 of the address selects sp_op1
-    const uint32_t qoff = ((uint32_t)q << 4) | ((uint32_t)OB << 16);
+    const uint32_t qoff = (uint32_t)q << 4;
 #pragma unroll
     for (int t = 0; t < SP_ROUNDS; ++t) {
         const uint32_t C = (cnt[t >> 1] >> (16 * (t & 1))) & 0xffffu;
@@ -208,32 +201,24 @@ __device__ __forceinline__ void sp_chunk(float4 (&acc)[SP_ROUNDS], SpStream& S, 
             float4 v;
             uint32_t sl;
             sp_take(S, ring, lane, v, sl);
-            sp_steps<4>(a, v, sl, qoff, ob);
+            sp_steps<4>(a, v, sl, qoff, sp_op);
         }
         if (rem) {
             float4 v;
             uint32_t sl;
             sp_take(S, ring, lane, v, sl);
-            if (rem == 1) sp_steps<1>(a, v, sl, qoff, ob);
-            else if (rem == 2) sp_steps<2>(a, v, sl, qoff, ob);
-            else sp_steps<3>(a, v, sl, qoff, ob);
+            if (rem == 1) sp_steps<1>(a, v, sl, qoff, sp_op);
+            else if (rem == 2) sp_steps<2>(a, v, sl, qoff, sp_op);
+            else sp_steps<3>(a, v, sl, qoff, sp_op);
         }
         acc[t] = a;
     }
     if (has_next) {
-        // loads complete in order: after a block change only the last block load
-        // may still be pending behind the operand DMA
-        if (S.sw) __builtin_amdgcn_s_waitcnt(0x0F71);    // vmcnt(1)
-        else __builtin_amdgcn_s_waitcnt(0x0F70);         // vmcnt(0)
+        __syncthreads();
+        sp_dma_operand<MODE>(next, rowlist, Y, w, lane);
+        __builtin_amdgcn_s_waitcnt(0x0F70);    // vmcnt(0): this wave's DMA (and its ring block load)
+        __syncthreads();
     }
-    // A plain barrier: every wave's operand DMA is complete (waited above) and
-    // every read of this chunk's operand has been consumed by its FMAs; the
-    // block load still in flight belongs to this wave's own ring and may stay
-    // pending (a __syncthreads fence would wait for it).  The empty asm keeps
-    // the compiler from moving memory operations across the barrier.
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
 }
 
 // A chunk descriptor in scalar registers (zeros when absent).
@@ -253,7 +238,7 @@ __device__ __forceinline__ int4 sp_chunk_desc(const int4* __restrict__ chd, int 
 // {first operand row (forward: internal column; backward: offset into the
 // row list), rows}; per (chunk, wave): 16 round step counts (uint16).
 template <int MODE>
-__global__ __launch_bounds__(SP_THREADS) void k_spmm(const int* __restrict__ lperm, const int4* __restrict__ wgt,
+__global__ __launch_bounds__(SP_THREADS, 4) void k_spmm(const int* __restrict__ lperm, const int4* __restrict__ wgt,
                                                      const int64_t* __restrict__ wblk, const int* __restrict__ wnb,
                                                      const int4* __restrict__ chd, const uint32_t* __restrict__ ctab,
                                                      const uint4* __restrict__ stream, const int* __restrict__ rowlist,
@@ -282,19 +267,15 @@ __global__ __launch_bounds__(SP_THREADS) void k_spmm(const int* __restrict__ lpe
         if (S.nb > 0) *(uint4*)(ring + lane * 16) = S.src[lane];
         if (S.nb > 1) *(uint4*)(ring + SP_BLK + lane * 16) = S.src[SP_BLK / 16 + lane];
         if (S.nb > 2) S.st = S.src[2 * (SP_BLK / 16) + lane];
-        sp_dma_operand<MODE, 0>(sp_chunk_desc(chd, chunk0, true), rowlist, Y, w, lane);
+        sp_dma_operand<MODE>(sp_chunk_desc(chd, chunk0, true), rowlist, Y, w, lane);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         S.off = sp_next(S, ring, lane);     // the first record: its slot word ahead
         S.sl = *(const uint32_t*)(ring + S.off + 64 + g * 4);
-        for (int c = 0; c < nch; c += 2) {
+        for (int c = 0; c < nch; ++c) {
             const int4 n1 = sp_chunk_desc(chd, chunk0 + c + 1, c + 1 < nch);
-            sp_chunk<MODE, 0>(acc, S, ring, ctab + ((int64_t)(chunk0 + c) * SP_WAVES + w) * (SP_ROUNDS / 2),
-                              c + 1 < nch, n1, rowlist, Y, w, lane);
-            if (c + 1 >= nch) break;
-            const int4 n2 = sp_chunk_desc(chd, chunk0 + c + 2, c + 2 < nch);
-            sp_chunk<MODE, 1>(acc, S, ring, ctab + ((int64_t)(chunk0 + c + 1) * SP_WAVES + w) * (SP_ROUNDS / 2),
-                              c + 2 < nch, n2, rowlist, Y, w, lane);
+            sp_chunk<MODE>(acc, S, ring, ctab + ((int64_t)(chunk0 + c) * SP_WAVES + w) * (SP_ROUNDS / 2), c + 1 < nch,
+                           n1, rowlist, Y, w, lane);
         }
     }
 
@@ -302,17 +283,15 @@ __global__ __launch_bounds__(SP_THREADS) void k_spmm(const int* __restrict__ lpe
         const uint32_t p0 = (uint32_t)d.z, np = (uint32_t)d.w;
 #pragma unroll
         for (int t = 0; t < SP_ROUNDS; ++t) {
-            const uint32_t i = (uint32_t)(t * 64 + w * 4 + g);
+            const uint32_t i = (uint32_t)(t * (4 * SP_WAVES) + w * 4 + g);
             if (i < np) *(float4*)(ep.out + (int64_t)(p0 + i) * SP_RHS + q * 4) = acc[t];
         }
     } else {
-        const int64_t F = d.z;
-        const uint32_t pat = (uint32_t)d.w;
+        const int64_t G = d.z;
 #pragma unroll
         for (int t = 0; t < SP_ROUNDS; ++t) {
-            const uint32_t i = (uint32_t)(t * 64 + w * 4 + g);
-            const uint32_t j = (pat >> (4 * (i >> 8))) & 15u;
-            const int64_t px = F * SP_FT + j * 256 + (i & 255u);
+            const uint32_t i = (uint32_t)(t * (4 * SP_WAVES) + w * 4 + g);
+            const int64_t px = G * SP_TARGETS + i;
             float4* xp = (float4*)(ep.out + px * SP_RHS + q * 4);
             const float4 xo = *xp;
             const double dd = ep.D ? (double)ep.D[px] : 1.0;
@@ -462,8 +441,8 @@ __global__ void k_sp_fcells(const uint64_t* __restrict__ ks, const uint32_t* __r
         const uint64_t key = ks[e];
         const uint32_t F = (uint32_t)(key >> 44), sub = (uint32_t)(key >> 8) & 15u;
         const uint32_t p = pex[e] + pflag[e] - 1u, qp = p - tfp[F];
-        const uint32_t B = bb[F] + (qp >> 10), i = qp & 1023u;
-        const uint32_t t = i >> 6, w = (i >> 2) & 15u, g = i & 3u;
+        const uint32_t B = bb[F] + qp / SP_TARGETS, i = qp % SP_TARGETS;
+        const uint32_t t = i / (4 * SP_WAVES), w = (i >> 2) % SP_WAVES, g = i & 3u;
         const uint32_t k = (uint32_t)e - segstart[sex[e] + sflag[e] - 1u];
         const uint32_t cell = (B * 16u + sub) * SP_CELLS + w * 16u + t;
         atomicMax(&cellC[cell], k + 1u);
@@ -500,23 +479,13 @@ __global__ void k_sp_fscatter(const uint64_t* __restrict__ ks, const uint32_t* _
     }
 }
 
-// backward key: 32 x 32 block (22 bits) | row (32) | pixel slot in the block (10)
+// backward key: block of 512 consecutive internal columns (23 bits) | row (32)
+// | column inside the block (9)
 __global__ void k_sp_bkeys(const int* __restrict__ idx, const int* __restrict__ rowof, const int* __restrict__ perm,
-                           int64_t nnz, int image, uint64_t* __restrict__ key, uint32_t* __restrict__ iota) {
+                           int64_t nnz, uint64_t* __restrict__ key, uint32_t* __restrict__ iota) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t c = (uint32_t)perm[idx[e]];
-        const uint32_t F = c >> 12, j = (c >> 8) & 15u, within = c & 255u;
-        uint32_t qd, k4;
-        if (image) {   // sub-tile j = (x / 16) * 4 + (y / 16) inside the 64 x 64 tile
-            const uint32_t sxb = j >> 2, syb = j & 3u;
-            qd = ((sxb >> 1) << 1) | (syb >> 1);
-            k4 = ((sxb & 1u) << 1) | (syb & 1u);
-        } else {
-            qd = j >> 2;
-            k4 = j & 3u;
-        }
-        const uint64_t G = (uint64_t)F * 4u + qd;
-        key[e] = (G << 42) | ((uint64_t)(uint32_t)rowof[e] << 10) | (k4 * 256u + within);
+        const uint64_t c = (uint32_t)perm[idx[e]];
+        key[e] = ((c >> SP_BB) << (SP_BB + 32)) | ((uint64_t)(uint32_t)rowof[e] << SP_BB) | (c & (SP_TARGETS - 1));
         iota[e] = (uint32_t)e;
     }
 }
@@ -527,8 +496,8 @@ __global__ void k_sp_buniq(const uint64_t* __restrict__ ks, const uint32_t* __re
                            uint32_t* __restrict__ nu) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         if (uflag[e]) {
-            urow[uex[e]] = (int)((ks[e] >> 10) & 0xffffffffu);
-            atomicAdd(&nu[ks[e] >> 42], 1u);
+            urow[uex[e]] = (int)((ks[e] >> SP_BB) & 0xffffffffu);
+            atomicAdd(&nu[ks[e] >> (SP_BB + 32)], 1u);
         }
 }
 
@@ -539,9 +508,9 @@ __global__ void k_sp_bchunk(const uint64_t* __restrict__ ks, const uint32_t* __r
                             unsigned char* __restrict__ slot_of, uint32_t* __restrict__ iota) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = ks[e];
-        const uint32_t G = (uint32_t)(key >> 42);
+        const uint32_t G = (uint32_t)(key >> (SP_BB + 32));
         const uint32_t r = uex[e] + uflag[e] - 1u - tfu[G];
-        key2[e] = (cb[G] + (r >> 8)) * 1024u + (uint32_t)(key & 1023u);
+        key2[e] = (cb[G] + (r >> 8)) * (uint32_t)SP_TARGETS + (uint32_t)(key & (SP_TARGETS - 1));
         slot_of[e] = (unsigned char)(r & 255u);
         iota[e] = (uint32_t)e;
     }
@@ -551,8 +520,8 @@ __global__ void k_sp_bcells(const uint32_t* __restrict__ ks2, const uint32_t* __
                             const uint32_t* __restrict__ ex, const uint32_t* __restrict__ segstart, int64_t n,
                             uint32_t* __restrict__ cellC, uint32_t* __restrict__ ecell, uint32_t* __restrict__ ek) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t key = ks2[e], ch = key >> 10, i = key & 1023u;
-        const uint32_t t = i >> 6, w = (i >> 2) & 15u, g = i & 3u;
+        const uint32_t key = ks2[e], ch = key / SP_TARGETS, i = key % SP_TARGETS;
+        const uint32_t t = i / (4 * SP_WAVES), w = (i >> 2) % SP_WAVES, g = i & 3u;
         const uint32_t k = (uint32_t)e - segstart[ex[e] + flag[e] - 1u];
         const uint32_t cell = ch * SP_CELLS + w * 16u + t;
         atomicMax(&cellC[cell], k + 1u);
@@ -823,13 +792,13 @@ bool build_spmm(const DevSparse& A, const TiledOp& T, SpmmOp& S, hipStream_t s) 
 
     // ================= backward: 32 x 32 blocks of pixels, chunks of 256 distinct rows
     {
-        const int64_t nG = nF * 4;
+        const int64_t nG = T.n_int / SP_TARGETS;
         k_sp_bkeys<<<sp_grid(nnz), 256, 0, s>>>(A.idx.as<int>(), rowof.as<int>(), T.perm.as<int>(), nnz,
-                                                T.N > 0 ? 1 : 0, key.as<uint64_t>(), iota.as<uint32_t>());
+                                                key.as<uint64_t>(), iota.as<uint32_t>());
         AIR_HIP(hipGetLastError());
         sp_sort(key.as<uint64_t>(), ks.as<uint64_t>(), iota.as<uint32_t>(), ord.as<uint32_t>(), nnz,
-                42 + sp_bits((uint64_t)nG), s);
-        k_sp_flags<<<sp_grid(nnz), 256, 0, s>>>(ks.as<uint64_t>(), nnz, 10, f1.as<uint32_t>());   // (block, row)
+                SP_BB + 32 + sp_bits((uint64_t)nG), s);
+        k_sp_flags<<<sp_grid(nnz), 256, 0, s>>>(ks.as<uint64_t>(), nnz, SP_BB, f1.as<uint32_t>());   // (block, row)
         AIR_HIP(hipGetLastError());
         sp_scan(f1.as<uint32_t>(), x1.as<uint32_t>(), nnz, s);
         uint32_t lastx = 0, lastf = 0;
@@ -855,7 +824,7 @@ bool build_spmm(const DevSparse& A, const TiledOp& T, SpmmOp& S, hipStream_t s) 
             nch += (hnu[G] + SP_ROWS - 1) / SP_ROWS;
         }
         AIR_CHECK(U == NU, AIR_E_STATE, "spmm layout: distinct rows do not add up");
-        AIR_CHECK(nch * 1024 < (int64_t)UINT32_MAX && nch * SP_CELLS < (int64_t)UINT32_MAX, AIR_E_UNSUPPORTED,
+        AIR_CHECK(nch * SP_TARGETS < (int64_t)UINT32_MAX && nch * SP_CELLS < (int64_t)UINT32_MAX, AIR_E_UNSUPPORTED,
                   "spmm layout: too many row chunks");
         DevBuf dtfu, dcb, key2, ks2, slot_of, cellC;
         dtfu.alloc(4 * (size_t)nG);
@@ -874,7 +843,7 @@ bool build_spmm(const DevSparse& A, const TiledOp& T, SpmmOp& S, hipStream_t s) 
         AIR_HIP(hipGetLastError());
         // stable: entries of one (chunk, pixel) keep their row order
         sp_sort(key2.as<uint32_t>(), ks2.as<uint32_t>(), iota2.as<uint32_t>(), ord2.as<uint32_t>(), nnz,
-                sp_bits((uint64_t)nch * 1024), s);
+                sp_bits((uint64_t)nch * SP_TARGETS), s);
         k_sp_flags32<<<sp_grid(nnz), 256, 0, s>>>(ks2.as<uint32_t>(), nnz, f2.as<uint32_t>());
         AIR_HIP(hipGetLastError());
         sp_scan(f2.as<uint32_t>(), x2.as<uint32_t>(), nnz, s);
@@ -896,20 +865,8 @@ bool build_spmm(const DevSparse& A, const TiledOp& T, SpmmOp& S, hipStream_t s) 
             for (int64_t c = 0; c < nc; ++c)
                 chunk_op[cb[G] + c] = make_int4((int)(tfu[G] + c * SP_ROWS),
                                                 (int)std::min<int64_t>(SP_ROWS, hnu[G] - c * SP_ROWS), 0, 0);
-            const int F = (int)(G / 4), qd = (int)(G % 4);
-            uint32_t pat = 0;
-            for (int k4 = 0; k4 < 4; ++k4) {
-                uint32_t j;
-                if (T.N > 0) {   // block qd = (x half, y half); sub-tile (2 xh + k4 / 2, 2 yh + k4 % 2)
-                    const uint32_t sxb = 2 * (qd >> 1) + (k4 >> 1), syb = 2 * (qd & 1) + (k4 & 1);
-                    j = sxb * 4 + syb;
-                } else {
-                    j = (uint32_t)(qd * 4 + k4);
-                }
-                pat |= j << (4 * k4);
-            }
-            AIR_CHECK((int64_t)F * SP_FT + SP_FT <= T.n_int, AIR_E_STATE, "spmm layout: block out of range");
-            wgs.push_back({(int)cb[G], (int)nc, make_int4((int)cb[G], (int)nc, F, (int)pat)});
+            AIR_CHECK((G + 1) * SP_TARGETS <= T.n_int, AIR_E_STATE, "spmm layout: block out of range");
+            wgs.push_back({(int)cb[G], (int)nc, make_int4((int)cb[G], (int)nc, (int)G, 0)});
         }
         std::vector<uint32_t> cell_r0;
         std::vector<int> chunk_wg;
