@@ -1185,11 +1185,16 @@ int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int3
                             int matrix_free, int dtype) {
     return guard([&] {
         AIR_CHECK(e && angles && N > 0 && nA >= 0 && numCircles > 0, AIR_E_ARG, "invalid sphericaltomo arguments");
-        AIR_CHECK(!matrix_free, AIR_E_UNSUPPORTED,
-                  "sphericaltomo: the matrix-free form is provided for the line-model tracers only");
         use_device(e);
         reset_operator(e);
         (void)dtype_size(dtype);
+        if (matrix_free) {     // sphericaltomo.m:90-93: rows re-formed on every product
+            circles_mf_prepare(e->geo, N, angles, nA, numCircles, e->stream);
+            e->kind = 2;
+            e->m = (int64_t)nA * numCircles;
+            e->n = (int64_t)N * N;
+            return;
+        }
         circles_to_csr(N, angles, nA, numCircles, dtype, e->A, e->stream);
         finish_matrix(e, N);
     });

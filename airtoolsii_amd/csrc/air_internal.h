@@ -162,10 +162,17 @@ struct Geometry {
     DevBuf pdet;          // double [p]: detector positions
     DevBuf ppart;         // double [pchunks * N * N]: per-chunk sums
     // model 1: Fresnel-zone rows of seismicwavetomo (nrays = rows)
-    int model = 0;        // 0: line rays, 1: Fresnel zones
+    int model = 0;        // 0: line rays, 1: Fresnel zones, 2: circles of sphericaltomo
     DevBuf frows;         // per-row source/receiver geometry
     DevBuf frowsum;       // double [rows]: sum of the kernel over the image
     double fw = 0;        // omega / N
+    // model 2: sphericaltomo circles (nrays = rows = angles x circles); per
+    // angle S samples, circle c at [ccum[c], ccum[c+1]) with its cos/sin table
+    int cnA = 0, cnC = 0;
+    int64_t cS = 0;
+    double cdx = 0, cimg = 0;
+    DevBuf ccos, csin, ccum, crad, cbase, cxi;
+    std::vector<double> cangles;
 };
 
 // ---------------------------------------------------------------- tracer (air_tracer.hip)
@@ -194,6 +201,11 @@ void row_action_run(const DevSparse& A, const DevSparse& AT, bool cart, int art_
 void fresnel_to_csr(int N, int s, int p, double omega, int dtype, DevSparse& A, hipStream_t st);
 // sphericaltomo: CSR of the circle-quadrature matrix (air_circles.hip).
 void circles_to_csr(int N, const double* angles, int nA, int nC, int dtype, DevSparse& A, hipStream_t s);
+void circles_mf_prepare(Geometry& g, int N, const double* angles, int nA, int nC, hipStream_t s);
+void circles_mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s);
+void circles_mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s);
+void circles_mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t s);
+void circles_mf_col_stats(const Geometry& g, double* colabs, int* colcnt, hipStream_t s);
 // Trace all rays into CSR (values narrowed to dtype).  Returns nnz.
 void trace_to_csr(const Geometry& g, int dtype, DevSparse& A, hipStream_t s);
 // Matrix-free products: y = A u (m) or g = A' v (n, FP64 accumulators).

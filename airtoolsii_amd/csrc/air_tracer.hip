@@ -1200,6 +1200,7 @@ void mf_prepare_parallel(Geometry& g, int nA, int p, double d, hipStream_t s) {
 
 void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_forward(g, u, y, s); return; }
+    if (g.model == 2) { circles_mf_forward(g, u, y, s); return; }
     if (g.nrays == 0) return;
     AIR_CHECK(g.T == MFT && g.splits >= 1, AIR_E_STATE, "matrix-free projector: tiles not built");
     const unsigned nblk = (unsigned)((int64_t)g.ntiles_side * g.ntiles_side * g.splits);
@@ -1216,6 +1217,7 @@ void mf_forward(const Geometry& g, const float* u, float* y, hipStream_t s) {
 // g = A' v (every pixel written: the tiles cover the image)
 void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_adjoint(g, v, gacc, s); return; }
+    if (g.model == 2) { circles_mf_adjoint(g, v, gacc, s); return; }
     if (g.nrays == 0) return;
     if (g.pnA > 0) {
         const int64_t n = (int64_t)g.N * g.N;
@@ -1247,6 +1249,7 @@ void mf_adjoint(const Geometry& g, const float* v, double* gacc, hipStream_t s) 
 
 void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab, double* sqw, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_row_stats(g, colw, sq, ab, sqw, s); return; }
+    if (g.model == 2) { circles_mf_row_stats(g, colw, sq, ab, sqw, s); return; }
     if (g.nrays == 0) return;
     if (g.T == MFT && g.splits >= 1 && g.nitems > 0) {
         DevBuf part;
@@ -1269,6 +1272,7 @@ void mf_row_stats(const Geometry& g, const double* colw, double* sq, double* ab,
 }
 void mf_col_stats(const Geometry& g, double* colabs, int* colcnt, hipStream_t s) {
     if (g.model == 1) { fresnel_mf_col_stats(g, colabs, colcnt, s); return; }
+    if (g.model == 2) { circles_mf_col_stats(g, colabs, colcnt, s); return; }
     if (g.nrays == 0) return;
     if (g.pnA > 0 && colabs && colcnt) {   // parallel beams: a thread per pixel, all angles, no atomics
         const int64_t n = (int64_t)g.N * g.N;

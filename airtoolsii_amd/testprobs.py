@@ -103,14 +103,15 @@ def seismictomo(N, s=None, p=None, isDisp=0, isMatrix=1, *, dtype="f32", device=
 def sphericaltomo(N, angles=None, numCircles=None, isDisp=0, isMatrix=1, *, dtype="f32", device=0,
                   return_all=False):
     """[A,b,x,angles,numCircles] = sphericaltomo(N,angles,numCircles,isDisp,isMatrix),
-    sphericaltomo.m:63-105.  Only the matrix form is provided (isMatrix=0 raises)."""
+    sphericaltomo.m:63-105.  isMatrix=0 gives the matrix-free operator, which
+    re-forms every row's circle samples on each product (sphericaltomo.m:90-93)."""
     N = int(N)
     angles = np.arange(0.0, 359.0, 2.0) if angles is None or np.size(angles) == 0 else f64(angles)
     nC = int(_matlab_round(np.sqrt(2) * N)) if numCircles is None else int(numCircles)
     e = Engine(device)
     check(load().air_build_sphericaltomo(e.h, N, ptr(angles), len(angles), nC, 0 if isMatrix else 1,
                                          DTYPES[dtype]))
-    A = AirOperator(e, dtype, False, {"kind": "spherical", "N": N, "angles": angles, "numCircles": nC})
+    A = AirOperator(e, dtype, not isMatrix, {"kind": "spherical", "N": N, "angles": angles, "numCircles": nC})
     return _finish(A, return_all, (angles, nC))
 
 

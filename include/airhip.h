@@ -157,7 +157,10 @@ int air_build_fancurvedtomo(air_engine* e, int32_t N, const double* theta, int32
 int air_build_seismictomo(air_engine* e, int32_t N, int32_t s, int32_t p, int matrix_free, int dtype);
 /* testprobs/sphericaltomo.m:1-270 (defaults resolved by the caller: numCircles
  * = round(sqrt(2) N)): row m*numCircles + n integrates circle n about centre m
- * (angles in degrees).  matrix_free must be 0 (AIR_E_UNSUPPORTED otherwise). */
+ * (angles in degrees).  matrix_free = 1: no stored matrix; every product
+ * re-forms the rows' circle samples (sphericaltomo.m:90-93,243-254) in FP64,
+ * equal to the reference's function-handle products up to FP64 rounding; the
+ * SIRT weights come from a temporary CSR (equal to the stored matrix's). */
 int air_build_sphericaltomo(air_engine* e, int32_t N, const double* angles, int32_t nA,
                             int32_t numCircles, int matrix_free, int dtype);
 /* testprobs/seismicwavetomo.m:1-288 (defaults resolved by the caller: s = N,

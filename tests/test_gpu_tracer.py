@@ -252,12 +252,49 @@ def test_spherical_narrowed_values_and_products(oracle, dtype):
     assert np.linalg.norm(A @ x - Ao @ x) <= tol * np.linalg.norm(Ao @ x)
 
 
-def test_spherical_return_all_and_matrix_free_refused():
+def test_spherical_return_all():
     A, b, x, angles, nC = air.sphericaltomo(24, return_all=True)
     assert A.shape == (180 * 34, 576) and nC == 34 and len(angles) == 180
     np.testing.assert_allclose(b, A @ x, rtol=1e-6)
-    with pytest.raises(Exception, match="matrix-free"):
-        air.sphericaltomo(8, [0.0], 4, 0, 0)
+    Af, bf, xf, _, _ = air.sphericaltomo(24, None, None, 0, 0, return_all=True)
+    assert Af.matrix_free and Af.shape == A.shape
+    assert np.array_equal(xf, x)
+    np.testing.assert_allclose(bf, b, rtol=1e-5, atol=1e-6 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("N,angles,nC", [(16, np.arange(0.0, 360.0, 37.0), 23), (33, np.arange(0.0, 360.0, 10.0), 20),
+                                         (64, None, None)])
+def test_spherical_matrix_free_products(oracle, N, angles, nC):
+    """sphericaltomo(..., isMatrix=0) re-forms the circle samples on every
+    product in FP64 (the reference's function handle, sphericaltomo.m:243-254):
+    A x and A' y equal the FP64 matrix's products up to the FP32 rounding of
+    the result, in every row and every pixel (including the samples outside
+    the image, rows that miss it and circles that leave and re-enter a pixel)."""
+    print(f"case spherical matrix-free N={N}", flush=True)
+    Af = air.sphericaltomo(N, angles, nC, 0, 0)
+    Ao = oracle.sphericaltomo(N, angles, Af.geometry["numCircles"] if nC is None else nC).tocsr()
+    assert Af.matrix_free and Af.shape == Ao.shape
+    Aabs = abs(Ao)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, Ao.shape[1]).astype(np.float32).astype(np.float64)
+    y = rng.uniform(-1, 1, Ao.shape[0]).astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(Af @ x - Ao @ x) <= 2e-7 * (Aabs @ np.abs(x)) + 1e-30)
+    assert np.all(np.abs(Af.T @ y - Ao.T @ y) <= 2e-7 * (Aabs.T @ np.abs(y)) + 1e-30)
+
+
+@pytest.mark.parametrize("method", ["cimmino", "cav", "drop", "sart"])
+def test_spherical_matrix_free_sirt_matches_matrix(method):
+    """Weights of the matrix-free operator (row and column statistics of the
+    binned circle samples) equal the stored matrix's; 20 SIRT iterations on
+    both operators agree."""
+    A = air.sphericaltomo(32, np.arange(0.0, 360.0, 4.0), 40)
+    Af = air.sphericaltomo(32, np.arange(0.0, 360.0, 4.0), 40, 0, 0)
+    x = air.phantomgallery("shepplogan", 32).ravel(order="F")
+    b = A @ x
+    opts = {"lbound": 0}
+    X, info, _ = air.sirt(method, A, b, 20, None, opts)
+    Xf, _, _ = air.sirt(method, Af, b, 20, None, dict(opts, rho=info["rho"]))
+    assert np.linalg.norm(Xf[:, -1] - X[:, -1]) <= 1e-5 * np.linalg.norm(X[:, -1])
 
 
 WAVE_CASES = {
