@@ -63,6 +63,17 @@ struct air_engine {
     air::Geometry geo;
     air::TiledOp T;            // LDS-tiled layout used by the SIRT loop when ready
     air::SpmmOp S;             // layout of the many-right-hand-side products (built on first use)
+    // grow-only device staging of x-like transfers (the FP32 user-order copy):
+    // no allocation and no device-synchronising free per call
+    air::DevBuf stg32;
+    hipEvent_t it_ev[2] = {nullptr, nullptr};   // timing of air_sirt_iterate (created on first use)
+    // pinned host staging of FP32 vectors (hipHostMalloc, grow-only) and the
+    // event after its last host-to-device copy (the buffer is rewritten only
+    // once that copy is done)
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    hipEvent_t pin_ev = nullptr;
+    bool pin_pending = false;
     bool no_tune = false;      // skip the kernel-shape autotuner (parts of a batch)
     // batch of independent problems (block-diagonal operator without CSR)
     std::vector<int64_t> b_row_off, b_col_off;
@@ -145,22 +156,44 @@ void upload(DevBuf& d, const T* h, int64_t count, hipStream_t s) {
     if (count > 0) AIR_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
 }
 
-// upload FP64 host values, narrow to FP32 on the device
-void upload_f32(DevBuf& d, const double* h, int64_t count, hipStream_t s) {
-    DevBuf tmp;
-    upload(tmp, h, count, s);
-    d.alloc(sizeof(float) * std::max<int64_t>(count, 1));
-    vec_f64_to_f32(tmp.as<double>(), d.as<float>(), count, s);
-    AIR_HIP(hipStreamSynchronize(s));
+// The engine's pinned staging buffer, at least `bytes` long, free to rewrite.
+float* pinned(air_engine* e, size_t bytes) {
+    if (e->pin_pending) {
+        AIR_HIP(hipEventSynchronize(e->pin_ev));
+        e->pin_pending = false;
+    }
+    if (bytes > e->pin_bytes || !e->pin) {
+        if (e->pin) (void)hipHostFree(e->pin);
+        e->pin = nullptr;
+        e->pin_bytes = 0;
+        AIR_HIP(hipHostMalloc(&e->pin, std::max<size_t>(bytes, 4096), hipHostMallocDefault));
+        e->pin_bytes = std::max<size_t>(bytes, 4096);
+    }
+    if (!e->pin_ev) AIR_HIP(hipEventCreateWithFlags(&e->pin_ev, hipEventDisableTiming));
+    return (float*)e->pin;
 }
 
-void download_f64(const float* d, double* h, int64_t count, hipStream_t s) {
+// upload FP64 host values as FP32: narrowed on the host (round to nearest
+// even, as on the device) into pinned memory and copied asynchronously
+// (stream-ordered; the pinned buffer is reused only after the copy)
+void upload_f32(air_engine* e, DevBuf& d, const double* h, int64_t count, hipStream_t s) {
+    d.alloc(sizeof(float) * std::max<int64_t>(count, 1));
     if (count <= 0) return;
-    DevBuf tmp;
-    tmp.alloc(sizeof(double) * count);
-    vec_f32_to_f64(d, tmp.as<double>(), count, s);
-    AIR_HIP(hipMemcpyAsync(h, tmp.p, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+    float* f = pinned(e, sizeof(float) * (size_t)count);
+    for (int64_t i = 0; i < count; ++i) f[i] = (float)h[i];
+    AIR_HIP(hipMemcpyAsync(d.p, f, sizeof(float) * count, hipMemcpyHostToDevice, s));
+    AIR_HIP(hipEventRecord(e->pin_ev, s));
+    e->pin_pending = true;
+}
+
+// FP32 device values to FP64 host values (copied in FP32 through pinned
+// memory, widened on the host)
+void download_f64(air_engine* e, const float* d, double* h, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    float* f = pinned(e, sizeof(float) * (size_t)count);
+    AIR_HIP(hipMemcpyAsync(f, d, sizeof(float) * count, hipMemcpyDeviceToHost, s));
     AIR_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < count; ++i) h[i] = (double)f[i];
 }
 
 template <typename V>
@@ -241,7 +274,7 @@ void ext_product(air_engine* e, int trans, const float* in) {
     const int64_t nin = trans ? e->m : e->n, nout = trans ? e->n : e->m;
     e->ext_in.resize(std::max<int64_t>(nin, 1));
     e->ext_out.assign(std::max<int64_t>(nout, 1), 0.0);
-    download_f64(in, e->ext_in.data(), nin, e->stream);
+    download_f64(e, in, e->ext_in.data(), nin, e->stream);
     ext_call(e, trans, e->ext_in.data(), e->ext_out.data());
 }
 void ext_apply(air_engine* e, int trans, const float* in, float* out) {
@@ -452,7 +485,7 @@ double lanczos_rho(air_engine* e, const std::vector<double>& M, const std::vecto
     hipStream_t s = e->stream;
     const int64_t m = e->m, n = e->n;
     DevBuf dM, dv, dy, dg;
-    upload_f32(dM, M.data(), m, s);
+    upload_f32(e, dM, M.data(), m, s);
     dv.alloc(4 * std::max<int64_t>(n, 1));
     dy.alloc(4 * std::max<int64_t>(m, 1));
     dg.alloc(4 * std::max<int64_t>(n, 1));
@@ -733,26 +766,23 @@ int64_t xlen(const air_engine* e) { return use_tiled(e) ? e->T.n_int : e->n; }
 // user-order FP64 n-vector -> FP32 vector of the loop's column order
 void upload_xlike(air_engine* e, DevBuf& d, const double* h, hipStream_t s) {
     if (!use_tiled(e)) {
-        upload_f32(d, h, e->n, s);
+        upload_f32(e, d, h, e->n, s);
         return;
     }
-    DevBuf tmp;
-    upload_f32(tmp, h, e->n, s);
+    upload_f32(e, e->stg32, h, e->n, s);
     d.alloc(4 * (size_t)e->T.n_int);
-    permute_to_internal(e->T, tmp.as<float>(), d.as<float>(), s);
-    AIR_HIP(hipStreamSynchronize(s));
+    permute_to_internal(e->T, e->stg32.as<float>(), d.as<float>(), s);
 }
 
 // FP32 vector of the loop's column order -> user-order FP64 host vector
 void download_xlike(air_engine* e, const float* d, double* h, hipStream_t s) {
     if (!use_tiled(e)) {
-        download_f64(d, h, e->n, s);
+        download_f64(e, d, h, e->n, s);
         return;
     }
-    DevBuf tmp;
-    tmp.alloc(4 * (size_t)std::max<int64_t>(e->n, 1));
-    permute_to_user(e->T, d, tmp.as<float>(), s);
-    download_f64(tmp.as<float>(), h, e->n, s);
+    e->stg32.alloc(4 * (size_t)std::max<int64_t>(e->n, 1));
+    permute_to_user(e->T, d, e->stg32.as<float>(), s);
+    download_f64(e, e->stg32.as<float>(), h, e->n, s);
 }
 
 void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, const double* x0, int kmax) {
@@ -784,20 +814,31 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     st.m = e->m;
     st.n = e->n;
     const int64_t m = st.m, n = st.n;
-    // M, D (get_mfun_dfun.m)
-    std::vector<double> M, D;
+    // M, D (get_mfun_dfun.m); a built-in method's weights are the engine's
+    // cached ones, used in place
+    std::vector<double> Mc, Dc;
+    const std::vector<double>* pM = &Mc;
+    const std::vector<double>* pD = &Dc;
     if (o->method == AIR_CUSTOM) {
-        M.assign(m, 1.0);
-        D.assign(n, 1.0);
-        if (o->M) std::copy(o->M, o->M + m, M.begin());
-        if (o->D) std::copy(o->D, o->D + n, D.begin());
+        Mc.assign(m, 1.0);
+        Dc.assign(n, 1.0);
+        if (o->M) std::copy(o->M, o->M + m, Mc.begin());
+        if (o->D) std::copy(o->D, o->D + n, Dc.begin());
         st.hasM = o->M != nullptr;
         st.hasD = o->D != nullptr;
     } else {
-        compute_weights(e, o->method, M, D);
+        AIR_CHECK(o->method >= AIR_LANDWEBER && o->method <= AIR_SART, AIR_E_ARG, "SIRT method not defined.");
+        if (!e->wvalid[o->method]) {
+            compute_weights_uncached(e, o->method, e->wM[o->method], e->wD[o->method]);
+            e->wvalid[o->method] = true;
+        }
+        pM = &e->wM[o->method];
+        pD = &e->wD[o->method];
         st.hasM = o->method != AIR_LANDWEBER;
         st.hasD = o->method == AIR_DROP || o->method == AIR_SART;
     }
+    const std::vector<double>& M = *pM;
+    const std::vector<double>& D = *pD;
     // rho and relaxpar (sirt.m:164-169, calc_relaxpar.m:89-171)
     double rho = o->rho;
     if (o->method == AIR_SART) rho = 1.0;
@@ -814,7 +855,7 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     st.relax_kind = o->relax_kind;
     relax_schedule(o->relax_kind, o->relaxpar, o->relax_vec, rho, kmax, st.lam);
     // device vectors
-    upload_f32(st.b, b, m, s);
+    upload_f32(e, st.b, b, m, s);
     if (x0) upload_xlike(e, st.x, x0, s);
     else {                                            // x0 = 0 (check_inputs.m:93)
         st.x.alloc(4 * (size_t)std::max<int64_t>(xlen(e), 1));
@@ -822,7 +863,7 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     }
     const float *dMp = nullptr, *dDp = nullptr;
     if (o->method == AIR_CUSTOM) {
-        if (st.hasM) upload_f32(st.M, M.data(), m, s);
+        if (st.hasM) upload_f32(e, st.M, M.data(), m, s);
         if (st.hasD) upload_xlike(e, st.D, D.data(), s);
         dMp = st.hasM ? st.M.as<float>() : nullptr;
         dDp = st.hasD ? st.D.as<float>() : nullptr;
@@ -830,7 +871,7 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
         // device copies of a built-in method's weights are kept with the operator
         const int q = o->method;
         if (!e->wdev[q]) {
-            if (st.hasM) upload_f32(e->wMdev[q], M.data(), m, s);
+            if (st.hasM) upload_f32(e, e->wMdev[q], M.data(), m, s);
             if (st.hasD) upload_xlike(e, e->wDdev[q], D.data(), s);
             e->wdev[q] = true;
         }
@@ -998,6 +1039,10 @@ void air_destroy(air_engine* e) {
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
     for (auto& v : e->ev) (void)hipEventDestroy(v);
+    for (auto& v : e->it_ev)
+        if (v) (void)hipEventDestroy(v);
+    if (e->pin_ev) (void)hipEventDestroy(e->pin_ev);
+    if (e->pin) (void)hipHostFree(e->pin);
     reset_operator(e);
     e->st = SirtState();
     (void)hipStreamDestroy(e->stream);
@@ -1262,10 +1307,10 @@ int air_apply(air_engine* e, int transp, const double* in, double* out) {
         hipStream_t s = e->stream;
         int64_t nin = transp ? e->m : e->n, nout = transp ? e->n : e->m;
         DevBuf din, dout;
-        upload_f32(din, in, nin, s);
+        upload_f32(e, din, in, nin, s);
         dout.alloc(4 * std::max<int64_t>(nout, 1));
         apply_dev(e, transp, din.as<float>(), dout.as<float>());
-        download_f64(dout.as<float>(), out, nout, s);
+        download_f64(e, dout.as<float>(), out, nout, s);
     });
 }
 
@@ -1325,9 +1370,9 @@ int air_fbp(air_engine* e, const double* b, int32_t na, int filter, double* x) {
         DevBuf din, dout;
         dout.alloc(4 * std::max<int64_t>(e->n, 1));
         if (filter == AIR_FBP_NONE) {                     // fbp.m:37
-            upload_f32(din, b, e->m, s);
+            upload_f32(e, din, b, e->m, s);
             apply_dev(e, 1, din.as<float>(), dout.as<float>());
-            download_f64(dout.as<float>(), x, e->n, s);
+            download_f64(e, dout.as<float>(), x, e->n, s);
             return;
         }
         // fbp.m:40-43: reshape(b, nr, na) needs m to be a multiple of na
@@ -1336,7 +1381,7 @@ int air_fbp(air_engine* e, const double* b, int32_t na, int filter, double* x) {
         din.alloc(4 * std::max<int64_t>(e->m, 1));
         fbp_filter_sinogram(b, nr, na, filter, din.as<float>(), s);
         apply_dev(e, 1, din.as<float>(), dout.as<float>());
-        download_f64(dout.as<float>(), x, e->n, s);
+        download_f64(e, dout.as<float>(), x, e->n, s);
         for (int64_t j = 0; j < e->n; ++j) x[j] = x[j] * M_PI / (2.0 * na);   // fbp.m:80
     });
 }
@@ -1388,9 +1433,11 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         AIR_CHECK(e->st.prepared, AIR_E_STATE, "air_sirt_prepare must be called first");
         use_device(e);
         SirtState& st = e->st;
-        hipEvent_t a, b;
-        AIR_HIP(hipEventCreate(&a));
-        AIR_HIP(hipEventCreate(&b));
+        if (!e->it_ev[0]) {
+            AIR_HIP(hipEventCreate(&e->it_ev[0]));
+            AIR_HIP(hipEventCreate(&e->it_ev[1]));
+        }
+        hipEvent_t a = e->it_ev[0], b = e->it_ev[1];
         auto t0 = std::chrono::steady_clock::now();
         timing_begin(e, niter);
         AIR_HIP(hipEventRecord(a, e->stream));
@@ -1410,8 +1457,6 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         timing_end(e);
         float ms = 0;
         AIR_HIP(hipEventElapsedTime(&ms, a, b));
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         st.iter_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (gpu_ms) *gpu_ms = ms;
     });
@@ -1424,15 +1469,19 @@ int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info*
         use_device(e);
         SirtState& st = e->st;
         hipStream_t s = e->stream;
+        // x (whatever the stop), the control block and the norm history are
+        // queued together and waited for once
         Ctrl c;
         AIR_HIP(hipMemcpyAsync(&c, st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+        std::vector<double> hist;
+        if (resnorm) {
+            hist.resize((size_t)st.kmax + 1);
+            AIR_HIP(hipMemcpyAsync(hist.data(), st.hist.p, 8 * hist.size(), hipMemcpyDeviceToHost, s));
+        }
+        if (x_out) download_xlike(e, st.x.as<float>(), x_out, s);
         AIR_HIP(hipStreamSynchronize(s));
         int fin = c.stop ? c.finaliter : st.k;
-        if (x_out) download_xlike(e, st.x.as<float>(), x_out, s);
-        if (resnorm) {
-            AIR_HIP(hipMemcpyAsync(resnorm, st.hist.p, 8 * (size_t)(fin + 1), hipMemcpyDeviceToHost, s));
-            AIR_HIP(hipStreamSynchronize(s));
-        }
+        if (resnorm) std::copy(hist.begin(), hist.begin() + (fin + 1), resnorm);
         if (info) {
             info->stoprule = c.stop ? c.stoprule : 0;
             info->finaliter = fin;
@@ -1862,11 +1911,11 @@ int air_sirt_multi(air_engine* e, const air_sirt_options* o, int32_t nrhs, const
             dMR.alloc(4 * (size_t)m * L);
             AIR_HIP(hipMemcpyAsync(dB.p, hB.data(), 4 * hB.size(), hipMemcpyHostToDevice, s));
             AIR_HIP(hipMemcpyAsync(dX.p, hX.data(), 4 * hX.size(), hipMemcpyHostToDevice, s));
-            if (hasM) upload_f32(dM, M.data(), m, s);
+            if (hasM) upload_f32(e, dM, M.data(), m, s);
             if (hasD) {
                 std::vector<double> Di((size_t)ni, 0.0);
                 for (int64_t c = 0; c < n; ++c) Di[perm[c]] = D[c];
-                upload_f32(dD, Di.data(), ni, s);
+                upload_f32(e, dD, Di.data(), ni, s);
             }
             const unsigned gr = spmm_rows_grid(m, e->num_cu);
             part.alloc(8 * (size_t)gr * L);
@@ -1916,8 +1965,8 @@ int air_sirt_multi(air_engine* e, const air_sirt_options* o, int32_t nrhs, const
         dMR.alloc(4 * (size_t)m * L);
         AIR_HIP(hipMemcpyAsync(dB.p, hB.data(), 4 * hB.size(), hipMemcpyHostToDevice, s));
         AIR_HIP(hipMemcpyAsync(dX.p, hX.data(), 4 * hX.size(), hipMemcpyHostToDevice, s));
-        if (hasM) upload_f32(dM, M.data(), m, s);
-        if (hasD) upload_f32(dD, D.data(), n, s);
+        if (hasM) upload_f32(e, dM, M.data(), m, s);
+        if (hasD) upload_f32(e, dD, D.data(), n, s);
         const unsigned gr = multi_grid(m, e->num_cu), gc = multi_grid(n, e->num_cu);
         part.alloc(8 * (size_t)gr * L);
         nrm.alloc(8 * L);
