@@ -1680,6 +1680,7 @@ int air_layout_info(air_engine* e, air_layout* out) {
             L.bwd_step = tiled_bwd_step(T) ? 1 : 0;
             L.bwd_step_ok = T.bstep_ok ? 1 : 0;
             L.bwd_step_entries = T.nbse;
+            L.fwd_waves_per_block = T.fwd_wpb;
         }
         *out = L;
     });

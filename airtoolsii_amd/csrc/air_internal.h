@@ -262,6 +262,7 @@ struct TiledOp {
     // stream form of the forward product: waves own runs of whole pieces
     DevBuf wblk_tile, wave_p0, wave_e;     // wave_e: first stream entry of every wave (wave_p0's pieces)
     int nblk_w = 0, nwaves = 0, wave_target = 4096;
+    int fwd_wpb = 4;                       // waves per block of the step-coded forward (4 or 16; other forms 4)
     // backward codes carry a column-start flag in bit 15 when every tile has
     // fewer than 32768 distinct rows (required by the stream form)
     bool bwd_flags = false;

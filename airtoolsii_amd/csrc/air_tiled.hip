@@ -534,71 +534,104 @@ __device__ __forceinline__ void stream_lanes(int64_t E0, int64_t E1, const V* __
 //   bits 14+2i, 15+2i (i = 1..7)  entry i's move: x step, y step (both 0 on
 //                padding entries, which carry value 0)
 // and the tile of the operand sits in LDS in the same x * 64 + y order.
+// The build checks every lane's moves against the real tile positions (a
+// word whose decode leaves the staircase rejects the layout), so the decoded
+// positions are exact and stay inside the tile.  They are carried as LDS byte
+// offsets: per entry one bit extract and shift-add for x (+256 B) and one
+// bit extract and 24-bit multiply-add for y (+-4 B).
 __device__ __forceinline__ float step_products(const float (&v)[8], unsigned w, const float* lds) {
-    unsigned p = w & 0xfffu;
-    const int yd = (w & 0x2000u) ? -1 : 1;
-    float s = v[0] * lds[p];
+    const char* base = (const char*)lds;
+    unsigned pb = (w & 0xfffu) << 2;
+    const int yd4 = ((((int)(w << 18)) >> 31) << 3) + 4;    // -4 with bit 13 (dy = -1), else +4
+    float s = v[0] * *(const float*)(base + pb);
 #pragma unroll
     for (int i = 1; i < 8; ++i) {
-        const unsigned sx = (w >> (14 + 2 * i)) & 1u, sy = (w >> (15 + 2 * i)) & 1u;
-        p = (unsigned)((int)(p + (sx << 6)) + __mul24((int)sy, yd));
-        s += v[i] * lds[p & 0xfffu];
+        pb += ((w >> (14 + 2 * i)) & 1u) << 8;
+        pb = (unsigned)((int)pb + __mul24((int)((w >> (15 + 2 * i)) & 1u), yd4));
+        s += v[i] * *(const float*)(base + pb);
     }
     return s;
 }
 
-// One round of a step-coded stream: lane l's 8 values and its word (zeros past E1).
+// A wave-uniform 64-bit value (every lane holds the same) moved to scalar registers.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(uint64_t)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// One round of a step-coded stream (lane l's 8 values and its word) with rb
+// wave-uniform: a scalar base address plus the lane's 32-bit entry offset
+// le (8 * lane for a round, 0 for a one-line broadcast), and no condition.
+// The stream arrays keep AIR_SPARSE_PAD (2048) entries past their end, so a
+// round read from any rb up to 512 entries past the last one stays inside
+// them.  Loads that no branch skips keep the compiler's waits for them exact
+// (a wait at the first use of one round does not also wait for the next
+// round's loads).
 template <typename V>
-__device__ __forceinline__ void steps_load_round(const V* __restrict__ val, const unsigned* __restrict__ words,
-                                                 int64_t rb, int64_t E1, float (&vv)[8], unsigned& ww) {
+__device__ __forceinline__ void steps_load_at(const V* __restrict__ val, const unsigned* __restrict__ words,
+                                              int64_t rb, int le, float (&vv)[8], unsigned& ww) {
     constexpr int VE = VecOf<V>::VE, NV = 8 / VE;
-    const int64_t q = rb + 8 * (int64_t)(threadIdx.x & 63);
-    if (q < E1) {
+    const V* vp = val + rb;
+    const unsigned* wp = words + (rb >> 3);
 #pragma unroll
-        for (int h = 0; h < NV; ++h) {
-            float t[VE];
-            load_vals<V, float>(val + q + VE * h, t);
+    for (int h = 0; h < NV; ++h) {
+        float t[VE];
+        load_vals<V, float>(vp + le + VE * h, t);
 #pragma unroll
-            for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
-        }
-        ww = __builtin_nontemporal_load(words + (q >> 3));
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) vv[i] = 0.f;
-        ww = 0u;
+        for (int i = 0; i < VE; ++i) vv[VE * h + i] = t[i];
     }
+    ww = __builtin_nontemporal_load(wp + (le >> 3));
 }
 
 // The wave's stream [E0, E1) with its first round (v, w) already loaded by
-// the caller, so that those loads can be in flight with the block's prologue.
+// the caller (steps_load_at at E0), so that those loads can be in flight with
+// the block's prologue.  E0, E1 and k0 are wave-uniform.  Lanes past E1 in
+// the last round hold entries that are not the wave's: their start flags are
+// masked, so they close nothing, and their sums only reach higher lanes.
+// With PF the rounds alternate between two register sets (the next round
+// loads into the set the current one is not using; on the last round the
+// load is a one-line broadcast), so no values are copied between rounds.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                              const unsigned* __restrict__ words, const float* lds, int k0,
                                              float (&v)[8], unsigned w, Close&& close) {
     static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
     const int lane = (int)(threadIdx.x & 63);
-    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) { steps_load_round<V>(val, words, rb, E1, vv, ww); };
+    const int le = 8 * lane;
     int64_t rb = E0;
     int knext = k0;
     bool open = false;
     float carry = 0.f;
-    while (rb < E1) {
-        const bool last_round = rb + 512 >= E1;
-        float vn[8];
-        unsigned wn = 0u;
-        if (PF && !last_round) load_round(rb + 512, vn, wn);
-        const float s = step_products(v, w, lds);
+    auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
+        const float s = step_products(vv, ww, lds);
         const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
-        lanes_round(s, ((w >> 12) & 1u) != 0u, last_round, nvalid, lane, knext, open, carry, close);
-        rb += 512;
-        if (!last_round) {
-            if (PF) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = vn[i];
-                w = wn;
-            } else {
-                load_round(rb, v, w);
-            }
+        const bool start = ((ww >> 12) & 1u) != 0u && lane < nvalid;
+        lanes_round(s, start, last_round, nvalid, lane, knext, open, carry, close);
+    };
+    if (rb >= E1) return;
+    if constexpr (PF != 0) {
+        float vb[8];
+        unsigned wb;
+        for (;;) {
+            bool last_round = rb + 512 >= E1;
+            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, vb, wb);
+            run_round(v, w, last_round);
+            if (last_round) break;
+            rb += 512;
+            last_round = rb + 512 >= E1;
+            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, v, w);
+            run_round(vb, wb, last_round);
+            if (last_round) break;
+            rb += 512;
+        }
+    } else {
+        for (;;) {
+            const bool last_round = rb + 512 >= E1;
+            run_round(v, w, last_round);
+            if (last_round) break;
+            rb += 512;
+            steps_load_at<V>(val, words, rb, le, v, w);
         }
     }
 }
@@ -648,38 +681,41 @@ __device__ __host__ __forceinline__ unsigned nested_to_plain(unsigned q) {
 // Forward product in the step-coded lane form: as k_tfwd_flat, with the tile
 // of x staged in LDS in plain x * 64 + y order (each aligned group of 4
 // nested entries is 4 consecutive y of one x, so it moves as one float4).
-template <typename V, int PF>
-__global__ __launch_bounds__(BLOCK) void k_tfwd_step(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
-                                                     int nwaves, int nblk, const int64_t* __restrict__ wave_e,
-                                                     const V* __restrict__ fval, const unsigned* __restrict__ fstep,
-                                                     const float* __restrict__ x, float* __restrict__ partial,
-                                                     const Ctrl* ctrl) {
+// WPB waves share one staged tile: 4, or 16 for tiles with many entries
+// (one staging of the 16 KB tile per 16 waves instead of per 4).
+template <typename V, int PF, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_tfwd_step(const int* __restrict__ blk_tile, const int* __restrict__ wave_p0,
+                                                        int nwaves, int nblk, const int64_t* __restrict__ wave_e,
+                                                        const V* __restrict__ fval, const unsigned* __restrict__ fstep,
+                                                        const float* __restrict__ x, float* __restrict__ partial,
+                                                        const Ctrl* ctrl) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    constexpr int WPB = BLOCK / 64;
+    constexpr int NT = WPB * 64;
+    static_assert(FT % (4 * NT) == 0, "the tile of x moves as whole float4 per thread");
     __shared__ __align__(16) float xs[FT];
     if (ctrl && load_stop(ctrl)) return;
     const int b = xcd_block(blockIdx.x, nblk);
     const int t = blk_tile[b];
     // the wave's range and first round are loaded with the tile of x, so the
     // block waits for one round trip before the tile is staged
-    const int w = b * WPB + (int)(threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(b * WPB + (int)(threadIdx.x >> 6));
     int pw0 = 0;
     int64_t E0 = 0, E1 = 0;
     if (w < nwaves) {
-        pw0 = wave_p0[w];
-        E0 = wave_e[w];
-        E1 = wave_e[w + 1];
+        pw0 = __builtin_amdgcn_readfirstlane(wave_p0[w]);
+        E0 = wave_uniform(wave_e[w]);
+        E1 = wave_uniform(wave_e[w + 1]);
     }
     const f4* xt = (const f4*)(x + (int64_t)t * FT);
-    f4 tmp[FT / (4 * BLOCK)];
+    f4 tmp[FT / (4 * NT)];
 #pragma unroll
-    for (int i = 0; i < FT / (4 * BLOCK); ++i) tmp[i] = xt[threadIdx.x + BLOCK * i];
+    for (int i = 0; i < FT / (4 * NT); ++i) tmp[i] = xt[threadIdx.x + NT * i];
     float v[8];
     unsigned wd;
-    steps_load_round<V>(fval, fstep, E0, E1, v, wd);
+    steps_load_at<V>(fval, fstep, E0, E0 < E1 ? 8 * (int)(threadIdx.x & 63) : 0, v, wd);
 #pragma unroll
-    for (int i = 0; i < FT / (4 * BLOCK); ++i)
-        ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + BLOCK * i)) >> 2] = tmp[i];
+    for (int i = 0; i < FT / (4 * NT); ++i)
+        ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + NT * i)) >> 2] = tmp[i];
     __syncthreads();
     if (E0 >= E1) return;
     auto close = [&](int k, float sum) { partial[k] = sum; };
@@ -1011,35 +1047,51 @@ __device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w,
 }
 
 // As stream_steps, for the backward words; the first round (v, w) is loaded
-// by the caller before the block stages its rows.
+// by the caller (steps_load_at at E0) before the block stages its rows.  E0,
+// E1 and k0 are wave-uniform.  A word past E1 may belong to another tile,
+// whose slots this tile's staging does not hold, so the last round zeroes the
+// words of its lanes past E1 (a zero word reads slots 0-7, which the staging
+// always holds) as well as their start flags.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                               const unsigned* __restrict__ words, const float* ys, int R, int k0,
                                               float (&v)[8], unsigned w, Close&& close) {
     static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
     const int lane = (int)(threadIdx.x & 63);
-    auto load_round = [&](int64_t rb, float (&vv)[8], unsigned& ww) { steps_load_round<V>(val, words, rb, E1, vv, ww); };
+    const int le = 8 * lane;
     int64_t rb = E0;
     int knext = k0;
     bool open = false;
     float carry = 0.f;
-    while (rb < E1) {
-        const bool last_round = rb + 512 >= E1;
-        float vn[8];
-        unsigned wn = 0u;
-        if (PF && !last_round) load_round(rb + 512, vn, wn);
-        const float s = bstep_products(v, w, ys, R);
+    auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
         const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
-        lanes_round(s, (w >> 31) != 0u, last_round, nvalid, lane, knext, open, carry, close);
-        rb += 512;
-        if (!last_round) {
-            if (PF) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = vn[i];
-                w = wn;
-            } else {
-                load_round(rb, v, w);
-            }
+        ww = lane < nvalid ? ww : 0u;
+        const float s = bstep_products(vv, ww, ys, R);
+        lanes_round(s, (ww >> 31) != 0u, last_round, nvalid, lane, knext, open, carry, close);
+    };
+    if (rb >= E1) return;
+    if constexpr (PF != 0) {
+        float vb[8];
+        unsigned wb;
+        for (;;) {
+            bool last_round = rb + 512 >= E1;
+            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, vb, wb);
+            run_round(v, w, last_round);
+            if (last_round) break;
+            rb += 512;
+            last_round = rb + 512 >= E1;
+            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, v, w);
+            run_round(vb, wb, last_round);
+            if (last_round) break;
+            rb += 512;
+        }
+    } else {
+        for (;;) {
+            const bool last_round = rb + 512 >= E1;
+            run_round(v, w, last_round);
+            if (last_round) break;
+            rb += 512;
+            steps_load_at<V>(val, words, rb, le, v, w);
         }
     }
 }
@@ -1116,12 +1168,12 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
     const int64_t c0 = (int64_t)t * MAX_BT;
     // the wave's range and first round first, so that they are in flight
     // while the block stages the tile's rows
-    const int wv = (int)(threadIdx.x >> 6);
-    const int64_t ea = a.wave_e[2 * (t * NW + wv)], eb = a.wave_e[2 * (t * NW + wv) + 1];
-    const int ka = a.wave_k[t * NW + wv];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t ea = wave_uniform(a.wave_e[2 * (t * NW + wv)]), eb = wave_uniform(a.wave_e[2 * (t * NW + wv) + 1]);
+    const int ka = __builtin_amdgcn_readfirstlane(a.wave_k[t * NW + wv]);
     float v[8];
     unsigned wd;
-    steps_load_round<V>(bval, a.words, ea, eb, v, wd);
+    steps_load_at<V>(bval, a.words, ea, ea < eb ? 8 * (int)(threadIdx.x & 63) : 0, v, wd);
     const int64_t Et0 = a.col_ptr[c0];
     const int R = a.R[t];
     const int c = (int)threadIdx.x;
@@ -1576,7 +1628,7 @@ void set_fwd_blocks(TiledOp& T, int ppb, hipStream_t s) {
 // (one block = 4 waves of one tile) of about target_entries each, cut at
 // piece boundaries.
 void set_fwd_waves(TiledOp& T, int target, hipStream_t s) {
-    constexpr int WPB = BLOCK / 64;
+    const int WPB = T.fwd_wpb;
     const std::vector<int>& ptile = T.piece_tile_h;
     const std::vector<int64_t>& pp = T.piece_ptr_h;
     const int64_t P = (int64_t)ptile.size();
@@ -2087,12 +2139,14 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                                                 T.fcode.as<unsigned short>(), x_int, T.partial.as<float>(), ctrl);
             };
             auto launch_step = [&](auto kern) {
-                kern<<<T.nblk_w, BLOCK, 0, s>>>(T.wblk_tile.as<int>(), T.wave_p0.as<int>(), T.nwaves, T.nblk_w,
-                                                T.wave_e.as<int64_t>(), T.fval.as<V>(), T.fstep.as<unsigned>(),
-                                                x_int, T.partial.as<float>(), ctrl);
+                kern<<<T.nblk_w, 64 * T.fwd_wpb, 0, s>>>(T.wblk_tile.as<int>(), T.wave_p0.as<int>(), T.nwaves,
+                                                         T.nblk_w, T.wave_e.as<int64_t>(), T.fval.as<V>(),
+                                                         T.fstep.as<unsigned>(), x_int, T.partial.as<float>(), ctrl);
             };
             int sv = cfg - N_FWD_CFG;
             if (sv >= 6 && !T.fstep_ok) sv -= 2;            // no step words: the lane form
+            AIR_CHECK(sv >= 6 || T.fwd_wpb == 4, AIR_E_STATE,
+                      "stream forward: the wave layout has 16 waves per block, which only the step-coded form runs");
             if constexpr (std::is_same<V, double>::value) {
                 if (sv >= 4) sv -= 4;
                 switch (sv) {
@@ -2109,8 +2163,14 @@ static void launch_fwd_pieces(const TiledOp& T, const float* x_int, const Ctrl* 
                 case 3: launch(k_tfwd_flat<V, 16, 1>); break;
                 case 4: launch(k_tfwd_flat<V, 0, 0>); break;
                 case 5: launch(k_tfwd_flat<V, 0, 1>); break;
-                case 6: launch_step(k_tfwd_step<V, 0>); break;
-                default: launch_step(k_tfwd_step<V, 1>); break;
+                case 6:
+                    if (T.fwd_wpb == 16) launch_step(k_tfwd_step<V, 0, 16>);
+                    else launch_step(k_tfwd_step<V, 0, 4>);
+                    break;
+                default:
+                    if (T.fwd_wpb == 16) launch_step(k_tfwd_step<V, 1, 16>);
+                    else launch_step(k_tfwd_step<V, 1, 4>);
+                    break;
                 }
             }
             return 0;
@@ -2258,8 +2318,8 @@ void tiled_shapes(const TiledOp& T, int* fl, int* fu, int* bl, int* bu) {
 void autotune_tiled(TiledOp& T, hipStream_t s) {
     const char* fixed = std::getenv("AIR_TILED_CFG");
     if (fixed && fixed[0]) {
-        int f = 0, b = 0, par = 0;
-        const int got = std::sscanf(fixed, "%d,%d,%d", &f, &b, &par);
+        int f = 0, b = 0, par = 0, wpb = 4;
+        const int got = std::sscanf(fixed, "%d,%d,%d,%d", &f, &b, &par, &wpb);
         if (got >= 2) {
             T.fwd_cfg = std::min(std::max(f, 0), N_FWD_CFG + N_FWD_STREAM - 1);
             T.bwd_cfg = std::min(std::max(b, 0), N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0) - 1);
@@ -2269,8 +2329,11 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
                 if (T.fwd_cfg >= N_FWD_CFG + 4) T.fwd_cfg -= 4;
                 if (T.bwd_cfg >= N_BWD_CFG + 3) T.bwd_cfg = N_BWD_CFG;
             }
-            if (got == 3 && T.fwd_cfg < N_FWD_CFG) set_fwd_blocks(T, std::min(std::max(par, 64), 4096), s);
-            if (got == 3 && T.fwd_cfg >= N_FWD_CFG) set_fwd_waves(T, std::min(std::max(par, 512), 1 << 20), s);
+            if (got >= 3 && T.fwd_cfg < N_FWD_CFG) set_fwd_blocks(T, std::min(std::max(par, 64), 4096), s);
+            if (got >= 3 && T.fwd_cfg >= N_FWD_CFG) {
+                T.fwd_wpb = (got == 4 && wpb == 16 && tiled_fwd_step(T)) ? 16 : 4;
+                set_fwd_waves(T, std::min(std::max(par, 512), 1 << 20), s);
+            }
             return;
         }
     }
@@ -2305,8 +2368,9 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
         }
     }
     set_fwd_blocks(T, best_ppb, s);
-    // stream forms at WAVE_TARGETS entries per wave
-    int best_wt = T.wave_target;
+    // stream forms at WAVE_TARGETS entries per wave, 4 waves per block
+    int best_wt = T.wave_target, best_wpb = 4;
+    T.fwd_wpb = 4;
     for (int wi = 0; wi < N_WAVE_TARGETS; ++wi) {
         set_fwd_waves(T, WAVE_TARGETS[wi], s);
         for (int sv = 0; sv < N_FWD_STREAM; ++sv) {
@@ -2318,6 +2382,20 @@ void autotune_tiled(TiledOp& T, hipStream_t s) {
             if (t < best) { best = t; T.fwd_cfg = c; best_wt = WAVE_TARGETS[wi]; }
         }
     }
+    // the step-coded form with 16 waves per block
+    if (T.fstep_ok && lane_form_ok(T.dtype)) {
+        T.fwd_wpb = 16;
+        for (int wi = 0; wi < N_WAVE_TARGETS; ++wi) {
+            set_fwd_waves(T, WAVE_TARGETS[wi], s);
+            for (int sv = 6; sv < 8; ++sv) {
+                const int c = N_FWD_CFG + sv;
+                float t = time_it([&] { launch_fwd_pieces(T, x.as<float>(), nullptr, s, c); });
+                T.tune_fwd_ms[3 * N_FWD_CFG + N_WAVE_TARGETS * N_FWD_STREAM + N_WAVE_TARGETS * (sv - 6) + wi] = t / 4;
+                if (t < best) { best = t; T.fwd_cfg = c; best_wt = WAVE_TARGETS[wi]; best_wpb = 16; }
+            }
+        }
+    }
+    T.fwd_wpb = best_wpb;
     set_fwd_waves(T, best_wt, s);
     best = 1e30f;
     for (int c = 0; c < N_BWD_CFG + (T.bwd_flags ? N_BWD_STREAM : 0); ++c) {

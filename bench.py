@@ -529,7 +529,8 @@ def main():
                             "bwd_step_entries": L["bwd_step_entries"],
                             "shapes": [L["fwd_lanes"], L["fwd_loads"], L["fwd_pieces_per_block"], L["bwd_lanes"],
                                        L["bwd_loads"]],
-                            "tune_fwd_us": L["tune_fwd_us"][:47], "tune_bwd_us": L["tune_bwd_us"][:11],
+                            "fwd_waves_per_block": L["fwd_waves_per_block"],
+                            "tune_fwd_us": L["tune_fwd_us"][:55], "tune_bwd_us": L["tune_bwd_us"][:11],
                             "fwd_entries": L["fwd_entries"], "bwd_entries": L["bwd_entries"]}
     out["setup_s"] = {"trace_csr_csc": round(t_trace, 3), "weights_rho_r0": round(t_setup, 3)}
     # the whole C-ABI call as a host caller sees it: b and x0 from host memory,

@@ -277,7 +277,9 @@ typedef struct {
                                        block -> [3c+i], c < 5; stream form v (v < 4: 8 or 16 entries per
                                        lane, without/with prefetch; v = 4, 5: the lane form without/with
                                        prefetch; v = 6, 7: the step-coded lane form without/with prefetch)
-                                       at 2048/4096/8192/16384 entries per wave -> [15+4v+i]; 0: not run */
+                                       at 2048/4096/8192/16384 entries per wave -> [15+4v+i]; the
+                                       step-coded form v = 6, 7 with 16 waves per block -> [47+4(v-6)+i];
+                                       0: not run */
     float   tune_bwd_us[16];        /* backward shapes: per-column 0-3, stream form 4-10 (8 entries per
                                        lane with prefetch, 16 without, 16 with, lane form without, with,
                                        step-coded lane form without, with) */
@@ -291,6 +293,8 @@ typedef struct {
                                        staging fits LDS */
     int64_t bwd_step_entries;       /* entries of the step-coded backward stream (bridging zeros and
                                        lane padding included) */
+    int     fwd_waves_per_block;    /* stream forward: waves sharing one staged tile of x (4, or 16 for
+                                       the step-coded form) */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 

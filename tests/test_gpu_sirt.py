@@ -221,19 +221,24 @@ def test_train_relaxpar_matches_oracle(oracle):
 # (one word of unit moves per 8 entries); backward shapes 0-3 are
 # per-column lane groups, 4-6 the stream form (8 entries with prefetch, 16
 # without, 16 with), 7-8 the lane form.  N=96 gives 4 forward tiles with
-# padded columns.
+# padded columns.  A fourth field of 16 puts 16 waves in each block of the
+# step-coded forward (one staged tile of x per 16 waves); the other forms keep 4.
 @pytest.mark.parametrize("cfg", ["0,0,256", "1,1,512", "2,2,1024", "3,3,256", "4,0,512",
                                  "5,1,512", "6,2,2048", "7,3,8192", "8,0,100000",
                                  "5,4,2048", "7,5,4096", "0,6,256", "8,6,512",
                                  "9,7,2048", "10,8,8192", "9,8,512", "10,7,100000", "0,7,256", "10,0,4096",
                                  "5,8,1024", "11,7,4096", "12,8,2048", "11,0,100000", "12,5,512",
-                                 "11,9,4096", "12,10,2048", "0,9,256", "9,10,512", "5,9,8192"])
+                                 "11,9,4096", "12,10,2048", "0,9,256", "9,10,512", "5,9,8192",
+                                 "11,10,4096,16", "12,10,2048,16", "12,9,512,16", "11,8,100000,16",
+                                 "12,0,1024,16", "10,10,2048,16"])
 def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
     N, theta, p = 96, np.arange(0.0, 180.0, 2.0), 137
     monkeypatch.setenv("AIR_TILED_CFG", cfg)
     A = air.paralleltomo(N, theta, p)
     L = A.layout()
-    f, bcfg, par = (int(v) for v in cfg.split(","))
+    f, bcfg, par, *rest = (int(v) for v in cfg.split(","))
+    wpb = rest[0] if rest else 4
+    assert L["fwd_waves_per_block"] == (wpb if f >= 11 else 4)
     assert L["tiled"] == 1 and L["fwd_stream"] == (1 if f >= 5 else 0)
     assert L["bwd_lanes"] == {4: 8, 5: 16, 6: 16, 7: 0, 8: 0, 9: 0, 10: 0}.get(bcfg, L["bwd_lanes"])
     assert L["bwd_step_ok"] == 1 and L["bwd_step"] == (1 if bcfg >= 9 else 0)
@@ -259,7 +264,8 @@ def test_tiled_kernel_shapes(oracle, monkeypatch, cfg):
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16", "f64"])
 @pytest.mark.parametrize("cfg", ["1,1,256", "6,1,2048", "8,6,4096", "7,5,512", "10,8,4096", "9,7,512",
-                                 "11,8,4096", "12,7,512", "12,10,4096", "11,9,512"])
+                                 "11,8,4096", "12,7,512", "12,10,4096", "11,9,512", "12,10,4096,16",
+                                 "11,9,512,16"])
 def test_tiled_shapes_narrow_storage(c1, monkeypatch, dtype, cfg):
     """Per-piece and stream forward kernels agree on every storage type."""
     opts = {"rho": c1["rho"]["sart"], "lbound": 0}
