@@ -584,14 +584,25 @@ __device__ __forceinline__ void steps_load_at(const V* __restrict__ val, const u
     ww = __builtin_nontemporal_load(wp + (le >> 3));
 }
 
+// The lane offset for round r of a stream that ends at E1: 8 * lane, except
+// that in a last, partial round the lanes past E1 read the last valid lane's
+// entries (the same cache lines, so the round reads nothing past E1); a round
+// that does not exist (r >= E1) becomes a one-line broadcast.
+__device__ __forceinline__ int steps_lane_offset(int le, int64_t r, int64_t E1) {
+    if (r >= E1) return 0;
+    const int64_t left = E1 - r < 512 ? E1 - r : 512;
+    return min(le, (int)left - 8);
+}
+
 // The wave's stream [E0, E1) with its first round (v, w) already loaded by
 // the caller (steps_load_at at E0), so that those loads can be in flight with
 // the block's prologue.  E0, E1 and k0 are wave-uniform.  Lanes past E1 in
-// the last round hold entries that are not the wave's: their start flags are
-// masked, so they close nothing, and their sums only reach higher lanes.
-// With PF the rounds alternate between two register sets (the next round
-// loads into the set the current one is not using; on the last round the
-// load is a one-line broadcast), so no values are copied between rounds.
+// the last round hold a copy of the last valid lane's entries
+// (steps_lane_offset): their start flags are masked, so they close nothing,
+// and their sums only reach higher lanes.  With PF the rounds alternate
+// between two register sets (the next round loads into the set the current
+// one is not using; on the last round the load re-reads one line of the
+// current round), so no values are copied between rounds.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                              const unsigned* __restrict__ words, const float* lds, int k0,
@@ -615,12 +626,12 @@ __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __
         unsigned wb;
         for (;;) {
             bool last_round = rb + 512 >= E1;
-            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, vb, wb);
+            steps_load_at<V>(val, words, last_round ? rb : rb + 512, steps_lane_offset(le, rb + 512, E1), vb, wb);
             run_round(v, w, last_round);
             if (last_round) break;
             rb += 512;
             last_round = rb + 512 >= E1;
-            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, v, w);
+            steps_load_at<V>(val, words, last_round ? rb : rb + 512, steps_lane_offset(le, rb + 512, E1), v, w);
             run_round(vb, wb, last_round);
             if (last_round) break;
             rb += 512;
@@ -631,7 +642,7 @@ __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __
             run_round(v, w, last_round);
             if (last_round) break;
             rb += 512;
-            steps_load_at<V>(val, words, rb, le, v, w);
+            steps_load_at<V>(val, words, rb, steps_lane_offset(le, rb, E1), v, w);
         }
     }
 }
@@ -712,7 +723,7 @@ __global__ __launch_bounds__(WPB * 64) void k_tfwd_step(const int* __restrict__ 
     for (int i = 0; i < FT / (4 * NT); ++i) tmp[i] = xt[threadIdx.x + NT * i];
     float v[8];
     unsigned wd;
-    steps_load_at<V>(fval, fstep, E0, E0 < E1 ? 8 * (int)(threadIdx.x & 63) : 0, v, wd);
+    steps_load_at<V>(fval, fstep, E0, steps_lane_offset(8 * (int)(threadIdx.x & 63), E0, E1), v, wd);
 #pragma unroll
     for (int i = 0; i < FT / (4 * NT); ++i)
         ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + NT * i)) >> 2] = tmp[i];
@@ -1048,10 +1059,9 @@ __device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w,
 
 // As stream_steps, for the backward words; the first round (v, w) is loaded
 // by the caller (steps_load_at at E0) before the block stages its rows.  E0,
-// E1 and k0 are wave-uniform.  A word past E1 may belong to another tile,
-// whose slots this tile's staging does not hold, so the last round zeroes the
-// words of its lanes past E1 (a zero word reads slots 0-7, which the staging
-// always holds) as well as their start flags.
+// E1 and k0 are wave-uniform.  Lanes past E1 in the last round hold a copy of
+// the last valid lane's word, one of this tile's, so their slots are inside
+// the tile's staging; their start flags are masked.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* __restrict__ val,
                                               const unsigned* __restrict__ words, const float* ys, int R, int k0,
@@ -1065,9 +1075,8 @@ __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* _
     float carry = 0.f;
     auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
         const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
-        ww = lane < nvalid ? ww : 0u;
         const float s = bstep_products(vv, ww, ys, R);
-        lanes_round(s, (ww >> 31) != 0u, last_round, nvalid, lane, knext, open, carry, close);
+        lanes_round(s, (ww >> 31) != 0u && lane < nvalid, last_round, nvalid, lane, knext, open, carry, close);
     };
     if (rb >= E1) return;
     if constexpr (PF != 0) {
@@ -1075,12 +1084,12 @@ __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* _
         unsigned wb;
         for (;;) {
             bool last_round = rb + 512 >= E1;
-            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, vb, wb);
+            steps_load_at<V>(val, words, last_round ? rb : rb + 512, steps_lane_offset(le, rb + 512, E1), vb, wb);
             run_round(v, w, last_round);
             if (last_round) break;
             rb += 512;
             last_round = rb + 512 >= E1;
-            steps_load_at<V>(val, words, rb + 512, last_round ? 0 : le, v, w);
+            steps_load_at<V>(val, words, last_round ? rb : rb + 512, steps_lane_offset(le, rb + 512, E1), v, w);
             run_round(vb, wb, last_round);
             if (last_round) break;
             rb += 512;
@@ -1091,7 +1100,7 @@ __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* _
             run_round(v, w, last_round);
             if (last_round) break;
             rb += 512;
-            steps_load_at<V>(val, words, rb, le, v, w);
+            steps_load_at<V>(val, words, rb, steps_lane_offset(le, rb, E1), v, w);
         }
     }
 }
@@ -1173,7 +1182,7 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
     const int ka = __builtin_amdgcn_readfirstlane(a.wave_k[t * NW + wv]);
     float v[8];
     unsigned wd;
-    steps_load_at<V>(bval, a.words, ea, ea < eb ? 8 * (int)(threadIdx.x & 63) : 0, v, wd);
+    steps_load_at<V>(bval, a.words, ea, steps_lane_offset(8 * (int)(threadIdx.x & 63), ea, eb), v, wd);
     const int64_t Et0 = a.col_ptr[c0];
     const int R = a.R[t];
     const int c = (int)threadIdx.x;
