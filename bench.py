@@ -14,6 +14,9 @@
         direct launches against graph mode at N = 64 .. 512
     python bench.py --study art
         ART/CART sweeps on the GPU against the CPU oracle at N = 64 and 256
+    python bench.py --study c3 [--c3-K K]
+        SART at fanlineartomo N=1024, 720 angles, p=1448: the matrix stored in
+        FP32, BF16 and FP16, and matrix-free
     python bench.py --study learned [--learn-train T --learn-eval E --learn-K K]
         learned relaxation at paralleltomo N=256, 180 angles, p=362: greedy
         trajectories of T seeded problems, network trained on the GPU, then
@@ -372,6 +375,41 @@ def graph_study(args):
     print(json.dumps(out), flush=True)
 
 
+def c3_study(args):
+    """SART at the fan-beam scale: fanlineartomo N=1024, 720 angles over 360
+    degrees, p=1448 (R=2N, dw=2.5N, sd=3N), with the matrix stored in FP32, BF16
+    and FP16 and without a stored matrix (isMatrix=0)."""
+    import airtoolsii_amd as air
+    N, theta, p, K = 1024, np.arange(0.0, 360.0, 0.5), 1448, args.c3_K
+    x = air.phantomgallery("threephases", N).ravel(order="F")
+    rows = []
+    b = None
+    for dt, mf in (("f32", False), ("bf16", False), ("f16", False), ("f32", True)):
+        t0 = time.perf_counter()
+        A = air.fanlineartomo(N, theta, p, isMatrix=0 if mf else 1, dtype=dt)
+        t_build = time.perf_counter() - t0
+        if b is None:
+            b, _ = air.add_noise(A.matvec(x), 0.01, seed=1)
+        opts = {"lbound": 0.0, "ubound": 1.0}
+        t0 = time.perf_counter()
+        run = air.SirtRun("sart", A, b, K + 2, None, opts)
+        run.iterate(2)                                   # weights, rho, caches
+        t_setup = time.perf_counter() - t0
+        gpu_ms = run.iterate(K)
+        xk = run.fetch()[0]
+        rows.append({"storage": "none (matrix-free)" if mf else dt, "build_s": round(t_build, 3),
+                     "setup_s": round(t_setup, 3), "ms_per_iteration": round(gpu_ms / K, 4),
+                     "iter_per_s": round(K / (gpu_ms * 1e-3), 1),
+                     "rel_err_to_phantom": round(float(np.linalg.norm(xk - x) / np.linalg.norm(x)), 5)})
+        del run, A
+    out = {"metric": "SART iter/s at the fan-beam scale, stored matrix by storage type and matrix-free",
+           "unit": "iter/s", "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": f"fanlineartomo N={N}, theta=0:0.5:359.5 ({len(theta)} angles), p={p}, "
+                                  f"R=2N, dw=2.5N, sd=3N, SART, box [0,1], threephases, 1% noise, K={K}"},
+           "cases": rows}
+    print(json.dumps(out), flush=True)
+
+
 def art_study(args):
     """ART (kaczmarz) and CART (columnaction) sweeps on the GPU against the CPU
     oracle's literal sweeps (one core), on the same matrices and inputs."""
@@ -419,7 +457,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--study", choices=["learned", "batch", "spmm", "graph", "art"], default=None)
+    ap.add_argument("--study", choices=["learned", "batch", "spmm", "graph", "art", "c3"], default=None)
+    ap.add_argument("--c3-K", type=int, default=20)
     ap.add_argument("--graph-iters", type=int, default=32)
     ap.add_argument("--spmm-K", type=int, default=50)
     ap.add_argument("--batch-problems", type=int, default=256)
@@ -457,6 +496,9 @@ def main():
         return
     if args.study == "art":
         art_study(args)
+        return
+    if args.study == "c3":
+        c3_study(args)
         return
 
     import airtoolsii_amd as air
