@@ -3,17 +3,19 @@
 // staged in LDS.
 //
 // Vectors are RHS-minor: row i of X, B or M R is 64 FP32 values (256 bytes),
-// one per right-hand side.  Both products use one kernel shape, a 1024-thread
-// work-group that owns 1024 accumulation targets and walks a list of operand
-// chunks of up to 256 rows (64 KB of LDS):
-//   - a wave holds 64 targets in 16 rounds of 4 (target i = 64 t + 4 w + g:
+// one per right-hand side.  Both products use one kernel shape, a 512-thread
+// work-group (8 waves) that owns 512 accumulation targets and walks a list of
+// operand chunks of up to 256 rows (64 KB of LDS); with the 16 KB of record
+// rings that is 80 KB, so two work-groups share a CU:
+//   - a wave holds 64 targets in 16 rounds of 4 (target i = 32 t + 4 w + g:
 //     round t, wave w, group g, so that every wave has targets spread over the
 //     whole work-group); its lanes form four groups of 16, group g works on
 //     target g of a round and lane q of a group holds right-hand sides
 //     4q..4q+3, so every entry is one 16-byte LDS read per lane, the group
 //     reading one whole 256-byte operand row (no bank conflicts), and four FMAs;
 //   - the 16 rounds of accumulators (64 VGPRs) stay in registers over all
-//     chunks of the work-group;
+//     chunks of the work-group (about 125 VGPRs in all, the budget of 4 waves
+//     per SIMD);
 //   - a round's entries in one chunk are stored in records of 4 steps x 4
 //     groups (80 bytes: 16 FP32 values, then 16 one-byte operand slots); the
 //     round's step count is the largest of its four targets' entry counts and
@@ -21,16 +23,18 @@
 //   - every wave reads its own records, in chunk and round order, from one
 //     contiguous stream of 1 KB blocks (12 records each) through a two-block
 //     ring in LDS, the next block waiting in four VGPRs; waves meet only at
-//     chunk changes.  The operand chunks are double-buffered and loaded by
-//     DMA (global_load_lds) one chunk ahead.
+//     chunk changes.  The operand chunk has one buffer: at a chunk change
+//     the waves meet at a barrier, the next chunk is loaded by DMA
+//     (global_load_lds) and waited for, and the CU's other work-group computes
+//     meanwhile.
 //
 //   A X    targets are pieces (one row's entries inside one 64 x 64 column tile
 //          of the tiled layout, in the tiled layout's piece order); a
-//          work-group takes 1024 consecutive pieces of one tile and its chunks
+//          work-group takes 512 consecutive pieces of one tile and its chunks
 //          are the tile's sixteen 16 x 16 sub-tiles of X.  It writes one
 //          256-byte partial per piece; the row pass sums a row's partials in
 //          tile order and forms r = b - A x, M r and the FP64 norms per RHS.
-//   A'(M R) targets are the 1024 pixels of a 32 x 32 block; its chunks are
+//   A'(M R) targets are a block of 512 consecutive internal columns; its chunks are
 //          consecutive runs of 256 of the block's distinct rows of M R; the
 //          epilogue is the update x <- P(x + lam D g) with scalar bounds.
 //
@@ -232,8 +236,8 @@ __device__ __forceinline__ int4 sp_chunk_desc(const int4* __restrict__ chd, int 
 }
 
 // Launch entry b runs work-group lperm[b] (-1: idle).  Work-group table
-// {first chunk, chunks, A, B}: forward A = first piece, B = pieces (<= 1024);
-// backward A = 64 x 64 tile, B = its four 16 x 16 sub-tiles (4 bits each).
+// {first chunk, chunks, A, B}: forward A = first piece, B = pieces (<= 512);
+// backward A = the block of 512 internal columns, B unused.
 // Per work-group and wave: first stream block and block count.  Per chunk:
 // {first operand row (forward: internal column; backward: offset into the
 // row list), rows}; per (chunk, wave): 16 round step counts (uint16).
@@ -501,7 +505,7 @@ __global__ void k_sp_buniq(const uint64_t* __restrict__ ks, const uint32_t* __re
         }
 }
 
-// chunk of 256 distinct rows and slot inside it; key2 = chunk * 1024 + pixel slot
+// chunk of 256 distinct rows and slot inside it; key2 = chunk * 512 + column slot
 __global__ void k_sp_bchunk(const uint64_t* __restrict__ ks, const uint32_t* __restrict__ uflag,
                             const uint32_t* __restrict__ uex, const uint32_t* __restrict__ tfu,
                             const uint32_t* __restrict__ cb, int64_t n, uint32_t* __restrict__ key2,
@@ -702,7 +706,7 @@ bool build_spmm(const DevSparse& A, const TiledOp& T, SpmmOp& S, hipStream_t s) 
     ecell.alloc(4 * (size_t)nnz);
     ek.alloc(4 * (size_t)nnz);
 
-    // ================= forward: pieces of each 64 x 64 tile in batches of 1024
+    // ================= forward: pieces of each 64 x 64 tile in batches of 512
     {
         k_sp_fkeys<<<sp_grid(nnz), 256, 0, s>>>(A.idx.as<int>(), rowof.as<int>(), T.perm.as<int>(), nnz,
                                                 key.as<uint64_t>(), iota.as<uint32_t>());
