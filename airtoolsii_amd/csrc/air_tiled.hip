@@ -614,11 +614,17 @@ __device__ __forceinline__ void stream_steps(int64_t E0, int64_t E1, const V* __
     int knext = k0;
     bool open = false;
     float carry = 0.f;
+    // a full round (all 64 lanes valid) takes the branch without the
+    // last-round tests; the branch is uniform
     auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
         const float s = step_products(vv, ww, lds);
-        const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
-        const bool start = ((ww >> 12) & 1u) != 0u && lane < nvalid;
-        lanes_round(s, start, last_round, nvalid, lane, knext, open, carry, close);
+        const bool st = ((ww >> 12) & 1u) != 0u;
+        if (last_round) {
+            const int nvalid = (int)((E1 - rb) >> 3);
+            lanes_round(s, st && lane < nvalid, true, nvalid, lane, knext, open, carry, close);
+        } else {
+            lanes_round(s, st, false, 64, lane, knext, open, carry, close);
+        }
     };
     if (rb >= E1) return;
     if constexpr (PF != 0) {
@@ -729,7 +735,7 @@ __global__ __launch_bounds__(WPB * 64) void k_tfwd_step(const int* __restrict__ 
         ((f4*)xs)[nested_to_plain(4u * (threadIdx.x + NT * i)) >> 2] = tmp[i];
     __syncthreads();
     if (E0 >= E1) return;
-    auto close = [&](int k, float sum) { partial[k] = sum; };
+    auto close = [&](int k, float sum) { partial[(uint32_t)k] = sum; };   // piece ordinals are < 2^31
     stream_steps<V, PF>(E0, E1, fval, fstep, xs, pw0, v, wd, close);
 }
 
@@ -1074,9 +1080,14 @@ __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* _
     bool open = false;
     float carry = 0.f;
     auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
-        const int nvalid = last_round ? (int)((E1 - rb) >> 3) : 64;
         const float s = bstep_products(vv, ww, ys, R);
-        lanes_round(s, (ww >> 31) != 0u && lane < nvalid, last_round, nvalid, lane, knext, open, carry, close);
+        const bool st = (ww >> 31) != 0u;
+        if (last_round) {
+            const int nvalid = (int)((E1 - rb) >> 3);
+            lanes_round(s, st && lane < nvalid, true, nvalid, lane, knext, open, carry, close);
+        } else {
+            lanes_round(s, st, false, 64, lane, knext, open, carry, close);
+        }
     };
     if (rb >= E1) return;
     if constexpr (PF != 0) {
