@@ -394,6 +394,45 @@ def test_deferred_norms_equal_direct_finalize(c1, oracle):
     np.testing.assert_allclose(info["resnorm"], info1["resnorm"], rtol=1e-12)
 
 
+@pytest.mark.parametrize("geom", ["parallel", "fanlinear", "fancurved"])
+def test_step_coded_forward_bit_identical_to_lane_form(oracle, monkeypatch, geom):
+    """The step-coded forward decodes each lane's 8 tile positions from one
+    word; the lane form reads them from 16-bit codes.  With the same wave
+    partition (entries per wave) and the same backward shape, both sum the
+    same products in the same order, so the iterates must agree bit for bit
+    (a single mis-decoded position would change them), and so must the step
+    form with and without the next round prefetched."""
+    N = 80
+    if geom == "fanlinear":
+        theta = np.arange(0.0, 360.0, 4.0)
+        make = lambda: air.fanlineartomo(N, theta, 113)
+        Ao = oracle.fanlineartomo(N, theta, 113)
+    elif geom == "fancurved":
+        theta = np.arange(0.0, 360.0, 4.0)
+        make = lambda: air.fancurvedtomo(N, theta, 113)
+        Ao = oracle.fancurvedtomo(N, theta, 113)
+    else:
+        theta = np.arange(0.0, 180.0, 2.0)
+        make = lambda: air.paralleltomo(N, theta, 113)
+        Ao = oracle.paralleltomo(N, theta, 113)
+    x = oracle.phantomgallery("shepplogan", N).ravel(order="F")
+    b = Ao @ x
+    opts = {"rho": oracle.spectral_radius(Ao, *oracle.weights("cimmino", Ao)), "lbound": 0}
+    out = {}
+    for cfg in ("9,7,2048", "11,7,2048", "10,7,4096", "12,7,4096", "11,7,4096"):
+        monkeypatch.setenv("AIR_TILED_CFG", cfg)
+        A = make()
+        L = A.layout()
+        assert L["fwd_step_ok"] == 1 and L["fwd_step"] == (1 if int(cfg.split(",")[0]) >= 11 else 0)
+        X, _, _ = air.cimmino(A, b, [1, 6], None, opts)
+        out[cfg] = X
+    assert np.array_equal(out["9,7,2048"], out["11,7,2048"]), geom          # lane form vs step form
+    assert np.array_equal(out["10,7,4096"], out["12,7,4096"]), geom         # the same, next round prefetched
+    assert np.array_equal(out["11,7,4096"], out["12,7,4096"]), geom         # step form without / with prefetch
+    Xo, _, _ = oracle.sirt("cimmino", Ao, b, [1, 6], None, opts)
+    assert rel(out["12,7,4096"][:, 1], Xo[:, 1]) <= TOL
+
+
 @pytest.mark.parametrize("geom", ["fanlinear", "fancurved", "seismic"])
 def test_step_coded_forward_other_geometries(oracle, monkeypatch, geom):
     """The step-coded forward and backward on other traced geometries: straight
