@@ -401,7 +401,7 @@ def test_step_coded_forward_bit_identical_to_lane_form(oracle, monkeypatch, geom
     partition (entries per wave) and the same backward shape, both sum the
     same products in the same order, so the iterates must agree bit for bit
     (a single mis-decoded position would change them), and so must the step
-    form with and without the next round prefetched."""
+    form with and without the next round prefetched, in either direction."""
     N = 80
     if geom == "fanlinear":
         theta = np.arange(0.0, 360.0, 4.0)
@@ -419,7 +419,7 @@ def test_step_coded_forward_bit_identical_to_lane_form(oracle, monkeypatch, geom
     b = Ao @ x
     opts = {"rho": oracle.spectral_radius(Ao, *oracle.weights("cimmino", Ao)), "lbound": 0}
     out = {}
-    for cfg in ("9,7,2048", "11,7,2048", "10,7,4096", "12,7,4096", "11,7,4096"):
+    for cfg in ("9,7,2048", "11,7,2048", "10,7,4096", "12,7,4096", "11,7,4096", "11,9,4096", "11,10,4096"):
         monkeypatch.setenv("AIR_TILED_CFG", cfg)
         A = make()
         L = A.layout()
@@ -429,6 +429,7 @@ def test_step_coded_forward_bit_identical_to_lane_form(oracle, monkeypatch, geom
     assert np.array_equal(out["9,7,2048"], out["11,7,2048"]), geom          # lane form vs step form
     assert np.array_equal(out["10,7,4096"], out["12,7,4096"]), geom         # the same, next round prefetched
     assert np.array_equal(out["11,7,4096"], out["12,7,4096"]), geom         # step form without / with prefetch
+    assert np.array_equal(out["11,9,4096"], out["11,10,4096"]), geom        # step backward without / with prefetch
     Xo, _, _ = oracle.sirt("cimmino", Ao, b, [1, 6], None, opts)
     assert rel(out["12,7,4096"][:, 1], Xo[:, 1]) <= TOL
 
