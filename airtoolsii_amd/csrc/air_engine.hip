@@ -90,8 +90,11 @@ struct air_engine {
     int graph_iters = 0;
     hipGraphExec_t gexec = nullptr;
     int64_t graph_launches = 0;   // graphs launched since the engine was created
-    // per-kernel timing
-    bool timing = false;
+    // per-kernel timing: events bracket the kernels of every timing-th
+    // iteration (0 = off); ev_now says whether the current one is bracketed
+    int timing = 0;
+    int64_t ev_tick = 0;
+    bool ev_now = false;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
     double t_rows_ms = 0, t_cols_ms = 0, t_red_ms = 0;
@@ -626,12 +629,12 @@ void timing_begin(air_engine* e, int niter) {
 }
 
 void record(air_engine* e, int slot) {
-    if (!e->timing) return;
+    if (!e->ev_now) return;
     AIR_HIP(hipEventRecord(e->ev[(size_t)e->ev_used * EV_PER_ITER + slot], e->stream));
 }
 
 hipEvent_t mid_event(air_engine* e) {
-    return e->timing ? e->ev[(size_t)e->ev_used * EV_PER_ITER + 2] : nullptr;
+    return e->ev_now ? e->ev[(size_t)e->ev_used * EV_PER_ITER + 2] : nullptr;
 }
 
 // one SIRT iteration k (sirt.m:213-243): the A'(M r) + update pass, then the
@@ -671,6 +674,7 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
     }
     const bool line = device_lambda(st.relax_kind);
     double lam = line ? 0.0 : st.lam[k - 1];
+    e->ev_now = e->timing > 0 && e->ev_tick++ % e->timing == 0;
     record(e, 0);
     if (st.relax_kind == AIR_RELAX_LEARNED) {
         if (!fused_select(st.bufs)) launch_learned_select(st.bufs, k, s);
@@ -681,7 +685,7 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
         record(e, 1);
         launch_tiled_resid(e->T, st.bufs, st.sp, k, s, mid_event(e));
         record(e, 3);
-        if (e->timing) e->ev_used++;
+        if (e->ev_now) e->ev_used++;
         if (st.bufs.ring && k - st.defer_first + 1 >= st.bufs.ring_slots) flush_deferred(e);
         return;
     } else if (e->kind == 1) {
@@ -704,7 +708,7 @@ void enqueue_iteration(air_engine* e, int k, bool devk = false) {
         record(e, 2);
     }
     record(e, 3);
-    if (e->timing) e->ev_used++;
+    if (e->ev_now) e->ev_used++;
 }
 
 // Enqueue iterations st.k+1 .. st.k+count.  In graph mode whole runs of
@@ -1615,7 +1619,10 @@ int air_cart(air_engine* e, const int32_t* order, int64_t norder, const air_row_
 int air_set_kernel_timing(air_engine* e, int on) {
     return guard([&] {
         AIR_CHECK(e, AIR_E_ARG, "null engine");
-        e->timing = on != 0;
+        AIR_CHECK(on >= 0, AIR_E_ARG, "kernel timing takes 0 (off) or a sampling stride >= 1");
+        e->timing = on;
+        e->ev_tick = 0;
+        e->ev_now = false;
         e->t_rows_ms = e->t_cols_ms = e->t_red_ms = 0;
         e->launches = 0;
     });

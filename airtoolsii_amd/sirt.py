@@ -344,7 +344,10 @@ class SirtRun:
         return {"features": feat, "target": target, "out": out, "err": err}
 
     def kernel_timing(self, on=True):
-        check(load().air_set_kernel_timing(self.A.engine.h, 1 if on else 0))
+        """Per-kernel event timing: False/0 off, True/1 every iteration, an
+        integer s > 1 every s-th iteration."""
+        stride = (1 if on else 0) if isinstance(on, bool) else int(on)
+        check(load().air_set_kernel_timing(self.A.engine.h, stride))
 
     def kernel_times(self):
         c, r, d, n = _lib.C.c_double(), _lib.C.c_double(), _lib.C.c_double(), _lib.C.c_int64()

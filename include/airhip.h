@@ -235,10 +235,14 @@ int air_sirt_fetch(air_engine* e, double* x_out, double* resnorm, air_sirt_info*
  * x_true was given).  Any pointer may be NULL; rows beyond finaliter are not written. */
 int air_sirt_records(air_engine* e, float* feat, float* target, float* out, double* err);
 
-/* per-kernel HIP-event timing of the SIRT kernels (0 = off).  Times are
- * accumulated over iterate() calls while on; reset by enabling again.
- * cols: the A'(M r) + update pass; rows: the A x pass; reduce: the residual
- * epilogue kernel of the tiled layout (0 for the compressed-row layout). */
+/* per-kernel HIP-event timing of the SIRT kernels: on = 0 turns it off,
+ * on = s >= 1 brackets the kernels of every s-th iteration with events (each
+ * event record costs the stream a few microseconds, so s > 1 keeps that cost
+ * out of most iterations of a timed run).  Times are accumulated over the
+ * bracketed iterations of iterate() calls while on (launches counts them);
+ * reset by enabling again.  cols: the A'(M r) + update pass; rows: the A x
+ * pass; reduce: the residual epilogue kernel of the tiled layout (0 for the
+ * compressed-row layout). */
 int air_set_kernel_timing(air_engine* e, int on);
 int air_get_kernel_timing(air_engine* e, double* cols_ms, double* rows_ms, double* reduce_ms, int64_t* launches);
 

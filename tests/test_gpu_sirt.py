@@ -465,3 +465,30 @@ def test_step_coded_forward_other_geometries(oracle, monkeypatch, geom):
     Xo, _, _ = oracle.sirt("sart", Ao, b, [4, 20], None, opts)
     for j in range(2):
         assert rel(X[:, j], Xo[:, j]) <= TOL, (geom, j, rel(X[:, j], Xo[:, j]))
+
+
+@pytest.mark.parametrize("stride", [1, 4])
+def test_sampled_kernel_timing(c1, stride):
+    """Event timing with stride s brackets the kernels of iterations 0, s, 2s, ...
+    counted over the calls since it was enabled; the events change no iterate."""
+    from airtoolsii_amd.sirt import SirtRun
+    opts = {"rho": c1["rho"]["cimmino"], "lbound": 0}
+    run = SirtRun("cimmino", c1["A"], c1["b"], 40, None, opts)
+    run.iterate(3)
+    run.kernel_timing(stride)
+    for n in (10, 7, 13):
+        run.iterate(n)
+    kt = run.kernel_times()
+    assert kt["launches"] == -(-30 // stride)
+    assert kt["cols_ms"] > 0 and kt["rows_ms"] > 0 and kt["reduce_ms"] > 0
+    run.kernel_timing(False)
+    run.iterate(10)
+    assert run.kernel_times()["launches"] == 0
+    x1, info1 = run.fetch()
+    ref = SirtRun("cimmino", c1["A"], c1["b"], 40, None, opts)
+    ref.iterate(40)
+    x2, _ = ref.fetch()
+    assert info1["finaliter"] == 40 and np.array_equal(x1, x2)
+    from airtoolsii_amd._lib import AirError
+    with pytest.raises(AirError):
+        run.kernel_timing(-1)
