@@ -67,7 +67,8 @@ class Layout(C.Structure):
                 ("fwd_entries", C.c_int64), ("bwd_entries", C.c_int64),
                 ("tune_fwd_us", C.c_float * 64), ("tune_bwd_us", C.c_float * 16),
                 ("fwd_step", C.c_int), ("fwd_step_ok", C.c_int), ("bwd_step", C.c_int), ("bwd_step_ok", C.c_int),
-                ("bwd_step_entries", C.c_int64), ("fwd_waves_per_block", C.c_int)]
+                ("bwd_step_entries", C.c_int64), ("fwd_waves_per_block", C.c_int),
+                ("bwd_step_base_bits", C.c_int), ("bwd_step_lds_bytes", C.c_int64)]
 
 
 class RowOptions(C.Structure):

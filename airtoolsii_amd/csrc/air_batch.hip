@@ -74,7 +74,8 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
         maxrows = std::max(maxrows, T.maxrows);
         flags = flags && T.bwd_flags;
         step = step && T.fstep_ok;
-        bstep = bstep && T.bstep_ok;
+        bstep = bstep && T.bstep_ok && T.bstep_bb == parts[0].bstep_bb &&   // one word format
+                T.bstep_nw == parts[0].bstep_nw;                            // and block shape
         nbse += T.nbse;
         if (T.bstep_ok) lds_bs = std::max(lds_bs, T.lds_bs - 8 * (size_t)T.maxruns);   // the slots
         rpi += T.npieces;
@@ -118,11 +119,13 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     // step-coded backward: words and strides are tile-local; column pointers are
     // shifted; the staging's run tables take the batch's longest run list
     lds_bs += 8 * (size_t)maxruns;
-    bstep = bstep && lds_bs <= AIR_TILE_LDS_MAX;
+    bstep = bstep && lds_bs <= (parts[0].bstep_nw == 8 ? AIR_TILE_LDS_STEP_MAX : AIR_TILE_LDS_STEP16_MAX);
     B.bstep_ok = bstep;
     if (bstep) {
         B.nbse = nbse;
         B.lds_bs = lds_bs;
+        B.bstep_bb = parts[0].bstep_bb;
+        B.bstep_nw = parts[0].bstep_nw;
         B.bsval.alloc(vb * (size_t)(nbse + AIR_SPARSE_PAD));
         AIR_HIP(hipMemsetAsync((char*)B.bsval.p + vb * nbse, 0, vb * AIR_SPARSE_PAD, s));
         B.bsword.alloc(4 * (size_t)(nbse / 8 + AIR_SPARSE_PAD));
@@ -193,6 +196,7 @@ bool concat_tiled(std::vector<TiledOp>& parts, TiledOp& B, hipStream_t s) {
     if (bstep) {
         AIR_HIP(hipMemcpyAsync(B.bstep_ptr.as<int64_t>() + nint, &nbse, 8, hipMemcpyHostToDevice, s));
         set_bstep_waves(B, s);
+        tiled_bstep_lds_attr(B);
     }
     const int Ri = (int)R;
     AIR_HIP(hipMemcpyAsync(B.run_ptr.as<int>() + tb, &Ri, 4, hipMemcpyHostToDevice, s));

@@ -1688,6 +1688,8 @@ int air_layout_info(air_engine* e, air_layout* out) {
             L.bwd_step_ok = T.bstep_ok ? 1 : 0;
             L.bwd_step_entries = T.nbse;
             L.fwd_waves_per_block = T.fwd_wpb;
+            L.bwd_step_base_bits = T.bstep_ok ? T.bstep_bb : 0;
+            L.bwd_step_lds_bytes = T.bstep_ok ? (int64_t)T.lds_bs : 0;
         }
         *out = L;
     });

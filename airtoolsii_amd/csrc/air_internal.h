@@ -239,6 +239,11 @@ void csr_to_csc(const DevSparse& A, DevSparse& AT, hipStream_t s);
 //    consecutive rows) and each column is summed with LDS gathers.
 // dynamic LDS a backward tile may stage (two 512-thread blocks per CU)
 constexpr size_t AIR_TILE_LDS_MAX = 60 * 1024;
+// dynamic LDS of the step-coded backward's strided staging: two 512-thread
+// blocks per CU with the kernel's 2.6 KB of static LDS, or one 1024-thread
+// block per CU for larger stagings
+constexpr size_t AIR_TILE_LDS_STEP_MAX = 76 * 1024;
+constexpr size_t AIR_TILE_LDS_STEP16_MAX = 152 * 1024;
 
 struct TiledOp {
     bool ready = false;
@@ -278,6 +283,13 @@ struct TiledOp {
     int64_t nbse = 0;
     size_t lds_bs = 0;
     bool bstep_ok = false;
+    // bits of a lane word's base slot: 14 (offset of the first entry in its
+    // group up to 7) while every tile stages fewer than 16384 slots, else 15
+    // (offset up to 3, slots up to 32767)
+    int bstep_bb = 14;
+    // waves per tile of the step-coded backward: 8 (two blocks per CU) while
+    // the staging fits AIR_TILE_LDS_STEP_MAX, else 16 (one block per CU)
+    int bstep_nw = 8;
     float tune_fwd_ms[64] = {}, tune_bwd_ms[16] = {};
 };
 // Many right-hand sides (air_spmm.hip): per product, the launch order of the
@@ -467,6 +479,8 @@ void launch_learned_select(const SirtBufs& B, int k, hipStream_t s);
 bool fused_select(const SirtBufs& B);
 // blocks of the tiled row pass
 unsigned tiled_reduce_grid(int64_t m);
+// allow the step-coded backward kernels the tile's staging when it exceeds 64 KB
+void tiled_bstep_lds_attr(const TiledOp& T);
 // hist[k], k = k0..k1, from the deferred block partials; the stop check of k1
 void launch_finalize_deferred(const SirtBufs& B, int nblocks, StopParams sp, int k0, int k1, hipStream_t s);
 // matrix-free iteration pieces

@@ -1039,17 +1039,18 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_flat(BwdArgs a, const V* __res
 // run at base + R + d, base the slot of the previous group's first entry and
 // d in {-1, 0, 1}, which covers a ray sweep in fine angular steps; anything
 // else is bridged at build time by entries with value 0 (k_bstep_walk).  A
-// lane's 8 entries are one 32-bit word:
-//   bits  0-13  slot of the first entry's group start (base)
-//   bits 14-16  first entry's slot minus base
+// lane's 8 entries are one 32-bit word; with BB = 14 (tiles staging fewer than
+// 16384 slots) or 15 (fewer than 32768):
+//   bits  0..BB-1   slot of the first entry's group start (base)
+//   bits  BB..16    first entry's slot minus base (up to 7, or 3 with BB = 15)
 //   bits 15+2i, 16+2i (i = 1..7)  entry i: 0 continue the group (slot + 1),
 //                s > 0 open a group at base + R + s - 2
 //   bit   31    the lane's first entry starts a column
 // Padding entries continue the group with value 0; the staging leaves 8 zero
 // slots after the last run so they stay inside it.
-__device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w, const float* ys, int R) {
-    unsigned base = w & 0x3fffu;
-    unsigned addr = base + ((w >> 14) & 7u);
+__device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w, const float* ys, int R, int bb) {
+    unsigned base = __builtin_amdgcn_ubfe(w, 0u, (unsigned)bb);
+    unsigned addr = base + __builtin_amdgcn_ubfe(w, (unsigned)bb, 17u - (unsigned)bb);
     const int adv = R - 2;
     float s = v[0] * ys[addr];
 #pragma unroll
@@ -1070,8 +1071,8 @@ __device__ __forceinline__ float bstep_products(const float (&v)[8], unsigned w,
 // the tile's staging; their start flags are masked.
 template <typename V, int PF, typename Close>
 __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* __restrict__ val,
-                                              const unsigned* __restrict__ words, const float* ys, int R, int k0,
-                                              float (&v)[8], unsigned w, Close&& close) {
+                                              const unsigned* __restrict__ words, const float* ys, int R, int bb,
+                                              int k0, float (&v)[8], unsigned w, Close&& close) {
     static_assert(!std::is_same<V, double>::value, "the lane form accumulates in FP32");
     const int lane = (int)(threadIdx.x & 63);
     const int le = 8 * lane;
@@ -1080,7 +1081,7 @@ __device__ __forceinline__ void stream_bsteps(int64_t E0, int64_t E1, const V* _
     bool open = false;
     float carry = 0.f;
     auto run_round = [&](const float (&vv)[8], unsigned ww, bool last_round) {
-        const float s = bstep_products(vv, ww, ys, R);
+        const float s = bstep_products(vv, ww, ys, R, bb);
         const bool st = (ww >> 31) != 0u;
         if (last_round) {
             const int nvalid = (int)((E1 - rb) >> 3);
@@ -1163,6 +1164,7 @@ struct BStepArgs {
     const int* run_row;
     const int* run_len;
     int maxruns;
+    int bb;                   // bits of a word's base slot (14 or 15)
 };
 
 // Backward product in the step-coded lane form: as k_tbwd_flat with the
@@ -1225,7 +1227,7 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
     __syncthreads();
     {
         auto close = [&](int k, float sum) { gt[necol[k]] = sum; };
-        if (ea < eb) stream_bsteps<V, PF>(ea, eb, bval, a.words, ys, R, ka, v, wd, close);
+        if (ea < eb) stream_bsteps<V, PF>(ea, eb, bval, a.words, ys, R, a.bb, ka, v, wd, close);
     }
     __syncthreads();
     double dsum[1] = {0};
@@ -1248,6 +1250,27 @@ __global__ __launch_bounds__(NW * 64) void k_tbwd_step(BStepArgs a, const V* __r
         block_reduce<1, NT>(dsum, sh);
         store_block_partials<1>(dsum, red);
     }
+}
+
+// Staging beyond 64 KB of dynamic LDS has to be allowed per kernel, once,
+// outside any graph capture: every instance the launches below may pick.
+void tiled_bstep_lds_attr(const TiledOp& T) {
+    if (T.lds_bs <= 64 * 1024) return;
+    dispatch_dtype(T.dtype, [&](auto v) {
+        using V = decltype(v);
+        if constexpr (!std::is_same<V, double>::value) {
+            const void* k8[] = {(const void*)k_tbwd_step<V, 0, 8, 0>, (const void*)k_tbwd_step<V, 0, 8, 1>,
+                                (const void*)k_tbwd_step<V, 1, 8, 0>, (const void*)k_tbwd_step<V, 1, 8, 1>,
+                                (const void*)k_tbwd_step<V, 2, 8, 0>, (const void*)k_tbwd_step<V, 2, 8, 1>};
+            const void* k16[] = {(const void*)k_tbwd_step<V, 0, 16, 0>, (const void*)k_tbwd_step<V, 0, 16, 1>,
+                                 (const void*)k_tbwd_step<V, 1, 16, 0>, (const void*)k_tbwd_step<V, 1, 16, 1>,
+                                 (const void*)k_tbwd_step<V, 2, 16, 0>, (const void*)k_tbwd_step<V, 2, 16, 1>};
+            const void* const* ks = T.bstep_nw == 16 ? k16 : k8;
+            for (int i = 0; i < 6; ++i)
+                AIR_HIP(hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)T.lds_bs));
+        }
+        return 0;
+    });
 }
 
 __global__ void k_flag_col_starts(const int64_t* __restrict__ cell_ptr, int64_t ncols, unsigned short* __restrict__ blr) {
@@ -1514,13 +1537,15 @@ __global__ void k_step_words(const unsigned short* __restrict__ code, int64_t nl
 // group's first entry and the padded count per column; pass 2 writes values
 // and lane words at optr.  A column whose stream does not fit the word (a
 // slot outside the staging, a lane starting more than 7 entries into its
-// group) sets *bad and the matrix keeps the column codes.
+// group) sets *bad and the matrix keeps the column codes.  bb: bits of the
+// word's base slot (14: group offsets up to 7; 15: up to 3).
 template <typename V, bool FILL>
 __global__ void k_bstep_walk(const int64_t* __restrict__ cell_ptr, int64_t ncols, const unsigned short* __restrict__ blr,
                              const V* __restrict__ bval, const int* __restrict__ run_ptr,
-                             const int* __restrict__ run_off, const int* __restrict__ Rt, int* __restrict__ ubuf,
+                             const int* __restrict__ run_off, const int* __restrict__ Rt, int bb, int* __restrict__ ubuf,
                              int64_t* __restrict__ count, const int64_t* __restrict__ optr, V* __restrict__ oval,
                              unsigned* __restrict__ oword, unsigned* __restrict__ bad) {
+    const unsigned bmask = (1u << bb) - 1u, omax = (1u << (17 - bb)) - 1u;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncols; c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = cell_ptr[c], e = cell_ptr[c + 1];
         if (s == e) {
@@ -1566,8 +1591,8 @@ __global__ void k_bstep_walk(const int64_t* __restrict__ cell_ptr, int64_t ncols
             if (addr >= lim || base >= lim) ok = false;
             const int i = (int)(n & 7);
             if (i == 0) {
-                if (addr - base > 7u) ok = false;
-                word = (base & 0x3fffu) | (((addr - base) & 7u) << 14) | (n == 0 ? (1u << 31) : 0u);
+                if (addr - base > omax) ok = false;
+                word = (base & bmask) | (((addr - base) & omax) << bb) | (n == 0 ? (1u << 31) : 0u);
             } else {
                 word |= sym << (15 + 2 * i);
             }
@@ -1767,7 +1792,7 @@ static int64_t pad_segments(DevBuf& ptr, int64_t nseg, DevBuf& val, DevBuf& code
 // of its first non-empty column, so that the kernel can load its first round
 // before the tile's rows are staged.
 void set_bstep_waves(TiledOp& T, hipStream_t s) {
-    constexpr int NW = 8;
+    const int NW = T.bstep_nw;
     std::vector<int64_t> ptr(T.n_int + 1);
     AIR_HIP(hipMemcpyAsync(ptr.data(), T.bstep_ptr.p, 8 * ptr.size(), hipMemcpyDeviceToHost, s));
     AIR_HIP(hipStreamSynchronize(s));
@@ -2020,18 +2045,22 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         T.run_ptr.alloc(4 * rp.size());
         AIR_HIP(hipMemcpyAsync(T.run_ptr.p, rp.data(), 4 * rp.size(), hipMemcpyHostToDevice, s));
         AIR_HIP(hipStreamSynchronize(s));
-        // strided staging of the step-coded backward: slot stride R per tile (its longest run)
+        // strided staging of the step-coded backward: slot stride R per tile (its
+        // longest run); words address the slots with 14 bits, or 15 when a tile
+        // stages 16384 or more (up to 32767)
         if (N > 0 && T.dtype != AIR_F64 && T.bwd_flags) {
             std::vector<int> Rt(T.ntiles_b, 1);
             size_t need = 0;
-            bool fits = true;
+            int64_t maxslots = 0;
             for (int64_t t = 0; t < T.ntiles_b; ++t) {
                 for (int r = rp[t]; r < rp[t + 1]; ++r) Rt[t] = std::max(Rt[t], rl[r]);
                 const int64_t slots = (int64_t)(rp[t + 1] - rp[t]) * Rt[t] + 8;
-                fits = fits && slots < 16384;
+                maxslots = std::max(maxslots, slots);
                 need = std::max(need, 4 * (2 * (size_t)maxruns) + 4 * (size_t)slots);
             }
-            if (fits && need <= MAX_LDS_B) {
+            T.bstep_bb = maxslots < 16384 ? 14 : 15;
+            T.bstep_nw = need <= AIR_TILE_LDS_STEP_MAX ? 8 : 16;
+            if (maxslots < 32768 && need <= (T.bstep_nw == 8 ? AIR_TILE_LDS_STEP_MAX : AIR_TILE_LDS_STEP16_MAX)) {
                 T.lds_bs = need;
                 T.bstep_R.alloc(4 * Rt.size());
                 AIR_HIP(hipMemcpyAsync(T.bstep_R.p, Rt.data(), 4 * Rt.size(), hipMemcpyHostToDevice, s));
@@ -2050,8 +2079,8 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
             using V = decltype(v);
             k_bstep_walk<V, false><<<grid_for(T.n_int), BLOCK, 0, s>>>(
                 T.cell_ptr.as<int64_t>(), T.n_int, T.blr.as<unsigned short>(), T.bval.as<V>(), T.run_ptr.as<int>(),
-                T.run_off.as<int>(), T.bstep_R.as<int>(), ubuf.as<int>(), cnt.as<int64_t>(), nullptr, nullptr,
-                nullptr, bad.as<unsigned>());
+                T.run_off.as<int>(), T.bstep_R.as<int>(), T.bstep_bb, ubuf.as<int>(), cnt.as<int64_t>(), nullptr,
+                nullptr, nullptr, bad.as<unsigned>());
             return 0;
         });
         AIR_HIP(hipGetLastError());
@@ -2071,8 +2100,9 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
                 using V = decltype(v);
                 k_bstep_walk<V, true><<<grid_for(T.n_int), BLOCK, 0, s>>>(
                     T.cell_ptr.as<int64_t>(), T.n_int, T.blr.as<unsigned short>(), T.bval.as<V>(),
-                    T.run_ptr.as<int>(), T.run_off.as<int>(), T.bstep_R.as<int>(), ubuf.as<int>(), nullptr,
-                    T.bstep_ptr.as<int64_t>(), T.bsval.as<V>(), T.bsword.as<unsigned>(), bad.as<unsigned>());
+                    T.run_ptr.as<int>(), T.run_off.as<int>(), T.bstep_R.as<int>(), T.bstep_bb, ubuf.as<int>(),
+                    nullptr, T.bstep_ptr.as<int64_t>(), T.bsval.as<V>(), T.bsword.as<unsigned>(),
+                    bad.as<unsigned>());
                 return 0;
             });
             AIR_HIP(hipGetLastError());
@@ -2081,7 +2111,10 @@ bool build_tiled(const DevSparse& A, int N, TiledOp& T, hipStream_t s, bool tune
         }
         T.bstep_ok = hb == 0u;
         T.nbse = T.bstep_ok ? total : 0;
-        if (T.bstep_ok) set_bstep_waves(T, s);
+        if (T.bstep_ok) {
+            set_bstep_waves(T, s);
+            tiled_bstep_lds_attr(T);
+        }
         if (!T.bstep_ok) {
             T.bsval.release(); T.bsword.release(); T.bstep_ptr.release(); T.bstep_R.release();
             T.lds_bs = 0;
@@ -2237,14 +2270,15 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
             b.run_row = T.run_row.as<int>();
             b.run_len = T.run_len.as<int>();
             b.maxruns = T.maxruns;
+            b.bb = T.bstep_bb;
             if (MODE == 0)
-                kern<<<(unsigned)T.ntiles_b, 512, T.lds_bs, s>>>(b, T.bsval.as<V>(), y, gout, nullptr, nullptr,
-                                                                 nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f, nullptr,
-                                                                 nullptr, nullptr);
+                kern<<<(unsigned)T.ntiles_b, 64 * T.bstep_nw, T.lds_bs, s>>>(
+                    b, T.bsval.as<V>(), y, gout, nullptr, nullptr, nullptr, 0.0, 0, 0.f, nullptr, 0, 0.f, nullptr,
+                    nullptr, nullptr);
             else
-                kern<<<(unsigned)T.ntiles_b, 512, T.lds_bs, s>>>(b, T.bsval.as<V>(), y, nullptr, B->g, B->D, B->x,
-                                                                 lam, B->lb_kind, B->lb, B->lbv, B->ub_kind, B->ub,
-                                                                 B->ubv, B->partial_cols, B->ctrl);
+                kern<<<(unsigned)T.ntiles_b, 64 * T.bstep_nw, T.lds_bs, s>>>(
+                    b, T.bsval.as<V>(), y, nullptr, B->g, B->D, B->x, lam, B->lb_kind, B->lb, B->lbv, B->ub_kind,
+                    B->ub, B->ubv, B->partial_cols, B->ctrl);
         };
         if (cfg >= N_BWD_CFG) {
             int sv = cfg - N_BWD_CFG;
@@ -2263,8 +2297,14 @@ static void launch_bwd(const TiledOp& T, const float* y, float* gout, const Sirt
                 case 2: launch(k_tbwd_flat<V, MODE, 8, 16, 1>, 512); break;
                 case 3: launch(k_tbwd_flat<V, MODE, 8, 0, 0>, 512); break;
                 case 4: launch(k_tbwd_flat<V, MODE, 8, 0, 1>, 512); break;
-                case 5: launch_step(k_tbwd_step<V, MODE, 8, 0>); break;
-                default: launch_step(k_tbwd_step<V, MODE, 8, 1>); break;
+                case 5:
+                    if (T.bstep_nw == 16) launch_step(k_tbwd_step<V, MODE, 16, 0>);
+                    else launch_step(k_tbwd_step<V, MODE, 8, 0>);
+                    break;
+                default:
+                    if (T.bstep_nw == 16) launch_step(k_tbwd_step<V, MODE, 16, 1>);
+                    else launch_step(k_tbwd_step<V, MODE, 8, 1>);
+                    break;
                 }
             }
             return 0;

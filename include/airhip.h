@@ -299,6 +299,9 @@ typedef struct {
                                        lane padding included) */
     int     fwd_waves_per_block;    /* stream forward: waves sharing one staged tile of x (4, or 16 for
                                        the step-coded form) */
+    int     bwd_step_base_bits;     /* step-coded backward: bits of a word's base slot, 14 (tiles stage
+                                       fewer than 16384 slots) or 15 (fewer than 32768) */
+    int64_t bwd_step_lds_bytes;     /* dynamic LDS of the step-coded backward's strided staging */
 } air_layout;
 int air_layout_info(air_engine* e, air_layout* out);
 

@@ -66,7 +66,8 @@ def test_fanlinear_n1024_precision(oracle):
         x, info = run.fetch()
         err = rel(x, xo)
         out["dtypes"][dt] = {"rel_err_to_oracle": err, "rel_err_to_phantom": rel(x, x_true),
-                             "iter_per_s": round(K / t_iter, 1), "tiled": L["tiled"], "lds_bytes": L["lds_bytes"]}
+                             "iter_per_s": round(K / t_iter, 1), "tiled": L["tiled"], "lds_bytes": L["lds_bytes"],
+                             "bwd_step": L["bwd_step"], "bwd_step_base_bits": L["bwd_step_base_bits"]}
         del run, A
     print("C3_PRECISION " + json.dumps(out), flush=True)
     for dt, r in out["dtypes"].items():

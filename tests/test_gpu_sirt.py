@@ -492,3 +492,38 @@ def test_sampled_kernel_timing(c1, stride):
     from airtoolsii_amd._lib import AirError
     with pytest.raises(AirError):
         run.kernel_timing(-1)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-5), ("bf16", 2e-2)])
+def test_step_coded_backward_wide_staging(oracle, monkeypatch, dtype, tol):
+    """Fan beams over 720 angles stage up to 720 runs of 28 rows per 16 x 16
+    tile: more than 16384 slots (the lane words address them with 15 bits)
+    and more than 76 KB of LDS (16 waves per tile, one block per CU).  The
+    step-coded backward on that staging gives the oracle's A'y and SART
+    iterates, as the lane form does."""
+    N, theta, p, K = 128, np.arange(0.0, 360.0, 0.5), 181, 20
+    Ao = oracle.fanlineartomo(N, theta, p)
+    x = oracle.phantomgallery("threephases", N, None, 3).ravel(order="F")
+    b = Ao @ x
+    y = np.random.default_rng(5).standard_normal(Ao.shape[0])
+    opts = {"rho": 1.0, "lbound": 0}
+    Xo, _, _ = oracle.sirt("sart", Ao, b, K, None, opts)
+    out = {}
+    for cfg in ("11,10,4096,16", "11,8,4096,16"):          # step-coded backward, then the lane form
+        monkeypatch.setenv("AIR_TILED_CFG", cfg)
+        A = air.fanlineartomo(N, theta, p, dtype=dtype)
+        L = A.layout()
+        if cfg.startswith("11,10"):
+            assert L["bwd_step"] == 1 and L["bwd_step_base_bits"] == 15, L
+            assert L["bwd_step_lds_bytes"] > 76 * 1024, L
+        else:
+            assert L["bwd_step"] == 0, L
+        g = A.rmatvec(y)
+        X, _, _ = air.sart(A, b, K, None, opts)
+        out[cfg] = (g, X[:, -1])
+        del A
+    go = Ao.T @ y
+    for cfg, (g, xk) in out.items():
+        print(cfg, rel(g, go), rel(xk, Xo[:, -1]))
+        assert rel(g, go) <= (1e-6 if dtype == "f32" else 1e-2), cfg
+        assert rel(xk, Xo[:, -1]) <= tol, cfg
