@@ -498,14 +498,13 @@ def test_sampled_kernel_timing(c1, stride):
 def test_step_coded_backward_wide_staging(oracle, monkeypatch, dtype, tol):
     """Fan beams over 720 angles stage up to 720 runs of 28 rows per 16 x 16
     tile: more than 16384 slots (the lane words address them with 15 bits)
-    and more than 76 KB of LDS (16 waves per tile, one block per CU).  The
-    step-coded backward on that staging gives the oracle's A'y and SART
-    iterates, as the lane form does."""
+    and more than 76 KB of LDS (16 waves per tile, one block per CU).  SART
+    on the step-coded backward with that staging (the update pass, and the
+    line search's norm pass) gives the oracle's iterates, as the lane form does."""
     N, theta, p, K = 128, np.arange(0.0, 360.0, 0.5), 181, 20
     Ao = oracle.fanlineartomo(N, theta, p)
     x = oracle.phantomgallery("threephases", N, None, 3).ravel(order="F")
     b = Ao @ x
-    y = np.random.default_rng(5).standard_normal(Ao.shape[0])
     opts = {"rho": 1.0, "lbound": 0}
     Xo, _, _ = oracle.sirt("sart", Ao, b, K, None, opts)
     out = {}
@@ -518,12 +517,18 @@ def test_step_coded_backward_wide_staging(oracle, monkeypatch, dtype, tol):
             assert L["bwd_step_lds_bytes"] > 76 * 1024, L
         else:
             assert L["bwd_step"] == 0, L
-        g = A.rmatvec(y)
         X, _, _ = air.sart(A, b, K, None, opts)
-        out[cfg] = (g, X[:, -1])
+        out[cfg] = X[:, -1]
         del A
-    go = Ao.T @ y
-    for cfg, (g, xk) in out.items():
-        print(cfg, rel(g, go), rel(xk, Xo[:, -1]))
-        assert rel(g, go) <= (1e-6 if dtype == "f32" else 1e-2), cfg
+    for cfg, xk in out.items():
+        print(cfg, rel(xk, Xo[:, -1]))
         assert rel(xk, Xo[:, -1]) <= tol, cfg
+    if dtype == "f32":
+        # the line search's pass (the backward's norm mode) on the wide staging
+        monkeypatch.setenv("AIR_TILED_CFG", "11,10,4096,16")
+        A = air.fanlineartomo(N, theta, p, dtype=dtype)
+        lopts = {"relaxpar": "line", "rho": 1.0, "lbound": 0}
+        X, info, _ = air.sart(A, b, 10, None, lopts)
+        Xo2, info_o, _ = oracle.sirt("sart", Ao, b, 10, None, lopts)
+        assert rel(X[:, -1], Xo2[:, -1]) <= tol
+        assert abs(info["relaxpar"] - info_o["relaxpar"]) <= 1e-4 * abs(info_o["relaxpar"])
