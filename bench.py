@@ -224,8 +224,13 @@ def learned_block(args):
             "phantoms": list(Lm.KINDS), "noise": "1-5% Gaussian, seeded",
             "plain": f"lambda={lam:.4f} from train_relaxpar, DP with tau=1.02",
             "learned": "committed network, DP with tau=1.02 as the fallback stop",
+            "timing": "each method timed as one call to the first iteration at the plain run's final error "
+                      "(set-up, iterations, x to the host)",
             "time_to_plain_error_over_plain_median": summ["time_to_plain_error_over_plain_median"],
             "iters_to_plain_error_over_plain_median": summ["iters_to_plain_err_over_plain_median"],
+            "time_ratio_vs_plain_dp_run_median": summ["time_to_plain_error_over_plain_dp_run_median"],
+            "plain_time_to_err_ms_median": round(summ["plain_time_to_err_median"] * 1e3, 3),
+            "plain_dp_run_ms_median": round(summ["plain_dp_run_s_median"] * 1e3, 3),
             "reached_plain_error_frac": summ["reached_plain_err_frac"],
             "time_ratio_median_reached_only": summ["time_to_plain_error_over_plain_median_reached"],
             "time_ratio_p75": summ["time_to_plain_error_over_plain_p75"],
