@@ -1000,7 +1000,9 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
         ext_apply(e, 0, B.x, st.Ax.as<float>());
         launch_vec_resid(e->m, st.Ax.as<float>(), B, st.sp, 0, s);
     } else launch_mf_resid(e->geo, B, st.sp, 0, st.Ax.as<float>(), s);
-    AIR_HIP(hipStreamSynchronize(s));
+    // no synchronisation here: the first iterations queue behind r0 on the
+    // stream, host staging is guarded by pin_ev, and pageable host data
+    // (the lambda schedule, network parameters) is consumed by the copy call
     st.prepared = true;
     st.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st.iter_s = 0;
