@@ -1008,6 +1008,17 @@ void sirt_prepare(air_engine* e, const air_sirt_options* o, const double* b, con
     st.iter_s = 0;
 }
 
+// Iterations until the residual norm reaches target, from its decay between
+// two checks (h0 -> h1 over dk iterations), plus one; 4..maxc.
+int dp_chunk(double h0, double h1, int dk, double target, int maxc) {
+    if (!(h1 > target)) return 4;
+    if (!(h0 > h1) || dk <= 0 || !(target > 0)) return maxc;
+    const double rate = std::log(h1 / h0) / dk;          // < 0
+    const double need = std::log(target / h1) / rate;   // > 0
+    if (!(need < (double)maxc)) return maxc;
+    return std::min(maxc, std::max(4, (int)std::ceil(need) + 1));
+}
+
 int read_stop(air_engine* e) {
     Ctrl c;
     AIR_HIP(hipMemcpyAsync(&c, e->st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
@@ -1448,15 +1459,31 @@ int air_sirt_iterate(air_engine* e, int32_t niter, double* gpu_ms) {
         timing_begin(e, niter);
         AIR_HIP(hipEventRecord(a, e->stream));
         // with a rule that can stop early, check the stop flag every 32 iterations
-        // instead of enqueueing iterations that would only return
+        // instead of enqueueing iterations that would only return.  Under the
+        // discrepancy principle the next check comes where the residual's decay
+        // over the last chunk reaches tau*delta (4 to 32 iterations on), after a
+        // first chunk of 16, so few iterations are launched past the stop; a
+        // prediction within 4 of the iterations left takes them all.
         const bool early = st.sp.type != AIR_STOP_NONE || st.relax_kind == AIR_RELAX_LEARNED;
         const int chunk = graph_mode(e) ? e->graph_iters : (e->kind == 3 ? 1 : 32);
+        const bool dp = early && st.sp.type == AIR_STOP_DP && !graph_mode(e) && e->kind != 3;
         int left = std::min(niter, st.kmax - st.k);
+        int k_ref = st.k;                    // hist[k_ref] is written once the queued work is done
+        int c = early ? std::min(left, dp ? 16 : chunk) : left;
         while (left > 0) {
-            const int c = early ? std::min(left, chunk) : left;
             enqueue_iterations(e, c);
             left -= c;
-            if (early && left > 0 && read_stop(e)) break;
+            if (!early || left <= 0) break;
+            Ctrl cc;
+            double href = 0.0;
+            AIR_HIP(hipMemcpyAsync(&cc, st.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
+            if (dp) AIR_HIP(hipMemcpyAsync(&href, st.hist.as<double>() + k_ref, 8, hipMemcpyDeviceToHost, e->stream));
+            AIR_HIP(hipStreamSynchronize(e->stream));
+            if (cc.stop) break;
+            int next = chunk;
+            if (dp) next = dp_chunk(href, cc.last_norm, st.k - k_ref, st.sp.taudelta, chunk);
+            k_ref = st.k;
+            c = left - next < 4 ? left : next;
         }
         AIR_HIP(hipEventRecord(b, e->stream));
         AIR_HIP(hipEventSynchronize(b));
